@@ -1,0 +1,283 @@
+// hfm_p2g.hip — particle-to-grid transfer without float atomics (gfx950).
+//
+// The reference scatters every particle into up to ten lattices with float atomics
+// (solver.py:84-103, heat.py:12-18 via fields.py:39-48), so its sums depend on timing.  Here the
+// particles are binned by cell (counting sort: integer counts, exclusive scan, then the ids of a
+// bin sorted ascending), and every lattice node gathers its own contributions from the bins that
+// can reach it, in bin order then particle order.  The result is the same sum of the same terms
+// in a fixed order: deterministic run to run.
+//
+// A node at index a on an axis where the lattice offset is 1/2 is reached by particles in bins
+// a-1, a, a+1; where the offset is 0 (the face axis) by bins a-1, a.  Particles outside the box
+// are binned into the nearest edge bin, which is also where their clamped weights land.
+#include "hfm_device.h"
+#include "hfm_scene.h"
+
+namespace hfm {
+
+namespace {
+
+inline int nblocks(long count) { return (int)((count + kBlock - 1) / kBlock); }
+
+__device__ __forceinline__ int bin_of(float x, float h, int n) { return clampi((int)floorf(x / h), 0, n - 1); }
+
+__global__ __launch_bounds__(kBlock) void k_bin_count(long P, int n, float h, const float* pos, int* cnt, int* key,
+                                                      int* slot) {
+    long p = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= P) return;
+    int bx = bin_of(pos[3 * p], h, n), by = bin_of(pos[3 * p + 1], h, n), bz = bin_of(pos[3 * p + 2], h, n);
+    int c = (bx * n + by) * n + bz;
+    key[p] = c;
+    slot[p] = atomicAdd(&cnt[c], 1);
+}
+
+// Exclusive scan of cnt[0..count) into start[0..count], three passes: per-workgroup totals,
+// a scan of those totals, then per-workgroup scans with their offsets.
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+__device__ int block_exclusive_scan(int v, int* lds, int& total) {
+    // Hillis-Steele over the workgroup in LDS (256 entries)
+    lds[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+        int add = threadIdx.x >= off ? lds[threadIdx.x - off] : 0;
+        __syncthreads();
+        lds[threadIdx.x] += add;
+        __syncthreads();
+    }
+    int incl = lds[threadIdx.x];
+    total = lds[kBlock - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(const int* in, long count, int* tile_sums) {
+    __shared__ int lds[kBlock];
+    long base = (long)blockIdx.x * kScanTile + (long)threadIdx.x * kScanItems;
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q)
+        if (base + q < count) s += in[base + q];
+    int total;
+    block_exclusive_scan(s, lds, total);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_sums(int* tile_sums, int ntiles) {
+    __shared__ int lds[kBlock];
+    int carry = 0;
+    for (int b0 = 0; b0 < ntiles; b0 += kBlock) {
+        int idx = b0 + threadIdx.x;
+        int v = idx < ntiles ? tile_sums[idx] : 0;
+        int total;
+        int ex = block_exclusive_scan(v, lds, total);
+        if (idx < ntiles) tile_sums[idx] = carry + ex;
+        carry += total;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_final(const int* in, long count, const int* tile_offsets, int* out) {
+    __shared__ int lds[kBlock];
+    long base = (long)blockIdx.x * kScanTile + (long)threadIdx.x * kScanItems;
+    int vals[kScanItems];
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q) {
+        vals[q] = base + q < count ? in[base + q] : 0;
+        s += vals[q];
+    }
+    int total;
+    int ex = block_exclusive_scan(s, lds, total) + tile_offsets[blockIdx.x];
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q) {
+        if (base + q <= count) out[base + q] = ex;  // out has count + 1 entries
+        ex += vals[q];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bin_fill(long P, const int* key, const int* slot, const int* start,
+                                                     int* sorted) {
+    long p = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= P) return;
+    sorted[start[key[p]] + slot[p]] = (int)p;
+}
+
+// Ascending particle ids inside each bin (bins hold a handful of particles).
+__global__ __launch_bounds__(kBlock) void k_bin_sort(long N, const int* start, int* sorted) {
+    long c = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= N) return;
+    int b = start[c], e = start[c + 1];
+    for (int x = b + 1; x < e; ++x) {
+        int v = sorted[x];
+        int y = x - 1;
+        while (y >= b && sorted[y] > v) {
+            sorted[y + 1] = sorted[y];
+            --y;
+        }
+        sorted[y + 1] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// gather: gridDim.y = 0 cell lattice (alpha_s, alpha_p, tsum), 1..3 face lattices (q, K, m)
+// ---------------------------------------------------------------------------------------------
+struct GatherArgs {
+    int n;
+    long N, F;
+    float h, frac, k0;
+    const float *pos, *vel, *temp;
+    const int *start, *sorted;
+    const float* st_alpha;
+    const float* st_k[3];
+    const float* st_m[3];
+    float *alpha_s, *alpha_p, *tsum;
+    float* q[3];
+    float* kf[3];
+    float* mf[3];
+};
+
+// Weight with which particle (px,py,pz) reaches node (ni,nj,nk) of a lattice with offsets
+// (ox,oy,oz) and dims (dx,dy,dz): the sum over the corners whose clamped index is the node,
+// each added separately in the corner order of fields.py:41-48.
+__device__ __forceinline__ void node_terms(float px, float py, float pz, float ox, float oy, float oz, float h,
+                                           int dx, int dy, int dz, int ni, int nj, int nk, float* w_out,
+                                           int& count) {
+    int bi, bj, bk;
+    float fx, fy, fz;
+    base_index1(px, ox, h, bi, fx);
+    base_index1(py, oy, h, bj, fy);
+    base_index1(pz, oz, h, bk, fz);
+    count = 0;
+#pragma unroll
+    for (int cx = 0; cx < 2; ++cx) {
+        if (clampi(bi + cx, 0, dx - 1) != ni) continue;
+        float wx = cx ? fx : 1.0f - fx;
+#pragma unroll
+        for (int cy = 0; cy < 2; ++cy) {
+            if (clampi(bj + cy, 0, dy - 1) != nj) continue;
+            float wy = cy ? fy : 1.0f - fy;
+#pragma unroll
+            for (int cz = 0; cz < 2; ++cz) {
+                if (clampi(bk + cz, 0, dz - 1) != nk) continue;
+                float wz = cz ? fz : 1.0f - fz;
+                float w = wx * wy;
+                w_out[count++] = w * wz;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_p2g_gather(GatherArgs a) {
+    const int lat = blockIdx.y;  // 0 cell, 1 + axis for faces
+    const int n = a.n;
+    long node = (long)xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    const long count = lat == 0 ? a.N : a.F;
+    if (node >= count) return;
+    int dx = n, dy = n, dz = n;
+    float ox = 0.5f, oy = 0.5f, oz = 0.5f;
+    int axis = lat - 1;
+    if (axis == 0) { dx = n + 1; ox = 0.0f; }
+    if (axis == 1) { dy = n + 1; oy = 0.0f; }
+    if (axis == 2) { dz = n + 1; oz = 0.0f; }
+    int ni = (int)(node / ((long)dy * dz)), nj = (int)((node / dz) % dy), nk = (int)(node % dz);
+    // candidate bins per axis
+    int lo_x = ni - 1, hi_x = ox == 0.0f ? ni : ni + 1;
+    int lo_y = nj - 1, hi_y = oy == 0.0f ? nj : nj + 1;
+    int lo_z = nk - 1, hi_z = oz == 0.0f ? nk : nk + 1;
+    lo_x = lo_x < 0 ? 0 : lo_x; hi_x = hi_x > n - 1 ? n - 1 : hi_x;
+    lo_y = lo_y < 0 ? 0 : lo_y; hi_y = hi_y > n - 1 ? n - 1 : hi_y;
+    lo_z = lo_z < 0 ? 0 : lo_z; hi_z = hi_z > n - 1 ? n - 1 : hi_z;
+    float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f;  // cell: alpha, tsum ; face: q, K, m
+    const float h = a.h;
+    for (int bx = lo_x; bx <= hi_x; ++bx)
+        for (int by = lo_y; by <= hi_y; ++by) {
+            long row = ((long)bx * n + by) * n;
+            int b = a.start[row + lo_z], e = a.start[row + hi_z + 1];
+            for (int s = b; s < e; ++s) {
+                int p = a.sorted[s];
+                float px = a.pos[3 * p], py = a.pos[3 * p + 1], pz = a.pos[3 * p + 2];
+                float w[8];
+                int cnt;
+                node_terms(px, py, pz, ox, oy, oz, h, dx, dy, dz, ni, nj, nk, w, cnt);
+                if (cnt == 0) continue;
+                if (lat == 0) {
+                    float vt = a.frac * a.temp[p];
+                    for (int q = 0; q < cnt; ++q) {
+                        acc0 += w[q] * a.frac;
+                        acc1 += w[q] * vt;
+                    }
+                } else {
+                    float vq = a.frac * a.vel[3 * p + axis];
+                    float vm = a.k0 * a.vel[3 * p + axis];
+                    for (int q = 0; q < cnt; ++q) {
+                        acc0 += w[q] * vq;
+                        acc1 += w[q] * a.k0;
+                        acc2 += w[q] * vm;
+                    }
+                }
+            }
+        }
+    if (lat == 0) {
+        a.alpha_p[node] = acc0;
+        a.tsum[node] = acc1;
+        a.alpha_s[node] = acc0 + a.st_alpha[node];
+    } else {
+        int idx = axis == 0 ? ni : (axis == 1 ? nj : nk);
+        a.q[axis][node] = (idx == 0 || idx == n) ? 0.0f : acc0;
+        a.kf[axis][node] = acc1 + a.st_k[axis][node];
+        a.mf[axis][node] = acc2 + a.st_m[axis][node];
+    }
+}
+
+}  // namespace
+
+void launch_bin_particles(Scene* s) {
+    const long N = s->g.N, P = s->P;
+    (void)hipMemsetAsync(s->bin_cnt, 0, (N + 1) * sizeof(int), s->stream);
+    if (P > 0)
+        hipLaunchKernelGGL(k_bin_count, dim3(nblocks(P)), dim3(kBlock), 0, s->stream, P, s->g.n, s->g.h, s->pos,
+                           s->bin_cnt, s->pkey, s->pslot);
+    long count = N;  // scan N counts into N + 1 starts
+    int ntiles = (int)((count + 1 + kScanTile - 1) / kScanTile);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(kBlock), 0, s->stream, s->bin_cnt, count, s->scan_tmp);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, s->stream, s->scan_tmp, ntiles);
+    hipLaunchKernelGGL(k_scan_final, dim3(ntiles), dim3(kBlock), 0, s->stream, s->bin_cnt, count, s->scan_tmp,
+                       s->bin_start);
+    if (P > 0) {
+        hipLaunchKernelGGL(k_bin_fill, dim3(nblocks(P)), dim3(kBlock), 0, s->stream, P, s->pkey, s->pslot,
+                           s->bin_start, s->sorted);
+        hipLaunchKernelGGL(k_bin_sort, dim3(nblocks(N)), dim3(kBlock), 0, s->stream, N, s->bin_start, s->sorted);
+    }
+    s->bins_dirty = 0;
+}
+
+void launch_p2g(Scene* s) {
+    GatherArgs a;
+    a.n = s->g.n;
+    a.N = s->g.N;
+    a.F = s->g.F;
+    a.h = s->g.h;
+    a.frac = (float)s->prm.vol / (a.h * a.h * a.h);
+    a.k0 = (float)s->prm.k_particle;
+    a.pos = s->pos;
+    a.vel = s->vel;
+    a.temp = s->temp;
+    a.start = s->bin_start;
+    a.sorted = s->sorted;
+    a.st_alpha = s->st_alpha;
+    a.alpha_s = s->alpha_s;
+    a.alpha_p = s->alpha_p;
+    a.tsum = s->tsum;
+    for (int ax = 0; ax < 3; ++ax) {
+        a.st_k[ax] = s->st_k[ax];
+        a.st_m[ax] = s->st_m[ax];
+        a.q[ax] = s->q[ax];
+        a.kf[ax] = s->kf[ax];
+        a.mf[ax] = s->mf[ax];
+    }
+    hipLaunchKernelGGL(k_p2g_gather, dim3(nblocks(a.F), 4), dim3(kBlock), 0, s->stream, a);
+}
+
+}  // namespace hfm

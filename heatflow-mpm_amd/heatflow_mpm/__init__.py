@@ -1,0 +1,14 @@
+"""Heatflow-MPM air/heat substep on AMD MI355X (gfx950): HIP kernels behind a ctypes C ABI.
+
+Public names follow the reference package (heatflow_mpm/__init__.py:2-7).  The mesh seeding of
+shapes.py and Newton's ThermalMPM are not part of this path: particles come from a body object
+(StaticBody by default) and `scenes.ellipsoid_particles` seeds synthetic material.
+"""
+from .solver import HeatflowMPM, KAPPA_AIR, K_PARTICLE, scene_inputs  # noqa: F401
+from .grid import GridConfig, AirGrid  # noqa: F401
+from .heat import HeatField  # noqa: F401
+from .heater import Heater  # noqa: F401
+from .mpm import Material, StaticBody  # noqa: F401
+from .scenes import ellipsoid_particles, plume_scene, coupled_scene  # noqa: F401
+
+__version__ = "0.2.0"

@@ -1,0 +1,183 @@
+/* hfm.h — C ABI of libhfm.so, the MI355X (gfx950) air/heat substep of Heatflow-MPM.
+ *
+ * The reference exposes this path as the Python class heatflow_mpm.solver.HeatflowMPM
+ * (heatflow_mpm/solver.py:106-175) on top of AirGrid (heatflow_mpm/grid.py:177-263) and
+ * HeatField (heatflow_mpm/heat.py:177-210).  Every entry point below replaces one piece of
+ * that class; the Python package heatflow_mpm (heatflow-mpm_amd/heatflow_mpm) binds them with
+ * ctypes and keeps the reference's method names.
+ *
+ * Conventions
+ *   - Every function returns 0 (HFM_OK) or a negative HFM_ERR_* code; hfm_last_error() gives
+ *     the message of the last failure on the calling thread.
+ *   - Host buffers are owned by the caller and are copied; device buffers are owned by the
+ *     scene.  All arrays are C-order with the z index fastest, like the reference's Warp arrays:
+ *       cell lattice   (n,   n,   n  )  flat index (i*n + j)*n + k
+ *       x-face lattice (n+1, n,   n  )  flat index (i*n + j)*n + k
+ *       y-face lattice (n,   n+1, n  )  flat index (i*(n+1) + j)*n + k
+ *       z-face lattice (n,   n,   n+1)  flat index (i*n + j)*(n+1) + k
+ *   - One host thread per scene.  Each scene owns one HIP stream (default priority); different
+ *     scenes may be driven from different threads concurrently.
+ *   - CG non-convergence is reported in hfm_stats, not as an error (the reference discards
+ *     the residual: grid.py:248, heat.py:209).
+ */
+#ifndef HFM_H
+#define HFM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HFM_OK 0
+#define HFM_ERR_ARG (-1)       /* bad argument (null pointer, size, field id) */
+#define HFM_ERR_HIP (-2)       /* a HIP runtime call failed */
+#define HFM_ERR_OOM (-3)       /* device allocation failed */
+#define HFM_ERR_NONFINITE (-4) /* a non-finite value was detected in inputs or state */
+#define HFM_ERR_STATE (-5)     /* call not valid in the current scene state */
+
+/* Scene parameters.  Field names follow GridConfig (grid.py:13-22), Material (mpm.py:9-22),
+ * the module constants of solver.py:15-16 and the hard-coded solve settings of
+ * grid.py:248 / heat.py:198-209. */
+typedef struct hfm_params {
+    int32_t n;                /* cells per side, int(round(L/h)) (grid.py:183) */
+    int32_t buoyancy;         /* GridConfig.buoyancy (grid.py:19) */
+    int32_t cg_max_iter;      /* GridConfig.cg_max_iter, pressure PCG (grid.py:22) */
+    int32_t cg_check_every;   /* 20 (grid.py:248) */
+    int32_t heat_max_iter;    /* 200 (heat.py:198) */
+    int32_t heat_check_every; /* 10 (heat.py:209) */
+    int32_t precision;        /* HFM_PREC_* */
+    int32_t flags;            /* HFM_FLAG_* */
+    double h;                 /* cell size [m] (grid.py:16) */
+    double rho;               /* air density (grid.py:17) */
+    double t_amb;             /* room temperature [C] (grid.py:18) */
+    double gravity;           /* 9.81 (grid.py:219) */
+    double alpha_f_min;       /* floor of the air fraction (grid.py:20) */
+    double cg_tol;            /* pressure PCG relative tolerance (grid.py:21) */
+    double heat_tol;          /* heat PCG relative tolerance, 1e-6 (heat.py:198) */
+    double kappa_air;         /* KAPPA_AIR or the constructor argument (solver.py:15,108) */
+    double kappa_mat;         /* Material.kappa (mpm.py:17) */
+    double vol;               /* particle volume spacing^3 (solver.py:130) */
+    double k_particle;        /* K_PARTICLE (solver.py:16) */
+    double t_sol, t_liq;      /* melt window (mpm.py:12-13) */
+    double yield_solid;       /* mpm.py:14 */
+    double mu_solid;          /* viscosity_solid (mpm.py:15) */
+    double mu_liquid;         /* viscosity_liquid (mpm.py:16) */
+    double t_max;             /* max(t_rod, t_cap, t_amb, t_init) (solver.py:122) */
+} hfm_params;
+
+#define HFM_PREC_FP32 0       /* the reference's arithmetic: fp32 storage and math */
+
+#define HFM_FLAG_TIME_KERNELS 1  /* record HIP events around every CG kernel launch */
+
+/* Per-substep statistics of the last hfm_substep. */
+typedef struct hfm_stats {
+    int32_t p_iters;          /* pressure PCG count as warp.optim.linear.cg returns it
+                                 (multiple of check_every, or the iteration cap): AirGrid.cg_iters */
+    int32_t p_iters_exact;    /* iterations that changed the pressure */
+    int32_t h_iters;          /* heat PCG count as warp returns it: HeatField.cg_iters */
+    int32_t h_iters_exact;
+    int32_t p_converged;      /* 1 when ||r||^2 <= atol^2 at exit */
+    int32_t h_converged;
+    int64_t substeps;         /* substeps run since creation */
+    double p_rr;              /* final ||r||^2 of the pressure solve */
+    double p_atol;            /* max(tol * ||b||, tol) */
+    double h_rr;
+    double h_atol;
+    double rhs_mean;          /* mean removed from the pressure rhs (grid.py:239) */
+} hfm_stats;
+
+/* Field identifiers for hfm_get_field / hfm_set_field. */
+enum hfm_field {
+    HFM_FIELD_T = 0,      /* cell temperature, HeatField.T (heat.py:181) */
+    HFM_FIELD_T2,         /* HeatField.T2: after a substep, the previous T (swap, solver.py:150) */
+    HFM_FIELD_TW,         /* blended/fixed/sourced T before advection (solver.py:139-142) */
+    HFM_FIELD_U,          /* x-face velocity (grid.py:186) */
+    HFM_FIELD_V,          /* y-face velocity */
+    HFM_FIELD_W,          /* z-face velocity */
+    HFM_FIELD_P,          /* cell pressure, warm start of the next solve (grid.py:192,248) */
+    HFM_FIELD_RHS,        /* pressure rhs after mean removal (grid.py:237-241) */
+    HFM_FIELD_ALPHA_S,    /* static + particle solid fraction (grid.py:192,206-213) */
+    HFM_FIELD_ALPHA_P,    /* particle-only solid fraction, HeatField.alpha (heat.py:12-18) */
+    HFM_FIELD_TSUM,       /* particle heat sum, HeatField.tsum */
+    HFM_FIELD_KAPPA,      /* cell diffusivity (heat.py:44-47) */
+    HFM_FIELD_AX, HFM_FIELD_AY, HFM_FIELD_AZ,   /* face air fractions (grid.py:46-59) */
+    HFM_FIELD_QX, HFM_FIELD_QY, HFM_FIELD_QZ,   /* particle flux, zeroed on walls (solver.py:95-97) */
+    HFM_FIELD_KX, HFM_FIELD_KY, HFM_FIELD_KZ,   /* relaxation weights, particle + static */
+    HFM_FIELD_MX, HFM_FIELD_MY, HFM_FIELD_MZ,   /* relaxation momenta, particle + static */
+    HFM_FIELD_PDIAG,      /* pressure Jacobi preconditioner (grid.py:118-137) */
+    HFM_FIELD_HRHS,       /* heat rhs (heat.py:108-118) */
+    HFM_FIELD_HDIAG,      /* heat Jacobi preconditioner (heat.py:121-144) */
+    HFM_FIELD_COUNT
+};
+
+typedef struct hfm_scene hfm_scene;
+
+/* HeatflowMPM.__init__ allocation part: AirGrid + HeatField state (grid.py:181-196,
+ * heat.py:178-187).  T, T2, Tw start at t_amb, velocities and pressure at zero. */
+int hfm_scene_create(const hfm_params* params, int device, hfm_scene** out);
+int hfm_scene_destroy(hfm_scene* s);
+
+/* AirGrid.set_static (grid.py:198-200): static solid fraction (cell lattice) and static
+ * relaxation weights / momenta per face lattice, added every substep (grid.py:206-213). */
+int hfm_set_static(hfm_scene* s, const float* alpha, const float* kx, const float* ky,
+                   const float* kz, const float* mx, const float* my, const float* mz);
+
+/* HeatField.set_fixed (heat.py:189-192): Dirichlet cells; also applies them to T. */
+int hfm_set_fixed(hfm_scene* s, const int32_t* fixed, const float* tfix);
+
+/* HeatField.set_source (heat.py:194-196): capped volumetric source on mask cells. */
+int hfm_set_source(hfm_scene* s, const int32_t* mask, double rate, double t_cap);
+
+/* The material enters as particle positions, velocities and temperatures (the state the
+ * reference reads from ThermalMPM: mpm.py:54-60 and solver.py:125).  P may be 0. */
+int hfm_set_particles(hfm_scene* s, int64_t P, const float* pos, const float* vel,
+                      const float* temp);
+
+/* Positions/velocities only (a moving body between substeps); temperatures are kept. */
+int hfm_update_particles(hfm_scene* s, const float* pos, const float* vel);
+
+/* HeatflowMPM.substep(dt) without Newton's b.step (solver.py:128-151). */
+int hfm_substep(hfm_scene* s, double dt);
+
+/* K substeps back to back; *ms (may be NULL) receives the HIP-event time on the scene's
+ * stream from before the first to after the last substep. */
+int hfm_run(hfm_scene* s, double dt, int32_t K, double* ms);
+
+/* Copy a field to / from the host (hfm_field; lattice size per the table above). */
+int hfm_get_field(hfm_scene* s, int32_t field, float* out);
+int hfm_set_field(hfm_scene* s, int32_t field, const float* in);
+int64_t hfm_field_size(const hfm_scene* s, int32_t field);
+
+/* AirGrid.cell_velocity (grid.py:156-162,253-255): (n^3, 3) cell-centred velocity. */
+int hfm_get_cell_velocity(hfm_scene* s, float* out);
+
+/* Per-particle temperature, melt fraction, yield stress and viscosity (heat.py:161-174). */
+int hfm_get_particles(hfm_scene* s, float* temp, float* melt, float* yield_stress,
+                      float* viscosity);
+
+/* AirGrid.divergence_max (grid.py:257-260) and AirGrid.speed_max (grid.py:262-263). */
+int hfm_diagnostics(hfm_scene* s, double* divergence_max, double* speed_max);
+
+int hfm_get_stats(hfm_scene* s, hfm_stats* out);
+
+/* With HFM_FLAG_TIME_KERNELS set at creation: total time and launch count of one kernel
+ * class (HFM_KT_*) since the last reset. */
+#define HFM_KT_PCG_DIR 0      /* pressure PCG pass A: direction update + SpMV + p.Ap */
+#define HFM_KT_PCG_UPD 1      /* pressure PCG pass B: x, r update + r.r, r.z */
+#define HFM_KT_COUNT 2
+int hfm_kernel_time(hfm_scene* s, int32_t kernel_class, double* total_ms, int64_t* launches);
+int hfm_kernel_time_reset(hfm_scene* s);
+
+int hfm_synchronize(hfm_scene* s);
+
+/* Number of HIP devices visible (0 without a GPU). */
+int hfm_device_count(void);
+
+const char* hfm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HFM_H */
