@@ -145,27 +145,49 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double (*lds)[kWaves]
     __syncthreads();
 }
 
+// Cross-workgroup traffic of the reductions uses relaxed agent-scope atomics only: they are
+// coherent across the XCDs without the L2 write-back / invalidate that an agent-scope release /
+// acquire fence costs.  Ordering between a partial's store and the arrival ticket comes from
+// waiting for the store to complete (s_waitcnt 0) before the ticket is taken.
 __device__ __forceinline__ void store_agent(double* p, double v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ double load_agent(const double* p) {
     return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ void wait_stores() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+// arrival ticket: returns the number of workgroups that arrived before this one
+__device__ __forceinline__ unsigned take_ticket(unsigned* ticket) {
+    wait_stores();
+    return __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
-// Grid-wide deterministic sum.  Every workgroup stores its partial (partials[q*nb + b]); the
-// workgroup that arrives last (ticket) sums all partials in index order with a fixed tree and
-// returns true in all its threads with the totals in out[] (valid in thread 0).  The ticket is
-// reset by that workgroup, so the counter is ready for the next launch.
+// Grid-wide deterministic sum in two levels.  Workgroup b stores its partial and takes a ticket
+// in bucket b % B (B = min(nb, kBuckets)); the last arrival of a bucket sums that bucket's
+// partials (in index order) and takes a ticket in the top counter; the last bucket leader sums
+// the bucket sums in bucket order.  Spreading the arrivals over kBuckets counters keeps them
+// from serialising on one address.  Returns true in all threads of the one workgroup that holds
+// the totals (in thread 0's out[]).  Tickets are reset by their last user, ready for the next
+// launch.  partials needs NV * (nb + kBuckets) doubles; tickets kBuckets + 1 counters.
+constexpr int kBuckets = 64;
+
 template <int NV>
-__device__ bool grid_sum_last(double (&v)[NV], double* __restrict__ partials, unsigned* ticket, int nb,
+__device__ bool grid_sum_last(double (&v)[NV], double* __restrict__ partials, unsigned* tickets, int nb,
                               double (*lds)[kWaves], double (&out)[NV]) {
     __shared__ int is_last;
+    const int B = nb < kBuckets ? nb : kBuckets;
+    const int bucket = blockIdx.x % B;
+    const int members = nb / B + (bucket < nb % B ? 1 : 0);
+    double* bpart = partials + NV * nb;
     block_sum<NV>(v, lds);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) store_agent(&partials[q * nb + blockIdx.x], v[q]);
-        unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (t == (unsigned)(nb - 1));
+        is_last = (take_ticket(&tickets[bucket]) == (unsigned)(members - 1));
     }
     __syncthreads();
     if (!is_last) return false;
@@ -173,14 +195,25 @@ __device__ bool grid_sum_last(double (&v)[NV], double* __restrict__ partials, un
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
         double s = 0.0;
-        for (int b = threadIdx.x; b < nb; b += kBlock) s += load_agent(&partials[q * nb + b]);
+        for (int m = threadIdx.x; m < members; m += kBlock) s += load_agent(&partials[q * nb + bucket + B * m]);
         acc[q] = s;
     }
     block_sum<NV>(acc, lds);
     if (threadIdx.x == 0) {
 #pragma unroll
+        for (int q = 0; q < NV; ++q) store_agent(&bpart[q * kBuckets + bucket], acc[q]);
+        __hip_atomic_store(&tickets[bucket], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        is_last = (take_ticket(&tickets[kBuckets]) == (unsigned)(B - 1));
+    }
+    __syncthreads();
+    if (!is_last) return false;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) acc[q] = threadIdx.x < B ? load_agent(&bpart[q * kBuckets + threadIdx.x]) : 0.0;
+    block_sum<NV>(acc, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
         for (int q = 0; q < NV; ++q) out[q] = acc[q];
-        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&tickets[kBuckets], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     return true;
 }

@@ -271,6 +271,7 @@ struct MeanArgs {
 
 constexpr int kChunk = 8192;
 constexpr int kMaxLeaves = 256;
+constexpr int kMaxChunksLds = 2048;
 
 __device__ float leaf_sum(const float* a, int len) {
     if (len < 8) {
@@ -317,14 +318,26 @@ __global__ __launch_bounds__(kBlock) void k_rhs_mean(MeanArgs a) {
             }
         }
         __hip_atomic_store(&a.chunk_sums[c], stk[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned tk = __hip_atomic_fetch_add(&a.ctl->ticket2, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (tk == (unsigned)(a.nchunks - 1));
+        is_last = (take_ticket(&a.ctl->ticket2) == (unsigned)(a.nchunks - 1));
     }
     __syncthreads();
-    if (!is_last || threadIdx.x != 0) return;
-    float s = 0.0f;
-    for (int q = 0; q < a.nchunks; ++q)
-        s += __hip_atomic_load(&a.chunk_sums[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!is_last) return;
+    // the chunk sums are loaded by all threads at once, then added in chunk order by one thread
+    __shared__ float cs[kMaxChunksLds];
+    for (int q0 = 0; q0 < a.nchunks; q0 += kMaxChunksLds) {
+        int m = a.nchunks - q0 < kMaxChunksLds ? a.nchunks - q0 : kMaxChunksLds;
+        for (int q = threadIdx.x; q < m; q += kBlock)
+            cs[q] = __hip_atomic_load(&a.chunk_sums[q0 + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float s = q0 == 0 ? 0.0f : leaves[0];
+            for (int q = 0; q < m; ++q) s += cs[q];
+            leaves[0] = s;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    float s = leaves[0];
     float mean = (float)((double)s / (double)a.N);
     a.ctl->mean = mean;
     __hip_atomic_store(&a.mirror->mean, (double)mean, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);

@@ -46,6 +46,7 @@ struct PcgArgs {
     CgMirror* mirror;
     double* partials;
     int remove_mean;       // pressure init: b -= mean when mean != 0 (grid.py:240-241)
+    int report;            // upd: last iteration of an enqueued batch, post the progress
 };
 
 template <int OP>
@@ -115,22 +116,37 @@ __device__ __forceinline__ void cell_ijk(long t, int n, int& i, int& j, int& k) 
     k = (int)(t % n);
 }
 
-__device__ __forceinline__ void mirror_progress(CgMirror* m, const CgCtl* c) {
+// Publish progress to pinned host memory.  Relaxed system-scope stores go straight to host
+// memory.  While the solve runs only the iteration count is posted, by the last iteration of
+// each batch the host enqueued (`report`), without waiting; at the end the results are stored,
+// their completion waited for, and only then `done` — so the host never sees `done` before the
+// results, and no system-scope release fence (a whole-L2 write-back) is used.
+__device__ __forceinline__ void mirror_progress(CgMirror* m, const CgCtl* c, int report) {
+    if (!c->done) {
+        if (report) __hip_atomic_store(&m->it, c->it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        return;
+    }
     __hip_atomic_store(&m->rr, (double)c->rr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(&m->atol, c->atol, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(&m->converged, c->converged, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    __hip_atomic_store(&m->it, c->it, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-    if (c->done) __hip_atomic_store(&m->done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&m->it, c->it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    wait_stores();
+    __hip_atomic_store(&m->done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// Cells of logical workgroup L: CPT consecutive tiles of kBlock cells.
+#define FOR_CELLS(CPT, nb)                                                   \
+    const int L_ = xcd_tile(blockIdx.x, nb);                                 \
+    _Pragma("unroll") for (int u_ = 0; u_ < CPT; ++u_)                       \
+        if (long t = ((long)L_ * CPT + u_) * kBlock + threadIdx.x; t < a.N)
+
 // r = b - A x ; z = D r ; sums b.b, r.r, r.z ; then atol and the convergence test
-template <int OP>
+template <int OP, int CPT>
 __global__ __launch_bounds__(kBlock) void k_pcg_init(PcgArgs a) {
     __shared__ double lds[3][kWaves];
     const int nb = gridDim.x;
-    long t = (long)xcd_tile(blockIdx.x, nb) * kBlock + threadIdx.x;
     double v[3] = {0.0, 0.0, 0.0};
-    if (t < a.N) {
+    FOR_CELLS(CPT, nb) {
         int i, j, k;
         cell_ijk(t, a.n, i, j, k);
         float bc = a.b[t];
@@ -149,12 +165,12 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(PcgArgs a) {
         float zc = a.D[t] * rc;
         a.r[t] = rc;
         a.z[t] = zc;
-        v[0] = (double)(bc * bc);
-        v[1] = (double)(rc * rc);
-        v[2] = (double)(rc * zc);
+        v[0] += (double)(bc * bc);
+        v[1] += (double)(rc * rc);
+        v[2] += (double)(rc * zc);
     }
     double tot[3];
-    if (!grid_sum_last<3>(v, a.partials, &a.ctl->ticket, nb, lds, tot)) return;
+    if (!grid_sum_last<3>(v, a.partials, a.ctl->tickets, nb, lds, tot)) return;
     if (threadIdx.x != 0) return;
     CgCtl* c = a.ctl;
     float bb = (float)tot[0], rr = (float)tot[1], rz = (float)tot[2];
@@ -170,20 +186,19 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(PcgArgs a) {
     c->first = 1;
     c->converged = rr <= c->atol_sq;
     c->done = c->converged;
-    mirror_progress(a.mirror, c);
+    mirror_progress(a.mirror, c, 1);
 }
 
 // pass A: direction update at the cell and its neighbours, SpMV, p.Ap ; then alpha
-template <int OP>
+template <int OP, int CPT>
 __global__ __launch_bounds__(kBlock) void k_pcg_dir(PcgArgs a) {
     __shared__ double lds[1][kWaves];
     if (a.ctl->done) return;
     const int nb = gridDim.x;
-    long t = (long)xcd_tile(blockIdx.x, nb) * kBlock + threadIdx.x;
     const float beta = a.ctl->beta;
     const int first = a.ctl->first;
     double v[1] = {0.0};
-    if (t < a.N) {
+    FOR_CELLS(CPT, nb) {
         int i, j, k;
         cell_ijk(t, a.n, i, j, k);
         const float* zs = a.z;
@@ -195,10 +210,10 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(PcgArgs a) {
         float apc = Operator<OP>::combine(a, acc, pc, fixed, 1.0f);
         a.d_new[t] = pc;
         a.Ap[t] = apc;
-        v[0] = (double)(pc * apc);
+        v[0] += (double)(pc * apc);
     }
     double tot[1];
-    if (!grid_sum_last<1>(v, a.partials, &a.ctl->ticket, nb, lds, tot)) return;
+    if (!grid_sum_last<1>(v, a.partials, a.ctl->tickets, nb, lds, tot)) return;
     if (threadIdx.x != 0) return;
     CgCtl* c = a.ctl;
     float pAp = (float)tot[0];
@@ -206,26 +221,25 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(PcgArgs a) {
 }
 
 // pass B: x, r update, z = D r, sums r.r and r.z ; then beta and the convergence test
-template <int OP>
+template <int OP, int CPT>
 __global__ __launch_bounds__(kBlock) void k_pcg_upd(PcgArgs a) {
     __shared__ double lds[2][kWaves];
     if (a.ctl->done) return;
     const int nb = gridDim.x;
-    long t = (long)xcd_tile(blockIdx.x, nb) * kBlock + threadIdx.x;
     const float alpha = a.ctl->alpha;
     double v[2] = {0.0, 0.0};
-    if (t < a.N) {
+    FOR_CELLS(CPT, nb) {
         float xc = a.x[t] + alpha * a.d_new[t];
         float rc = a.r[t] - alpha * a.Ap[t];
         float zc = a.D[t] * rc;
         a.x[t] = xc;
         a.r[t] = rc;
         a.z[t] = zc;
-        v[0] = (double)(rc * rc);
-        v[1] = (double)(rc * zc);
+        v[0] += (double)(rc * rc);
+        v[1] += (double)(rc * zc);
     }
     double tot[2];
-    if (!grid_sum_last<2>(v, a.partials, &a.ctl->ticket, nb, lds, tot)) return;
+    if (!grid_sum_last<2>(v, a.partials, a.ctl->tickets, nb, lds, tot)) return;
     if (threadIdx.x != 0) return;
     CgCtl* c = a.ctl;
     float rr = (float)tot[0], rz = (float)tot[1];
@@ -240,7 +254,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_upd(PcgArgs a) {
     } else if (c->it >= c->maxeff) {
         c->done = 1;
     }
-    mirror_progress(a.mirror, c);
+    mirror_progress(a.mirror, c, a.report);
 }
 
 PcgArgs make_args(Scene* s, int op, float dt) {
@@ -272,41 +286,51 @@ PcgArgs make_args(Scene* s, int op, float dt) {
     a.mirror = &s->mirror_dev[op];
     a.partials = s->partials;
     a.remove_mean = op == 1;
+    a.report = 0;
     return a;
 }
 
 }  // namespace
 
-int pcg_blocks(const Scene* s) { return (int)((s->g.N + kBlock - 1) / kBlock); }
+int pcg_blocks(const Scene* s) {
+    long tiles = (s->g.N + kBlock - 1) / kBlock;
+    int cpt = s->pcg_cpt > 0 ? s->pcg_cpt : 1;
+    return (int)((tiles + cpt - 1) / cpt);
+}
+
+// dispatch over (operator, cells per thread)
+#define PCG_LAUNCH(KERNEL, s, op, a)                                                                         \
+    do {                                                                                                    \
+        dim3 grid_(pcg_blocks(s)), block_(kBlock);                                                          \
+        int cpt_ = (s)->pcg_cpt;                                                                            \
+        if (op == 1) {                                                                                      \
+            if (cpt_ == 4) hipLaunchKernelGGL((KERNEL<1, 4>), grid_, block_, 0, (s)->stream, a);            \
+            else if (cpt_ == 2) hipLaunchKernelGGL((KERNEL<1, 2>), grid_, block_, 0, (s)->stream, a);       \
+            else hipLaunchKernelGGL((KERNEL<1, 1>), grid_, block_, 0, (s)->stream, a);                      \
+        } else {                                                                                            \
+            if (cpt_ == 4) hipLaunchKernelGGL((KERNEL<0, 4>), grid_, block_, 0, (s)->stream, a);            \
+            else if (cpt_ == 2) hipLaunchKernelGGL((KERNEL<0, 2>), grid_, block_, 0, (s)->stream, a);       \
+            else hipLaunchKernelGGL((KERNEL<0, 1>), grid_, block_, 0, (s)->stream, a);                      \
+        }                                                                                                   \
+    } while (0)
 
 void launch_pcg_init(Scene* s, int op, float dt) {
     PcgArgs a = make_args(s, op, dt);
-    dim3 grid(pcg_blocks(s)), block(kBlock);
-    if (op == 1)
-        hipLaunchKernelGGL(k_pcg_init<1>, grid, block, 0, s->stream, a);
-    else
-        hipLaunchKernelGGL(k_pcg_init<0>, grid, block, 0, s->stream, a);
+    PCG_LAUNCH(k_pcg_init, s, op, a);
 }
 
 void launch_pcg_dir(Scene* s, int op, float dt, int parity) {
     PcgArgs a = make_args(s, op, dt);
     a.d_old = s->cg_d + (long)(parity ^ 1) * s->g.N;
     a.d_new = s->cg_d + (long)parity * s->g.N;
-    dim3 grid(pcg_blocks(s)), block(kBlock);
-    if (op == 1)
-        hipLaunchKernelGGL(k_pcg_dir<1>, grid, block, 0, s->stream, a);
-    else
-        hipLaunchKernelGGL(k_pcg_dir<0>, grid, block, 0, s->stream, a);
+    PCG_LAUNCH(k_pcg_dir, s, op, a);
 }
 
-void launch_pcg_upd(Scene* s, int op, int parity) {
+void launch_pcg_upd(Scene* s, int op, int parity, int report) {
     PcgArgs a = make_args(s, op, 0.0f);
+    a.report = report;
     a.d_new = s->cg_d + (long)parity * s->g.N;
-    dim3 grid(pcg_blocks(s)), block(kBlock);
-    if (op == 1)
-        hipLaunchKernelGGL(k_pcg_upd<1>, grid, block, 0, s->stream, a);
-    else
-        hipLaunchKernelGGL(k_pcg_upd<0>, grid, block, 0, s->stream, a);
+    PCG_LAUNCH(k_pcg_upd, s, op, a);
 }
 
 }  // namespace hfm

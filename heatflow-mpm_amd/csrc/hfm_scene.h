@@ -23,9 +23,8 @@ struct CgCtl {
     double tol;       // relative tolerance
     double atol;      // max(tol * ||b||, tol)
     float mean;       // pressure rhs mean to remove (numpy float32 mean)
-    unsigned ticket;  // arrival counter of the grid reductions
-    unsigned ticket2;
-    unsigned pad;
+    unsigned ticket2; // arrival counter of the rhs-mean chunks
+    unsigned tickets[65];  // arrival counters of the grid reductions (kBuckets + 1)
 };
 
 // Pinned host memory written by the device with system-scope stores: CG progress and results,
@@ -116,6 +115,7 @@ struct Scene {
     int bins_dirty;
 
     hfm_stats stats;
+    int pcg_cpt;          // cells per thread of the PCG kernels (1, 2 or 4)
     int heat_pred;        // last exact heat iteration count (chunk sizing)
     int pres_pred;
 
@@ -149,6 +149,6 @@ void launch_p2g(Scene* s);
 int pcg_blocks(const Scene* s);
 void launch_pcg_init(Scene* s, int op, float dt);
 void launch_pcg_dir(Scene* s, int op, float dt, int parity);
-void launch_pcg_upd(Scene* s, int op, int parity);
+void launch_pcg_upd(Scene* s, int op, int parity, int report);
 
 }  // namespace hfm

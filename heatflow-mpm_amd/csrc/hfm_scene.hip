@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <chrono>
@@ -151,6 +152,17 @@ int time_end(Scene* s) {
     return HFM_OK;
 }
 
+int enable_timing(Scene* s) {
+    if (!s->ev_pool) {
+        s->ev_cap = 4096;
+        s->ev_pool = new hipEvent_t[s->ev_cap];
+        s->ev_class = new int[s->ev_cap];
+        for (int i = 0; i < s->ev_cap; ++i) HIP_OK(hipEventCreate(&s->ev_pool[i]));
+    }
+    s->timing = 1;
+    return HFM_OK;
+}
+
 // ---- PCG driver ----------------------------------------------------------------------------
 
 // Enqueue one PCG solve (op 0 heat, 1 pressure) and return when the device reports it done.
@@ -163,20 +175,28 @@ int run_pcg(Scene* s, int op, float dt) {
     const int maxiter = op ? s->prm.cg_max_iter : s->prm.heat_max_iter;
     const int maxeff = maxiter + (maxiter & 1);
     int& pred = op ? s->pres_pred : s->heat_pred;
+    // Iterations are enqueued in batches, at most two in flight; the last iteration of each
+    // batch posts its count, which releases the next batch.  Batch size follows the iteration
+    // count of the previous solve so that few launches are wasted after convergence.
     int launched = 0;
+    int batch_end[2] = {0, 0};
+    int inflight = 0;
     auto t0 = std::chrono::steady_clock::now();
     long spins = 0;
     for (;;) {
         if (__atomic_load_n(&m->done, __ATOMIC_ACQUIRE)) break;
         int prog = __atomic_load_n(&m->it, __ATOMIC_ACQUIRE);
-        int cap = pred - prog;
-        cap = cap < 2 ? 2 : (cap > 16 ? 16 : cap);
-        if (launched < maxeff && launched - prog < cap) {
-            int g = cap - (launched - prog);
+        while (inflight > 0 && prog >= batch_end[0]) {
+            batch_end[0] = batch_end[1];
+            --inflight;
+        }
+        if (inflight < 2 && launched < maxeff) {
+            int g = pred - launched;
+            g = g < 2 ? 2 : (g > 16 ? 16 : g);
             if (g > maxeff - launched) g = maxeff - launched;
+            const bool timed = op == 1 && s->timing;
             for (int q = 0; q < g; ++q) {
                 int k = launched + q;
-                const bool timed = op == 1 && s->timing;
                 int rc = timed ? time_begin(s, HFM_KT_PCG_DIR) : HFM_OK;
                 if (rc) return rc;
                 launch_pcg_dir(s, op, dt, k & 1);
@@ -184,12 +204,13 @@ int run_pcg(Scene* s, int op, float dt) {
                 if (rc) return rc;
                 rc = timed ? time_begin(s, HFM_KT_PCG_UPD) : HFM_OK;
                 if (rc) return rc;
-                launch_pcg_upd(s, op, k & 1);
+                launch_pcg_upd(s, op, k & 1, q == g - 1);
                 rc = timed ? time_end(s) : HFM_OK;
                 if (rc) return rc;
             }
             CHECK_LAUNCH();
             launched += g;
+            batch_end[inflight++] = launched;
             continue;
         }
         if ((++spins & 0x3ff) == 0) {
@@ -213,6 +234,7 @@ int run_pcg(Scene* s, int op, float dt) {
     int ce = op ? s->prm.cg_check_every : s->prm.heat_check_every;
     int reported = it == 0 && conv ? 0 : warp_reported_count(it, conv, maxiter, ce);
     if (op == 1) {
+        s->stats.p_iters_total += it;
         s->stats.p_iters_exact = it;
         s->stats.p_iters = reported;
         s->stats.p_converged = conv;
@@ -220,6 +242,7 @@ int run_pcg(Scene* s, int op, float dt) {
         s->stats.p_atol = m->atol;
         s->stats.rhs_mean = m->mean;
     } else {
+        s->stats.h_iters_total += it;
         s->stats.h_iters_exact = it;
         s->stats.h_iters = reported;
         s->stats.h_converged = conv;
@@ -364,6 +387,13 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
     s->g.F = (long)(prm->n + 1) * prm->n * prm->n;
     s->g.h = (float)prm->h;
     s->heat_pred = 2;
+    {
+        // cells per thread of the PCG kernels: measured best per grid size (n <= 64: 1,
+        // n <= 128: 2, larger: 4); HFM_PCG_CPT overrides
+        const char* e = getenv("HFM_PCG_CPT");
+        int c = e ? atoi(e) : (prm->n <= 64 ? 1 : (prm->n <= 128 ? 2 : 4));
+        s->pcg_cpt = (c == 2 || c == 4) ? c : 1;
+    }
     s->pres_pred = 8;
     s->mean_prog_tail = -1;
     int rc = HFM_OK;
@@ -407,8 +437,8 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
     A(s->cg_z, N);
     A(s->cg_d, 2 * N);
     A(s->cg_Ap, N);
-    s->max_blocks = pcg_blocks(s);
-    A(s->partials, 3 * s->max_blocks);
+    s->max_blocks = (N + kBlock - 1) / kBlock;  // the most workgroups any cells-per-thread choice uses
+    A(s->partials, 3 * (s->max_blocks + kBuckets));
     A(s->ctl, 2);
     long nchunks = (N + 8191) / 8192;
     A(s->chunk_sums, nchunks);
@@ -461,11 +491,11 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
         return HFM_ERR_HIP;
     }
     if (prm->flags & HFM_FLAG_TIME_KERNELS) {
-        s->timing = 1;
-        s->ev_cap = 4096;
-        s->ev_pool = new hipEvent_t[s->ev_cap];
-        s->ev_class = new int[s->ev_cap];
-        for (int i = 0; i < s->ev_cap; ++i) (void)hipEventCreate(&s->ev_pool[i]);
+        int rc2 = enable_timing(s);
+        if (rc2) {
+            hfm_scene_destroy((hfm_scene*)o);
+            return rc2;
+        }
     }
     *out = (hfm_scene*)o;
     return HFM_OK;
@@ -779,6 +809,14 @@ int hfm_kernel_time_reset(hfm_scene* h) {
         s->kt_n[c] = 0;
     }
     return HFM_OK;
+}
+
+int hfm_set_kernel_timing(hfm_scene* h, int32_t on) {
+    SCENE_OR_FAIL(h);
+    if (on) return enable_timing(s);
+    int rc = flush_timing(s);
+    s->timing = 0;
+    return rc;
 }
 
 int hfm_synchronize(hfm_scene* h) {

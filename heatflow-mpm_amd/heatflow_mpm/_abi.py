@@ -44,7 +44,8 @@ class Stats(C.Structure):
     _fields_ = [("p_iters", C.c_int32), ("p_iters_exact", C.c_int32), ("h_iters", C.c_int32),
                 ("h_iters_exact", C.c_int32), ("p_converged", C.c_int32), ("h_converged", C.c_int32),
                 ("substeps", C.c_int64), ("p_rr", C.c_double), ("p_atol", C.c_double), ("h_rr", C.c_double),
-                ("h_atol", C.c_double), ("rhs_mean", C.c_double)]
+                ("h_atol", C.c_double), ("rhs_mean", C.c_double), ("p_iters_total", C.c_int64),
+                ("h_iters_total", C.c_int64)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -74,6 +75,7 @@ SIGNATURES = {
     "hfm_get_stats": (C.c_int, [_VP, C.POINTER(Stats)]),
     "hfm_kernel_time": (C.c_int, [_VP, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "hfm_kernel_time_reset": (C.c_int, [_VP]),
+    "hfm_set_kernel_timing": (C.c_int, [_VP, C.c_int32]),
     "hfm_synchronize": (C.c_int, [_VP]),
     "hfm_device_count": (C.c_int, []),
     "hfm_last_error": (C.c_char_p, []),

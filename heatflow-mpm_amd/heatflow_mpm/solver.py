@@ -177,6 +177,17 @@ class _Scene:
     def synchronize(self) -> None:
         _abi.check(self.lib.hfm_synchronize(self.handle), "hfm_synchronize")
 
+    def set_kernel_timing(self, on: bool) -> None:
+        _abi.check(self.lib.hfm_set_kernel_timing(self.handle, int(bool(on))), "hfm_set_kernel_timing")
+
+    def kernel_time(self, cls: int):
+        ms, n = C.c_double(0.0), C.c_int64(0)
+        _abi.check(self.lib.hfm_kernel_time(self.handle, int(cls), C.byref(ms), C.byref(n)), "hfm_kernel_time")
+        return ms.value, n.value
+
+    def kernel_time_reset(self) -> None:
+        _abi.check(self.lib.hfm_kernel_time_reset(self.handle), "hfm_kernel_time_reset")
+
 
 class HeatflowMPM:
     """Eulerian air and temperature coupled to a particle material; substep on the GPU.

@@ -85,6 +85,8 @@ typedef struct hfm_stats {
     double h_rr;
     double h_atol;
     double rhs_mean;          /* mean removed from the pressure rhs (grid.py:239) */
+    int64_t p_iters_total;    /* sum of p_iters_exact over all substeps */
+    int64_t h_iters_total;    /* sum of h_iters_exact over all substeps */
 } hfm_stats;
 
 /* Field identifiers for hfm_get_field / hfm_set_field. */
@@ -161,13 +163,17 @@ int hfm_diagnostics(hfm_scene* s, double* divergence_max, double* speed_max);
 
 int hfm_get_stats(hfm_scene* s, hfm_stats* out);
 
-/* With HFM_FLAG_TIME_KERNELS set at creation: total time and launch count of one kernel
- * class (HFM_KT_*) since the last reset. */
+/* With kernel timing on (HFM_FLAG_TIME_KERNELS at creation, or hfm_set_kernel_timing): total
+ * time and launch count of one kernel class (HFM_KT_*) since the last reset, from HIP events
+ * recorded around every launch on the scene's stream. */
 #define HFM_KT_PCG_DIR 0      /* pressure PCG pass A: direction update + SpMV + p.Ap */
 #define HFM_KT_PCG_UPD 1      /* pressure PCG pass B: x, r update + r.r, r.z */
 #define HFM_KT_COUNT 2
 int hfm_kernel_time(hfm_scene* s, int32_t kernel_class, double* total_ms, int64_t* launches);
 int hfm_kernel_time_reset(hfm_scene* s);
+
+/* Switch the kernel timing of HFM_FLAG_TIME_KERNELS on (1) or off (0) after creation. */
+int hfm_set_kernel_timing(hfm_scene* s, int32_t on);
 
 int hfm_synchronize(hfm_scene* s);
 

@@ -767,6 +767,7 @@ static void st_diffuse(orc_scene* s, double dtd) {
     cgres r;
     pcg(&op, s->hdiag, s->hrhs, s->T2, s->N, s->prm.heat_tol, s->prm.heat_max_iter, s->prm.heat_check_every, s->cr,
         s->cz, s->cp, s->cAp, &r);
+    s->st.h_iters_total += r.exact;
     s->st.h_iters = r.reported;
     s->st.h_iters_exact = r.exact;
     s->st.h_converged = r.converged;
@@ -944,6 +945,7 @@ static void st_pressure_cg(orc_scene* s) {
     cgres r;
     pcg(&op, s->pdiag, s->rhs, s->p, s->N, s->prm.cg_tol, s->prm.cg_max_iter, s->prm.cg_check_every, s->cr, s->cz,
         s->cp, s->cAp, &r);
+    s->st.p_iters_total += r.exact;
     s->st.p_iters = r.reported;
     s->st.p_iters_exact = r.exact;
     s->st.p_converged = r.converged;

@@ -140,6 +140,14 @@ class OracleSim:
         self.L.orc_set_fixed(self.s, np.ascontiguousarray(fixed, np.int32).reshape(n, n, n),
                              np.ascontiguousarray(values, np.float32).reshape(n, n, n))
 
+    def set_particle_temperatures(self, temp) -> None:
+        """Replace the particle temperatures (positions and velocities kept)."""
+        pos, vel, _ = self._keep
+        temp = np.ascontiguousarray(temp, np.float32).reshape(-1)
+        self.L.orc_set_particles(self.s, self.n_particles, pos.ctypes.data_as(C.c_void_p),
+                                 vel.ctypes.data_as(C.c_void_p), temp.ctypes.data_as(C.c_void_p))
+        self._keep = (pos, vel, temp)
+
     def substep(self, dt: float) -> None:
         self.L.orc_substep(self.s, float(dt))
 
