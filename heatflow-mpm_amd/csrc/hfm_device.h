@@ -14,6 +14,8 @@ namespace hfm {
 constexpr int kBlock = 256;      // threads per workgroup (4 wave64)
 constexpr int kWaves = kBlock / 64;
 constexpr int kXcds = 8;         // MI355X: 8 XCDs, workgroups dispatched round-robin over them
+constexpr int kMaxPcgBlocks = 1024;     // pass B / init workgroups (4 per CU); bounds their partial sums
+constexpr int kMaxDirBlocks = 2048;     // pass A workgroups
 
 // cell flags (1 byte per cell)
 constexpr uint8_t FLAG_FIXED = 1;
@@ -122,10 +124,11 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Sum of one double per thread over the workgroup; the result is valid in thread 0.  The tree
-// is fixed (lane butterfly, then waves in order), so the value does not depend on timing.
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double (*lds)[kWaves]) {
+// Sum of one double per thread over a workgroup of BS threads; the result is valid in thread 0.
+// The tree is fixed (lane butterfly, then waves in order), so the value does not depend on
+// timing.  lds must hold NV x (BS / 64) doubles.
+template <int NV, int BS = kBlock>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double (*lds)[BS / 64]) {
     int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
@@ -138,7 +141,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double (*lds)[kWaves]
         for (int q = 0; q < NV; ++q) {
             double s = lds[q][0];
 #pragma unroll
-            for (int w = 1; w < kWaves; ++w) s += lds[q][w];
+            for (int w = 1; w < BS / 64; ++w) s += lds[q][w];
             v[q] = s;
         }
     }

@@ -11,8 +11,14 @@
 // One iteration is two fused passes over the cells:
 //   pass A (dir): p = z + beta p_old at the cell and its six neighbours, Ap = A p, sum p.Ap
 //   pass B (upd): x += alpha p, r -= alpha Ap, z = D r, sums r.r and r.z
-// The scalars are formed by the workgroup that finishes last (fixed summation tree), which
-// also publishes the progress to pinned host memory so the host can stop enqueueing.
+// Reductions use no atomics: every workgroup stores its partial sums with plain stores, and
+// every workgroup of the next kernel sums all of them itself in one fixed order, so all of them
+// hold the same bit-identical scalars (the kernel boundary makes the partials visible).  Pass A
+// of iteration k+1 therefore also finishes iteration k: it forms r.r and r.z, tests convergence
+// and derives beta; pass B forms p.Ap and alpha.  Scalars that live across kernels (r.z of the
+// last two iterations, atol) are written by workgroup 0 into parity slots, never read and
+// written by the same kernel.  The workgroup count is capped (kMaxPcgBlocks) so the redundant
+// prologue sums stay small; each workgroup then covers several consecutive tiles.
 // The search direction is double-buffered (pass A reads neighbours of the old direction while
 // other workgroups write the new one).
 //
@@ -44,9 +50,16 @@ struct PcgArgs {
     const float* D;
     CgCtl* ctl;
     CgMirror* mirror;
-    double* partials;
+    double* part_dir;      // [nb]       p.Ap partials of pass A
+    double* part_upd;      // [3][nb]    r.r, r.z (and b.b from init) partials of pass B / init
+    int cpt;               // tiles per workgroup
+    int nb_dir;            // workgroups of the pass A launch (rows of part_dir)
+    int nb_upd;            // workgroups of the init / pass B launches (rows of part_upd)
+    // LDS-tiled pressure pass A: 16 x 16 (j, k) tiles, ichunk planes per workgroup (0: flat)
+    int tiles_j, tiles_k, ichunk;
+    int k;                 // completed iterations before this launch
     int remove_mean;       // pressure init: b -= mean when mean != 0 (grid.py:240-241)
-    int report;            // upd: last iteration of an enqueued batch, post the progress
+    int report;            // dir: first launch of an enqueued batch, post the progress
 };
 
 template <int OP>
@@ -117,36 +130,123 @@ __device__ __forceinline__ void cell_ijk(long t, int n, int& i, int& j, int& k) 
 }
 
 // Publish progress to pinned host memory.  Relaxed system-scope stores go straight to host
-// memory.  While the solve runs only the iteration count is posted, by the last iteration of
-// each batch the host enqueued (`report`), without waiting; at the end the results are stored,
-// their completion waited for, and only then `done` — so the host never sees `done` before the
+// memory.  While the solve runs only the iteration count is posted (by the first pass of each
+// batch the host enqueued), without waiting; at the end the results are stored, their
+// completion waited for, and only then `done` — so the host never sees `done` before the
 // results, and no system-scope release fence (a whole-L2 write-back) is used.
-__device__ __forceinline__ void mirror_progress(CgMirror* m, const CgCtl* c, int report) {
-    if (!c->done) {
-        if (report) __hip_atomic_store(&m->it, c->it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        return;
-    }
-    __hip_atomic_store(&m->rr, (double)c->rr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    __hip_atomic_store(&m->atol, c->atol, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    __hip_atomic_store(&m->converged, c->converged, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    __hip_atomic_store(&m->it, c->it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+__device__ __forceinline__ void mirror_final(CgMirror* m, int it, float rr, double atol, int converged) {
+    __hip_atomic_store(&m->rr, (double)rr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&m->atol, atol, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&m->converged, converged, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&m->it, it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     wait_stores();
     __hip_atomic_store(&m->done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// Cells of logical workgroup L: CPT consecutive tiles of kBlock cells.
-#define FOR_CELLS(CPT, nb)                                                   \
-    const int L_ = xcd_tile(blockIdx.x, nb);                                 \
-    _Pragma("unroll") for (int u_ = 0; u_ < CPT; ++u_)                       \
-        if (long t = ((long)L_ * CPT + u_) * kBlock + threadIdx.x; t < a.N)
+// Whether the solve has stopped, read once per workgroup (thread 0, broadcast through LDS) so
+// that every wave of the workgroup takes the same branch: workgroup 0 of a pass A may set the
+// flag while other workgroups of the same launch start.
+__device__ __forceinline__ bool solve_stopped(const CgCtl* c) {
+    __shared__ int stopped;
+    if (threadIdx.x == 0) stopped = c->done;
+    __syncthreads();
+    return stopped != 0;
+}
 
-// r = b - A x ; z = D r ; sums b.b, r.r, r.z ; then atol and the convergence test
-template <int OP, int CPT>
+// Sum NV rows of per-workgroup partials (row q at P + q*nb) in a fixed order; every thread of the
+// BS-thread workgroup receives the totals.
+template <int NV, int BS = kBlock>
+__device__ __forceinline__ void sum_partials(const double* __restrict__ P, int nb, double (*lds)[BS / 64],
+                                             double (&out)[NV]) {
+    __shared__ double bcast[3];
+    double acc[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += BS)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) acc[q] += P[q * nb + b];
+    block_sum<NV, BS>(acc, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) bcast[q] = acc[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NV; ++q) out[q] = bcast[q];
+}
+
+// Store this workgroup's partial sums (thread 0 after the block sum).
+template <int NV, int BS = kBlock>
+__device__ __forceinline__ void store_partials(double (&v)[NV], double* __restrict__ P, int nb,
+                                               double (*lds)[BS / 64]) {
+    block_sum<NV, BS>(v, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) P[q * nb + blockIdx.x] = v[q];
+}
+
+// Cells of logical workgroup L: cpt consecutive tiles of kBlock cells.
+#define FOR_CELLS(nb)                                                        \
+    const int L_ = xcd_tile(blockIdx.x, nb);                                 \
+    for (int u_ = 0; u_ < a.cpt; ++u_)                                       \
+        if (long t = ((long)L_ * a.cpt + u_) * kBlock + threadIdx.x; t < a.N)
+
+// Prologue of every pass A: finish iteration k from pass B's partial sums (r.r, r.z; b.b too
+// at k = 0, where atol is formed), test convergence and the iteration cap, and derive beta.
+// Workgroup 0 records the state.  Returns true when the solve stops (in every thread).
+template <int BS = kBlock>
+__device__ bool finish_iteration(const PcgArgs& a, double (*lds)[BS / 64], float& beta) {
+    CgCtl* c = a.ctl;
+    const int k = a.k;
+    float rr, rz, atol_sq;
+    double atol;
+    if (k == 0) {
+        double tot[3];
+        sum_partials<3, BS>(a.part_upd, a.nb_upd, lds, tot);
+        rr = (float)tot[0];
+        rz = (float)tot[1];
+        float bb = (float)tot[2];
+        // warp: atol = max(tol * ||b||, tol)
+        atol = c->tol * sqrt((double)bb);
+        if (atol < c->tol) atol = c->tol;
+        atol_sq = (float)(atol * atol);
+    } else {
+        double tot[2];
+        sum_partials<2, BS>(a.part_upd, a.nb_upd, lds, tot);
+        rr = (float)tot[0];
+        rz = (float)tot[1];
+        atol = c->atol;
+        atol_sq = c->atol_sq;
+    }
+    const bool conv = rr <= atol_sq;
+    const bool stop = conv || k >= c->maxeff;
+    // beta of iteration k (warp: rz_new / rz_old while rr > atol^2)
+    beta = (k == 0) ? 0.0f : ((rr > atol_sq) ? rz / c->slot_rz[(k + 1) & 1] : 0.0f);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c->slot_rr[k & 1] = rr;
+        c->slot_rz[k & 1] = rz;
+        if (k == 0) {
+            c->atol = atol;
+            c->atol_sq = atol_sq;
+        }
+        if (stop) {
+            c->done = 1;
+            c->converged = conv;
+            c->it = k;
+            mirror_final(a.mirror, k, rr, atol, conv);
+        } else if (a.report) {
+            __hip_atomic_store(&a.mirror->it, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
+    return stop;
+}
+
+// r = b - A x ; z = D r ; partial sums r.r, r.z, b.b
+template <int OP>
 __global__ __launch_bounds__(kBlock) void k_pcg_init(PcgArgs a) {
     __shared__ double lds[3][kWaves];
     const int nb = gridDim.x;
     double v[3] = {0.0, 0.0, 0.0};
-    FOR_CELLS(CPT, nb) {
+    FOR_CELLS(nb) {
         int i, j, k;
         cell_ijk(t, a.n, i, j, k);
         float bc = a.b[t];
@@ -165,70 +265,187 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(PcgArgs a) {
         float zc = a.D[t] * rc;
         a.r[t] = rc;
         a.z[t] = zc;
-        v[0] += (double)(bc * bc);
-        v[1] += (double)(rc * rc);
-        v[2] += (double)(rc * zc);
+        v[0] += (double)(rc * rc);
+        v[1] += (double)(rc * zc);
+        v[2] += (double)(bc * bc);
     }
-    double tot[3];
-    if (!grid_sum_last<3>(v, a.partials, a.ctl->tickets, nb, lds, tot)) return;
-    if (threadIdx.x != 0) return;
-    CgCtl* c = a.ctl;
-    float bb = (float)tot[0], rr = (float)tot[1], rz = (float)tot[2];
-    double atol = c->tol * sqrt((double)bb);
-    if (atol < c->tol) atol = c->tol;
-    c->atol = atol;
-    c->atol_sq = (float)(atol * atol);
-    c->rr = rr;
-    c->rz = rz;
-    c->alpha = 0.0f;
-    c->beta = 0.0f;
-    c->it = 0;
-    c->first = 1;
-    c->converged = rr <= c->atol_sq;
-    c->done = c->converged;
-    mirror_progress(a.mirror, c, 1);
+    store_partials<3>(v, a.part_upd, nb, lds);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // a new solve: clear the previous one's stop
+        a.ctl->done = 0;
+        a.ctl->converged = 0;
+        a.ctl->it = 0;
+    }
 }
 
-// pass A: direction update at the cell and its neighbours, SpMV, p.Ap ; then alpha
-template <int OP, int CPT>
+// pass A of iteration k+1: finish iteration k (r.r, r.z, convergence, beta), then the direction
+// update at the cell and its neighbours, SpMV and partial p.Ap
+template <int OP>
 __global__ __launch_bounds__(kBlock) void k_pcg_dir(PcgArgs a) {
-    __shared__ double lds[1][kWaves];
-    if (a.ctl->done) return;
+    __shared__ double lds[3][kWaves];
+    CgCtl* c = a.ctl;
+    if (solve_stopped(c)) return;
     const int nb = gridDim.x;
-    const float beta = a.ctl->beta;
-    const int first = a.ctl->first;
+    const int k = a.k;
+    float beta;
+    if (finish_iteration(a, lds, beta)) return;
     double v[1] = {0.0};
-    FOR_CELLS(CPT, nb) {
-        int i, j, k;
-        cell_ijk(t, a.n, i, j, k);
+    const bool first = k == 0;
+    FOR_CELLS(nb) {
+        int i, j, kk;
+        cell_ijk(t, a.n, i, j, kk);
         const float* zs = a.z;
         const float* ds = a.d_old;
         auto dir = [zs, ds, beta, first](long q) { return first ? zs[q] : zs[q] + beta * ds[q]; };
         float pc = dir(t);
-        float acc = Operator<OP>::apply(a, t, i, j, k, dir);
+        float acc = Operator<OP>::apply(a, t, i, j, kk, dir);
         bool fixed = OP == 0 && (a.flags[t] & FLAG_FIXED);
         float apc = Operator<OP>::combine(a, acc, pc, fixed, 1.0f);
         a.d_new[t] = pc;
         a.Ap[t] = apc;
         v[0] += (double)(pc * apc);
     }
-    double tot[1];
-    if (!grid_sum_last<1>(v, a.partials, a.ctl->tickets, nb, lds, tot)) return;
-    if (threadIdx.x != 0) return;
-    CgCtl* c = a.ctl;
-    float pAp = (float)tot[0];
-    c->alpha = (c->rr > c->atol_sq) ? c->rz / pAp : 0.0f;
+    store_partials<1>(v, a.part_dir, nb, lds);
 }
 
-// pass B: x, r update, z = D r, sums r.r and r.z ; then beta and the convergence test
-template <int OP, int CPT>
+// Pass A for the pressure operator, LDS-tiled.  A workgroup of 1024 threads owns a 16 x 16 x D
+// block of cells.  It first forms the new direction p = z + beta p_old and loads the solid
+// fraction on the block framed by a one-cell halo ((D+2) x 18 x 18 cells, all loads in flight at
+// once) into LDS, then every thread applies the stencil to D/4 cells from LDS.  Per cell the
+// arithmetic is exactly that of k_pcg_dir<1>.
+constexpr int kT3 = 16, kF3 = kT3 + 2, kBlock3 = 1024;
+
+template <int D>
+__global__ __launch_bounds__(kBlock3) void k_pcg_dir_tile3(PcgArgs a) {
+    __shared__ float P[D + 2][kF3][kF3];
+    __shared__ float S[D + 2][kF3][kF3];
+    __shared__ double lds[3][kBlock3 / 64];
+    {
+        __shared__ int stopped;
+        if (threadIdx.x == 0) stopped = a.ctl->done;
+        __syncthreads();
+        if (stopped) return;
+    }
+    const int nb = gridDim.x;
+    const int n = a.n;
+    const int L = xcd_tile(blockIdx.x, nb);
+    const int tjk = a.tiles_j * a.tiles_k;
+    const int ti = L / tjk, rem = L % tjk;
+    const int i0 = ti * D, j0 = (rem / a.tiles_k) * kT3, k0 = (rem % a.tiles_k) * kT3;
+    const bool first = a.k == 0;
+    const float* __restrict__ zs = a.z;
+    const float* __restrict__ ds = a.d_old;
+    const float* __restrict__ as = a.alpha_s;
+    // frame operands are loaded before the prologue (they do not depend on beta), so the
+    // latency of the partial-sum prologue overlaps theirs
+    constexpr int kFrame = (D + 2) * kF3 * kF3;
+    constexpr int kPer = (kFrame + kBlock3 - 1) / kBlock3;
+    float fz[kPer], fd[kPer], fa[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int f = threadIdx.x + u * kBlock3;
+        fz[u] = 0.0f;
+        fd[u] = 0.0f;
+        fa[u] = 0.0f;
+        if (f < kFrame) {
+            const int fi = f / (kF3 * kF3), fr = f % (kF3 * kF3);
+            const int fj = fr / kF3, fk = fr % kF3;
+            const int gi = i0 + fi - 1, gj = j0 + fj - 1, gk = k0 + fk - 1;
+            if (gi >= 0 && gi < n && gj >= 0 && gj < n && gk >= 0 && gk < n) {
+                const long t = ((long)gi * n + gj) * n + gk;
+                fz[u] = zs[t];
+                if (!first) fd[u] = ds[t];
+                fa[u] = as[t];
+            }
+        }
+    }
+    float beta;
+    if (finish_iteration<kBlock3>(a, lds, beta)) return;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int f = threadIdx.x + u * kBlock3;
+        if (f < kFrame) {
+            const int fi = f / (kF3 * kF3), fr = f % (kF3 * kF3);
+            const int fj = fr / kF3, fk = fr % kF3;
+            P[fi][fj][fk] = first ? fz[u] : fz[u] + beta * fd[u];
+            S[fi][fj][fk] = fa[u];
+        }
+    }
+    __syncthreads();
+    const float amin = a.amin;
+    const int lk = threadIdx.x % kT3, lj = (threadIdx.x / kT3) % kT3, lq = threadIdx.x / (kT3 * kT3);
+    const int j = j0 + lj, k = k0 + lk;
+    double v[1] = {0.0};
+    if (j < n && k < n) {
+        const int fj = lj + 1, fk = lk + 1;
+#pragma unroll
+        for (int m = 0; m < D / 4; ++m) {
+            const int li = lq + 4 * m, i = i0 + li;
+            if (i >= n) break;
+            const int fi = li + 1;
+            const float pc = P[fi][fj][fk], ac = S[fi][fj][fk];
+            float acc = 0.0f;
+            if (i > 0) acc += face_fraction(S[fi - 1][fj][fk], ac, amin) * (pc - P[fi - 1][fj][fk]);
+            if (i < n - 1) acc += face_fraction(ac, S[fi + 1][fj][fk], amin) * (pc - P[fi + 1][fj][fk]);
+            if (j > 0) acc += face_fraction(S[fi][fj - 1][fk], ac, amin) * (pc - P[fi][fj - 1][fk]);
+            if (j < n - 1) acc += face_fraction(ac, S[fi][fj + 1][fk], amin) * (pc - P[fi][fj + 1][fk]);
+            if (k > 0) acc += face_fraction(S[fi][fj][fk - 1], ac, amin) * (pc - P[fi][fj][fk - 1]);
+            if (k < n - 1) acc += face_fraction(ac, S[fi][fj][fk + 1], amin) * (pc - P[fi][fj][fk + 1]);
+            const float apc = 1.0f * acc / a.hh;
+            const long t = ((long)i * n + j) * n + k;
+            a.d_new[t] = pc;
+            a.Ap[t] = apc;
+            v[0] += (double)(pc * apc);
+        }
+    }
+    store_partials<1, kBlock3>(v, a.part_dir, nb, lds);
+}
+
+// pass B of iteration k+1: alpha from p.Ap, x and r update, z = D r, partial r.r and r.z
+template <int OP>
 __global__ __launch_bounds__(kBlock) void k_pcg_upd(PcgArgs a) {
     __shared__ double lds[2][kWaves];
-    if (a.ctl->done) return;
+    CgCtl* c = a.ctl;
+    if (solve_stopped(c)) return;
     const int nb = gridDim.x;
-    const float alpha = a.ctl->alpha;
+    const int L = xcd_tile(blockIdx.x, nb);
+    // the operands of the first kPre tiles are loaded before the p.Ap prologue, so its latency
+    // overlaps theirs
+    constexpr int kPre = 4;
+    float px[kPre], pd[kPre], pr[kPre], pa[kPre], pD[kPre];
+#pragma unroll
+    for (int u = 0; u < kPre; ++u) {
+        long t = ((long)L * a.cpt + u) * kBlock + threadIdx.x;
+        if (u < a.cpt && t < a.N) {
+            px[u] = a.x[t];
+            pd[u] = a.d_new[t];
+            pr[u] = a.r[t];
+            pa[u] = a.Ap[t];
+            pD[u] = a.D[t];
+        }
+    }
+    double tot[1];
+    sum_partials<1>(a.part_dir, a.nb_dir, lds, tot);
+    const float pAp = (float)tot[0];
+    const float rr = c->slot_rr[a.k & 1], rz = c->slot_rz[a.k & 1];
+    const float alpha = (rr > c->atol_sq) ? rz / pAp : 0.0f;
     double v[2] = {0.0, 0.0};
-    FOR_CELLS(CPT, nb) {
+#pragma unroll
+    for (int u = 0; u < kPre; ++u) {
+        long t = ((long)L * a.cpt + u) * kBlock + threadIdx.x;
+        if (u < a.cpt && t < a.N) {
+            float xc = px[u] + alpha * pd[u];
+            float rc = pr[u] - alpha * pa[u];
+            float zc = pD[u] * rc;
+            a.x[t] = xc;
+            a.r[t] = rc;
+            a.z[t] = zc;
+            v[0] += (double)(rc * rc);
+            v[1] += (double)(rc * zc);
+        }
+    }
+    for (int u = kPre; u < a.cpt; ++u) {
+        long t = ((long)L * a.cpt + u) * kBlock + threadIdx.x;
+        if (t >= a.N) break;
         float xc = a.x[t] + alpha * a.d_new[t];
         float rc = a.r[t] - alpha * a.Ap[t];
         float zc = a.D[t] * rc;
@@ -238,24 +455,14 @@ __global__ __launch_bounds__(kBlock) void k_pcg_upd(PcgArgs a) {
         v[0] += (double)(rc * rc);
         v[1] += (double)(rc * zc);
     }
-    double tot[2];
-    if (!grid_sum_last<2>(v, a.partials, a.ctl->tickets, nb, lds, tot)) return;
-    if (threadIdx.x != 0) return;
-    CgCtl* c = a.ctl;
-    float rr = (float)tot[0], rz = (float)tot[1];
-    c->beta = (rr > c->atol_sq) ? rz / c->rz : 0.0f;
-    c->rz = rz;
-    c->rr = rr;
-    c->first = 0;
-    c->it = c->it + 1;
-    if (rr <= c->atol_sq) {
-        c->converged = 1;
-        c->done = 1;
-    } else if (c->it >= c->maxeff) {
-        c->done = 1;
-    }
-    mirror_progress(a.mirror, c, a.report);
+    store_partials<2>(v, a.part_upd, nb, lds);
 }
+
+}  // namespace
+int pcg_cpt(const Scene* s);
+int pcg_blocks(const Scene* s);
+void pcg_dir_geometry(const Scene* s, int op, int* nb, int* tiles_j, int* tiles_k, int* ichunk);
+namespace {
 
 PcgArgs make_args(Scene* s, int op, float dt) {
     PcgArgs a;
@@ -284,7 +491,12 @@ PcgArgs make_args(Scene* s, int op, float dt) {
     a.Ap = s->cg_Ap;
     a.ctl = &s->ctl[op];
     a.mirror = &s->mirror_dev[op];
-    a.partials = s->partials;
+    a.part_dir = s->partials;
+    a.part_upd = s->partials + kMaxDirBlocks;
+    a.cpt = pcg_cpt(s);
+    a.nb_upd = pcg_blocks(s);
+    pcg_dir_geometry(s, op, &a.nb_dir, &a.tiles_j, &a.tiles_k, &a.ichunk);
+    a.k = 0;
     a.remove_mean = op == 1;
     a.report = 0;
     return a;
@@ -292,26 +504,29 @@ PcgArgs make_args(Scene* s, int op, float dt) {
 
 }  // namespace
 
+
+// tiles of kBlock cells per PCG workgroup: enough that at most kMaxPcgBlocks workgroups run
+// (the scene's pcg_cpt raises it further)
+int pcg_cpt(const Scene* s) {
+    long tiles = (s->g.N + kBlock - 1) / kBlock;
+    long need = (tiles + kMaxPcgBlocks - 1) / kMaxPcgBlocks;
+    int cpt = s->pcg_cpt > 0 ? s->pcg_cpt : 1;
+    return (int)(need > cpt ? need : cpt);
+}
+
 int pcg_blocks(const Scene* s) {
     long tiles = (s->g.N + kBlock - 1) / kBlock;
-    int cpt = s->pcg_cpt > 0 ? s->pcg_cpt : 1;
+    int cpt = pcg_cpt(s);
     return (int)((tiles + cpt - 1) / cpt);
 }
 
-// dispatch over (operator, cells per thread)
-#define PCG_LAUNCH(KERNEL, s, op, a)                                                                         \
-    do {                                                                                                    \
-        dim3 grid_(pcg_blocks(s)), block_(kBlock);                                                          \
-        int cpt_ = (s)->pcg_cpt;                                                                            \
-        if (op == 1) {                                                                                      \
-            if (cpt_ == 4) hipLaunchKernelGGL((KERNEL<1, 4>), grid_, block_, 0, (s)->stream, a);            \
-            else if (cpt_ == 2) hipLaunchKernelGGL((KERNEL<1, 2>), grid_, block_, 0, (s)->stream, a);       \
-            else hipLaunchKernelGGL((KERNEL<1, 1>), grid_, block_, 0, (s)->stream, a);                      \
-        } else {                                                                                            \
-            if (cpt_ == 4) hipLaunchKernelGGL((KERNEL<0, 4>), grid_, block_, 0, (s)->stream, a);            \
-            else if (cpt_ == 2) hipLaunchKernelGGL((KERNEL<0, 2>), grid_, block_, 0, (s)->stream, a);       \
-            else hipLaunchKernelGGL((KERNEL<0, 1>), grid_, block_, 0, (s)->stream, a);                      \
-        }                                                                                                   \
+#define PCG_LAUNCH(KERNEL, s, op, a)                                                        \
+    do {                                                                                   \
+        dim3 grid_(pcg_blocks(s)), block_(kBlock);                                         \
+        if (op == 1)                                                                       \
+            hipLaunchKernelGGL((KERNEL<1>), grid_, block_, 0, (s)->stream, a);             \
+        else                                                                               \
+            hipLaunchKernelGGL((KERNEL<0>), grid_, block_, 0, (s)->stream, a);             \
     } while (0)
 
 void launch_pcg_init(Scene* s, int op, float dt) {
@@ -319,17 +534,53 @@ void launch_pcg_init(Scene* s, int op, float dt) {
     PCG_LAUNCH(k_pcg_init, s, op, a);
 }
 
-void launch_pcg_dir(Scene* s, int op, float dt, int parity) {
-    PcgArgs a = make_args(s, op, dt);
-    a.d_old = s->cg_d + (long)(parity ^ 1) * s->g.N;
-    a.d_new = s->cg_d + (long)parity * s->g.N;
-    PCG_LAUNCH(k_pcg_dir, s, op, a);
+// Pass A geometry: the LDS-tiled kernel for the pressure operator (unless HFM_PCG_FLAT_DIR) with
+// planes D = 8, else 16, else 4, the first that keeps at most kMaxDirBlocks workgroups
+// (HFM_DIR_ICHUNK forces D), else the flat kernel with the pass B geometry.
+void pcg_dir_geometry(const Scene* s, int op, int* nb, int* tiles_j, int* tiles_k, int* ichunk) {
+    const int n = s->g.n;
+    const int tj = (n + kT3 - 1) / kT3, tk = tj;
+    int D = 0;
+    if (op == 1 && !s->flat_dir) {
+        const int cand[3] = {8, 16, 4};
+        for (int c = 0; c < 3 && !D; ++c) {
+            int d = cand[c];
+            if (s->dir_ichunk > 0 && d != s->dir_ichunk) continue;
+            long blocks = (long)tj * tk * ((n + d - 1) / d);
+            if (blocks <= kMaxDirBlocks) D = d;
+        }
+    }
+    if (!D) {
+        *nb = pcg_blocks(s);
+        *tiles_j = *tiles_k = *ichunk = 0;
+        return;
+    }
+    *nb = tj * tk * ((n + D - 1) / D);
+    *tiles_j = tj;
+    *tiles_k = tk;
+    *ichunk = D;
 }
 
-void launch_pcg_upd(Scene* s, int op, int parity, int report) {
-    PcgArgs a = make_args(s, op, 0.0f);
+void launch_pcg_dir(Scene* s, int op, float dt, int k, int report) {
+    PcgArgs a = make_args(s, op, dt);
+    a.k = k;
     a.report = report;
-    a.d_new = s->cg_d + (long)parity * s->g.N;
+    a.d_old = s->cg_d + (long)((k & 1) ^ 1) * s->g.N;
+    a.d_new = s->cg_d + (long)(k & 1) * s->g.N;
+    if (a.ichunk == 4)
+        hipLaunchKernelGGL(k_pcg_dir_tile3<4>, dim3(a.nb_dir), dim3(kBlock3), 0, s->stream, a);
+    else if (a.ichunk == 8)
+        hipLaunchKernelGGL(k_pcg_dir_tile3<8>, dim3(a.nb_dir), dim3(kBlock3), 0, s->stream, a);
+    else if (a.ichunk == 16)
+        hipLaunchKernelGGL(k_pcg_dir_tile3<16>, dim3(a.nb_dir), dim3(kBlock3), 0, s->stream, a);
+    else
+        PCG_LAUNCH(k_pcg_dir, s, op, a);
+}
+
+void launch_pcg_upd(Scene* s, int op, int k) {
+    PcgArgs a = make_args(s, op, 0.0f);
+    a.k = k;
+    a.d_new = s->cg_d + (long)(k & 1) * s->g.N;
     PCG_LAUNCH(k_pcg_upd, s, op, a);
 }
 

@@ -8,23 +8,21 @@
 
 namespace hfm {
 
-// Device-resident control block of one PCG solve (warp.optim.linear.cg semantics).
+// Device-resident control block of one PCG solve (warp.optim.linear.cg semantics).  Written by
+// workgroup 0 of a kernel and read by later kernels only.
 struct CgCtl {
+    float slot_rr[2]; // ||r||^2 after iteration k, in slot k & 1
+    float slot_rz[2]; // r.z after iteration k, in slot k & 1
     float atol_sq;    // fp32 threshold on ||r||^2
-    float rr;         // ||r||^2 after the last completed iteration (or after init)
-    float rz;         // r.z of the last completed iteration (rz_new; the next iteration's rz_old)
-    float alpha;      // step of the running iteration, set after p.Ap
-    float beta;       // direction update for the next iteration
-    int it;           // completed iterations
+    int it;           // iterations done when the solve stopped
     int done;         // 1: converged or iteration cap reached; every later CG kernel returns at once
     int converged;
     int maxeff;       // iteration cap rounded up to the 2-iteration cycle
-    int first;        // 1 before the first iteration: the direction is z itself
+    int pad0;
     double tol;       // relative tolerance
     double atol;      // max(tol * ||b||, tol)
     float mean;       // pressure rhs mean to remove (numpy float32 mean)
     unsigned ticket2; // arrival counter of the rhs-mean chunks
-    unsigned tickets[65];  // arrival counters of the grid reductions (kBuckets + 1)
 };
 
 // Pinned host memory written by the device with system-scope stores: CG progress and results,
@@ -86,7 +84,7 @@ struct Scene {
     float* cg_z;
     float* cg_d;          // search direction, two buffers of N (double-buffered by iteration parity)
     float* cg_Ap;
-    double* partials;     // per-workgroup partial sums
+    double* partials;     // per-workgroup partial sums: [max_blocks] pass A, [3][max_blocks] pass B
     long max_blocks;
     CgCtl* ctl;           // [0] heat, [1] pressure (device)
     CgMirror* mirror;     // [0] heat, [1] pressure (pinned host)
@@ -115,7 +113,9 @@ struct Scene {
     int bins_dirty;
 
     hfm_stats stats;
-    int pcg_cpt;          // cells per thread of the PCG kernels (1, 2 or 4)
+    int pcg_cpt;          // minimum tiles per PCG workgroup (HFM_PCG_CPT)
+    int flat_dir;         // 1: pressure pass A without LDS tiling (HFM_PCG_FLAT_DIR)
+    int dir_ichunk;       // planes per workgroup of the tiled pass A, 0 = automatic (HFM_DIR_ICHUNK)
     int heat_pred;        // last exact heat iteration count (chunk sizing)
     int pres_pred;
 
@@ -148,7 +148,8 @@ void launch_p2g(Scene* s);
 // ---- kernel launchers (hfm_pcg.hip); op 0 = heat, 1 = pressure ----
 int pcg_blocks(const Scene* s);
 void launch_pcg_init(Scene* s, int op, float dt);
-void launch_pcg_dir(Scene* s, int op, float dt, int parity);
-void launch_pcg_upd(Scene* s, int op, int parity, int report);
+void launch_pcg_dir(Scene* s, int op, float dt, int k, int report);
+void launch_pcg_upd(Scene* s, int op, int k);
+int pcg_cpt(const Scene* s);
 
 }  // namespace hfm

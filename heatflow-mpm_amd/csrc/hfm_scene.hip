@@ -175,12 +175,15 @@ int run_pcg(Scene* s, int op, float dt) {
     const int maxiter = op ? s->prm.cg_max_iter : s->prm.heat_max_iter;
     const int maxeff = maxiter + (maxiter & 1);
     int& pred = op ? s->pres_pred : s->heat_pred;
-    // Iterations are enqueued in batches, at most two in flight; the last iteration of each
-    // batch posts its count, which releases the next batch.  Batch size follows the iteration
+    // Iterations are enqueued in batches of (pass A, pass B) pairs, at most two batches in
+    // flight.  Pass A of iteration k+1 finishes iteration k, so the first pass A of a batch
+    // posts how many iterations the previous batch completed, which releases the next batch;
+    // after the last pair one more pass A closes the solve.  Batch size follows the iteration
     // count of the previous solve so that few launches are wasted after convergence.
     int launched = 0;
     int batch_end[2] = {0, 0};
     int inflight = 0;
+    bool closed = false;
     auto t0 = std::chrono::steady_clock::now();
     long spins = 0;
     for (;;) {
@@ -199,18 +202,24 @@ int run_pcg(Scene* s, int op, float dt) {
                 int k = launched + q;
                 int rc = timed ? time_begin(s, HFM_KT_PCG_DIR) : HFM_OK;
                 if (rc) return rc;
-                launch_pcg_dir(s, op, dt, k & 1);
+                launch_pcg_dir(s, op, dt, k, q == 0);
                 rc = timed ? time_end(s) : HFM_OK;
                 if (rc) return rc;
                 rc = timed ? time_begin(s, HFM_KT_PCG_UPD) : HFM_OK;
                 if (rc) return rc;
-                launch_pcg_upd(s, op, k & 1, q == g - 1);
+                launch_pcg_upd(s, op, k);
                 rc = timed ? time_end(s) : HFM_OK;
                 if (rc) return rc;
             }
             CHECK_LAUNCH();
             launched += g;
             batch_end[inflight++] = launched;
+            continue;
+        }
+        if (launched >= maxeff && !closed) {
+            launch_pcg_dir(s, op, dt, maxeff, 1);
+            CHECK_LAUNCH();
+            closed = true;
             continue;
         }
         if ((++spins & 0x3ff) == 0) {
@@ -388,11 +397,15 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
     s->g.h = (float)prm->h;
     s->heat_pred = 2;
     {
-        // cells per thread of the PCG kernels: measured best per grid size (n <= 64: 1,
-        // n <= 128: 2, larger: 4); HFM_PCG_CPT overrides
+        // minimum tiles per PCG workgroup (the workgroup cap may raise it); HFM_PCG_CPT overrides
         const char* e = getenv("HFM_PCG_CPT");
-        int c = e ? atoi(e) : (prm->n <= 64 ? 1 : (prm->n <= 128 ? 2 : 4));
-        s->pcg_cpt = (c == 2 || c == 4) ? c : 1;
+        int c = e ? atoi(e) : 1;
+        s->pcg_cpt = c >= 1 && c <= 64 ? c : 1;
+        e = getenv("HFM_PCG_FLAT_DIR");
+        s->flat_dir = e && atoi(e) != 0;
+        e = getenv("HFM_DIR_ICHUNK");
+        int ic = e ? atoi(e) : 0;
+        s->dir_ichunk = ic >= 1 && ic <= 1024 ? ic : 0;
     }
     s->pres_pred = 8;
     s->mean_prog_tail = -1;
@@ -437,8 +450,8 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
     A(s->cg_z, N);
     A(s->cg_d, 2 * N);
     A(s->cg_Ap, N);
-    s->max_blocks = (N + kBlock - 1) / kBlock;  // the most workgroups any cells-per-thread choice uses
-    A(s->partials, 3 * (s->max_blocks + kBuckets));
+    s->max_blocks = kMaxPcgBlocks;
+    A(s->partials, kMaxDirBlocks + 3 * kMaxPcgBlocks);
     A(s->ctl, 2);
     long nchunks = (N + 8191) / 8192;
     A(s->chunk_sums, nchunks);
