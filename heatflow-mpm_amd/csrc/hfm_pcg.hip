@@ -26,6 +26,8 @@
 //   pressure (grid.py:94-115): sum over in-range faces a_f (p_c - p_nb) / h^2, a_f recomputed
 //                              from the solid fraction (grid.py:46-59)
 //   heat (heat.py:55-105):     fixed cells identity; free cells x + dt/h^2 sum kappa_f terms
+#include <hip/hip_ext.h>
+
 #include "hfm_device.h"
 #include "hfm_scene.h"
 
@@ -561,27 +563,44 @@ void pcg_dir_geometry(const Scene* s, int op, int* nb, int* tiles_j, int* tiles_
     *ichunk = D;
 }
 
-void launch_pcg_dir(Scene* s, int op, float dt, int k, int report) {
+// With ev0/ev1 the launch carries its own start/stop events (hipExtLaunchKernelGGL): the events
+// time the dispatch itself, with no separate event packets around it.
+template <class K>
+static void launch_timed(K kernel, dim3 grid, dim3 block, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1,
+                         const PcgArgs& a) {
+    if (ev0)
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, st, ev0, ev1, 0, a);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, 0, st, a);
+}
+
+void launch_pcg_dir(Scene* s, int op, float dt, int k, int report, hipEvent_t ev0, hipEvent_t ev1) {
     PcgArgs a = make_args(s, op, dt);
     a.k = k;
     a.report = report;
     a.d_old = s->cg_d + (long)((k & 1) ^ 1) * s->g.N;
     a.d_new = s->cg_d + (long)(k & 1) * s->g.N;
+    const dim3 blk3(kBlock3), blk(kBlock);
     if (a.ichunk == 4)
-        hipLaunchKernelGGL(k_pcg_dir_tile3<4>, dim3(a.nb_dir), dim3(kBlock3), 0, s->stream, a);
+        launch_timed(k_pcg_dir_tile3<4>, dim3(a.nb_dir), blk3, s->stream, ev0, ev1, a);
     else if (a.ichunk == 8)
-        hipLaunchKernelGGL(k_pcg_dir_tile3<8>, dim3(a.nb_dir), dim3(kBlock3), 0, s->stream, a);
+        launch_timed(k_pcg_dir_tile3<8>, dim3(a.nb_dir), blk3, s->stream, ev0, ev1, a);
     else if (a.ichunk == 16)
-        hipLaunchKernelGGL(k_pcg_dir_tile3<16>, dim3(a.nb_dir), dim3(kBlock3), 0, s->stream, a);
+        launch_timed(k_pcg_dir_tile3<16>, dim3(a.nb_dir), blk3, s->stream, ev0, ev1, a);
+    else if (op == 1)
+        launch_timed(k_pcg_dir<1>, dim3(a.nb_dir), blk, s->stream, ev0, ev1, a);
     else
-        PCG_LAUNCH(k_pcg_dir, s, op, a);
+        launch_timed(k_pcg_dir<0>, dim3(a.nb_dir), blk, s->stream, ev0, ev1, a);
 }
 
-void launch_pcg_upd(Scene* s, int op, int k) {
+void launch_pcg_upd(Scene* s, int op, int k, hipEvent_t ev0, hipEvent_t ev1) {
     PcgArgs a = make_args(s, op, 0.0f);
     a.k = k;
     a.d_new = s->cg_d + (long)(k & 1) * s->g.N;
-    PCG_LAUNCH(k_pcg_upd, s, op, a);
+    if (op == 1)
+        launch_timed(k_pcg_upd<1>, dim3(a.nb_upd), dim3(kBlock), s->stream, ev0, ev1, a);
+    else
+        launch_timed(k_pcg_upd<0>, dim3(a.nb_upd), dim3(kBlock), s->stream, ev0, ev1, a);
 }
 
 }  // namespace hfm

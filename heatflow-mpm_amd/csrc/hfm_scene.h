@@ -148,8 +148,9 @@ void launch_p2g(Scene* s);
 // ---- kernel launchers (hfm_pcg.hip); op 0 = heat, 1 = pressure ----
 int pcg_blocks(const Scene* s);
 void launch_pcg_init(Scene* s, int op, float dt);
-void launch_pcg_dir(Scene* s, int op, float dt, int k, int report);
-void launch_pcg_upd(Scene* s, int op, int k);
+void launch_pcg_dir(Scene* s, int op, float dt, int k, int report, hipEvent_t ev0 = nullptr,
+                    hipEvent_t ev1 = nullptr);
+void launch_pcg_upd(Scene* s, int op, int k, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 int pcg_cpt(const Scene* s);
 
 }  // namespace hfm

@@ -135,20 +135,17 @@ int flush_timing(Scene* s) {
     return HFM_OK;
 }
 
-int time_begin(Scene* s, int cls) {
+// A pair of events for one timed launch of kernel class cls (flushing the pool when full).
+int timing_events(Scene* s, int cls, hipEvent_t* ev0, hipEvent_t* ev1) {
+    *ev0 = *ev1 = nullptr;
     if (!s->timing) return HFM_OK;
     if (s->ev_used + 2 > s->ev_cap) {
         int rc = flush_timing(s);
         if (rc) return rc;
     }
     s->ev_class[s->ev_used] = cls;
-    HIP_OK(hipEventRecord(s->ev_pool[s->ev_used++], s->stream));
-    return HFM_OK;
-}
-
-int time_end(Scene* s) {
-    if (!s->timing) return HFM_OK;
-    HIP_OK(hipEventRecord(s->ev_pool[s->ev_used++], s->stream));
+    *ev0 = s->ev_pool[s->ev_used++];
+    *ev1 = s->ev_pool[s->ev_used++];
     return HFM_OK;
 }
 
@@ -200,16 +197,13 @@ int run_pcg(Scene* s, int op, float dt) {
             const bool timed = op == 1 && s->timing;
             for (int q = 0; q < g; ++q) {
                 int k = launched + q;
-                int rc = timed ? time_begin(s, HFM_KT_PCG_DIR) : HFM_OK;
+                hipEvent_t e0 = nullptr, e1 = nullptr;
+                int rc = timed ? timing_events(s, HFM_KT_PCG_DIR, &e0, &e1) : HFM_OK;
                 if (rc) return rc;
-                launch_pcg_dir(s, op, dt, k, q == 0);
-                rc = timed ? time_end(s) : HFM_OK;
+                launch_pcg_dir(s, op, dt, k, q == 0, e0, e1);
+                rc = timed ? timing_events(s, HFM_KT_PCG_UPD, &e0, &e1) : HFM_OK;
                 if (rc) return rc;
-                rc = timed ? time_begin(s, HFM_KT_PCG_UPD) : HFM_OK;
-                if (rc) return rc;
-                launch_pcg_upd(s, op, k);
-                rc = timed ? time_end(s) : HFM_OK;
-                if (rc) return rc;
+                launch_pcg_upd(s, op, k, e0, e1);
             }
             CHECK_LAUNCH();
             launched += g;
