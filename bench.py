@@ -39,7 +39,24 @@ HBM_PEAK_GBS = 8000.0
 #   dir: reads z, p_old, alpha_s; writes p, Ap                      -> 5 words
 #   upd: reads x, p, r, Ap, D; writes x, r, z                         -> 8 words
 KERNEL_WORDS = {_abi.KT_PCG_DIR: 5, _abi.KT_PCG_UPD: 8}
-KERNEL_NAMES = {_abi.KT_PCG_DIR: "k_pcg_dir<pressure>", _abi.KT_PCG_UPD: "k_pcg_upd<pressure>"}
+KERNEL_NAMES = {_abi.KT_PCG_DIR: "k_pcg_dir_tile3", _abi.KT_PCG_UPD: "k_pcg_upd<1>"}
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "pcg_pressure_n96_traffic.json")
+
+
+def measured_traffic(kernel: str, n: int):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    (calibrated against a plain copy), when they were taken on this workload; else None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("n") != n:
+        return None
+    for name, rec in t.get("kernels", {}).items():
+        if name.startswith(kernel):
+            return rec["traffic_bytes"]
+    return None
 
 
 def substep_bytes(N: int, P: int, I_p: float, I_h: float) -> float:
@@ -123,8 +140,10 @@ def main() -> int:
     total_ms, launches = kt[dom]
     bytes_per_launch = KERNEL_WORDS[dom] * 4.0 * N
     achieved = bytes_per_launch * iters / (total_ms * 1e-3) / 1e9
+    traffic = measured_traffic(KERNEL_NAMES[dom], n)
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel=KERNEL_NAMES[dom],
+                    frac=round(achieved / HBM_PEAK_GBS, 4),
+                    traffic=None if traffic is None else round(traffic), kernel=KERNEL_NAMES[dom],
                     avg_launch_us=round(total_ms * 1e3 / max(launches, 1), 3), launches=launches,
                     iterations=iters, bytes_per_launch=bytes_per_launch)
 
