@@ -17,6 +17,8 @@ constexpr int kXcds = 8;         // MI355X: 8 XCDs, workgroups dispatched round-
 constexpr int kMaxPcgBlocks = 1024;     // pass B / init workgroups (4 per CU); bounds their partial sums
 constexpr int kMaxDirBlocks = 2048;     // pass A workgroups
 
+__host__ __device__ inline int nblocks(long count) { return (int)((count + kBlock - 1) / kBlock); }
+
 // cell flags (1 byte per cell)
 constexpr uint8_t FLAG_FIXED = 1;
 constexpr uint8_t FLAG_SOURCE = 2;
@@ -167,58 +169,6 @@ __device__ __forceinline__ void wait_stores() {
 __device__ __forceinline__ unsigned take_ticket(unsigned* ticket) {
     wait_stores();
     return __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Grid-wide deterministic sum in two levels.  Workgroup b stores its partial and takes a ticket
-// in bucket b % B (B = min(nb, kBuckets)); the last arrival of a bucket sums that bucket's
-// partials (in index order) and takes a ticket in the top counter; the last bucket leader sums
-// the bucket sums in bucket order.  Spreading the arrivals over kBuckets counters keeps them
-// from serialising on one address.  Returns true in all threads of the one workgroup that holds
-// the totals (in thread 0's out[]).  Tickets are reset by their last user, ready for the next
-// launch.  partials needs NV * (nb + kBuckets) doubles; tickets kBuckets + 1 counters.
-constexpr int kBuckets = 64;
-
-template <int NV>
-__device__ bool grid_sum_last(double (&v)[NV], double* __restrict__ partials, unsigned* tickets, int nb,
-                              double (*lds)[kWaves], double (&out)[NV]) {
-    __shared__ int is_last;
-    const int B = nb < kBuckets ? nb : kBuckets;
-    const int bucket = blockIdx.x % B;
-    const int members = nb / B + (bucket < nb % B ? 1 : 0);
-    double* bpart = partials + NV * nb;
-    block_sum<NV>(v, lds);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) store_agent(&partials[q * nb + blockIdx.x], v[q]);
-        is_last = (take_ticket(&tickets[bucket]) == (unsigned)(members - 1));
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    double acc[NV];
-#pragma unroll
-    for (int q = 0; q < NV; ++q) {
-        double s = 0.0;
-        for (int m = threadIdx.x; m < members; m += kBlock) s += load_agent(&partials[q * nb + bucket + B * m]);
-        acc[q] = s;
-    }
-    block_sum<NV>(acc, lds);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) store_agent(&bpart[q * kBuckets + bucket], acc[q]);
-        __hip_atomic_store(&tickets[bucket], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (take_ticket(&tickets[kBuckets]) == (unsigned)(B - 1));
-    }
-    __syncthreads();
-    if (!is_last) return false;
-#pragma unroll
-    for (int q = 0; q < NV; ++q) acc[q] = threadIdx.x < B ? load_agent(&bpart[q * kBuckets + threadIdx.x]) : 0.0;
-    block_sum<NV>(acc, lds);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) out[q] = acc[q];
-        __hip_atomic_store(&tickets[kBuckets], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    return true;
 }
 
 }  // namespace hfm

@@ -1,4 +1,4 @@
-// hfm_fields.hip — per-cell and per-face passes of the substep (gfx950).
+// hfm_fields.hip — per-cell and per-face passes of the substep (gfx950), batched over scenes.
 //
 // Each pass fuses a run of the reference's Warp launches that touch the same cells, keeping
 // their arithmetic order:
@@ -9,145 +9,16 @@
 //   rhs_mean    numpy float32 mean of the rhs                          (grid.py:238-241)
 //   project     k_project                                              (grid.py:140-153)
 //   g2p         k_particle_update                                      (heat.py:161-174)
+// Buffer roles are fixed, so the per-scene descriptors never change between substeps: heat_prep
+// keeps the previous T in T2 (what the reference's swap leaves there, solver.py:150) and
+// advection writes the new T in place; the face pass writes the advected velocity to us and the
+// projection writes the projected velocity back to u.
 #include "hfm_device.h"
 #include "hfm_scene.h"
 
 namespace hfm {
 
 namespace {
-
-inline int nblocks(long count) { return (int)((count + kBlock - 1) / kBlock); }
-
-// ---------------------------------------------------------------------------------------------
-// heat prep: Tw = blend(T) ; fixed ; capped source
-// ---------------------------------------------------------------------------------------------
-struct HeatPrepArgs {
-    long N;
-    const float* T;
-    const float* alpha_p;  // null without particles
-    const float* tsum;
-    const uint8_t* flags;
-    const float* tfix;
-    float* Tw;
-    float rate, t_cap, dt;
-    int rate_on;
-};
-
-__global__ __launch_bounds__(kBlock) void k_heat_prep(HeatPrepArgs a) {
-    long t = (long)xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (t >= a.N) return;
-    float tw = a.T[t];
-    if (a.alpha_p) {
-        float al = a.alpha_p[t];
-        if (al > 0.05f) tw = a.tsum[t] / al;
-    }
-    uint8_t fl = a.flags[t];
-    if (fl & FLAG_FIXED) tw = a.tfix[t];
-    if (a.rate_on && (fl & FLAG_SOURCE)) {
-        float up = tw + a.rate * a.dt;
-        float lo = tw > a.t_cap ? tw : a.t_cap;
-        tw = up < lo ? up : lo;
-    }
-    a.Tw[t] = tw;
-}
-
-// ---------------------------------------------------------------------------------------------
-// T advection + conductivity + heat rhs/diag
-// ---------------------------------------------------------------------------------------------
-struct AdvectTArgs {
-    int n;
-    long N;
-    float h, dt, dt_hh, ka, km;
-    const float *u, *v, *w;
-    const float* Tw;
-    const float* alpha_s;
-    const uint8_t* flags;
-    const float* tfix;
-    float *T2, *kap, *hrhs, *hdiag;
-};
-
-__global__ __launch_bounds__(kBlock) void k_advect_T(AdvectTArgs a) {
-    long t = (long)xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (t >= a.N) return;
-    const int n = a.n;
-    const long nn = (long)n * n;
-    int i = (int)(t / nn), j = (int)((t / n) % n), k = (int)(t % n);
-    float ac = a.alpha_s[t];
-    float t2;
-    if (ac > 0.5f) {
-        t2 = a.Tw[t];
-    } else {
-        float x = ((float)i + 0.5f) * a.h, y = ((float)j + 0.5f) * a.h, z = ((float)k + 0.5f) * a.h;
-        backtrace(a.u, a.v, a.w, x, y, z, a.h, n, a.dt);
-        t2 = sample(a.Tw, x, y, z, 0.5f, 0.5f, 0.5f, a.h, n, n, n);
-    }
-    a.T2[t] = t2;
-    float kc = conductivity(ac, a.ka, a.km);
-    a.kap[t] = kc;
-    if (a.flags[t] & FLAG_FIXED) {
-        a.hrhs[t] = a.tfix[t];
-        a.hdiag[t] = 1.0f;
-        return;
-    }
-    long nb[6] = {t - nn, t + nn, t - n, t + n, t - 1, t + 1};
-    bool in[6] = {i > 0, i < n - 1, j > 0, j < n - 1, k > 0, k < n - 1};
-    float ntf = 0.0f, sd = 0.0f;
-#pragma unroll
-    for (int s = 0; s < 6; ++s) {
-        if (!in[s]) continue;
-        float kf = face_kappa(kc, conductivity(a.alpha_s[nb[s]], a.ka, a.km));
-        sd += kf;
-        if (a.flags[nb[s]] & FLAG_FIXED) ntf += kf * a.tfix[nb[s]];
-    }
-    a.hrhs[t] = t2 + a.dt_hh * ntf;
-    a.hdiag[t] = 1.0f / (1.0f + a.dt_hh * sd);
-}
-
-// ---------------------------------------------------------------------------------------------
-// particle update (G2P)
-// ---------------------------------------------------------------------------------------------
-struct G2PArgs {
-    long P;
-    int n;
-    float h;
-    const float* pos;
-    const float *T_before, *T_after;
-    float *temp, *melt, *yield, *visc;
-    float t_sol, t_liq, ys, mus, mul, t_max;
-};
-
-__global__ __launch_bounds__(kBlock) void k_g2p(G2PArgs a) {
-    long p = (long)blockIdx.x * kBlock + threadIdx.x;
-    if (p >= a.P) return;
-    float x = a.pos[3 * p], y = a.pos[3 * p + 1], z = a.pos[3 * p + 2];
-    const int n = a.n;
-    float dT = sample(a.T_after, x, y, z, 0.5f, 0.5f, 0.5f, a.h, n, n, n) -
-               sample(a.T_before, x, y, z, 0.5f, 0.5f, 0.5f, a.h, n, n, n);
-    float tp = a.temp[p] + dT;
-    tp = tp < 0.0f ? 0.0f : (tp > a.t_max ? a.t_max : tp);
-    a.temp[p] = tp;
-    float f = (tp - a.t_sol) / (a.t_liq - a.t_sol);
-    f = f < 0.0f ? 0.0f : (f > 1.0f ? 1.0f : f);
-    a.melt[p] = f;
-    a.yield[p] = a.ys * (1.0f - f);
-    a.visc[p] = a.mus * (1.0f - f) + a.mul * f;
-}
-
-// ---------------------------------------------------------------------------------------------
-// air face pass: advect (old u, v, w) -> buoyancy (new T) -> relax; gridDim.y = axis
-// ---------------------------------------------------------------------------------------------
-struct FaceArgs {
-    int n;
-    long F;
-    float h, dt, g, t_amb, rho, amin;
-    int buoyancy;
-    const float* u[3];
-    float* out[3];
-    const float* kf[3];
-    const float* mf[3];
-    const float* T;
-    const float* alpha_s;
-};
 
 __device__ __forceinline__ void face_ijk(int axis, long f, int n, int& i, int& j, int& k) {
     if (axis == 0) {
@@ -165,15 +36,110 @@ __device__ __forceinline__ void face_ijk(int axis, long f, int n, int& i, int& j
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_air_faces(FaceArgs a) {
-    const int axis = blockIdx.y;
-    long f = (long)xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+// ---------------------------------------------------------------------------------------------
+// heat prep: T2 = T (previous temperature); Tw = blend(T); fixed; capped source
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_heat_prep(const SceneDev* __restrict__ tab,
+                                                      const MapEntry* __restrict__ map, float dt) {
+    HFM_BATCH_ENTRY(tab, map);
+    const long t = (long)xcd_tile(lb, nblocks(a.N)) * kBlock + threadIdx.x;
+    if (t >= a.N) return;
+    const float Tc = a.T[t];
+    a.T2[t] = Tc;
+    float tw = Tc;
+    if (a.P > 0) {
+        float al = a.alpha_p[t];
+        if (al > 0.05f) tw = a.tsum[t] / al;
+    }
+    const uint8_t fl = a.flags[t];
+    if (fl & FLAG_FIXED) tw = a.tfix[t];
+    if (a.rate_on && (fl & FLAG_SOURCE)) {
+        float up = tw + a.rate * dt;
+        float lo = tw > a.t_cap ? tw : a.t_cap;
+        tw = up < lo ? up : lo;
+    }
+    a.Tw[t] = tw;
+}
+
+// ---------------------------------------------------------------------------------------------
+// T advection (into T) + conductivity + heat rhs/diag
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_advect_T(const SceneDev* __restrict__ tab,
+                                                     const MapEntry* __restrict__ map, float dt) {
+    HFM_BATCH_ENTRY(tab, map);
+    const long t = (long)xcd_tile(lb, nblocks(a.N)) * kBlock + threadIdx.x;
+    if (t >= a.N) return;
+    const int n = a.n;
+    const long nn = (long)n * n;
+    const int i = (int)(t / nn), j = (int)((t / n) % n), k = (int)(t % n);
+    const float h = a.h;
+    const float ac = a.alpha_s[t];
+    float t2;
+    if (ac > 0.5f) {
+        t2 = a.Tw[t];
+    } else {
+        float x = ((float)i + 0.5f) * h, y = ((float)j + 0.5f) * h, z = ((float)k + 0.5f) * h;
+        backtrace(a.u[0], a.u[1], a.u[2], x, y, z, h, n, dt);
+        t2 = sample(a.Tw, x, y, z, 0.5f, 0.5f, 0.5f, h, n, n, n);
+    }
+    a.T[t] = t2;
+    const float kc = conductivity(ac, a.ka, a.km);
+    a.kap[t] = kc;
+    if (a.flags[t] & FLAG_FIXED) {
+        a.hrhs[t] = a.tfix[t];
+        a.hdiag[t] = 1.0f;
+        return;
+    }
+    const float dt_hh = dt / (h * h);
+    const long nb[6] = {t - nn, t + nn, t - n, t + n, t - 1, t + 1};
+    const bool in[6] = {i > 0, i < n - 1, j > 0, j < n - 1, k > 0, k < n - 1};
+    float ntf = 0.0f, sd = 0.0f;
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        if (!in[s]) continue;
+        float kf = face_kappa(kc, conductivity(a.alpha_s[nb[s]], a.ka, a.km));
+        sd += kf;
+        if (a.flags[nb[s]] & FLAG_FIXED) ntf += kf * a.tfix[nb[s]];
+    }
+    a.hrhs[t] = t2 + dt_hh * ntf;
+    a.hdiag[t] = 1.0f / (1.0f + dt_hh * sd);
+}
+
+// ---------------------------------------------------------------------------------------------
+// particle update (G2P): grid temperature before diffusion = Tw, after = T
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_g2p(const SceneDev* __restrict__ tab, const MapEntry* __restrict__ map) {
+    HFM_BATCH_ENTRY(tab, map);
+    const long p = (long)lb * kBlock + threadIdx.x;
+    if (p >= a.P) return;
+    const float x = a.pos[3 * p], y = a.pos[3 * p + 1], z = a.pos[3 * p + 2];
+    const int n = a.n;
+    const float dT = sample(a.T, x, y, z, 0.5f, 0.5f, 0.5f, a.h, n, n, n) -
+                     sample(a.Tw, x, y, z, 0.5f, 0.5f, 0.5f, a.h, n, n, n);
+    float tp = a.temp[p] + dT;
+    tp = tp < 0.0f ? 0.0f : (tp > a.t_max ? a.t_max : tp);
+    a.temp[p] = tp;
+    float f = (tp - a.t_sol) / (a.t_liq - a.t_sol);
+    f = f < 0.0f ? 0.0f : (f > 1.0f ? 1.0f : f);
+    a.melt[p] = f;
+    a.yield[p] = a.ys * (1.0f - f);
+    a.visc[p] = a.mus * (1.0f - f) + a.mul * f;
+}
+
+// ---------------------------------------------------------------------------------------------
+// air face pass: advect (old u, v, w) -> buoyancy (new T) -> relax, into us; sub-range = axis
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_air_faces(const SceneDev* __restrict__ tab,
+                                                      const MapEntry* __restrict__ map, float dt) {
+    HFM_BATCH_ENTRY(tab, map);
+    const int axis = sub;
+    const long f = (long)xcd_tile(lb, nblocks(a.F)) * kBlock + threadIdx.x;
     if (f >= a.F) return;
     const int n = a.n;
     int i, j, k;
     face_ijk(axis, f, n, i, j, k);
-    int idx = axis == 0 ? i : (axis == 1 ? j : k);
-    float* out = a.out[axis];
+    const int idx = axis == 0 ? i : (axis == 1 ? j : k);
+    float* out = a.us[axis];
     if (idx == 0 || idx == n) {
         out[f] = 0.0f;
         return;
@@ -191,56 +157,48 @@ __global__ __launch_bounds__(kBlock) void k_air_faces(FaceArgs a) {
         x = ((float)i + 0.5f) * h; y = ((float)j + 0.5f) * h; z = (float)k * h;
         ox = 0.5f; oy = 0.5f; oz = 0.0f; dz = n + 1;
     }
-    backtrace(a.u[0], a.u[1], a.u[2], x, y, z, h, n, a.dt);
+    backtrace(a.u[0], a.u[1], a.u[2], x, y, z, h, n, dt);
     float val = sample(a.u[axis], x, y, z, ox, oy, oz, h, dx, dy, dz);
-    long c = ((long)i * n + j) * n + k;
+    const long c = ((long)i * n + j) * n + k;
     if (axis == 2 && a.buoyancy) {
         float tf = 0.5f * (a.T[c - 1] + a.T[c]);
-        val = val + a.dt * a.g * (tf - a.t_amb) / (a.t_amb + 273.15f);
+        val = val + dt * a.g * (tf - a.t_amb) / (a.t_amb + 273.15f);
     }
-    long cl = axis == 0 ? c - (long)n * n : (axis == 1 ? c - n : c - 1);
-    float af = face_fraction(a.alpha_s[cl], a.alpha_s[c], a.amin);
-    float am = af > a.amin ? af : a.amin;
-    float cc = 1.0f / (a.rho * am * h * h * h);
-    val = (val + a.dt * cc * a.mf[axis][f]) / (1.0f + a.dt * cc * a.kf[axis][f]);
+    const long cl = axis == 0 ? c - (long)n * n : (axis == 1 ? c - n : c - 1);
+    const float af = face_fraction(a.alpha_s[cl], a.alpha_s[c], a.amin);
+    const float am = af > a.amin ? af : a.amin;
+    const float cc = 1.0f / (a.rho * am * h * h * h);
+    val = (val + dt * cc * a.mf[axis][f]) / (1.0f + dt * cc * a.kf[axis][f]);
     out[f] = val;
 }
 
 // ---------------------------------------------------------------------------------------------
 // divergence rhs (+ Jacobi diagonal)
 // ---------------------------------------------------------------------------------------------
-struct DivArgs {
-    int n;
-    long N;
-    float h, scale, amin;
-    const float *u, *v, *w, *qx, *qy, *qz;
-    const float* alpha_s;
-    float* rhs;
-    float* pdiag;  // null: no diagonal
-};
-
-__global__ __launch_bounds__(kBlock) void k_divergence(DivArgs a) {
-    long t = (long)xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (t >= a.N) return;
+__device__ __forceinline__ void divergence_cell(const SceneDev& a, const float* const* vel, long t, float scale,
+                                                float* rhs, float* pdiag) {
     const int n = a.n;
     const long nn = (long)n * n;
-    int i = (int)(t / nn), j = (int)((t / n) % n), k = (int)(t % n);
+    const int i = (int)(t / nn), j = (int)((t / n) % n), k = (int)(t % n);
     const float* as = a.alpha_s;
-    float ac = as[t];
-    float ax0 = i > 0 ? face_fraction(as[t - nn], ac, a.amin) : 0.0f;
-    float ax1 = i < n - 1 ? face_fraction(ac, as[t + nn], a.amin) : 0.0f;
-    float ay0 = j > 0 ? face_fraction(as[t - n], ac, a.amin) : 0.0f;
-    float ay1 = j < n - 1 ? face_fraction(ac, as[t + n], a.amin) : 0.0f;
-    float az0 = k > 0 ? face_fraction(as[t - 1], ac, a.amin) : 0.0f;
-    float az1 = k < n - 1 ? face_fraction(ac, as[t + 1], a.amin) : 0.0f;
-    long fx0 = ((long)i * n + j) * n + k, fx1 = fx0 + nn;
-    long fy0 = ((long)i * (n + 1) + j) * n + k, fy1 = fy0 + n;
-    long fz0 = ((long)i * n + j) * (n + 1) + k, fz1 = fz0 + 1;
-    float div = ax1 * a.u[fx1] + a.qx[fx1] - ax0 * a.u[fx0] - a.qx[fx0] + ay1 * a.v[fy1] + a.qy[fy1] -
-                ay0 * a.v[fy0] - a.qy[fy0] + az1 * a.w[fz1] + a.qz[fz1] - az0 * a.w[fz0] - a.qz[fz0];
+    const float amin = a.amin;
+    const float ac = as[t];
+    const float ax0 = i > 0 ? face_fraction(as[t - nn], ac, amin) : 0.0f;
+    const float ax1 = i < n - 1 ? face_fraction(ac, as[t + nn], amin) : 0.0f;
+    const float ay0 = j > 0 ? face_fraction(as[t - n], ac, amin) : 0.0f;
+    const float ay1 = j < n - 1 ? face_fraction(ac, as[t + n], amin) : 0.0f;
+    const float az0 = k > 0 ? face_fraction(as[t - 1], ac, amin) : 0.0f;
+    const float az1 = k < n - 1 ? face_fraction(ac, as[t + 1], amin) : 0.0f;
+    const long fx0 = ((long)i * n + j) * n + k, fx1 = fx0 + nn;
+    const long fy0 = ((long)i * (n + 1) + j) * n + k, fy1 = fy0 + n;
+    const long fz0 = ((long)i * n + j) * (n + 1) + k, fz1 = fz0 + 1;
+    const float *u = vel[0], *v = vel[1], *w = vel[2];
+    const float *qx = a.q[0], *qy = a.q[1], *qz = a.q[2];
+    float div = ax1 * u[fx1] + qx[fx1] - ax0 * u[fx0] - qx[fx0] + ay1 * v[fy1] + qy[fy1] - ay0 * v[fy0] - qy[fy0] +
+                az1 * w[fz1] + qz[fz1] - az0 * w[fz0] - qz[fz0];
     div = div / a.h;
-    a.rhs[t] = -a.scale * div;
-    if (a.pdiag) {
+    rhs[t] = -scale * div;
+    if (pdiag) {
         float s = 0.0f;
         if (i > 0) s += ax0;
         if (i < n - 1) s += ax1;
@@ -248,9 +206,24 @@ __global__ __launch_bounds__(kBlock) void k_divergence(DivArgs a) {
         if (j < n - 1) s += ay1;
         if (k > 0) s += az0;
         if (k < n - 1) s += az1;
-        float m = s > 1.0e-12f ? s : 1.0e-12f;
-        a.pdiag[t] = (a.h * a.h) / m;
+        const float m = s > 1.0e-12f ? s : 1.0e-12f;
+        pdiag[t] = (a.h * a.h) / m;
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_divergence(const SceneDev* __restrict__ tab,
+                                                       const MapEntry* __restrict__ map, double dt) {
+    HFM_BATCH_ENTRY(tab, map);
+    const long t = (long)xcd_tile(lb, nblocks(a.N)) * kBlock + threadIdx.x;
+    if (t >= a.N) return;
+    const float scale = (float)(a.rho_d / dt);  // rho/dt formed in double (grid.py:237)
+    divergence_cell(a, a.us, t, scale, a.rhs, a.pdiag);
+}
+
+__global__ __launch_bounds__(kBlock) void k_divergence1(SceneDev a, float* out) {
+    const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= a.N) return;
+    divergence_cell(a, a.u, t, 1.0f, out, nullptr);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -258,17 +231,6 @@ __global__ __launch_bounds__(kBlock) void k_divergence(DivArgs a) {
 // 128 elements with 8 interleaved accumulators), chunk sums added in order to a float32 total.
 // One workgroup per chunk; the chunk's tree is given as a leaf table and a postfix program.
 // ---------------------------------------------------------------------------------------------
-struct MeanArgs {
-    long N;
-    int nchunks;
-    const float* rhs;
-    float* chunk_sums;
-    const int* prog_full;   // [nleaves, nops, leaves(off,len)..., ops...] for a full chunk
-    const int* prog_tail;   // same for the last, shorter chunk (null if N % 8192 == 0)
-    CgCtl* ctl;
-    CgMirror* mirror;
-};
-
 constexpr int kChunk = 8192;
 constexpr int kMaxLeaves = 256;
 constexpr int kMaxChunksLds = 2048;
@@ -292,16 +254,19 @@ __device__ float leaf_sum(const float* a, int len) {
     return res;
 }
 
-__global__ __launch_bounds__(kBlock) void k_rhs_mean(MeanArgs a) {
+__global__ __launch_bounds__(kBlock) void k_rhs_mean(const SceneDev* __restrict__ tab,
+                                                     const MapEntry* __restrict__ map) {
+    HFM_BATCH_ENTRY(tab, map);
     __shared__ float leaves[kMaxLeaves];
     __shared__ int is_last;
-    const int c = blockIdx.x;
-    const bool tail = (c == a.nchunks - 1) && a.prog_tail;
-    const int* prog = tail ? a.prog_tail : a.prog_full;
+    const int c = lb;
+    const bool tail = (c == a.nchunks - 1) && a.mean_tail >= 0;
+    const int* prog = tail ? a.mean_prog + a.mean_tail : a.mean_prog;
     const int nleaves = prog[0], nops = prog[1];
     const int* lv = prog + 2;
     const int* ops = prog + 2 + 2 * nleaves;
     const float* base = a.rhs + (long)c * kChunk;
+    CgCtl* ctl = a.ctl + 1;
     for (int l = threadIdx.x; l < nleaves; l += kBlock) leaves[l] = leaf_sum(base + lv[2 * l], lv[2 * l + 1]);
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -318,14 +283,14 @@ __global__ __launch_bounds__(kBlock) void k_rhs_mean(MeanArgs a) {
             }
         }
         __hip_atomic_store(&a.chunk_sums[c], stk[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (take_ticket(&a.ctl->ticket2) == (unsigned)(a.nchunks - 1));
+        is_last = (take_ticket(&ctl->ticket2) == (unsigned)(a.nchunks - 1));
     }
     __syncthreads();
     if (!is_last) return;
     // the chunk sums are loaded by all threads at once, then added in chunk order by one thread
     __shared__ float cs[kMaxChunksLds];
     for (int q0 = 0; q0 < a.nchunks; q0 += kMaxChunksLds) {
-        int m = a.nchunks - q0 < kMaxChunksLds ? a.nchunks - q0 : kMaxChunksLds;
+        const int m = a.nchunks - q0 < kMaxChunksLds ? a.nchunks - q0 : kMaxChunksLds;
         for (int q = threadIdx.x; q < m; q += kBlock)
             cs[q] = __hip_atomic_load(&a.chunk_sums[q0 + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
@@ -337,55 +302,49 @@ __global__ __launch_bounds__(kBlock) void k_rhs_mean(MeanArgs a) {
         __syncthreads();
     }
     if (threadIdx.x != 0) return;
-    float s = leaves[0];
-    float mean = (float)((double)s / (double)a.N);
-    a.ctl->mean = mean;
-    __hip_atomic_store(&a.mirror->mean, (double)mean, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    __hip_atomic_store(&a.ctl->ticket2, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const float s = leaves[0];
+    const float mean = (float)((double)s / (double)a.N);
+    ctl->mean = mean;
+    __hip_atomic_store(&a.mirror[1].mean, (double)mean, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&ctl->ticket2, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------------------------------------
-// projection: q -= (dt/rho) grad p / h on interior faces, walls 0; gridDim.y = axis
+// projection: u = us - (dt/rho) grad p / h on interior faces, walls 0; sub-range = axis
 // ---------------------------------------------------------------------------------------------
-struct ProjArgs {
-    int n;
-    long F;
-    float h, dor;
-    const float* p;
-    float* q[3];
-};
-
-__global__ __launch_bounds__(kBlock) void k_project(ProjArgs a) {
-    const int axis = blockIdx.y;
-    long f = (long)xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kBlock) void k_project(const SceneDev* __restrict__ tab,
+                                                    const MapEntry* __restrict__ map, double dt) {
+    HFM_BATCH_ENTRY(tab, map);
+    const int axis = sub;
+    const long f = (long)xcd_tile(lb, nblocks(a.F)) * kBlock + threadIdx.x;
     if (f >= a.F) return;
     const int n = a.n;
     int i, j, k;
     face_ijk(axis, f, n, i, j, k);
-    int idx = axis == 0 ? i : (axis == 1 ? j : k);
-    float* q = a.q[axis];
+    const int idx = axis == 0 ? i : (axis == 1 ? j : k);
     if (idx == 0 || idx == n) {
-        q[f] = 0.0f;
+        a.u[axis][f] = 0.0f;
         return;
     }
-    long c = ((long)i * n + j) * n + k;
-    long cl = axis == 0 ? c - (long)n * n : (axis == 1 ? c - n : c - 1);
-    float gp = a.p[c] - a.p[cl];
-    q[f] = q[f] - a.dor * gp / a.h;
+    const float dor = (float)(dt / a.rho_d);  // dt/rho formed in double (grid.py:251)
+    const long c = ((long)i * n + j) * n + k;
+    const long cl = axis == 0 ? c - (long)n * n : (axis == 1 ? c - n : c - 1);
+    const float gp = a.p[c] - a.p[cl];
+    a.u[axis][f] = a.us[axis][f] - dor * gp / a.h;
 }
 
 // ---------------------------------------------------------------------------------------------
-// accessors and diagnostics
+// single-scene accessors and diagnostics
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_cell_velocity(int n, long N, const float* u, const float* v,
                                                           const float* w, float* out) {
-    long t = (long)blockIdx.x * kBlock + threadIdx.x;
+    const long t = (long)blockIdx.x * kBlock + threadIdx.x;
     if (t >= N) return;
     const long nn = (long)n * n;
-    int i = (int)(t / nn), j = (int)((t / n) % n), k = (int)(t % n);
-    long fx0 = ((long)i * n + j) * n + k;
-    long fy0 = ((long)i * (n + 1) + j) * n + k;
-    long fz0 = ((long)i * n + j) * (n + 1) + k;
+    const int i = (int)(t / nn), j = (int)((t / n) % n), k = (int)(t % n);
+    const long fx0 = ((long)i * n + j) * n + k;
+    const long fy0 = ((long)i * (n + 1) + j) * n + k;
+    const long fz0 = ((long)i * n + j) * (n + 1) + k;
     out[3 * t + 0] = 0.5f * (u[fx0] + u[fx0 + nn]);
     out[3 * t + 1] = 0.5f * (v[fy0] + v[fy0 + n]);
     out[3 * t + 2] = 0.5f * (w[fz0] + w[fz0 + 1]);
@@ -393,17 +352,17 @@ __global__ __launch_bounds__(kBlock) void k_cell_velocity(int n, long N, const f
 
 __global__ __launch_bounds__(kBlock) void k_face_fraction(int n, long F, int axis, float amin, const float* alpha_s,
                                                           float* out) {
-    long f = (long)blockIdx.x * kBlock + threadIdx.x;
+    const long f = (long)blockIdx.x * kBlock + threadIdx.x;
     if (f >= F) return;
     int i, j, k;
     face_ijk(axis, f, n, i, j, k);
-    int idx = axis == 0 ? i : (axis == 1 ? j : k);
+    const int idx = axis == 0 ? i : (axis == 1 ? j : k);
     if (idx == 0 || idx == n) {
         out[f] = 0.0f;
         return;
     }
-    long c = ((long)i * n + j) * n + k;
-    long cl = axis == 0 ? c - (long)n * n : (axis == 1 ? c - n : c - 1);
+    const long c = ((long)i * n + j) * n + k;
+    const long cl = axis == 0 ? c - (long)n * n : (axis == 1 ? c - n : c - 1);
     out[f] = face_fraction(alpha_s[cl], alpha_s[c], amin);
 }
 
@@ -423,157 +382,50 @@ __global__ __launch_bounds__(kBlock) void k_abs_max(const float* a, long count, 
 }
 
 __global__ __launch_bounds__(kBlock) void k_apply_fixed(long N, const uint8_t* flags, const float* tfix, float* T) {
-    long t = (long)blockIdx.x * kBlock + threadIdx.x;
+    const long t = (long)blockIdx.x * kBlock + threadIdx.x;
     if (t >= N) return;
     if (flags[t] & FLAG_FIXED) T[t] = tfix[t];
 }
 
 }  // namespace
 
-void launch_heat_prep(Scene* s, float dt) {
-    HeatPrepArgs a;
-    a.N = s->g.N;
-    a.T = s->T;
-    a.alpha_p = s->P > 0 ? s->alpha_p : nullptr;
-    a.tsum = s->tsum;
-    a.flags = s->flags;
-    a.tfix = s->tfix;
-    a.Tw = s->Tw;
-    a.rate = (float)s->rate;
-    a.t_cap = (float)s->t_cap;
-    a.dt = dt;
-    a.rate_on = s->rate > 0;
-    hipLaunchKernelGGL(k_heat_prep, dim3(nblocks(a.N)), dim3(kBlock), 0, s->stream, a);
+#define HFM_BATCH_LAUNCH(KERNEL, L, ...)                                                              \
+    do {                                                                                              \
+        if ((L).blocks > 0)                                                                           \
+            hipLaunchKernelGGL(KERNEL, dim3((L).blocks), dim3(kBlock), 0, (L).stream, (L).tab, (L).map, \
+                               ##__VA_ARGS__);                                                        \
+    } while (0)
+
+void launch_heat_prep(const Launch& L, float dt) { HFM_BATCH_LAUNCH(k_heat_prep, L, dt); }
+void launch_advect_T(const Launch& L, float dt) { HFM_BATCH_LAUNCH(k_advect_T, L, dt); }
+void launch_g2p(const Launch& L) { HFM_BATCH_LAUNCH(k_g2p, L); }
+void launch_air_faces(const Launch& L, float dt) { HFM_BATCH_LAUNCH(k_air_faces, L, dt); }
+void launch_divergence(const Launch& L, double dt) { HFM_BATCH_LAUNCH(k_divergence, L, dt); }
+void launch_rhs_mean(const Launch& L) { HFM_BATCH_LAUNCH(k_rhs_mean, L); }
+void launch_project(const Launch& L, double dt) { HFM_BATCH_LAUNCH(k_project, L, dt); }
+
+void launch_divergence_scale1(const Scene* s, hipStream_t st, float* out) {
+    hipLaunchKernelGGL(k_divergence1, dim3(nblocks(s->N)), dim3(kBlock), 0, st, s->dev, out);
 }
 
-void launch_advect_T(Scene* s, float dt) {
-    AdvectTArgs a;
-    a.n = s->g.n;
-    a.N = s->g.N;
-    a.h = s->g.h;
-    a.dt = dt;
-    a.dt_hh = dt / (a.h * a.h);
-    a.ka = (float)s->prm.kappa_air;
-    a.km = (float)s->prm.kappa_mat;
-    a.u = s->u[0];
-    a.v = s->u[1];
-    a.w = s->u[2];
-    a.Tw = s->Tw;
-    a.alpha_s = s->alpha_s;
-    a.flags = s->flags;
-    a.tfix = s->tfix;
-    a.T2 = s->T2;
-    a.kap = s->kap;
-    a.hrhs = s->hrhs;
-    a.hdiag = s->hdiag;
-    hipLaunchKernelGGL(k_advect_T, dim3(nblocks(a.N)), dim3(kBlock), 0, s->stream, a);
+void launch_cell_velocity(const Scene* s, hipStream_t st, float* out) {
+    hipLaunchKernelGGL(k_cell_velocity, dim3(nblocks(s->N)), dim3(kBlock), 0, st, s->n, s->N, s->dev.u[0],
+                       s->dev.u[1], s->dev.u[2], out);
 }
 
-void launch_g2p(Scene* s) {
-    G2PArgs a;
-    a.P = s->P;
-    a.n = s->g.n;
-    a.h = s->g.h;
-    a.pos = s->pos;
-    a.T_before = s->Tw;
-    a.T_after = s->T2;
-    a.temp = s->temp;
-    a.melt = s->melt;
-    a.yield = s->yield;
-    a.visc = s->visc;
-    a.t_sol = (float)s->prm.t_sol;
-    a.t_liq = (float)s->prm.t_liq;
-    a.ys = (float)s->prm.yield_solid;
-    a.mus = (float)s->prm.mu_solid;
-    a.mul = (float)s->prm.mu_liquid;
-    a.t_max = (float)s->prm.t_max;
-    hipLaunchKernelGGL(k_g2p, dim3(nblocks(a.P)), dim3(kBlock), 0, s->stream, a);
+void launch_face_fraction(const Scene* s, hipStream_t st, int axis, float* out) {
+    hipLaunchKernelGGL(k_face_fraction, dim3(nblocks(s->F)), dim3(kBlock), 0, st, s->n, s->F, axis, s->dev.amin,
+                       s->dev.alpha_s, out);
 }
 
-void launch_air_faces(Scene* s, float dt) {
-    FaceArgs a;
-    a.n = s->g.n;
-    a.F = s->g.F;
-    a.h = s->g.h;
-    a.dt = dt;
-    a.g = (float)s->prm.gravity;
-    a.t_amb = (float)s->prm.t_amb;
-    a.rho = (float)s->prm.rho;
-    a.amin = (float)s->prm.alpha_f_min;
-    a.buoyancy = s->prm.buoyancy;
-    for (int ax = 0; ax < 3; ++ax) {
-        a.u[ax] = s->u[ax];
-        a.out[ax] = s->us[ax];
-        a.kf[ax] = s->kf[ax];
-        a.mf[ax] = s->mf[ax];
-    }
-    a.T = s->T;
-    a.alpha_s = s->alpha_s;
-    hipLaunchKernelGGL(k_air_faces, dim3(nblocks(a.F), 3), dim3(kBlock), 0, s->stream, a);
-}
-
-void launch_divergence(Scene* s, const float* const* vel, float scale, float* rhs_out, int with_diag) {
-    DivArgs a;
-    a.n = s->g.n;
-    a.N = s->g.N;
-    a.h = s->g.h;
-    a.scale = scale;
-    a.amin = (float)s->prm.alpha_f_min;
-    a.u = vel[0];
-    a.v = vel[1];
-    a.w = vel[2];
-    a.qx = s->q[0];
-    a.qy = s->q[1];
-    a.qz = s->q[2];
-    a.alpha_s = s->alpha_s;
-    a.rhs = rhs_out;
-    a.pdiag = with_diag ? s->pdiag : nullptr;
-    hipLaunchKernelGGL(k_divergence, dim3(nblocks(a.N)), dim3(kBlock), 0, s->stream, a);
-}
-
-void launch_rhs_mean(Scene* s, const int* prog_full, const int* prog_tail) {
-    MeanArgs a;
-    a.N = s->g.N;
-    a.nchunks = (int)((s->g.N + kChunk - 1) / kChunk);
-    a.rhs = s->rhs;
-    a.chunk_sums = s->chunk_sums;
-    a.prog_full = prog_full;
-    a.prog_tail = prog_tail;
-    a.ctl = &s->ctl[1];
-    a.mirror = &s->mirror_dev[1];
-    hipLaunchKernelGGL(k_rhs_mean, dim3(a.nchunks), dim3(kBlock), 0, s->stream, a);
-}
-
-void launch_project(Scene* s, float dt_over_rho) {
-    ProjArgs a;
-    a.n = s->g.n;
-    a.F = s->g.F;
-    a.h = s->g.h;
-    a.dor = dt_over_rho;
-    a.p = s->p;
-    for (int ax = 0; ax < 3; ++ax) a.q[ax] = s->us[ax];
-    hipLaunchKernelGGL(k_project, dim3(nblocks(a.F), 3), dim3(kBlock), 0, s->stream, a);
-}
-
-void launch_cell_velocity(Scene* s, float* out) {
-    hipLaunchKernelGGL(k_cell_velocity, dim3(nblocks(s->g.N)), dim3(kBlock), 0, s->stream, s->g.n, s->g.N, s->u[0],
-                       s->u[1], s->u[2], out);
-}
-
-void launch_face_fraction(Scene* s, int axis, float* out) {
-    hipLaunchKernelGGL(k_face_fraction, dim3(nblocks(s->g.F)), dim3(kBlock), 0, s->stream, s->g.n, s->g.F, axis,
-                       (float)s->prm.alpha_f_min, s->alpha_s, out);
-}
-
-void launch_abs_max(Scene* s, const float* a, long count, unsigned* slot) {
+void launch_abs_max(hipStream_t st, const float* a, long count, unsigned* slot) {
     int nb = nblocks(count);
     if (nb > 1024) nb = 1024;
-    hipLaunchKernelGGL(k_abs_max, dim3(nb), dim3(kBlock), 0, s->stream, a, count, slot);
+    hipLaunchKernelGGL(k_abs_max, dim3(nb), dim3(kBlock), 0, st, a, count, slot);
 }
 
-void launch_apply_fixed(Scene* s, float* T) {
-    hipLaunchKernelGGL(k_apply_fixed, dim3(nblocks(s->g.N)), dim3(kBlock), 0, s->stream, s->g.N, s->flags, s->tfix,
-                       T);
+void launch_apply_fixed(const Scene* s, hipStream_t st, float* T) {
+    hipLaunchKernelGGL(k_apply_fixed, dim3(nblocks(s->N)), dim3(kBlock), 0, st, s->N, s->dev.flags, s->dev.tfix, T);
 }
 
 }  // namespace hfm

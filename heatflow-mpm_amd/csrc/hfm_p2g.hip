@@ -17,8 +17,6 @@ namespace hfm {
 
 namespace {
 
-inline int nblocks(long count) { return (int)((count + kBlock - 1) / kBlock); }
-
 __device__ __forceinline__ int bin_of(float x, float h, int n) { return clampi((int)floorf(x / h), 0, n - 1); }
 
 __global__ __launch_bounds__(kBlock) void k_bin_count(long P, int n, float h, const float* pos, int* cnt, int* key,
@@ -121,23 +119,8 @@ __global__ __launch_bounds__(kBlock) void k_bin_sort(long N, const int* start, i
 }
 
 // ---------------------------------------------------------------------------------------------
-// gather: gridDim.y = 0 cell lattice (alpha_s, alpha_p, tsum), 1..3 face lattices (q, K, m)
+// gather: sub-range 0 = cell lattice (alpha_s, alpha_p, tsum), 1..3 = face lattices (q, K, m)
 // ---------------------------------------------------------------------------------------------
-struct GatherArgs {
-    int n;
-    long N, F;
-    float h, frac, k0;
-    const float *pos, *vel, *temp;
-    const int *start, *sorted;
-    const float* st_alpha;
-    const float* st_k[3];
-    const float* st_m[3];
-    float *alpha_s, *alpha_p, *tsum;
-    float* q[3];
-    float* kf[3];
-    float* mf[3];
-};
-
 // Weight with which particle (px,py,pz) reaches node (ni,nj,nk) of a lattice with offsets
 // (ox,oy,oz) and dims (dx,dy,dz): the sum over the corners whose clamped index is the node,
 // each added separately in the corner order of fields.py:41-48.
@@ -169,15 +152,17 @@ __device__ __forceinline__ void node_terms(float px, float py, float pz, float o
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_p2g_gather(GatherArgs a) {
-    const int lat = blockIdx.y;  // 0 cell, 1 + axis for faces
+__global__ __launch_bounds__(kBlock) void k_p2g_gather(const SceneDev* __restrict__ tab,
+                                                       const MapEntry* __restrict__ map) {
+    HFM_BATCH_ENTRY(tab, map);
+    const int lat = sub;  // 0 cell, 1 + axis for faces
     const int n = a.n;
-    long node = (long)xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     const long count = lat == 0 ? a.N : a.F;
+    const long node = (long)xcd_tile(lb, nblocks(count)) * kBlock + threadIdx.x;
     if (node >= count) return;
     int dx = n, dy = n, dz = n;
     float ox = 0.5f, oy = 0.5f, oz = 0.5f;
-    int axis = lat - 1;
+    const int axis = lat - 1;
     if (axis == 0) { dx = n + 1; ox = 0.0f; }
     if (axis == 1) { dy = n + 1; oy = 0.0f; }
     if (axis == 2) { dz = n + 1; oz = 0.0f; }
@@ -194,7 +179,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_gather(GatherArgs a) {
     for (int bx = lo_x; bx <= hi_x; ++bx)
         for (int by = lo_y; by <= hi_y; ++by) {
             long row = ((long)bx * n + by) * n;
-            int b = a.start[row + lo_z], e = a.start[row + hi_z + 1];
+            int b = a.bin_start[row + lo_z], e = a.bin_start[row + hi_z + 1];
             for (int s = b; s < e; ++s) {
                 int p = a.sorted[s];
                 float px = a.pos[3 * p], py = a.pos[3 * p + 1], pz = a.pos[3 * p + 2];
@@ -233,51 +218,28 @@ __global__ __launch_bounds__(kBlock) void k_p2g_gather(GatherArgs a) {
 
 }  // namespace
 
-void launch_bin_particles(Scene* s) {
-    const long N = s->g.N, P = s->P;
-    (void)hipMemsetAsync(s->bin_cnt, 0, (N + 1) * sizeof(int), s->stream);
+void launch_bin_particles(Scene* s, hipStream_t st) {
+    const long N = s->N, P = s->dev.P;
+    (void)hipMemsetAsync(s->bin_cnt, 0, (N + 1) * sizeof(int), st);
     if (P > 0)
-        hipLaunchKernelGGL(k_bin_count, dim3(nblocks(P)), dim3(kBlock), 0, s->stream, P, s->g.n, s->g.h, s->pos,
+        hipLaunchKernelGGL(k_bin_count, dim3(nblocks(P)), dim3(kBlock), 0, st, P, s->n, s->dev.h, s->dev.pos,
                            s->bin_cnt, s->pkey, s->pslot);
-    long count = N;  // scan N counts into N + 1 starts
-    int ntiles = (int)((count + 1 + kScanTile - 1) / kScanTile);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(kBlock), 0, s->stream, s->bin_cnt, count, s->scan_tmp);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, s->stream, s->scan_tmp, ntiles);
-    hipLaunchKernelGGL(k_scan_final, dim3(ntiles), dim3(kBlock), 0, s->stream, s->bin_cnt, count, s->scan_tmp,
-                       s->bin_start);
+    const long count = N;  // scan N counts into N + 1 starts
+    const int ntiles = (int)((count + 1 + kScanTile - 1) / kScanTile);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(kBlock), 0, st, s->bin_cnt, count, s->scan_tmp);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, st, s->scan_tmp, ntiles);
+    hipLaunchKernelGGL(k_scan_final, dim3(ntiles), dim3(kBlock), 0, st, s->bin_cnt, count, s->scan_tmp,
+                       s->dev.bin_start);
     if (P > 0) {
-        hipLaunchKernelGGL(k_bin_fill, dim3(nblocks(P)), dim3(kBlock), 0, s->stream, P, s->pkey, s->pslot,
-                           s->bin_start, s->sorted);
-        hipLaunchKernelGGL(k_bin_sort, dim3(nblocks(N)), dim3(kBlock), 0, s->stream, N, s->bin_start, s->sorted);
+        hipLaunchKernelGGL(k_bin_fill, dim3(nblocks(P)), dim3(kBlock), 0, st, P, s->pkey, s->pslot,
+                           s->dev.bin_start, s->dev.sorted);
+        hipLaunchKernelGGL(k_bin_sort, dim3(nblocks(N)), dim3(kBlock), 0, st, N, s->dev.bin_start, s->dev.sorted);
     }
     s->bins_dirty = 0;
 }
 
-void launch_p2g(Scene* s) {
-    GatherArgs a;
-    a.n = s->g.n;
-    a.N = s->g.N;
-    a.F = s->g.F;
-    a.h = s->g.h;
-    a.frac = (float)s->prm.vol / (a.h * a.h * a.h);
-    a.k0 = (float)s->prm.k_particle;
-    a.pos = s->pos;
-    a.vel = s->vel;
-    a.temp = s->temp;
-    a.start = s->bin_start;
-    a.sorted = s->sorted;
-    a.st_alpha = s->st_alpha;
-    a.alpha_s = s->alpha_s;
-    a.alpha_p = s->alpha_p;
-    a.tsum = s->tsum;
-    for (int ax = 0; ax < 3; ++ax) {
-        a.st_k[ax] = s->st_k[ax];
-        a.st_m[ax] = s->st_m[ax];
-        a.q[ax] = s->q[ax];
-        a.kf[ax] = s->kf[ax];
-        a.mf[ax] = s->mf[ax];
-    }
-    hipLaunchKernelGGL(k_p2g_gather, dim3(nblocks(a.F), 4), dim3(kBlock), 0, s->stream, a);
+void launch_p2g(const Launch& L) {
+    if (L.blocks > 0) hipLaunchKernelGGL(k_p2g_gather, dim3(L.blocks), dim3(kBlock), 0, L.stream, L.tab, L.map);
 }
 
 }  // namespace hfm

@@ -1,32 +1,40 @@
-// hfm_scene.h — scene state and kernel launchers of libhfm.so (host side).
+// hfm_scene.h — scenes, batches and kernel launchers of libhfm.so.
+//
+// Every substep kernel runs over a batch: a set of scenes (grids of any size on one device) that
+// advance together.  Each launch covers all scenes of the batch: workgroup b reads a 64-bit
+// entry of a block map (scene, sub-range, local workgroup) and the scene's SceneDev descriptor,
+// so one launch per stage serves every scene and the fixed cost of a launch is shared.  A
+// single scene runs as a batch of one.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <vector>
 
 #include "../../include/hfm.h"
 
 namespace hfm {
 
 // Device-resident control block of one PCG solve (warp.optim.linear.cg semantics).  Written by
-// workgroup 0 of a kernel and read by later kernels only.
+// local workgroup 0 of a scene in a kernel and read by later kernels only.
 struct CgCtl {
-    float slot_rr[2]; // ||r||^2 after iteration k, in slot k & 1
-    float slot_rz[2]; // r.z after iteration k, in slot k & 1
-    float atol_sq;    // fp32 threshold on ||r||^2
-    int it;           // iterations done when the solve stopped
-    int done;         // 1: converged or iteration cap reached; every later CG kernel returns at once
+    float slot_rr[2];  // ||r||^2 after iteration k, in slot k & 1
+    float slot_rz[2];  // r.z after iteration k, in slot k & 1
+    float atol_sq;     // fp32 threshold on ||r||^2
+    int it;            // iterations done when the solve stopped
+    int done;          // 1: converged or iteration cap reached; every later CG kernel skips the scene
     int converged;
-    int maxeff;       // iteration cap rounded up to the 2-iteration cycle
+    int maxeff;        // iteration cap rounded up to the 2-iteration cycle
     int pad0;
-    double tol;       // relative tolerance
-    double atol;      // max(tol * ||b||, tol)
-    float mean;       // pressure rhs mean to remove (numpy float32 mean)
-    unsigned ticket2; // arrival counter of the rhs-mean chunks
+    double tol;        // relative tolerance
+    double atol;       // max(tol * ||b||, tol)
+    float mean;        // pressure rhs mean to remove (numpy float32 mean)
+    unsigned ticket2;  // arrival counter of the rhs-mean chunks
 };
 
-// Pinned host memory written by the device with system-scope stores: CG progress and results,
-// polled by the host to decide how far ahead to enqueue iterations.
+// Pinned host memory written by the device with system-scope stores (results of a solve),
+// polled by the host.
 struct CgMirror {
     int it;
     int done;
@@ -37,120 +45,154 @@ struct CgMirror {
     double mean;
 };
 
-// Fixed launch geometry of a scene.
-struct Geo {
-    int n;
-    long N;       // n^3 cells
-    long F;       // (n+1) n^2 faces per axis
-    float h;
+// Per-scene view read by the kernels (one entry per scene of a batch, in device memory).
+// Parameters are fp32 as the reference passes them to its kernels; rho is also kept in double
+// because rho/dt and dt/rho are formed in double on the reference's host (grid.py:237,251).
+struct SceneDev {
+    int n, P;
+    long N, F;
+    float h, hh;
+    float rho, t_amb, g, amin, ka, km, frac, k0, t_sol, t_liq, ys, mus, mul, t_max, rate, t_cap;
+    int rate_on, buoyancy;
+    double rho_d;
+    float *u[3], *us[3], *q[3], *kf[3], *mf[3], *st_k[3], *st_m[3];
+    float *p, *rhs, *pdiag, *alpha_s, *st_alpha;
+    float *T, *T2, *Tw, *kap, *alpha_p, *tsum, *tfix, *hrhs, *hdiag;
+    uint8_t* flags;
+    float *cg_r, *cg_z, *cg_d, *cg_Ap;
+    double* partials;        // [kMaxDirBlocks] pass A partials, then [3][nb_upd] pass B / init
+    CgCtl* ctl;              // [0] heat, [1] pressure
+    CgMirror* mirror;        // device view of the pinned mirrors, [0] heat, [1] pressure
+    int cpt, nb_upd, nb_dir, tiles_j, tiles_k, dtile;
+    float* chunk_sums;
+    const int* mean_prog;
+    int mean_tail;           // offset of the tail chunk's program in mean_prog, -1 if none
+    int nchunks;
+    float *pos, *vel;
+    float *temp, *melt, *yield, *visc;
+    int *bin_start, *sorted;
 };
+
+// Block-map entries: scene (24 bits) | sub-range (8 bits) | local workgroup (32 bits).
+typedef unsigned long long MapEntry;
+constexpr MapEntry kNoBlock = ~0ull;
+constexpr int kMaxBatch = 4096;
+__host__ __device__ inline MapEntry map_entry(int s, int sub, long l) {
+    return ((MapEntry)s << 40) | ((MapEntry)sub << 32) | (MapEntry)(unsigned)l;
+}
+
+// Workgroup prologue of a batched kernel: `a` is the scene's descriptor, `sub` the sub-range
+// and `lb` the workgroup's index inside it.  Padding entries end the workgroup at once.
+#define HFM_BATCH_ENTRY(tab, map)                                 \
+    const ::hfm::MapEntry e_ = (map)[blockIdx.x];                 \
+    if (e_ == ::hfm::kNoBlock) return;                            \
+    const int sub = (int)((e_ >> 32) & 0xFFu);                    \
+    const int lb = (int)(e_ & 0xFFFFFFFFu);                       \
+    const ::hfm::SceneDev& a = (tab)[e_ >> 40];                   \
+    (void)sub;                                                    \
+    (void)lb;
+
+// Launch families: how many workgroups each scene needs in each sub-range.
+enum Family {
+    FAM_CELL = 0,   // one workgroup per 256 cells
+    FAM_FACE3,      // 3 axes x one workgroup per 256 faces
+    FAM_P2G,        // cell lattice + 3 face lattices (scenes with particles)
+    FAM_MEAN,       // one workgroup per 8192-cell rhs chunk
+    FAM_PCG,        // PCG pass B / init / flat pass A workgroups
+    FAM_G2P,        // one workgroup per 256 particles (scenes with particles)
+    FAM_DIR4,       // tiled pressure pass A, 16x16x4 blocks (1024 threads)
+    FAM_DIR8,
+    FAM_DIR16,
+    FAM_DIRFLAT,    // pressure pass A without tiling (PCG geometry)
+    FAM_COUNT
+};
+
+struct Batch;
 
 struct Scene {
     hfm_params prm;
-    Geo g;
     int device;
-    hipStream_t stream;
-
-    // air (face-major, each axis on its own lattice)
-    float* u[3];          // velocity
-    float* us[3];         // advected + buoyancy + relaxed velocity, projected in place, then swapped
-    float* q[3];          // particle flux, zero on wall faces (solver.py:95-97, grid.py:62-67)
-    float* kf[3];         // particle + static relaxation weights
-    float* mf[3];         // particle + static relaxation momenta
-    float* st_k[3];
-    float* st_m[3];
-    float* p;             // pressure (warm start across substeps)
-    float* rhs;           // pressure rhs (mean removed)
-    float* pdiag;         // pressure Jacobi preconditioner
-    float* alpha_s;       // static + particle solid fraction
-    float* st_alpha;
+    hipStream_t stream;     // the scene's own stream (its solo batch runs on it)
+    Batch* solo;            // the scene as a batch of one
+    Batch* current;         // the batch whose stream last ran work for the scene
+    int members;            // user batches that hold the scene
+    SceneDev dev;           // descriptor (host copy)
+    CgMirror* mirror;       // pinned result mirrors, [0] heat, [1] pressure (host view)
+    int n;
+    long N, F;
     int has_static;
-
-    // heat
-    float* T;
-    float* T2;
-    float* Tw;
-    float* kap;
-    float* alpha_p;       // particle-only solid fraction (heat.py:12-18)
-    float* tsum;
-    float* tfix;
-    float* hrhs;
-    float* hdiag;
-    uint8_t* flags;       // FLAG_FIXED | FLAG_SOURCE per cell
     double rate, t_cap;
-
-    // CG work vectors (the heat and pressure solves run one after the other)
-    float* cg_r;
-    float* cg_z;
-    float* cg_d;          // search direction, two buffers of N (double-buffered by iteration parity)
-    float* cg_Ap;
-    double* partials;     // per-workgroup partial sums: [max_blocks] pass A, [3][max_blocks] pass B
-    long max_blocks;
-    CgCtl* ctl;           // [0] heat, [1] pressure (device)
-    CgMirror* mirror;     // [0] heat, [1] pressure (pinned host)
-    CgMirror* mirror_dev; // device view of mirror
-    float* chunk_sums;    // numpy-mean chunk sums
-    int* mean_prog;       // pairwise-sum programs: full chunk, then the tail chunk
-    int mean_prog_tail;   // offset of the tail program in mean_prog, -1 if none
-    float* scratch_field; // face-sized temporary for getters and diagnostics
-    unsigned* umax;       // diagnostics maxima
-
-    // particles
-    long P;
     long P_cap;
-    float* pos;           // (P, 3)
-    float* vel;           // (P, 3)
-    float* temp;
-    float* melt;
-    float* yield;
-    float* visc;
-    int* pkey;            // bin (cell) of each particle
-    int* pslot;           // arrival slot inside its bin
-    int* sorted;          // particle ids grouped by bin, ascending inside a bin
-    int* bin_cnt;         // N + 1
-    int* bin_start;       // N + 1 (exclusive scan)
+    int* pkey;
+    int* pslot;
+    int* bin_cnt;
     int* scan_tmp;
     int bins_dirty;
-
+    unsigned version;       // bumped whenever the descriptor changes
     hfm_stats stats;
-    int pcg_cpt;          // minimum tiles per PCG workgroup (HFM_PCG_CPT)
-    int flat_dir;         // 1: pressure pass A without LDS tiling (HFM_PCG_FLAT_DIR)
-    int dir_ichunk;       // planes per workgroup of the tiled pass A, 0 = automatic (HFM_DIR_ICHUNK)
-    int heat_pred;        // last exact heat iteration count (chunk sizing)
-    int pres_pred;
-
-    // kernel timing (HFM_FLAG_TIME_KERNELS)
-    int timing;
-    hipEvent_t* ev_pool;
-    int ev_cap, ev_used;
-    int* ev_class;
-    double kt_ms[HFM_KT_COUNT];
-    long long kt_n[HFM_KT_COUNT];
+    int heat_pred, pres_pred;
+    float* scratch_field;   // face-sized temporary for getters and diagnostics
+    unsigned* umax;
+    int flat_dir, dir_ichunk, pcg_cpt;
 };
 
-// ---- kernel launchers (hfm_fields.hip) ----
-void launch_heat_prep(Scene* s, float dt);
-void launch_advect_T(Scene* s, float dt);
-void launch_g2p(Scene* s);
-void launch_air_faces(Scene* s, float dt);
-void launch_divergence(Scene* s, const float* const* vel, float scale, float* rhs_out, int with_diag);
-void launch_rhs_mean(Scene* s, const int* prog_full, const int* prog_tail);
-void launch_project(Scene* s, float dt_over_rho);
-void launch_cell_velocity(Scene* s, float* out);
-void launch_face_fraction(Scene* s, int axis, float* out);
-void launch_abs_max(Scene* s, const float* a, long count, unsigned* slot);
-void launch_apply_fixed(Scene* s, float* T);
+struct Batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    bool user = false;                       // created through the batch API
+    std::vector<Scene*> scenes;
+    SceneDev* table = nullptr;               // device copy of the descriptors
+    MapEntry* maps[FAM_COUNT] = {};          // device block maps
+    int map_len[FAM_COUNT] = {};
+    std::vector<unsigned> versions;          // scene versions the maps and table were built for
+    unsigned long long* pace = nullptr;      // pinned: (solve tag << 32) | iterations completed
+    unsigned long long* pace_dev = nullptr;
+    unsigned solve_tag = 0;
+    // kernel timing (HFM_FLAG_TIME_KERNELS)
+    int timing = 0;
+    hipEvent_t* ev_pool = nullptr;
+    int ev_cap = 0, ev_used = 0;
+    int* ev_class = nullptr;
+    double kt_ms[HFM_KT_COUNT] = {};
+    long long kt_n[HFM_KT_COUNT] = {};
+};
 
-// ---- kernel launchers (hfm_p2g.hip) ----
-void launch_bin_particles(Scene* s);
-void launch_p2g(Scene* s);
+struct Launch {
+    const SceneDev* tab;
+    const MapEntry* map;
+    int blocks;
+    hipStream_t stream;
+};
 
-// ---- kernel launchers (hfm_pcg.hip); op 0 = heat, 1 = pressure ----
-int pcg_blocks(const Scene* s);
-void launch_pcg_init(Scene* s, int op, float dt);
-void launch_pcg_dir(Scene* s, int op, float dt, int k, int report, hipEvent_t ev0 = nullptr,
-                    hipEvent_t ev1 = nullptr);
-void launch_pcg_upd(Scene* s, int op, int k, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
-int pcg_cpt(const Scene* s);
+inline Launch family(const Batch* b, Family f) { return Launch{b->table, b->maps[f], b->map_len[f], b->stream}; }
+
+// ---- batched kernel launchers (hfm_fields.hip) ----
+void launch_heat_prep(const Launch& L, float dt);
+void launch_advect_T(const Launch& L, float dt);
+void launch_g2p(const Launch& L);
+void launch_air_faces(const Launch& L, float dt);
+void launch_divergence(const Launch& L, double dt);
+void launch_rhs_mean(const Launch& L);
+void launch_project(const Launch& L, double dt);
+
+// ---- single-scene helpers (hfm_fields.hip) ----
+void launch_divergence_scale1(const Scene* s, hipStream_t st, float* out);
+void launch_cell_velocity(const Scene* s, hipStream_t st, float* out);
+void launch_face_fraction(const Scene* s, hipStream_t st, int axis, float* out);
+void launch_abs_max(hipStream_t st, const float* a, long count, unsigned* slot);
+void launch_apply_fixed(const Scene* s, hipStream_t st, float* T);
+
+// ---- particles (hfm_p2g.hip) ----
+void launch_bin_particles(Scene* s, hipStream_t st);
+void launch_p2g(const Launch& L);
+
+// ---- PCG (hfm_pcg.hip); op 0 = heat, 1 = pressure ----
+void pcg_geometry(const Scene* s, int* cpt, int* nb_upd, int* nb_dir, int* tiles_j, int* tiles_k, int* dtile);
+void launch_pcg_init(const Launch& L, int op, float dt);
+// pace: when non-null, workgroup 0 posts (tag << 32) | k to it as the launch starts
+void launch_pcg_dir(const Launch& L, Family fam, int op, float dt, int k, unsigned long long* pace, unsigned tag,
+                    hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+void launch_pcg_upd(const Launch& L, int op, int k, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 
 }  // namespace hfm

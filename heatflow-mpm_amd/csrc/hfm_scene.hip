@@ -1,18 +1,22 @@
 // hfm_scene.hip — C ABI of libhfm.so and the host-side order of the substep.
 //
-// HeatflowMPM.substep (solver.py:128-151) as launched here, one HIP stream per scene:
+// HeatflowMPM.substep (solver.py:128-151) as launched here, for a batch of scenes on one stream
+// (a single scene is a batch of one), one launch per stage over all scenes:
 //   [particles] bin (when positions changed) + deterministic P2G gather
-//   heat_prep -> advect_T -> heat PCG (x = T2 in place) -> [particles] G2P -> swap T, T2
-//   air_faces -> divergence -> numpy-exact rhs mean -> pressure PCG (warm start) -> project
+//   heat_prep (T2 = T) -> advect_T (into T) -> heat PCG (x = T) -> [particles] G2P
+//   air_faces (u -> us) -> divergence -> numpy-exact rhs mean -> pressure PCG (warm start)
+//   -> project (us -> u)
 // The host never waits for the GPU inside a substep except to decide how many PCG iterations
-// to enqueue: the last workgroup of every iteration publishes its progress to pinned host
-// memory, and the host keeps a bounded number of iterations in flight.
+// to enqueue: the first pass of each group of iterations posts the batch's progress to pinned
+// host memory, every scene posts the end of its solve there, and the host keeps a bounded
+// number of iterations in flight.
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <string>
 #include <thread>
@@ -22,6 +26,10 @@
 #include "hfm_scene.h"
 
 using namespace hfm;
+
+struct hfm_batch {
+    Batch b;
+};
 
 namespace {
 
@@ -52,6 +60,12 @@ void set_err(const char* fmt, ...) {
             set_err("kernel launch failed: %s (%s:%d)", hipGetErrorString(e_), __FILE__, __LINE__); \
             return HFM_ERR_HIP;                                                              \
         }                                                                                    \
+    } while (0)
+
+#define RC(expr)                   \
+    do {                           \
+        int rc_ = (expr);          \
+        if (rc_) return rc_;       \
     } while (0)
 
 struct Owner {
@@ -120,110 +134,301 @@ int warp_reported_count(int exact, int converged, int maxiter, int check_every) 
     return cur;
 }
 
+// Scalar part of the descriptor, from the scene's parameters and state (fp32 as the reference
+// passes them to its kernels).
+void refresh_dev(Scene* s) {
+    SceneDev& d = s->dev;
+    const hfm_params& p = s->prm;
+    d.n = s->n;
+    d.N = s->N;
+    d.F = s->F;
+    d.h = (float)p.h;
+    d.hh = d.h * d.h;
+    d.rho = (float)p.rho;
+    d.rho_d = p.rho;
+    d.t_amb = (float)p.t_amb;
+    d.g = (float)p.gravity;
+    d.amin = (float)p.alpha_f_min;
+    d.ka = (float)p.kappa_air;
+    d.km = (float)p.kappa_mat;
+    d.frac = (float)p.vol / (d.h * d.h * d.h);
+    d.k0 = (float)p.k_particle;
+    d.t_sol = (float)p.t_sol;
+    d.t_liq = (float)p.t_liq;
+    d.ys = (float)p.yield_solid;
+    d.mus = (float)p.mu_solid;
+    d.mul = (float)p.mu_liquid;
+    d.t_max = (float)p.t_max;
+    d.rate = (float)s->rate;
+    d.t_cap = (float)s->t_cap;
+    d.rate_on = s->rate > 0;
+    d.buoyancy = p.buoyancy;
+    s->version += 1;
+}
+
+// ---- batches -------------------------------------------------------------------------------
+
+int batch_init(Batch* b, int device, hipStream_t stream, bool own_stream) {
+    b->device = device;
+    b->stream = stream;
+    b->own_stream = own_stream;
+    HIP_OK(hipHostMalloc((void**)&b->pace, sizeof(unsigned long long), hipHostMallocMapped | hipHostMallocCoherent));
+    *b->pace = 0;
+    HIP_OK(hipHostGetDevicePointer((void**)&b->pace_dev, b->pace, 0));
+    return HFM_OK;
+}
+
+void batch_free_maps(Batch* b) {
+    if (b->table) (void)hipFree(b->table);
+    b->table = nullptr;
+    for (int f = 0; f < FAM_COUNT; ++f) {
+        if (b->maps[f]) (void)hipFree(b->maps[f]);
+        b->maps[f] = nullptr;
+        b->map_len[f] = 0;
+    }
+    b->versions.clear();
+}
+
+void batch_release(Batch* b) {
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_free_maps(b);
+    if (b->pace) (void)hipHostFree(b->pace);
+    b->pace = nullptr;
+    if (b->ev_pool) {
+        for (int i = 0; i < b->ev_cap; ++i) (void)hipEventDestroy(b->ev_pool[i]);
+        delete[] b->ev_pool;
+        delete[] b->ev_class;
+        b->ev_pool = nullptr;
+    }
+    if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
+    b->stream = nullptr;
+}
+
+// Workgroups scene s needs in sub-range sub of a launch family.
+long family_blocks(const Scene* s, int f, int sub) {
+    const SceneDev& d = s->dev;
+    switch (f) {
+        case FAM_CELL: return sub == 0 ? nblocks(s->N) : 0;
+        case FAM_FACE3: return sub < 3 ? nblocks(s->F) : 0;
+        case FAM_P2G: return d.P > 0 ? (sub == 0 ? nblocks(s->N) : nblocks(s->F)) : 0;
+        case FAM_MEAN: return sub == 0 ? d.nchunks : 0;
+        case FAM_PCG: return sub == 0 ? d.nb_upd : 0;
+        case FAM_G2P: return sub == 0 && d.P > 0 ? nblocks(d.P) : 0;
+        case FAM_DIR4: return sub == 0 && d.dtile == 4 ? d.nb_dir : 0;
+        case FAM_DIR8: return sub == 0 && d.dtile == 8 ? d.nb_dir : 0;
+        case FAM_DIR16: return sub == 0 && d.dtile == 16 ? d.nb_dir : 0;
+        case FAM_DIRFLAT: return sub == 0 && d.dtile == 0 ? d.nb_dir : 0;
+        default: return 0;
+    }
+}
+
+// (Re)build the device descriptor table and block maps when a scene's descriptor changed.  Each
+// (scene, sub-range) run of workgroups starts at a multiple of 8, so a workgroup's index inside
+// its scene has the XCD of its hardware index and the XCD-aware tile mapping keeps working.
+int batch_prepare(Batch* b) {
+    const size_t ns = b->scenes.size();
+    bool fresh = b->table && b->versions.size() == ns;
+    for (size_t i = 0; fresh && i < ns; ++i) fresh = b->versions[i] == b->scenes[i]->version;
+    if (fresh) return HFM_OK;
+    HIP_OK(hipStreamSynchronize(b->stream));
+    batch_free_maps(b);
+    std::vector<SceneDev> tab(ns);
+    for (size_t i = 0; i < ns; ++i) tab[i] = b->scenes[i]->dev;
+    void* q = nullptr;
+    HIP_OK(hipMalloc(&q, ns * sizeof(SceneDev)));
+    b->table = (SceneDev*)q;
+    HIP_OK(hipMemcpy(b->table, tab.data(), ns * sizeof(SceneDev), hipMemcpyHostToDevice));
+    std::vector<MapEntry> m;
+    for (int f = 0; f < FAM_COUNT; ++f) {
+        m.clear();
+        for (size_t i = 0; i < ns; ++i)
+            for (int sub = 0; sub < 4; ++sub) {
+                const long nb = family_blocks(b->scenes[i], f, sub);
+                for (long l = 0; l < nb; ++l) m.push_back(map_entry((int)i, sub, l));
+                while (m.size() % kXcds) m.push_back(kNoBlock);
+            }
+        if (m.size() > (size_t)0x7fffffff) {
+            set_err("batch too large: %zu workgroups in one launch", m.size());
+            return HFM_ERR_ARG;
+        }
+        b->map_len[f] = (int)m.size();
+        if (m.empty()) continue;
+        HIP_OK(hipMalloc(&q, m.size() * sizeof(MapEntry)));
+        b->maps[f] = (MapEntry*)q;
+        HIP_OK(hipMemcpy(b->maps[f], m.data(), m.size() * sizeof(MapEntry), hipMemcpyHostToDevice));
+    }
+    b->versions.resize(ns);
+    for (size_t i = 0; i < ns; ++i) b->versions[i] = b->scenes[i]->version;
+    return HFM_OK;
+}
+
+// Make b the batch that runs the work of its scenes: work a scene has queued on another stream
+// is waited for first, so work on a scene stays ordered across batches.
+int batch_adopt(Batch* b) {
+    for (Scene* s : b->scenes)
+        if (s->current != b) {
+            if (s->current) HIP_OK(hipStreamSynchronize(s->current->stream));
+            s->current = b;
+        }
+    return HFM_OK;
+}
+
+// A scene-level call: its own stream, after any batch work on the scene.
+int scene_adopt(Scene* s) { return batch_adopt(s->solo); }
+
 // ---- kernel timing -------------------------------------------------------------------------
 
-int flush_timing(Scene* s) {
-    if (!s->timing || s->ev_used == 0) return HFM_OK;
-    HIP_OK(hipStreamSynchronize(s->stream));
-    for (int e = 0; e + 1 < s->ev_used; e += 2) {
+int flush_timing(Batch* b) {
+    if (!b->timing || b->ev_used == 0) return HFM_OK;
+    HIP_OK(hipStreamSynchronize(b->stream));
+    for (int e = 0; e + 1 < b->ev_used; e += 2) {
         float ms = 0.f;
-        HIP_OK(hipEventElapsedTime(&ms, s->ev_pool[e], s->ev_pool[e + 1]));
-        s->kt_ms[s->ev_class[e]] += ms;
-        s->kt_n[s->ev_class[e]] += 1;
+        HIP_OK(hipEventElapsedTime(&ms, b->ev_pool[e], b->ev_pool[e + 1]));
+        b->kt_ms[b->ev_class[e]] += ms;
+        b->kt_n[b->ev_class[e]] += 1;
     }
-    s->ev_used = 0;
+    b->ev_used = 0;
     return HFM_OK;
 }
 
 // A pair of events for one timed launch of kernel class cls (flushing the pool when full).
-int timing_events(Scene* s, int cls, hipEvent_t* ev0, hipEvent_t* ev1) {
+int timing_events(Batch* b, int cls, hipEvent_t* ev0, hipEvent_t* ev1) {
     *ev0 = *ev1 = nullptr;
-    if (!s->timing) return HFM_OK;
-    if (s->ev_used + 2 > s->ev_cap) {
-        int rc = flush_timing(s);
-        if (rc) return rc;
-    }
-    s->ev_class[s->ev_used] = cls;
-    *ev0 = s->ev_pool[s->ev_used++];
-    *ev1 = s->ev_pool[s->ev_used++];
+    if (!b->timing) return HFM_OK;
+    if (b->ev_used + 2 > b->ev_cap) RC(flush_timing(b));
+    b->ev_class[b->ev_used] = cls;
+    *ev0 = b->ev_pool[b->ev_used++];
+    *ev1 = b->ev_pool[b->ev_used++];
     return HFM_OK;
 }
 
-int enable_timing(Scene* s) {
-    if (!s->ev_pool) {
-        s->ev_cap = 4096;
-        s->ev_pool = new hipEvent_t[s->ev_cap];
-        s->ev_class = new int[s->ev_cap];
-        for (int i = 0; i < s->ev_cap; ++i) HIP_OK(hipEventCreate(&s->ev_pool[i]));
+int enable_timing(Batch* b) {
+    if (!b->ev_pool) {
+        b->ev_cap = 4096;
+        b->ev_pool = new hipEvent_t[b->ev_cap];
+        b->ev_class = new int[b->ev_cap];
+        for (int i = 0; i < b->ev_cap; ++i) HIP_OK(hipEventCreate(&b->ev_pool[i]));
     }
-    s->timing = 1;
+    b->timing = 1;
+    return HFM_OK;
+}
+
+int set_timing(Batch* b, int on) {
+    if (on) return enable_timing(b);
+    int rc = flush_timing(b);
+    b->timing = 0;
+    return rc;
+}
+
+int kernel_time(Batch* b, int cls, double* total_ms, int64_t* launches) {
+    if (cls < 0 || cls >= HFM_KT_COUNT) {
+        set_err("bad kernel class %d", cls);
+        return HFM_ERR_ARG;
+    }
+    RC(flush_timing(b));
+    if (total_ms) *total_ms = b->kt_ms[cls];
+    if (launches) *launches = b->kt_n[cls];
+    return HFM_OK;
+}
+
+int kernel_time_reset(Batch* b) {
+    RC(flush_timing(b));
+    for (int c = 0; c < HFM_KT_COUNT; ++c) {
+        b->kt_ms[c] = 0.0;
+        b->kt_n[c] = 0;
+    }
     return HFM_OK;
 }
 
 // ---- PCG driver ----------------------------------------------------------------------------
 
-// Enqueue one PCG solve (op 0 heat, 1 pressure) and return when the device reports it done.
-int run_pcg(Scene* s, int op, float dt) {
-    CgMirror* m = &s->mirror[op];
-    __atomic_store_n(&m->it, 0, __ATOMIC_RELAXED);
-    __atomic_store_n(&m->done, 0, __ATOMIC_RELEASE);
-    launch_pcg_init(s, op, dt);
+// Enqueue one PCG solve (op 0 heat, 1 pressure) of every scene of the batch and return when
+// the device reports all of them done.
+int run_pcg(Batch* b, int op, float dt) {
+    const unsigned tag = ++b->solve_tag;
+    int maxeff = 0, pred = 2;
+    for (Scene* s : b->scenes) {
+        CgMirror* m = &s->mirror[op];
+        __atomic_store_n(&m->it, 0, __ATOMIC_RELAXED);
+        __atomic_store_n(&m->done, 0, __ATOMIC_RELEASE);
+        const int mi = op ? s->prm.cg_max_iter : s->prm.heat_max_iter;
+        maxeff = std::max(maxeff, mi + (mi & 1));
+        pred = std::max(pred, op ? s->pres_pred : s->heat_pred);
+    }
+    launch_pcg_init(family(b, FAM_PCG), op, dt);
     CHECK_LAUNCH();
-    const int maxiter = op ? s->prm.cg_max_iter : s->prm.heat_max_iter;
-    const int maxeff = maxiter + (maxiter & 1);
-    int& pred = op ? s->pres_pred : s->heat_pred;
-    // Iterations are enqueued in batches of (pass A, pass B) pairs, at most two batches in
-    // flight.  Pass A of iteration k+1 finishes iteration k, so the first pass A of a batch
-    // posts how many iterations the previous batch completed, which releases the next batch;
-    // after the last pair one more pass A closes the solve.  Batch size follows the iteration
+    // pass A launches: the heat operator uses the flat kernel with the PCG geometry; the
+    // pressure operator one launch per tiling present in the batch
+    Family dirs[4];
+    int ndir = 0;
+    if (op == 0) {
+        dirs[ndir++] = FAM_PCG;
+    } else {
+        for (Family f : {FAM_DIR4, FAM_DIR8, FAM_DIR16, FAM_DIRFLAT})
+            if (b->map_len[f] > 0) dirs[ndir++] = f;
+    }
+    // Iterations are enqueued in groups of (pass A, pass B) pairs, at most two groups in
+    // flight.  Pass A of iteration k+1 finishes iteration k, so the first pass A of a group
+    // posts how many iterations the previous groups completed, which releases the next group;
+    // after the last pair one more pass A closes every solve.  Group size follows the iteration
     // count of the previous solve so that few launches are wasted after convergence.
     int launched = 0;
-    int batch_end[2] = {0, 0};
+    int group_end[2] = {0, 0};
     int inflight = 0;
     bool closed = false;
-    auto t0 = std::chrono::steady_clock::now();
+    const auto t0 = std::chrono::steady_clock::now();
     long spins = 0;
+    const bool timed = op == 1 && b->timing;
     for (;;) {
-        if (__atomic_load_n(&m->done, __ATOMIC_ACQUIRE)) break;
-        int prog = __atomic_load_n(&m->it, __ATOMIC_ACQUIRE);
-        while (inflight > 0 && prog >= batch_end[0]) {
-            batch_end[0] = batch_end[1];
+        bool all_done = true;
+        for (Scene* s : b->scenes)
+            if (!__atomic_load_n(&s->mirror[op].done, __ATOMIC_ACQUIRE)) {
+                all_done = false;
+                break;
+            }
+        if (all_done) break;
+        const unsigned long long w = __atomic_load_n(b->pace, __ATOMIC_ACQUIRE);
+        const int prog = (unsigned)(w >> 32) == tag ? (int)(w & 0xffffffffu) : 0;
+        while (inflight > 0 && prog >= group_end[0]) {
+            group_end[0] = group_end[1];
             --inflight;
         }
         if (inflight < 2 && launched < maxeff) {
             int g = pred - launched;
             g = g < 2 ? 2 : (g > 16 ? 16 : g);
             if (g > maxeff - launched) g = maxeff - launched;
-            const bool timed = op == 1 && s->timing;
             for (int q = 0; q < g; ++q) {
-                int k = launched + q;
+                const int k = launched + q;
+                for (int d = 0; d < ndir; ++d) {
+                    hipEvent_t e0 = nullptr, e1 = nullptr;
+                    if (timed) RC(timing_events(b, HFM_KT_PCG_DIR, &e0, &e1));
+                    launch_pcg_dir(family(b, dirs[d]), dirs[d], op, dt, k, q == 0 && d == 0 ? b->pace_dev : nullptr,
+                                   tag, e0, e1);
+                }
                 hipEvent_t e0 = nullptr, e1 = nullptr;
-                int rc = timed ? timing_events(s, HFM_KT_PCG_DIR, &e0, &e1) : HFM_OK;
-                if (rc) return rc;
-                launch_pcg_dir(s, op, dt, k, q == 0, e0, e1);
-                rc = timed ? timing_events(s, HFM_KT_PCG_UPD, &e0, &e1) : HFM_OK;
-                if (rc) return rc;
-                launch_pcg_upd(s, op, k, e0, e1);
+                if (timed) RC(timing_events(b, HFM_KT_PCG_UPD, &e0, &e1));
+                launch_pcg_upd(family(b, FAM_PCG), op, k, e0, e1);
             }
             CHECK_LAUNCH();
             launched += g;
-            batch_end[inflight++] = launched;
+            group_end[inflight++] = launched;
             continue;
         }
         if (launched >= maxeff && !closed) {
-            launch_pcg_dir(s, op, dt, maxeff, 1);
+            for (int d = 0; d < ndir; ++d)
+                launch_pcg_dir(family(b, dirs[d]), dirs[d], op, dt, maxeff, d == 0 ? b->pace_dev : nullptr, tag);
             CHECK_LAUNCH();
             closed = true;
             continue;
         }
         if ((++spins & 0x3ff) == 0) {
-            double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-            if (el > 30.0) {
-                set_err("PCG (%s) did not finish within 30 s (iteration %d of %d launched)",
+            const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (el > 60.0) {
+                set_err("PCG (%s) did not finish within 60 s (progress %d, %d iterations launched)",
                         op ? "pressure" : "heat", prog, launched);
                 return HFM_ERR_HIP;
             }
-            hipError_t e = hipStreamQuery(s->stream);
+            hipError_t e = hipStreamQuery(b->stream);
             if (e != hipSuccess && e != hipErrorNotReady) {
                 set_err("stream error during PCG: %s", hipGetErrorString(e));
                 return HFM_ERR_HIP;
@@ -231,86 +436,105 @@ int run_pcg(Scene* s, int op, float dt) {
             std::this_thread::yield();
         }
     }
-    int it = __atomic_load_n(&m->it, __ATOMIC_ACQUIRE);
-    int conv = __atomic_load_n(&m->converged, __ATOMIC_ACQUIRE);
-    pred = it > 2 ? it : 2;
-    int ce = op ? s->prm.cg_check_every : s->prm.heat_check_every;
-    int reported = it == 0 && conv ? 0 : warp_reported_count(it, conv, maxiter, ce);
-    if (op == 1) {
-        s->stats.p_iters_total += it;
-        s->stats.p_iters_exact = it;
-        s->stats.p_iters = reported;
-        s->stats.p_converged = conv;
-        s->stats.p_rr = m->rr;
-        s->stats.p_atol = m->atol;
-        s->stats.rhs_mean = m->mean;
-    } else {
-        s->stats.h_iters_total += it;
-        s->stats.h_iters_exact = it;
-        s->stats.h_iters = reported;
-        s->stats.h_converged = conv;
-        s->stats.h_rr = m->rr;
-        s->stats.h_atol = m->atol;
+    for (Scene* s : b->scenes) {
+        CgMirror* m = &s->mirror[op];
+        const int it = __atomic_load_n(&m->it, __ATOMIC_ACQUIRE);
+        const int conv = __atomic_load_n(&m->converged, __ATOMIC_ACQUIRE);
+        const int maxiter = op ? s->prm.cg_max_iter : s->prm.heat_max_iter;
+        const int ce = op ? s->prm.cg_check_every : s->prm.heat_check_every;
+        const int reported = it == 0 && conv ? 0 : warp_reported_count(it, conv, maxiter, ce);
+        if (op == 1) {
+            s->pres_pred = it > 2 ? it : 2;
+            s->stats.p_iters_total += it;
+            s->stats.p_iters_exact = it;
+            s->stats.p_iters = reported;
+            s->stats.p_converged = conv;
+            s->stats.p_rr = m->rr;
+            s->stats.p_atol = m->atol;
+            s->stats.rhs_mean = m->mean;
+        } else {
+            s->heat_pred = it > 2 ? it : 2;
+            s->stats.h_iters_total += it;
+            s->stats.h_iters_exact = it;
+            s->stats.h_iters = reported;
+            s->stats.h_converged = conv;
+            s->stats.h_rr = m->rr;
+            s->stats.h_atol = m->atol;
+        }
     }
     return HFM_OK;
 }
 
-int substep(Owner* o, double dt) {
-    Scene* s = &o->s;
+int batch_substep(Batch* b, double dt) {
+    RC(batch_prepare(b));
+    RC(batch_adopt(b));
     const float dt32 = (float)dt;
-    if (s->P > 0) {
-        if (s->bins_dirty) launch_bin_particles(s);
-        launch_p2g(s);
-    }
-    launch_heat_prep(s, dt32);
-    launch_advect_T(s, dt32);
+    for (Scene* s : b->scenes)
+        if (s->dev.P > 0 && s->bins_dirty) launch_bin_particles(s, b->stream);
+    launch_p2g(family(b, FAM_P2G));
+    launch_heat_prep(family(b, FAM_CELL), dt32);
+    launch_advect_T(family(b, FAM_CELL), dt32);
     CHECK_LAUNCH();
-    int rc = run_pcg(s, 0, dt32);
-    if (rc) return rc;
-    if (s->P > 0) launch_g2p(s);
-    float* tmp = s->T;
-    s->T = s->T2;
-    s->T2 = tmp;
-    launch_air_faces(s, dt32);
-    launch_divergence(s, (const float* const*)s->us, (float)(s->prm.rho / dt), s->rhs, 1);
-    const int* pf = o->s.mean_prog;
-    const int* pt = s->mean_prog_tail >= 0 ? s->mean_prog + s->mean_prog_tail : nullptr;
-    launch_rhs_mean(s, pf, pt);
+    RC(run_pcg(b, 0, dt32));
+    launch_g2p(family(b, FAM_G2P));
+    launch_air_faces(family(b, FAM_FACE3), dt32);
+    launch_divergence(family(b, FAM_CELL), dt);
+    launch_rhs_mean(family(b, FAM_MEAN));
     CHECK_LAUNCH();
-    rc = run_pcg(s, 1, dt32);
-    if (rc) return rc;
-    launch_project(s, (float)(dt / s->prm.rho));
+    RC(run_pcg(b, 1, dt32));
+    launch_project(family(b, FAM_FACE3), dt);
     CHECK_LAUNCH();
-    for (int a = 0; a < 3; ++a) {
-        float* t = s->u[a];
-        s->u[a] = s->us[a];
-        s->us[a] = t;
-    }
-    s->stats.substeps += 1;
+    for (Scene* s : b->scenes) s->stats.substeps += 1;
     return HFM_OK;
+}
+
+int batch_run(Batch* b, double dt, int K, double* ms) {
+    RC(batch_prepare(b));
+    RC(batch_adopt(b));
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    int rc = HFM_OK;
+    if (hipEventRecord(e0, b->stream) != hipSuccess) rc = HFM_ERR_HIP;
+    for (int k = 0; k < K && !rc; ++k) rc = batch_substep(b, dt);
+    float el = 0.f;
+    if (!rc) {
+        hipError_t e = hipEventRecord(e1, b->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&el, e0, e1);
+        if (e != hipSuccess) {
+            set_err("run timing: %s", hipGetErrorString(e));
+            rc = HFM_ERR_HIP;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!rc && ms) *ms = el;
+    return rc;
 }
 
 float* field_ptr(Scene* s, int f, long* size) {
-    long N = s->g.N, F = s->g.F;
+    const long N = s->N, F = s->F;
+    SceneDev& d = s->dev;
     switch (f) {
-        case HFM_FIELD_T: *size = N; return s->T;
-        case HFM_FIELD_T2: *size = N; return s->T2;
-        case HFM_FIELD_TW: *size = N; return s->Tw;
-        case HFM_FIELD_U: *size = F; return s->u[0];
-        case HFM_FIELD_V: *size = F; return s->u[1];
-        case HFM_FIELD_W: *size = F; return s->u[2];
-        case HFM_FIELD_P: *size = N; return s->p;
-        case HFM_FIELD_RHS: *size = N; return s->rhs;
-        case HFM_FIELD_ALPHA_S: *size = N; return s->alpha_s;
-        case HFM_FIELD_ALPHA_P: *size = N; return s->alpha_p;
-        case HFM_FIELD_TSUM: *size = N; return s->tsum;
-        case HFM_FIELD_KAPPA: *size = N; return s->kap;
-        case HFM_FIELD_QX: case HFM_FIELD_QY: case HFM_FIELD_QZ: *size = F; return s->q[f - HFM_FIELD_QX];
-        case HFM_FIELD_KX: case HFM_FIELD_KY: case HFM_FIELD_KZ: *size = F; return s->kf[f - HFM_FIELD_KX];
-        case HFM_FIELD_MX: case HFM_FIELD_MY: case HFM_FIELD_MZ: *size = F; return s->mf[f - HFM_FIELD_MX];
-        case HFM_FIELD_PDIAG: *size = N; return s->pdiag;
-        case HFM_FIELD_HRHS: *size = N; return s->hrhs;
-        case HFM_FIELD_HDIAG: *size = N; return s->hdiag;
+        case HFM_FIELD_T: *size = N; return d.T;
+        case HFM_FIELD_T2: *size = N; return d.T2;
+        case HFM_FIELD_TW: *size = N; return d.Tw;
+        case HFM_FIELD_U: *size = F; return d.u[0];
+        case HFM_FIELD_V: *size = F; return d.u[1];
+        case HFM_FIELD_W: *size = F; return d.u[2];
+        case HFM_FIELD_P: *size = N; return d.p;
+        case HFM_FIELD_RHS: *size = N; return d.rhs;
+        case HFM_FIELD_ALPHA_S: *size = N; return d.alpha_s;
+        case HFM_FIELD_ALPHA_P: *size = N; return d.alpha_p;
+        case HFM_FIELD_TSUM: *size = N; return d.tsum;
+        case HFM_FIELD_KAPPA: *size = N; return d.kap;
+        case HFM_FIELD_QX: case HFM_FIELD_QY: case HFM_FIELD_QZ: *size = F; return d.q[f - HFM_FIELD_QX];
+        case HFM_FIELD_KX: case HFM_FIELD_KY: case HFM_FIELD_KZ: *size = F; return d.kf[f - HFM_FIELD_KX];
+        case HFM_FIELD_MX: case HFM_FIELD_MY: case HFM_FIELD_MZ: *size = F; return d.mf[f - HFM_FIELD_MX];
+        case HFM_FIELD_PDIAG: *size = N; return d.pdiag;
+        case HFM_FIELD_HRHS: *size = N; return d.hrhs;
+        case HFM_FIELD_HDIAG: *size = N; return d.hdiag;
         default: *size = 0; return nullptr;
     }
 }
@@ -326,14 +550,15 @@ int check_finite(const float* a, long n, const char* what) {
 
 // totals without particles: solid fraction and face weights are the static ones
 int reset_particle_fields(Scene* s) {
-    long N = s->g.N, F = s->g.F;
-    HIP_OK(hipMemsetAsync(s->alpha_p, 0, N * sizeof(float), s->stream));
-    HIP_OK(hipMemsetAsync(s->tsum, 0, N * sizeof(float), s->stream));
-    HIP_OK(hipMemcpyAsync(s->alpha_s, s->st_alpha, N * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
+    const long N = s->N, F = s->F;
+    SceneDev& d = s->dev;
+    HIP_OK(hipMemsetAsync(d.alpha_p, 0, N * sizeof(float), s->stream));
+    HIP_OK(hipMemsetAsync(d.tsum, 0, N * sizeof(float), s->stream));
+    HIP_OK(hipMemcpyAsync(d.alpha_s, d.st_alpha, N * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
     for (int a = 0; a < 3; ++a) {
-        HIP_OK(hipMemsetAsync(s->q[a], 0, F * sizeof(float), s->stream));
-        HIP_OK(hipMemcpyAsync(s->kf[a], s->st_k[a], F * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
-        HIP_OK(hipMemcpyAsync(s->mf[a], s->st_m[a], F * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
+        HIP_OK(hipMemsetAsync(d.q[a], 0, F * sizeof(float), s->stream));
+        HIP_OK(hipMemcpyAsync(d.kf[a], d.st_k[a], F * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
+        HIP_OK(hipMemcpyAsync(d.mf[a], d.st_m[a], F * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
     }
     return HFM_OK;
 }
@@ -345,7 +570,17 @@ int reset_particle_fields(Scene* s) {
     }                                             \
     Owner* o = owner(h);                          \
     Scene* s = &o->s;                             \
-    HIP_OK(hipSetDevice(s->device));
+    (void)o;                                      \
+    HIP_OK(hipSetDevice(s->device));              \
+    RC(scene_adopt(s));
+
+#define BATCH_OR_FAIL(h)                          \
+    if (!(h)) {                                   \
+        set_err("null batch");                    \
+        return HFM_ERR_ARG;                       \
+    }                                             \
+    Batch* b = &(h)->b;                           \
+    HIP_OK(hipSetDevice(b->device));
 
 }  // namespace
 
@@ -374,7 +609,7 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
         set_err("precision mode %d is not available", prm->precision);
         return HFM_ERR_ARG;
     }
-    int ndev = hfm_device_count();
+    const int ndev = hfm_device_count();
     if (device < 0 || device >= ndev) {
         set_err("device %d not available (%d HIP devices)", device, ndev);
         return HFM_ERR_ARG;
@@ -385,24 +620,24 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
     memset(s, 0, sizeof(Scene));
     s->prm = *prm;
     s->device = device;
-    s->g.n = prm->n;
-    s->g.N = (long)prm->n * prm->n * prm->n;
-    s->g.F = (long)(prm->n + 1) * prm->n * prm->n;
-    s->g.h = (float)prm->h;
+    s->n = prm->n;
+    s->N = (long)prm->n * prm->n * prm->n;
+    s->F = (long)(prm->n + 1) * prm->n * prm->n;
     s->heat_pred = 2;
+    s->pres_pred = 8;
     {
         // minimum tiles per PCG workgroup (the workgroup cap may raise it); HFM_PCG_CPT overrides
         const char* e = getenv("HFM_PCG_CPT");
-        int c = e ? atoi(e) : 1;
+        const int c = e ? atoi(e) : 1;
         s->pcg_cpt = c >= 1 && c <= 64 ? c : 1;
         e = getenv("HFM_PCG_FLAT_DIR");
         s->flat_dir = e && atoi(e) != 0;
         e = getenv("HFM_DIR_ICHUNK");
-        int ic = e ? atoi(e) : 0;
+        const int ic = e ? atoi(e) : 0;
         s->dir_ichunk = ic >= 1 && ic <= 1024 ? ic : 0;
     }
-    s->pres_pred = 8;
-    s->mean_prog_tail = -1;
+    SceneDev& d = s->dev;
+    d.mean_tail = -1;
     int rc = HFM_OK;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -410,59 +645,61 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
         delete o;
         return HFM_ERR_HIP;
     }
-    const long N = s->g.N, F = s->g.F;
+    const long N = s->N, F = s->F;
 #define A(ptr, cnt)                        \
     if ((rc = dalloc(o, &(ptr), (cnt)))) { \
         hfm_scene_destroy((hfm_scene*)o);  \
         return rc;                         \
     }
     for (int a = 0; a < 3; ++a) {
-        A(s->u[a], F);
-        A(s->us[a], F);
-        A(s->q[a], F);
-        A(s->kf[a], F);
-        A(s->mf[a], F);
-        A(s->st_k[a], F);
-        A(s->st_m[a], F);
+        A(d.u[a], F);
+        A(d.us[a], F);
+        A(d.q[a], F);
+        A(d.kf[a], F);
+        A(d.mf[a], F);
+        A(d.st_k[a], F);
+        A(d.st_m[a], F);
     }
-    A(s->p, N);
-    A(s->rhs, N);
-    A(s->pdiag, N);
-    A(s->alpha_s, N);
-    A(s->st_alpha, N);
-    A(s->T, N);
-    A(s->T2, N);
-    A(s->Tw, N);
-    A(s->kap, N);
-    A(s->alpha_p, N);
-    A(s->tsum, N);
-    A(s->tfix, N);
-    A(s->hrhs, N);
-    A(s->hdiag, N);
-    A(s->flags, N);
-    A(s->cg_r, N);
-    A(s->cg_z, N);
-    A(s->cg_d, 2 * N);
-    A(s->cg_Ap, N);
-    s->max_blocks = kMaxPcgBlocks;
-    A(s->partials, kMaxDirBlocks + 3 * kMaxPcgBlocks);
-    A(s->ctl, 2);
-    long nchunks = (N + 8191) / 8192;
-    A(s->chunk_sums, nchunks);
+    A(d.p, N);
+    A(d.rhs, N);
+    A(d.pdiag, N);
+    A(d.alpha_s, N);
+    A(d.st_alpha, N);
+    A(d.T, N);
+    A(d.T2, N);
+    A(d.Tw, N);
+    A(d.kap, N);
+    A(d.alpha_p, N);
+    A(d.tsum, N);
+    A(d.tfix, N);
+    A(d.hrhs, N);
+    A(d.hdiag, N);
+    A(d.flags, N);
+    A(d.cg_r, N);
+    A(d.cg_z, N);
+    A(d.cg_d, 2 * N);
+    A(d.cg_Ap, N);
+    A(d.partials, kMaxDirBlocks + 3 * kMaxPcgBlocks);
+    A(d.ctl, 2);
+    d.nchunks = (int)((N + 8191) / 8192);
+    A(d.chunk_sums, d.nchunks);
     A(s->scratch_field, F);
     A(s->umax, 4);
     A(s->bin_cnt, N + 1);
-    A(s->bin_start, N + 1);
+    A(d.bin_start, N + 1);
     A(s->scan_tmp, (N + 1) / 2048 + 16);
     // pairwise-sum programs for the numpy mean
     append_program(o->prog, (int)(N >= 8192 ? 8192 : N));
     if (N % 8192 != 0 && N > 8192) {
-        s->mean_prog_tail = (int)o->prog.size();
+        d.mean_tail = (int)o->prog.size();
         append_program(o->prog, (int)(N % 8192));
     }
-    A(s->mean_prog, (long)o->prog.size());
+    int* prog_dev = nullptr;
+    A(prog_dev, (long)o->prog.size());
 #undef A
-    e = hipMemcpy(s->mean_prog, o->prog.data(), o->prog.size() * sizeof(int), hipMemcpyHostToDevice);
+    d.mean_prog = prog_dev;
+    pcg_geometry(s, &d.cpt, &d.nb_upd, &d.nb_dir, &d.tiles_j, &d.tiles_k, &d.dtile);
+    e = hipMemcpy(prog_dev, o->prog.data(), o->prog.size() * sizeof(int), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         set_err("hipMemcpy: %s", hipGetErrorString(e));
         hfm_scene_destroy((hfm_scene*)o);
@@ -475,12 +712,12 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
     ctl[0].maxeff = prm->heat_max_iter + (prm->heat_max_iter & 1);
     ctl[1].tol = prm->cg_tol;
     ctl[1].maxeff = prm->cg_max_iter + (prm->cg_max_iter & 1);
-    e = hipMemcpy(s->ctl, ctl, sizeof(ctl), hipMemcpyHostToDevice);
+    e = hipMemcpy(d.ctl, ctl, sizeof(ctl), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipHostMalloc((void**)&s->mirror, 2 * sizeof(CgMirror), hipHostMallocMapped | hipHostMallocCoherent);
     if (e == hipSuccess) {
         memset(s->mirror, 0, 2 * sizeof(CgMirror));
-        e = hipHostGetDevicePointer((void**)&s->mirror_dev, s->mirror, 0);
+        e = hipHostGetDevicePointer((void**)&d.mirror, s->mirror, 0);
     }
     if (e != hipSuccess) {
         set_err("control block setup: %s", hipGetErrorString(e));
@@ -489,21 +726,24 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
     }
     // heat.py:182-183: T, T2, Tw start at the room temperature
     std::vector<float> tamb(N, (float)prm->t_amb);
-    e = hipMemcpy(s->T, tamb.data(), N * sizeof(float), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(s->T2, tamb.data(), N * sizeof(float), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(s->Tw, tamb.data(), N * sizeof(float), hipMemcpyHostToDevice);
+    e = hipMemcpy(d.T, tamb.data(), N * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d.T2, tamb.data(), N * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d.Tw, tamb.data(), N * sizeof(float), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         set_err("initial temperature: %s", hipGetErrorString(e));
         hfm_scene_destroy((hfm_scene*)o);
         return HFM_ERR_HIP;
     }
-    if (prm->flags & HFM_FLAG_TIME_KERNELS) {
-        int rc2 = enable_timing(s);
-        if (rc2) {
-            hfm_scene_destroy((hfm_scene*)o);
-            return rc2;
-        }
+    refresh_dev(s);
+    s->solo = new Batch();
+    s->solo->scenes.push_back(s);
+    rc = batch_init(s->solo, device, s->stream, false);
+    if (!rc && (prm->flags & HFM_FLAG_TIME_KERNELS)) rc = enable_timing(s->solo);
+    if (rc) {
+        hfm_scene_destroy((hfm_scene*)o);
+        return rc;
     }
+    s->current = s->solo;
     *out = (hfm_scene*)o;
     return HFM_OK;
 }
@@ -512,16 +752,20 @@ int hfm_scene_destroy(hfm_scene* h) {
     if (!h) return HFM_OK;
     Owner* o = owner(h);
     Scene* s = &o->s;
+    if (s->members > 0) {
+        set_err("scene is held by %d batch(es); destroy them first", s->members);
+        return HFM_ERR_STATE;
+    }
     (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->solo) {
+        batch_release(s->solo);
+        delete s->solo;
+        s->solo = nullptr;
+    }
     for (void* p : o->dev)
         if (p) (void)hipFree(p);
     if (s->mirror) (void)hipHostFree(s->mirror);
-    if (s->ev_pool) {
-        for (int i = 0; i < s->ev_cap; ++i) (void)hipEventDestroy(s->ev_pool[i]);
-        delete[] s->ev_pool;
-        delete[] s->ev_class;
-    }
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete o;
     return HFM_OK;
@@ -534,7 +778,7 @@ int hfm_set_static(hfm_scene* h, const float* alpha, const float* kx, const floa
         set_err("null array");
         return HFM_ERR_ARG;
     }
-    const long N = s->g.N, F = s->g.F;
+    const long N = s->N, F = s->F;
     const float* k[3] = {kx, ky, kz};
     const float* m[3] = {mx, my, mz};
     int rc = check_finite(alpha, N, "alpha");
@@ -544,15 +788,14 @@ int hfm_set_static(hfm_scene* h, const float* alpha, const float* kx, const floa
     }
     if (rc) return rc;
     HIP_OK(hipStreamSynchronize(s->stream));
-    HIP_OK(hipMemcpy(s->st_alpha, alpha, N * sizeof(float), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(s->dev.st_alpha, alpha, N * sizeof(float), hipMemcpyHostToDevice));
     for (int a = 0; a < 3; ++a) {
-        HIP_OK(hipMemcpy(s->st_k[a], k[a], F * sizeof(float), hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(s->st_m[a], m[a], F * sizeof(float), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(s->dev.st_k[a], k[a], F * sizeof(float), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(s->dev.st_m[a], m[a], F * sizeof(float), hipMemcpyHostToDevice));
     }
     s->has_static = 1;
-    if (s->P == 0) {
-        rc = reset_particle_fields(s);
-        if (rc) return rc;
+    if (s->dev.P == 0) {
+        RC(reset_particle_fields(s));
         HIP_OK(hipStreamSynchronize(s->stream));
     }
     return HFM_OK;
@@ -564,16 +807,15 @@ int hfm_set_fixed(hfm_scene* h, const int32_t* fixed, const float* tfix) {
         set_err("null array");
         return HFM_ERR_ARG;
     }
-    const long N = s->g.N;
-    int rc = check_finite(tfix, N, "tfix");
-    if (rc) return rc;
+    const long N = s->N;
+    RC(check_finite(tfix, N, "tfix"));
     std::vector<uint8_t> fl(N);
     HIP_OK(hipStreamSynchronize(s->stream));
-    HIP_OK(hipMemcpy(fl.data(), s->flags, N, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(fl.data(), s->dev.flags, N, hipMemcpyDeviceToHost));
     for (long t = 0; t < N; ++t) fl[t] = (uint8_t)((fl[t] & ~FLAG_FIXED) | (fixed[t] == 1 ? FLAG_FIXED : 0));
-    HIP_OK(hipMemcpy(s->flags, fl.data(), N, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(s->tfix, tfix, N * sizeof(float), hipMemcpyHostToDevice));
-    launch_apply_fixed(s, s->T);  // heat.py:192
+    HIP_OK(hipMemcpy(s->dev.flags, fl.data(), N, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(s->dev.tfix, tfix, N * sizeof(float), hipMemcpyHostToDevice));
+    launch_apply_fixed(s, s->stream, s->dev.T);  // heat.py:192
     CHECK_LAUNCH();
     HIP_OK(hipStreamSynchronize(s->stream));
     return HFM_OK;
@@ -589,14 +831,15 @@ int hfm_set_source(hfm_scene* h, const int32_t* mask, double rate, double t_cap)
         set_err("rate/t_cap not finite");
         return HFM_ERR_NONFINITE;
     }
-    const long N = s->g.N;
+    const long N = s->N;
     std::vector<uint8_t> fl(N);
     HIP_OK(hipStreamSynchronize(s->stream));
-    HIP_OK(hipMemcpy(fl.data(), s->flags, N, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(fl.data(), s->dev.flags, N, hipMemcpyDeviceToHost));
     for (long t = 0; t < N; ++t) fl[t] = (uint8_t)((fl[t] & ~FLAG_SOURCE) | (mask[t] == 1 ? FLAG_SOURCE : 0));
-    HIP_OK(hipMemcpy(s->flags, fl.data(), N, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(s->dev.flags, fl.data(), N, hipMemcpyHostToDevice));
     s->rate = rate;
     s->t_cap = t_cap;
+    refresh_dev(s);
     return HFM_OK;
 }
 
@@ -611,30 +854,37 @@ int hfm_set_particles(hfm_scene* h, int64_t P, const float* pos, const float* ve
     if (!rc && P > 0) rc = check_finite(temp, P, "temp");
     if (rc) return rc;
     HIP_OK(hipStreamSynchronize(s->stream));
+    SceneDev& d = s->dev;
     if (P > s->P_cap) {
-        void* olds[] = {s->pos, s->vel, s->temp, s->melt, s->yield, s->visc, s->pkey, s->pslot, s->sorted};
+        void* olds[] = {d.pos, d.vel, d.temp, d.melt, d.yield, d.visc, s->pkey, s->pslot, d.sorted};
         for (void* q : olds)
             if (q) dfree(o, q);
-        if ((rc = dalloc(o, &s->pos, 3 * P)) || (rc = dalloc(o, &s->vel, 3 * P)) || (rc = dalloc(o, &s->temp, P)) ||
-            (rc = dalloc(o, &s->melt, P)) || (rc = dalloc(o, &s->yield, P)) || (rc = dalloc(o, &s->visc, P)) ||
-            (rc = dalloc(o, &s->pkey, P)) || (rc = dalloc(o, &s->pslot, P)) || (rc = dalloc(o, &s->sorted, P)))
+        d.pos = d.vel = d.temp = d.melt = d.yield = d.visc = nullptr;
+        s->pkey = s->pslot = d.sorted = nullptr;
+        s->P_cap = 0;
+        if ((rc = dalloc(o, &d.pos, 3 * P)) || (rc = dalloc(o, &d.vel, 3 * P)) || (rc = dalloc(o, &d.temp, P)) ||
+            (rc = dalloc(o, &d.melt, P)) || (rc = dalloc(o, &d.yield, P)) || (rc = dalloc(o, &d.visc, P)) ||
+            (rc = dalloc(o, &s->pkey, P)) || (rc = dalloc(o, &s->pslot, P)) || (rc = dalloc(o, &d.sorted, P))) {
+            d.P = 0;
+            refresh_dev(s);
             return rc;
+        }
         s->P_cap = P;
     }
-    s->P = P;
+    d.P = (int)P;
+    refresh_dev(s);
     if (P > 0) {
-        HIP_OK(hipMemcpy(s->pos, pos, 3 * P * sizeof(float), hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(s->vel, vel, 3 * P * sizeof(float), hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(s->temp, temp, P * sizeof(float), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(d.pos, pos, 3 * P * sizeof(float), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(d.vel, vel, 3 * P * sizeof(float), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(d.temp, temp, P * sizeof(float), hipMemcpyHostToDevice));
         std::vector<float> init(P, (float)s->prm.yield_solid);
-        HIP_OK(hipMemcpy(s->yield, init.data(), P * sizeof(float), hipMemcpyHostToDevice));  // mpm.py:42
+        HIP_OK(hipMemcpy(d.yield, init.data(), P * sizeof(float), hipMemcpyHostToDevice));  // mpm.py:42
         std::fill(init.begin(), init.end(), (float)s->prm.mu_solid);
-        HIP_OK(hipMemcpy(s->visc, init.data(), P * sizeof(float), hipMemcpyHostToDevice));   // mpm.py:43
-        HIP_OK(hipMemset(s->melt, 0, P * sizeof(float)));                                     // solver.py:126
+        HIP_OK(hipMemcpy(d.visc, init.data(), P * sizeof(float), hipMemcpyHostToDevice));   // mpm.py:43
+        HIP_OK(hipMemset(d.melt, 0, P * sizeof(float)));                                     // solver.py:126
         s->bins_dirty = 1;
     } else {
-        rc = reset_particle_fields(s);
-        if (rc) return rc;
+        RC(reset_particle_fields(s));
         HIP_OK(hipStreamSynchronize(s->stream));
     }
     return HFM_OK;
@@ -642,16 +892,16 @@ int hfm_set_particles(hfm_scene* h, int64_t P, const float* pos, const float* ve
 
 int hfm_update_particles(hfm_scene* h, const float* pos, const float* vel) {
     SCENE_OR_FAIL(h);
-    if (s->P == 0) return HFM_OK;
+    const long P = s->dev.P;
+    if (P == 0) return HFM_OK;
     if (!pos || !vel) {
         set_err("null array");
         return HFM_ERR_ARG;
     }
-    int rc = check_finite(pos, 3 * s->P, "pos");
-    if (!rc) rc = check_finite(vel, 3 * s->P, "vel");
-    if (rc) return rc;
-    HIP_OK(hipMemcpyAsync(s->pos, pos, 3 * s->P * sizeof(float), hipMemcpyHostToDevice, s->stream));
-    HIP_OK(hipMemcpyAsync(s->vel, vel, 3 * s->P * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    RC(check_finite(pos, 3 * P, "pos"));
+    RC(check_finite(vel, 3 * P, "vel"));
+    HIP_OK(hipMemcpyAsync(s->dev.pos, pos, 3 * P * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIP_OK(hipMemcpyAsync(s->dev.vel, vel, 3 * P * sizeof(float), hipMemcpyHostToDevice, s->stream));
     HIP_OK(hipStreamSynchronize(s->stream));
     s->bins_dirty = 1;
     return HFM_OK;
@@ -663,41 +913,22 @@ int hfm_substep(hfm_scene* h, double dt) {
         set_err("bad dt %g", dt);
         return HFM_ERR_ARG;
     }
-    return substep(o, dt);
+    return batch_substep(s->solo, dt);
 }
 
 int hfm_run(hfm_scene* h, double dt, int32_t K, double* ms) {
     SCENE_OR_FAIL(h);
-    if (K < 0 || !(dt > 0)) {
+    if (K < 0 || !(dt > 0) || !(dt < 1e3)) {
         set_err("bad run arguments");
         return HFM_ERR_ARG;
     }
-    hipEvent_t e0, e1;
-    HIP_OK(hipEventCreate(&e0));
-    HIP_OK(hipEventCreate(&e1));
-    HIP_OK(hipEventRecord(e0, s->stream));
-    for (int k = 0; k < K; ++k) {
-        int rc = substep(o, dt);
-        if (rc) {
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            return rc;
-        }
-    }
-    HIP_OK(hipEventRecord(e1, s->stream));
-    HIP_OK(hipEventSynchronize(e1));
-    float el = 0.f;
-    HIP_OK(hipEventElapsedTime(&el, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (ms) *ms = el;
-    return HFM_OK;
+    return batch_run(s->solo, dt, K, ms);
 }
 
 int64_t hfm_field_size(const hfm_scene* h, int32_t field) {
     if (!h) return -1;
     Scene* s = &owner(const_cast<hfm_scene*>(h))->s;
-    if (field >= HFM_FIELD_AX && field <= HFM_FIELD_AZ) return s->g.F;
+    if (field >= HFM_FIELD_AX && field <= HFM_FIELD_AZ) return s->F;
     long size = 0;
     if (!field_ptr(s, field, &size)) return -1;
     return size;
@@ -710,10 +941,10 @@ int hfm_get_field(hfm_scene* h, int32_t field, float* out) {
         return HFM_ERR_ARG;
     }
     if (field >= HFM_FIELD_AX && field <= HFM_FIELD_AZ) {
-        launch_face_fraction(s, field - HFM_FIELD_AX, s->scratch_field);
+        launch_face_fraction(s, s->stream, field - HFM_FIELD_AX, s->scratch_field);
         CHECK_LAUNCH();
         HIP_OK(hipStreamSynchronize(s->stream));
-        HIP_OK(hipMemcpy(out, s->scratch_field, s->g.F * sizeof(float), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(out, s->scratch_field, s->F * sizeof(float), hipMemcpyDeviceToHost));
         return HFM_OK;
     }
     long size = 0;
@@ -731,8 +962,7 @@ int hfm_set_field(hfm_scene* h, int32_t field, const float* in) {
     }
     long size = 0;
     float* p = field_ptr(s, field, &size);
-    int rc = check_finite(in, size, "field");
-    if (rc) return rc;
+    RC(check_finite(in, size, "field"));
     HIP_OK(hipStreamSynchronize(s->stream));
     HIP_OK(hipMemcpy(p, in, size * sizeof(float), hipMemcpyHostToDevice));
     return HFM_OK;
@@ -745,10 +975,10 @@ int hfm_get_cell_velocity(hfm_scene* h, float* out) {
         return HFM_ERR_ARG;
     }
     float* tmp = nullptr;
-    HIP_OK(hipMalloc(&tmp, 3 * s->g.N * sizeof(float)));
-    launch_cell_velocity(s, tmp);
+    HIP_OK(hipMalloc(&tmp, 3 * s->N * sizeof(float)));
+    launch_cell_velocity(s, s->stream, tmp);
     hipError_t e = hipStreamSynchronize(s->stream);
-    if (e == hipSuccess) e = hipMemcpy(out, tmp, 3 * s->g.N * sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(out, tmp, 3 * s->N * sizeof(float), hipMemcpyDeviceToHost);
     (void)hipFree(tmp);
     HIP_OK(e);
     return HFM_OK;
@@ -757,21 +987,21 @@ int hfm_get_cell_velocity(hfm_scene* h, float* out) {
 int hfm_get_particles(hfm_scene* h, float* temp, float* melt, float* yield_stress, float* viscosity) {
     SCENE_OR_FAIL(h);
     HIP_OK(hipStreamSynchronize(s->stream));
-    if (s->P == 0) return HFM_OK;
-    size_t b = s->P * sizeof(float);
-    if (temp) HIP_OK(hipMemcpy(temp, s->temp, b, hipMemcpyDeviceToHost));
-    if (melt) HIP_OK(hipMemcpy(melt, s->melt, b, hipMemcpyDeviceToHost));
-    if (yield_stress) HIP_OK(hipMemcpy(yield_stress, s->yield, b, hipMemcpyDeviceToHost));
-    if (viscosity) HIP_OK(hipMemcpy(viscosity, s->visc, b, hipMemcpyDeviceToHost));
+    if (s->dev.P == 0) return HFM_OK;
+    const size_t bytes = (size_t)s->dev.P * sizeof(float);
+    if (temp) HIP_OK(hipMemcpy(temp, s->dev.temp, bytes, hipMemcpyDeviceToHost));
+    if (melt) HIP_OK(hipMemcpy(melt, s->dev.melt, bytes, hipMemcpyDeviceToHost));
+    if (yield_stress) HIP_OK(hipMemcpy(yield_stress, s->dev.yield, bytes, hipMemcpyDeviceToHost));
+    if (viscosity) HIP_OK(hipMemcpy(viscosity, s->dev.visc, bytes, hipMemcpyDeviceToHost));
     return HFM_OK;
 }
 
 int hfm_diagnostics(hfm_scene* h, double* divergence_max, double* speed_max) {
     SCENE_OR_FAIL(h);
     HIP_OK(hipMemsetAsync(s->umax, 0, 4 * sizeof(unsigned), s->stream));
-    launch_divergence(s, (const float* const*)s->u, 1.0f, s->scratch_field, 0);
-    launch_abs_max(s, s->scratch_field, s->g.N, &s->umax[0]);
-    for (int a = 0; a < 3; ++a) launch_abs_max(s, s->u[a], s->g.F, &s->umax[1]);
+    launch_divergence_scale1(s, s->stream, s->scratch_field);
+    launch_abs_max(s->stream, s->scratch_field, s->N, &s->umax[0]);
+    for (int a = 0; a < 3; ++a) launch_abs_max(s->stream, s->dev.u[a], s->F, &s->umax[1]);
     CHECK_LAUNCH();
     unsigned m[4];
     HIP_OK(hipStreamSynchronize(s->stream));
@@ -796,40 +1026,124 @@ int hfm_get_stats(hfm_scene* h, hfm_stats* out) {
 
 int hfm_kernel_time(hfm_scene* h, int32_t cls, double* total_ms, int64_t* launches) {
     SCENE_OR_FAIL(h);
-    if (cls < 0 || cls >= HFM_KT_COUNT) {
-        set_err("bad kernel class %d", cls);
-        return HFM_ERR_ARG;
-    }
-    int rc = flush_timing(s);
-    if (rc) return rc;
-    if (total_ms) *total_ms = s->kt_ms[cls];
-    if (launches) *launches = s->kt_n[cls];
-    return HFM_OK;
+    return kernel_time(s->solo, cls, total_ms, launches);
 }
 
 int hfm_kernel_time_reset(hfm_scene* h) {
     SCENE_OR_FAIL(h);
-    int rc = flush_timing(s);
-    if (rc) return rc;
-    for (int c = 0; c < HFM_KT_COUNT; ++c) {
-        s->kt_ms[c] = 0.0;
-        s->kt_n[c] = 0;
-    }
-    return HFM_OK;
+    return kernel_time_reset(s->solo);
 }
 
 int hfm_set_kernel_timing(hfm_scene* h, int32_t on) {
     SCENE_OR_FAIL(h);
-    if (on) return enable_timing(s);
-    int rc = flush_timing(s);
-    s->timing = 0;
-    return rc;
+    return set_timing(s->solo, on);
 }
 
 int hfm_synchronize(hfm_scene* h) {
     SCENE_OR_FAIL(h);
     HIP_OK(hipStreamSynchronize(s->stream));
     return HFM_OK;
+}
+
+// ---- batches -------------------------------------------------------------------------------
+
+int hfm_batch_create(hfm_scene* const* scenes, int32_t count, hfm_batch** out) {
+    if (!scenes || !out || count < 1 || count > kMaxBatch) {
+        set_err("bad batch arguments (count=%d, at most %d)", count, kMaxBatch);
+        return HFM_ERR_ARG;
+    }
+    *out = nullptr;
+    std::vector<Scene*> list;
+    for (int i = 0; i < count; ++i) {
+        if (!scenes[i]) {
+            set_err("null scene at %d", i);
+            return HFM_ERR_ARG;
+        }
+        Scene* s = &owner(scenes[i])->s;
+        if (std::find(list.begin(), list.end(), s) != list.end()) {
+            set_err("scene %d appears twice in the batch", i);
+            return HFM_ERR_ARG;
+        }
+        if (!list.empty() && s->device != list[0]->device) {
+            set_err("scenes of a batch must be on one device");
+            return HFM_ERR_ARG;
+        }
+        list.push_back(s);
+    }
+    HIP_OK(hipSetDevice(list[0]->device));
+    hfm_batch* hb = new hfm_batch();
+    Batch* b = &hb->b;
+    b->user = true;
+    b->scenes = list;
+    hipStream_t st = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        set_err("hipStreamCreate: %s", hipGetErrorString(e));
+        delete hb;
+        return HFM_ERR_HIP;
+    }
+    int rc = batch_init(b, list[0]->device, st, true);
+    if (rc) {
+        batch_release(b);
+        delete hb;
+        return rc;
+    }
+    for (Scene* s : list) s->members += 1;
+    *out = hb;
+    return HFM_OK;
+}
+
+int hfm_batch_destroy(hfm_batch* h) {
+    if (!h) return HFM_OK;
+    Batch* b = &h->b;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (Scene* s : b->scenes) {
+        if (s->current == b) s->current = s->solo;
+        s->members -= 1;
+    }
+    batch_release(b);
+    delete h;
+    return HFM_OK;
+}
+
+int hfm_batch_substep(hfm_batch* h, double dt) {
+    BATCH_OR_FAIL(h);
+    if (!(dt > 0) || !(dt < 1e3)) {
+        set_err("bad dt %g", dt);
+        return HFM_ERR_ARG;
+    }
+    return batch_substep(b, dt);
+}
+
+int hfm_batch_run(hfm_batch* h, double dt, int32_t K, double* ms) {
+    BATCH_OR_FAIL(h);
+    if (K < 0 || !(dt > 0) || !(dt < 1e3)) {
+        set_err("bad run arguments");
+        return HFM_ERR_ARG;
+    }
+    return batch_run(b, dt, K, ms);
+}
+
+int hfm_batch_synchronize(hfm_batch* h) {
+    BATCH_OR_FAIL(h);
+    HIP_OK(hipStreamSynchronize(b->stream));
+    return HFM_OK;
+}
+
+int hfm_batch_set_kernel_timing(hfm_batch* h, int32_t on) {
+    BATCH_OR_FAIL(h);
+    return set_timing(b, on);
+}
+
+int hfm_batch_kernel_time(hfm_batch* h, int32_t cls, double* total_ms, int64_t* launches) {
+    BATCH_OR_FAIL(h);
+    return kernel_time(b, cls, total_ms, launches);
+}
+
+int hfm_batch_kernel_time_reset(hfm_batch* h) {
+    BATCH_OR_FAIL(h);
+    return kernel_time_reset(b);
 }
 
 }  // extern "C"

@@ -87,7 +87,7 @@ class _Scene:
 
     def close(self) -> None:
         if getattr(self, "handle", None):
-            self.lib.hfm_scene_destroy(self.handle)
+            _abi.check(self.lib.hfm_scene_destroy(self.handle), "hfm_scene_destroy")
             self.handle = None
 
     def __del__(self):

@@ -177,6 +177,28 @@ int hfm_set_kernel_timing(hfm_scene* s, int32_t on);
 
 int hfm_synchronize(hfm_scene* s);
 
+/* ---- Batches: many scenes stepped together ------------------------------------------------
+ * No counterpart in the reference, which steps one HeatflowMPM at a time (solver.py:128-151):
+ * a batch runs the same substep for every scene it holds, each stage one kernel launch over all
+ * of them, so small grids fill the GPU together.  Scenes may differ in size, particles, heater
+ * and solver settings; each keeps its own PCG solves (own tolerances, iteration counts, stats)
+ * and its results are the same as when it is stepped alone.  All scenes must be on one device.
+ * A scene may be in several batches and still be stepped alone; work on it is ordered across
+ * them.  Destroy a batch before the scenes it holds (hfm_scene_destroy returns HFM_ERR_STATE
+ * while a batch holds the scene).  One host thread per batch. */
+typedef struct hfm_batch hfm_batch;
+int hfm_batch_create(hfm_scene* const* scenes, int32_t count, hfm_batch** out);
+int hfm_batch_destroy(hfm_batch* b);
+/* One substep of every scene (same dt). */
+int hfm_batch_substep(hfm_batch* b, double dt);
+/* K substeps back to back; *ms (may be NULL) receives the HIP-event time of the batch. */
+int hfm_batch_run(hfm_batch* b, double dt, int32_t K, double* ms);
+int hfm_batch_synchronize(hfm_batch* b);
+/* Kernel timing of the batch's launches, as hfm_kernel_time / hfm_set_kernel_timing. */
+int hfm_batch_set_kernel_timing(hfm_batch* b, int32_t on);
+int hfm_batch_kernel_time(hfm_batch* b, int32_t kernel_class, double* total_ms, int64_t* launches);
+int hfm_batch_kernel_time_reset(hfm_batch* b);
+
 /* Number of HIP devices visible (0 without a GPU). */
 int hfm_device_count(void);
 
