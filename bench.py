@@ -36,10 +36,12 @@ METRIC = "cell-substeps/s + HBM TB/s at 96^3; served p50/p99 vs plain at equal t
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes per cell of one PCG iteration, by kernel (fp32 words x 4):
-#   dir: reads z, p_old, alpha_s; writes p, Ap                      -> 5 words
-#   upd: reads x, p, r, Ap, D; writes x, r, z                         -> 8 words
-KERNEL_WORDS = {_abi.KT_PCG_DIR: 5, _abi.KT_PCG_UPD: 8}
-KERNEL_NAMES = {_abi.KT_PCG_DIR: "k_pcg_dir_tile3", _abi.KT_PCG_UPD: "k_pcg_upd<1>"}
+#   dir:   reads z, p_old, alpha_s; writes p, Ap                     -> 5 words
+#   upd:   reads x, p, r, Ap, D; writes x, r, z                        -> 8 words
+#   fused: reads x, r, p_old, D, alpha_s; writes x, r, p              -> 8 words (whole iteration)
+KERNEL_WORDS = {_abi.KT_PCG_DIR: 5, _abi.KT_PCG_UPD: 8, _abi.KT_PCG_FUSED: 8}
+KERNEL_NAMES = {_abi.KT_PCG_DIR: "k_pcg_dir_tile3", _abi.KT_PCG_UPD: "k_pcg_upd<1>",
+                _abi.KT_PCG_FUSED: "k_pcg_fused"}
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "pcg_pressure_n96_traffic.json")
 
 
@@ -59,13 +61,15 @@ def measured_traffic(kernel: str, n: int):
     return None
 
 
-def substep_bytes(N: int, P: int, I_p: float, I_h: float) -> float:
+def substep_bytes(N: int, P: int, I_p: float, I_h: float, fused: bool) -> float:
     """Algorithmic HBM bytes of one substep of this implementation (fp32 words x 4):
-    P2G gather 20, heat prep 5.25, T advection + kappa + heat rhs/diag 10.25, heat PCG init 6.25,
-    heat PCG 13.25 per iteration, air face pass 14, divergence + diagonal 9, rhs mean 1, pressure
-    PCG init 7, pressure PCG 13 per iteration, projection 7 words per cell; 8 words per particle
-    for G2P."""
-    words = N * (20 + 5.25 + 10.25 + 6.25 + 13.25 * I_h + 14 + 9 + 1 + 7 + 13 * I_p + 7) + 8 * P
+    P2G heat-sum gather 2 (static particles), heat prep 6.25, T advection + kappa + heat rhs/diag
+    10.25, heat PCG init 6.25, heat PCG 13.25 per iteration, air face pass 14, divergence +
+    diagonal 9, rhs mean 1, pressure PCG init 7 (+5 for the fused form's first scalars), pressure
+    PCG 13 (two-pass) or 8 (fused) per iteration, projection 7 words per cell; 8 words per
+    particle for G2P."""
+    p_init, p_iter = (12, 8) if fused else (7, 13)
+    words = N * (2 + 6.25 + 10.25 + 6.25 + 13.25 * I_h + 14 + 9 + 1 + p_init + p_iter * I_p + 7) + 8 * P
     return 4.0 * words
 
 
@@ -98,6 +102,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=96)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cg", choices=HeatflowMPM.CG_FORMS, default="warp",
+                    help="pressure PCG form: warp's two-pass iteration, or one fused launch per iteration")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the substep runs on one device (scene-level sharding across GPUs is "
@@ -105,7 +111,7 @@ def main() -> int:
         return 2
 
     kw, dt, _ = coupled_scene(a.n)
-    sim = HeatflowMPM(**kw, device="cuda:0")
+    sim = HeatflowMPM(**kw, device="cuda:0", cg=a.cg)
     n, N, P = sim.grid.n, sim.grid.n ** 3, sim.n
     if a.warmup:
         sim.run(dt, a.warmup)
@@ -124,7 +130,7 @@ def main() -> int:
     I_p = (st1["p_iters_total"] - st0["p_iters_total"]) / a.steps
     I_h = (st1["h_iters_total"] - st0["h_iters_total"]) / a.steps
     value = N * a.steps / wall
-    hbm_tbs = substep_bytes(N, P, I_p, I_h) * a.steps / wall / 1e12
+    hbm_tbs = substep_bytes(N, P, I_p, I_h, a.cg == "fused") * a.steps / wall / 1e12
 
     # second timed region with HIP events around every pressure-PCG launch
     sim._scene.set_kernel_timing(True)
@@ -154,6 +160,7 @@ def main() -> int:
                dtype="f32", data="synthetic",
                config=dict(workload=f"coupled scene n={n} (h=5 mm), static ellipsoid P={P}, heater 400 W cap 120 C fan "
                                     f"2 m/s, PCG tol 1e-6, dt 2 ms", n=n, particles=P, dt=dt, cg_tol=1e-6,
+                           pressure_cg=a.cg,
                            pressure_iters_per_substep=round(I_p, 2), heat_iters_per_substep=round(I_h, 2),
                            parallelism="1 GPU"),
                hbm_tb_s=round(hbm_tbs, 3), roofline=roofline, cpu_baseline=cpu)

@@ -110,6 +110,7 @@ __global__ __launch_bounds__(kBlock) void k_advect_T(const SceneDev* __restrict_
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_g2p(const SceneDev* __restrict__ tab, const MapEntry* __restrict__ map) {
     HFM_BATCH_ENTRY(tab, map);
+    if (lb == 0 && threadIdx.x == 0) a.ctl[0].geom_dirty = 0;  // this substep's gather rebuilt the geometry
     const long p = (long)lb * kBlock + threadIdx.x;
     if (p >= a.P) return;
     const float x = a.pos[3 * p], y = a.pos[3 * p + 1], z = a.pos[3 * p + 2];

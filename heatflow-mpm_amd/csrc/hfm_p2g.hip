@@ -7,6 +7,12 @@
 // can reach it, in bin order then particle order.  The result is the same sum of the same terms
 // in a fixed order: deterministic run to run.
 //
+// Particle geometry (solid fraction, flux, relaxation weights and momenta) depends on positions
+// and velocities only, so after a re-binning the gather rebuilds every lattice once; while the
+// particles stay put it recomputes only the heat sum, whose particle temperatures change every
+// substep.  Positions divided by h are stored in bin order at binning time (the same fp32
+// divide base_index performs), so the gather reads them contiguously and divides nothing.
+//
 // A node at index a on an axis where the lattice offset is 1/2 is reached by particles in bins
 // a-1, a, a+1; where the offset is 0 (the face axis) by bins a-1, a.  Particles outside the box
 // are binned into the nearest edge bin, which is also where their clamped weights land.
@@ -118,20 +124,37 @@ __global__ __launch_bounds__(kBlock) void k_bin_sort(long N, const int* start, i
     }
 }
 
+// pos / h in bin order; marks the particle geometry as changed for the next gather
+__global__ __launch_bounds__(kBlock) void k_bin_pack(long P, float h, const float* pos, const int* sorted, float* qpos,
+                                                     int* geom_dirty) {
+    long s = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= P) return;
+    const long p = sorted[s];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) qpos[3 * s + d] = pos[3 * p + d] / h;
+    if (s == 0) *geom_dirty = 1;
+}
+
 // ---------------------------------------------------------------------------------------------
 // gather: sub-range 0 = cell lattice (alpha_s, alpha_p, tsum), 1..3 = face lattices (q, K, m)
 // ---------------------------------------------------------------------------------------------
-// Weight with which particle (px,py,pz) reaches node (ni,nj,nk) of a lattice with offsets
-// (ox,oy,oz) and dims (dx,dy,dz): the sum over the corners whose clamped index is the node,
-// each added separately in the corner order of fields.py:41-48.
-__device__ __forceinline__ void node_terms(float px, float py, float pz, float ox, float oy, float oz, float h,
-                                           int dx, int dy, int dz, int ni, int nj, int nk, float* w_out,
-                                           int& count) {
+// fields.py:5-10 with q = pos / h already formed: g = q - off; i = floor(g); f = g - i
+__device__ __forceinline__ void base_index_q(float q, float off, int& i, float& f) {
+    const float g = q - off;
+    i = (int)floorf(g);
+    f = g - (float)i;
+}
+
+// Weight with which the particle at (qx,qy,qz) = pos / h reaches node (ni,nj,nk) of a lattice
+// with offsets (ox,oy,oz) and dims (dx,dy,dz): the sum over the corners whose clamped index is
+// the node, each added separately in the corner order of fields.py:41-48.
+__device__ __forceinline__ void node_terms(float qx, float qy, float qz, float ox, float oy, float oz, int dx, int dy,
+                                           int dz, int ni, int nj, int nk, float* w_out, int& count) {
     int bi, bj, bk;
     float fx, fy, fz;
-    base_index1(px, ox, h, bi, fx);
-    base_index1(py, oy, h, bj, fy);
-    base_index1(pz, oz, h, bk, fz);
+    base_index_q(qx, ox, bi, fx);
+    base_index_q(qy, oy, bj, fy);
+    base_index_q(qz, oz, bk, fz);
     count = 0;
 #pragma unroll
     for (int cx = 0; cx < 2; ++cx) {
@@ -156,6 +179,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g_gather(const SceneDev* __restric
                                                        const MapEntry* __restrict__ map) {
     HFM_BATCH_ENTRY(tab, map);
     const int lat = sub;  // 0 cell, 1 + axis for faces
+    const bool full = a.ctl[0].geom_dirty != 0;
+    if (!full && lat != 0) return;  // face lattices unchanged since the last gather
     const int n = a.n;
     const long count = lat == 0 ? a.N : a.F;
     const long node = (long)xcd_tile(lb, nblocks(count)) * kBlock + threadIdx.x;
@@ -175,24 +200,22 @@ __global__ __launch_bounds__(kBlock) void k_p2g_gather(const SceneDev* __restric
     lo_y = lo_y < 0 ? 0 : lo_y; hi_y = hi_y > n - 1 ? n - 1 : hi_y;
     lo_z = lo_z < 0 ? 0 : lo_z; hi_z = hi_z > n - 1 ? n - 1 : hi_z;
     float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f;  // cell: alpha, tsum ; face: q, K, m
-    const float h = a.h;
     for (int bx = lo_x; bx <= hi_x; ++bx)
         for (int by = lo_y; by <= hi_y; ++by) {
             long row = ((long)bx * n + by) * n;
             int b = a.bin_start[row + lo_z], e = a.bin_start[row + hi_z + 1];
             for (int s = b; s < e; ++s) {
-                int p = a.sorted[s];
-                float px = a.pos[3 * p], py = a.pos[3 * p + 1], pz = a.pos[3 * p + 2];
+                const float* qp = a.qpos + 3 * (long)s;
                 float w[8];
                 int cnt;
-                node_terms(px, py, pz, ox, oy, oz, h, dx, dy, dz, ni, nj, nk, w, cnt);
+                node_terms(qp[0], qp[1], qp[2], ox, oy, oz, dx, dy, dz, ni, nj, nk, w, cnt);
                 if (cnt == 0) continue;
+                const int p = a.sorted[s];
                 if (lat == 0) {
-                    float vt = a.frac * a.temp[p];
-                    for (int q = 0; q < cnt; ++q) {
-                        acc0 += w[q] * a.frac;
-                        acc1 += w[q] * vt;
-                    }
+                    const float vt = a.frac * a.temp[p];
+                    if (full)
+                        for (int q = 0; q < cnt; ++q) acc0 += w[q] * a.frac;
+                    for (int q = 0; q < cnt; ++q) acc1 += w[q] * vt;
                 } else {
                     float vq = a.frac * a.vel[3 * p + axis];
                     float vm = a.k0 * a.vel[3 * p + axis];
@@ -205,9 +228,11 @@ __global__ __launch_bounds__(kBlock) void k_p2g_gather(const SceneDev* __restric
             }
         }
     if (lat == 0) {
-        a.alpha_p[node] = acc0;
         a.tsum[node] = acc1;
-        a.alpha_s[node] = acc0 + a.st_alpha[node];
+        if (full) {
+            a.alpha_p[node] = acc0;
+            a.alpha_s[node] = acc0 + a.st_alpha[node];
+        }
     } else {
         int idx = axis == 0 ? ni : (axis == 1 ? nj : nk);
         a.q[axis][node] = (idx == 0 || idx == n) ? 0.0f : acc0;
@@ -234,6 +259,8 @@ void launch_bin_particles(Scene* s, hipStream_t st) {
         hipLaunchKernelGGL(k_bin_fill, dim3(nblocks(P)), dim3(kBlock), 0, st, P, s->pkey, s->pslot,
                            s->dev.bin_start, s->dev.sorted);
         hipLaunchKernelGGL(k_bin_sort, dim3(nblocks(N)), dim3(kBlock), 0, st, N, s->dev.bin_start, s->dev.sorted);
+        hipLaunchKernelGGL(k_bin_pack, dim3(nblocks(P)), dim3(kBlock), 0, st, P, s->dev.h, s->dev.pos,
+                           s->dev.sorted, s->dev.qpos, &s->dev.ctl[0].geom_dirty);
     }
     s->bins_dirty = 0;
 }

@@ -212,7 +212,7 @@ __device__ __forceinline__ bool solve_stopped(const CgCtl* c) {
 template <int NV, int BS = kBlock>
 __device__ __forceinline__ void sum_partials(const double* __restrict__ P, int nb, double (*lds)[BS / 64],
                                              double (&out)[NV]) {
-    __shared__ double bcast[3];
+    __shared__ double bcast[NV];
     double acc[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) acc[q] = 0.0;
@@ -467,6 +467,7 @@ template <int OP>
 __global__ __launch_bounds__(kBlock) void k_pcg_upd(const SceneDev* __restrict__ tab,
                                                     const MapEntry* __restrict__ map, int kit) {
     HFM_BATCH_ENTRY(tab, map);
+    if (OP == 1 && a.cg_fused) return;  // the fused form has no pass B
     __shared__ double lds[2][kWaves];
     const PcgView v = make_view<OP>(a, lb, kit, 0.0f);
     CgCtl* c = v.ctl;
@@ -520,6 +521,260 @@ __global__ __launch_bounds__(kBlock) void k_pcg_upd(const SceneDev* __restrict__
         s[1] += (double)(rc * zc);
     }
     store_partials<2>(s, v.part_upd, v.nb_upd, lb, lds);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Fused pressure PCG: one launch per iteration (HFM_FLAG_FUSED_CG).
+//
+// Launch k starts from x_k, r_k, p_{k-1} in memory and the partial sums of launch k-1:
+// r_k.r_k, r_k.z_k, z_k.A z_k and z_k.A p_{k-1} (z = D r).  Its prologue forms
+//   beta = r_k.z_k / r_{k-1}.z_{k-1}   (0 at k = 0, or once converged, as warp)
+//   p_k.A p_k = z.Az + 2 beta z.Ap_{k-1} + beta^2 p_{k-1}.A p_{k-1}   (exact algebra, A symmetric)
+//   alpha = r_k.z_k / p_k.A p_k        (0 once converged, as warp)
+// and then, on a 16 x 16 x D block with a two-cell halo staged in LDS:
+//   p_k = z_k + beta p_{k-1}           on block + 2 halo
+//   A p_k, r_{k+1} = r_k - alpha A p_k, z_{k+1} = D r_{k+1}   on block + 1 halo
+//   x_{k+1} = x_k + alpha p_k, A z_{k+1} and the four partial sums   on the block
+// Halo cells are recomputed by every block that needs them with the same operations on the same
+// inputs, so all copies agree bit for bit.  Memory per iteration: reads x, r, p, D, alpha_s,
+// writes x, r, p (8 words per cell) in one launch, against 12 words in two launches for the
+// two-pass form.  r and p are double-buffered by iteration parity (neighbours read the old ones
+// while owners write the new), as are the partial sums.
+// ---------------------------------------------------------------------------------------------
+constexpr int kF2 = kT3 + 4;
+
+template <int D>
+__global__ __launch_bounds__(kBlock3) void k_pcg_fused(const SceneDev* __restrict__ tab,
+                                                       const MapEntry* __restrict__ map, int kit,
+                                                       unsigned long long* pace, unsigned tag) {
+    post_pace(pace, tag, kit);
+    HFM_BATCH_ENTRY(tab, map);
+    __shared__ float P2[D + 4][kF2][kF2];
+    __shared__ float S2[D + 4][kF2][kF2];
+    __shared__ float Z1[D + 2][kF3][kF3];
+    __shared__ double lds[kFusedRows][kBlock3 / 64];
+    CgCtl* c = a.ctl + 1;
+    {
+        __shared__ int stopped;
+        if (threadIdx.x == 0) stopped = c->done;
+        __syncthreads();
+        if (stopped) return;
+    }
+    const int n = a.n;
+    const long N = a.N;
+    const int nb = a.nb_dir;
+    const int L = xcd_tile(lb, nb);
+    const int tjk = a.tiles_j * a.tiles_k;
+    const int ti = L / tjk, rem = L % tjk;
+    const int i0 = ti * D, j0 = (rem / a.tiles_k) * kT3, k0 = (rem % a.tiles_k) * kT3;
+    const bool first = kit == 0;
+    const float* __restrict__ rk = a.cg_r + (long)(kit & 1) * N;
+    float* __restrict__ rn = a.cg_r + (long)((kit + 1) & 1) * N;
+    const float* __restrict__ pold = a.cg_d + (long)((kit + 1) & 1) * N;
+    float* __restrict__ pk = a.cg_d + (long)(kit & 1) * N;
+    const float* __restrict__ Dg = a.pdiag;
+    const float* __restrict__ as = a.alpha_s;
+    float* __restrict__ x = a.p;
+    const double* part_in = a.partials + (long)(kit & 1) * kFusedRows * kMaxDirBlocks;
+    double* part_out = a.partials + (long)((kit + 1) & 1) * kFusedRows * kMaxDirBlocks;
+    // ---- operand loads, issued before the prologue (they do not depend on its scalars)
+    constexpr int kFr2 = (D + 4) * kF2 * kF2;
+    constexpr int kP2 = (kFr2 + kBlock3 - 1) / kBlock3;
+    float f_r[kP2], f_p[kP2], f_d[kP2], f_a[kP2];
+#pragma unroll
+    for (int u = 0; u < kP2; ++u) {
+        const int f = threadIdx.x + u * kBlock3;
+        f_r[u] = 0.0f;
+        f_p[u] = 0.0f;
+        f_d[u] = 0.0f;
+        f_a[u] = 0.0f;
+        if (f < kFr2) {
+            const int fi = f / (kF2 * kF2), fr = f % (kF2 * kF2);
+            const int gi = i0 + fi - 2, gj = j0 + fr / kF2 - 2, gk = k0 + fr % kF2 - 2;
+            if (gi >= 0 && gi < n && gj >= 0 && gj < n && gk >= 0 && gk < n) {
+                const long t = ((long)gi * n + gj) * n + gk;
+                f_r[u] = rk[t];
+                if (!first) f_p[u] = pold[t];
+                f_d[u] = Dg[t];
+                f_a[u] = as[t];
+            }
+        }
+    }
+    constexpr int kFr1 = (D + 2) * kF3 * kF3;
+    constexpr int kP1 = (kFr1 + kBlock3 - 1) / kBlock3;
+    float g_r[kP1], g_d[kP1], g_x[kP1];
+#pragma unroll
+    for (int u = 0; u < kP1; ++u) {
+        const int f = threadIdx.x + u * kBlock3;
+        g_r[u] = 0.0f;
+        g_d[u] = 0.0f;
+        g_x[u] = 0.0f;
+        if (f < kFr1) {
+            const int fi = f / (kF3 * kF3), fr = f % (kF3 * kF3);
+            const int fj = fr / kF3, fk = fr % kF3;
+            const int gi = i0 + fi - 1, gj = j0 + fj - 1, gk = k0 + fk - 1;
+            if (gi >= 0 && gi < n && gj >= 0 && gj < n && gk >= 0 && gk < n) {
+                const long t = ((long)gi * n + gj) * n + gk;
+                g_r[u] = rk[t];
+                g_d[u] = Dg[t];
+                if (fi >= 1 && fi <= D && fj >= 1 && fj <= kT3 && fk >= 1 && fk <= kT3) g_x[u] = x[t];
+            }
+        }
+    }
+    // ---- prologue: scalars of iteration k from launch k-1's partial sums
+    float rr, rz, atol_sq;
+    double zaz, zap, atol;
+    if (first) {
+        double tot[5];
+        sum_partials<5, kBlock3>(part_in, a.nb_upd, lds, tot);  // rows written by the init launch
+        rr = (float)tot[0];
+        rz = (float)tot[1];
+        zaz = tot[2];
+        zap = tot[3];
+        const float bb = (float)tot[4];
+        atol = c->tol * sqrt((double)bb);  // warp: atol = max(tol * ||b||, tol)
+        if (atol < c->tol) atol = c->tol;
+        atol_sq = (float)(atol * atol);
+    } else {
+        double tot[4];
+        sum_partials<4, kBlock3>(part_in, nb, lds, tot);
+        rr = (float)tot[0];
+        rz = (float)tot[1];
+        zaz = tot[2];
+        zap = tot[3];
+        atol = c->atol;
+        atol_sq = c->atol_sq;
+    }
+    const bool conv = rr <= atol_sq;
+    const bool stop = conv || kit >= c->maxeff;
+    const float beta = first ? 0.0f : (rr > atol_sq ? rz / c->slot_rz[(kit + 1) & 1] : 0.0f);
+    const double pap = first ? zaz
+                             : zaz + 2.0 * (double)beta * zap + (double)beta * (double)beta * c->slot_pap[(kit + 1) & 1];
+    const float alpha = rr > atol_sq ? rz / (float)pap : 0.0f;
+    if (lb == 0 && threadIdx.x == 0) {
+        c->slot_rr[kit & 1] = rr;
+        c->slot_rz[kit & 1] = rz;
+        c->slot_pap[kit & 1] = pap;
+        if (first) {
+            c->atol = atol;
+            c->atol_sq = atol_sq;
+        }
+        if (stop) {
+            c->done = 1;
+            c->converged = conv;
+            c->it = kit;
+            mirror_final(a.mirror + 1, kit, rr, atol, conv);
+        }
+    }
+    if (stop) return;
+    // ---- stage 1: p_k and alpha_s on block + 2 halo
+#pragma unroll
+    for (int u = 0; u < kP2; ++u) {
+        const int f = threadIdx.x + u * kBlock3;
+        if (f < kFr2) {
+            const int fi = f / (kF2 * kF2), fr = f % (kF2 * kF2);
+            const float z = f_d[u] * f_r[u];
+            P2[fi][fr / kF2][fr % kF2] = first ? z : z + beta * f_p[u];
+            S2[fi][fr / kF2][fr % kF2] = f_a[u];
+        }
+    }
+    __syncthreads();
+    const float amin = a.amin;
+    const float hh = a.hh;
+    // ---- stage 2: A p_k, r_{k+1}, z_{k+1} on block + 1 halo
+    float k_ap[kP1], k_rn[kP1], k_zn[kP1];
+#pragma unroll
+    for (int u = 0; u < kP1; ++u) {
+        const int f = threadIdx.x + u * kBlock3;
+        k_ap[u] = 0.0f;
+        k_rn[u] = 0.0f;
+        k_zn[u] = 0.0f;
+        if (f < kFr1) {
+            const int fi = f / (kF3 * kF3), fr = f % (kF3 * kF3);
+            const int fj = fr / kF3, fk = fr % kF3;
+            const int i = i0 + fi - 1, j = j0 + fj - 1, k = k0 + fk - 1;
+            float zn = 0.0f;
+            if (i >= 0 && i < n && j >= 0 && j < n && k >= 0 && k < n) {
+                const int ci = fi + 1, cj = fj + 1, ck = fk + 1;
+                const float pc = P2[ci][cj][ck], ac = S2[ci][cj][ck];
+                float acc = 0.0f;
+                if (i > 0) acc += face_fraction(S2[ci - 1][cj][ck], ac, amin) * (pc - P2[ci - 1][cj][ck]);
+                if (i < n - 1) acc += face_fraction(ac, S2[ci + 1][cj][ck], amin) * (pc - P2[ci + 1][cj][ck]);
+                if (j > 0) acc += face_fraction(S2[ci][cj - 1][ck], ac, amin) * (pc - P2[ci][cj - 1][ck]);
+                if (j < n - 1) acc += face_fraction(ac, S2[ci][cj + 1][ck], amin) * (pc - P2[ci][cj + 1][ck]);
+                if (k > 0) acc += face_fraction(S2[ci][cj][ck - 1], ac, amin) * (pc - P2[ci][cj][ck - 1]);
+                if (k < n - 1) acc += face_fraction(ac, S2[ci][cj][ck + 1], amin) * (pc - P2[ci][cj][ck + 1]);
+                const float ap = 1.0f * acc / hh;
+                const float rnew = g_r[u] - alpha * ap;
+                zn = g_d[u] * rnew;
+                k_ap[u] = ap;
+                k_rn[u] = rnew;
+                k_zn[u] = zn;
+            }
+            Z1[fi][fj][fk] = zn;
+        }
+    }
+    __syncthreads();
+    // ---- stage 3: x_{k+1}, A z_{k+1} and the partial sums on the block
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int u = 0; u < kP1; ++u) {
+        const int f = threadIdx.x + u * kBlock3;
+        if (f < kFr1) {
+            const int fi = f / (kF3 * kF3), fr = f % (kF3 * kF3);
+            const int fj = fr / kF3, fk = fr % kF3;
+            const int i = i0 + fi - 1, j = j0 + fj - 1, k = k0 + fk - 1;
+            const bool inner = fi >= 1 && fi <= D && fj >= 1 && fj <= kT3 && fk >= 1 && fk <= kT3;
+            if (inner && i < n && j < n && k < n) {
+                const int ci = fi + 1, cj = fj + 1, ck = fk + 1;
+                const float zc = Z1[fi][fj][fk], ac = S2[ci][cj][ck];
+                float acc = 0.0f;
+                if (i > 0) acc += face_fraction(S2[ci - 1][cj][ck], ac, amin) * (zc - Z1[fi - 1][fj][fk]);
+                if (i < n - 1) acc += face_fraction(ac, S2[ci + 1][cj][ck], amin) * (zc - Z1[fi + 1][fj][fk]);
+                if (j > 0) acc += face_fraction(S2[ci][cj - 1][ck], ac, amin) * (zc - Z1[fi][fj - 1][fk]);
+                if (j < n - 1) acc += face_fraction(ac, S2[ci][cj + 1][ck], amin) * (zc - Z1[fi][fj + 1][fk]);
+                if (k > 0) acc += face_fraction(S2[ci][cj][ck - 1], ac, amin) * (zc - Z1[fi][fj][fk - 1]);
+                if (k < n - 1) acc += face_fraction(ac, S2[ci][cj][ck + 1], amin) * (zc - Z1[fi][fj][fk + 1]);
+                const float az = 1.0f * acc / hh;
+                const float pc = P2[ci][cj][ck];
+                const long t = ((long)i * n + j) * n + k;
+                x[t] = g_x[u] + alpha * pc;
+                pk[t] = pc;
+                rn[t] = k_rn[u];
+                v[0] += (double)(k_rn[u] * k_rn[u]);
+                v[1] += (double)(k_rn[u] * k_zn[u]);
+                v[2] += (double)(k_zn[u] * az);
+                v[3] += (double)(k_zn[u] * k_ap[u]);
+            }
+        }
+    }
+    store_partials<4, kBlock3>(v, part_out, nb, lb, lds);
+}
+
+// Scalars of the first fused iteration from r_0 (written by k_pcg_init<1>): r.r, r.z, z.Az,
+// z.Ap_{-1} = 0 and b.b, into the partial-sum buffer launch 0 reads.
+__global__ __launch_bounds__(kBlock) void k_pcg_init_fused(const SceneDev* __restrict__ tab,
+                                                           const MapEntry* __restrict__ map) {
+    HFM_BATCH_ENTRY(tab, map);
+    if (!a.cg_fused) return;
+    __shared__ double lds[kFusedRows][kWaves];
+    const PcgView v = make_view<1>(a, lb, 0, 0.0f);
+    const float* r = a.cg_r;
+    const float* Dg = a.pdiag;
+    double s[kFusedRows] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    FOR_CELLS(v, v.nb_upd) {
+        int i, j, k;
+        cell_ijk(t, v.n, i, j, k);
+        auto zf = [r, Dg](long q) { return Dg[q] * r[q]; };
+        const float acc = Operator<1>::apply(v, t, i, j, k, zf);
+        const float az = Operator<1>::combine(v, acc, 0.0f, false, 1.0f);
+        const float rc = r[t], zc = zf(t), bc = v.b[t];
+        s[0] += (double)(rc * rc);
+        s[1] += (double)(rc * zc);
+        s[2] += (double)(zc * az);
+        s[4] += (double)(bc * bc);
+    }
+    store_partials<kFusedRows>(s, a.partials, v.nb_upd, lb, lds);
 }
 
 // With ev0/ev1 the launch carries its own start/stop events (hipExtLaunchKernelGGL): the events
@@ -595,6 +850,27 @@ void launch_pcg_dir(const Launch& L, Family fam, int op, float dt, int k, unsign
                 launch_timed(k_pcg_dir<1>, L.blocks, kBlock, L.stream, ev0, ev1, L.tab, L.map, dt, k, pace, tag);
             else
                 launch_timed(k_pcg_dir<0>, L.blocks, kBlock, L.stream, ev0, ev1, L.tab, L.map, dt, k, pace, tag);
+    }
+}
+
+void launch_pcg_init_fused(const Launch& L) {
+    launch_timed(k_pcg_init_fused, L.blocks, kBlock, L.stream, nullptr, nullptr, L.tab, L.map);
+}
+
+void launch_pcg_fused(const Launch& L, Family fam, int k, unsigned long long* pace, unsigned tag, hipEvent_t ev0,
+                      hipEvent_t ev1) {
+    switch (fam) {
+        case FAM_FUSED4:
+            launch_timed(k_pcg_fused<4>, L.blocks, kBlock3, L.stream, ev0, ev1, L.tab, L.map, k, pace, tag);
+            break;
+        case FAM_FUSED8:
+            launch_timed(k_pcg_fused<8>, L.blocks, kBlock3, L.stream, ev0, ev1, L.tab, L.map, k, pace, tag);
+            break;
+        case FAM_FUSED16:
+            launch_timed(k_pcg_fused<16>, L.blocks, kBlock3, L.stream, ev0, ev1, L.tab, L.map, k, pace, tag);
+            break;
+        default:
+            break;
     }
 }
 

@@ -21,12 +21,13 @@ namespace hfm {
 struct CgCtl {
     float slot_rr[2];  // ||r||^2 after iteration k, in slot k & 1
     float slot_rz[2];  // r.z after iteration k, in slot k & 1
+    double slot_pap[2];  // fused form: p.Ap of iteration k, in slot k & 1
     float atol_sq;     // fp32 threshold on ||r||^2
     int it;            // iterations done when the solve stopped
     int done;          // 1: converged or iteration cap reached; every later CG kernel skips the scene
     int converged;
     int maxeff;        // iteration cap rounded up to the 2-iteration cycle
-    int pad0;
+    int geom_dirty;    // ctl[0] only: particles were re-binned, the next gather rebuilds every lattice
     double tol;        // relative tolerance
     double atol;       // max(tol * ||b||, tol)
     float mean;        // pressure rhs mean to remove (numpy float32 mean)
@@ -60,10 +61,12 @@ struct SceneDev {
     float *T, *T2, *Tw, *kap, *alpha_p, *tsum, *tfix, *hrhs, *hdiag;
     uint8_t* flags;
     float *cg_r, *cg_z, *cg_d, *cg_Ap;
-    double* partials;        // [kMaxDirBlocks] pass A partials, then [3][nb_upd] pass B / init
+    double* partials;        // two-pass: [kMaxDirBlocks] pass A, then [3][nb_upd] pass B / init;
+                             // fused: two parity buffers of [kFusedRows][kMaxDirBlocks]
     CgCtl* ctl;              // [0] heat, [1] pressure
     CgMirror* mirror;        // device view of the pinned mirrors, [0] heat, [1] pressure
     int cpt, nb_upd, nb_dir, tiles_j, tiles_k, dtile;
+    int cg_fused;            // pressure PCG in one launch per iteration (tiled scenes only)
     float* chunk_sums;
     const int* mean_prog;
     int mean_tail;           // offset of the tail chunk's program in mean_prog, -1 if none
@@ -71,6 +74,7 @@ struct SceneDev {
     float *pos, *vel;
     float *temp, *melt, *yield, *visc;
     int *bin_start, *sorted;
+    float* qpos;             // pos / h per particle, in bin order (3 per particle)
 };
 
 // Block-map entries: scene (24 bits) | sub-range (8 bits) | local workgroup (32 bits).
@@ -104,6 +108,9 @@ enum Family {
     FAM_DIR8,
     FAM_DIR16,
     FAM_DIRFLAT,    // pressure pass A without tiling (PCG geometry)
+    FAM_FUSED4,     // fused pressure iteration, 16x16x4 blocks (scenes with cg_fused)
+    FAM_FUSED8,
+    FAM_FUSED16,
     FAM_COUNT
 };
 
@@ -194,5 +201,10 @@ void launch_pcg_init(const Launch& L, int op, float dt);
 void launch_pcg_dir(const Launch& L, Family fam, int op, float dt, int k, unsigned long long* pace, unsigned tag,
                     hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 void launch_pcg_upd(const Launch& L, int op, int k, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+// fused pressure form: first-iteration scalars after launch_pcg_init, then one launch per iteration
+constexpr int kFusedRows = 5;
+void launch_pcg_init_fused(const Launch& L);
+void launch_pcg_fused(const Launch& L, Family fam, int k, unsigned long long* pace, unsigned tag,
+                      hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 
 }  // namespace hfm

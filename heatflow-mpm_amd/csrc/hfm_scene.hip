@@ -214,10 +214,13 @@ long family_blocks(const Scene* s, int f, int sub) {
         case FAM_MEAN: return sub == 0 ? d.nchunks : 0;
         case FAM_PCG: return sub == 0 ? d.nb_upd : 0;
         case FAM_G2P: return sub == 0 && d.P > 0 ? nblocks(d.P) : 0;
-        case FAM_DIR4: return sub == 0 && d.dtile == 4 ? d.nb_dir : 0;
-        case FAM_DIR8: return sub == 0 && d.dtile == 8 ? d.nb_dir : 0;
-        case FAM_DIR16: return sub == 0 && d.dtile == 16 ? d.nb_dir : 0;
+        case FAM_DIR4: return sub == 0 && d.dtile == 4 && !d.cg_fused ? d.nb_dir : 0;
+        case FAM_DIR8: return sub == 0 && d.dtile == 8 && !d.cg_fused ? d.nb_dir : 0;
+        case FAM_DIR16: return sub == 0 && d.dtile == 16 && !d.cg_fused ? d.nb_dir : 0;
         case FAM_DIRFLAT: return sub == 0 && d.dtile == 0 ? d.nb_dir : 0;
+        case FAM_FUSED4: return sub == 0 && d.dtile == 4 && d.cg_fused ? d.nb_dir : 0;
+        case FAM_FUSED8: return sub == 0 && d.dtile == 8 && d.cg_fused ? d.nb_dir : 0;
+        case FAM_FUSED16: return sub == 0 && d.dtile == 16 && d.cg_fused ? d.nb_dir : 0;
         default: return 0;
     }
 }
@@ -356,22 +359,48 @@ int run_pcg(Batch* b, int op, float dt) {
         pred = std::max(pred, op ? s->pres_pred : s->heat_pred);
     }
     launch_pcg_init(family(b, FAM_PCG), op, dt);
-    CHECK_LAUNCH();
     // pass A launches: the heat operator uses the flat kernel with the PCG geometry; the
-    // pressure operator one launch per tiling present in the batch
-    Family dirs[4];
+    // pressure operator one launch per tiling present in the batch, two-pass or fused
+    Family dirs[8];
     int ndir = 0;
+    bool two_pass = op == 0;
     if (op == 0) {
         dirs[ndir++] = FAM_PCG;
     } else {
-        for (Family f : {FAM_DIR4, FAM_DIR8, FAM_DIR16, FAM_DIRFLAT})
+        for (Family f : {FAM_DIR4, FAM_DIR8, FAM_DIR16, FAM_DIRFLAT, FAM_FUSED4, FAM_FUSED8, FAM_FUSED16})
             if (b->map_len[f] > 0) dirs[ndir++] = f;
+        bool any_fused = false;
+        for (Scene* s : b->scenes) {
+            if (s->dev.cg_fused)
+                any_fused = true;
+            else
+                two_pass = true;
+        }
+        if (any_fused) launch_pcg_init_fused(family(b, FAM_PCG));
     }
-    // Iterations are enqueued in groups of (pass A, pass B) pairs, at most two groups in
-    // flight.  Pass A of iteration k+1 finishes iteration k, so the first pass A of a group
-    // posts how many iterations the previous groups completed, which releases the next group;
-    // after the last pair one more pass A closes every solve.  Group size follows the iteration
-    // count of the previous solve so that few launches are wasted after convergence.
+    CHECK_LAUNCH();
+    auto is_fused = [](Family f) { return f == FAM_FUSED4 || f == FAM_FUSED8 || f == FAM_FUSED16; };
+    // One launch of pass A (or of the fused iteration) per family for iteration k.
+    auto pass_a = [&](int k, unsigned long long* pace, bool timed) -> int {
+        for (int d = 0; d < ndir; ++d) {
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            unsigned long long* pc = d == 0 ? pace : nullptr;
+            if (is_fused(dirs[d])) {
+                if (timed) RC(timing_events(b, HFM_KT_PCG_FUSED, &e0, &e1));
+                launch_pcg_fused(family(b, dirs[d]), dirs[d], k, pc, tag, e0, e1);
+            } else {
+                if (timed) RC(timing_events(b, HFM_KT_PCG_DIR, &e0, &e1));
+                launch_pcg_dir(family(b, dirs[d]), dirs[d], op, dt, k, pc, tag, e0, e1);
+            }
+        }
+        return HFM_OK;
+    };
+    // Iterations are enqueued in groups, at most two groups in flight.  Each iteration is a
+    // pass A and a pass B (two-pass form) or one fused launch; pass A of iteration k+1 finishes
+    // iteration k, so the first launch of a group posts how many iterations the previous groups
+    // completed, which releases the next group; after the last iteration one more pass A closes
+    // every solve.  Group size follows the iteration count of the previous solve so that few
+    // launches are wasted after convergence.
     int launched = 0;
     int group_end[2] = {0, 0};
     int inflight = 0;
@@ -399,15 +428,12 @@ int run_pcg(Batch* b, int op, float dt) {
             if (g > maxeff - launched) g = maxeff - launched;
             for (int q = 0; q < g; ++q) {
                 const int k = launched + q;
-                for (int d = 0; d < ndir; ++d) {
+                RC(pass_a(k, q == 0 ? b->pace_dev : nullptr, timed));
+                if (two_pass) {
                     hipEvent_t e0 = nullptr, e1 = nullptr;
-                    if (timed) RC(timing_events(b, HFM_KT_PCG_DIR, &e0, &e1));
-                    launch_pcg_dir(family(b, dirs[d]), dirs[d], op, dt, k, q == 0 && d == 0 ? b->pace_dev : nullptr,
-                                   tag, e0, e1);
+                    if (timed) RC(timing_events(b, HFM_KT_PCG_UPD, &e0, &e1));
+                    launch_pcg_upd(family(b, FAM_PCG), op, k, e0, e1);
                 }
-                hipEvent_t e0 = nullptr, e1 = nullptr;
-                if (timed) RC(timing_events(b, HFM_KT_PCG_UPD, &e0, &e1));
-                launch_pcg_upd(family(b, FAM_PCG), op, k, e0, e1);
             }
             CHECK_LAUNCH();
             launched += g;
@@ -415,8 +441,7 @@ int run_pcg(Batch* b, int op, float dt) {
             continue;
         }
         if (launched >= maxeff && !closed) {
-            for (int d = 0; d < ndir; ++d)
-                launch_pcg_dir(family(b, dirs[d]), dirs[d], op, dt, maxeff, d == 0 ? b->pace_dev : nullptr, tag);
+            RC(pass_a(maxeff, b->pace_dev, false));
             CHECK_LAUNCH();
             closed = true;
             continue;
@@ -675,11 +700,11 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
     A(d.hrhs, N);
     A(d.hdiag, N);
     A(d.flags, N);
-    A(d.cg_r, N);
+    A(d.cg_r, 2 * N);
     A(d.cg_z, N);
     A(d.cg_d, 2 * N);
     A(d.cg_Ap, N);
-    A(d.partials, kMaxDirBlocks + 3 * kMaxPcgBlocks);
+    A(d.partials, std::max(kMaxDirBlocks + 3 * kMaxPcgBlocks, 2 * kFusedRows * kMaxDirBlocks));
     A(d.ctl, 2);
     d.nchunks = (int)((N + 8191) / 8192);
     A(d.chunk_sums, d.nchunks);
@@ -699,6 +724,8 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
 #undef A
     d.mean_prog = prog_dev;
     pcg_geometry(s, &d.cpt, &d.nb_upd, &d.nb_dir, &d.tiles_j, &d.tiles_k, &d.dtile);
+    // the fused iteration is built for 4- and 8-plane blocks (16 planes exceed its register budget)
+    d.cg_fused = (prm->flags & HFM_FLAG_FUSED_CG) && (d.dtile == 4 || d.dtile == 8);
     e = hipMemcpy(prog_dev, o->prog.data(), o->prog.size() * sizeof(int), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         set_err("hipMemcpy: %s", hipGetErrorString(e));
@@ -797,6 +824,8 @@ int hfm_set_static(hfm_scene* h, const float* alpha, const float* kx, const floa
     if (s->dev.P == 0) {
         RC(reset_particle_fields(s));
         HIP_OK(hipStreamSynchronize(s->stream));
+    } else {
+        s->bins_dirty = 1;  // the next gather adds the new static part to every lattice
     }
     return HFM_OK;
 }
@@ -856,15 +885,16 @@ int hfm_set_particles(hfm_scene* h, int64_t P, const float* pos, const float* ve
     HIP_OK(hipStreamSynchronize(s->stream));
     SceneDev& d = s->dev;
     if (P > s->P_cap) {
-        void* olds[] = {d.pos, d.vel, d.temp, d.melt, d.yield, d.visc, s->pkey, s->pslot, d.sorted};
+        void* olds[] = {d.pos, d.vel, d.temp, d.melt, d.yield, d.visc, s->pkey, s->pslot, d.sorted, d.qpos};
         for (void* q : olds)
             if (q) dfree(o, q);
-        d.pos = d.vel = d.temp = d.melt = d.yield = d.visc = nullptr;
+        d.pos = d.vel = d.temp = d.melt = d.yield = d.visc = d.qpos = nullptr;
         s->pkey = s->pslot = d.sorted = nullptr;
         s->P_cap = 0;
         if ((rc = dalloc(o, &d.pos, 3 * P)) || (rc = dalloc(o, &d.vel, 3 * P)) || (rc = dalloc(o, &d.temp, P)) ||
             (rc = dalloc(o, &d.melt, P)) || (rc = dalloc(o, &d.yield, P)) || (rc = dalloc(o, &d.visc, P)) ||
-            (rc = dalloc(o, &s->pkey, P)) || (rc = dalloc(o, &s->pslot, P)) || (rc = dalloc(o, &d.sorted, P))) {
+            (rc = dalloc(o, &s->pkey, P)) || (rc = dalloc(o, &s->pslot, P)) || (rc = dalloc(o, &d.sorted, P)) ||
+            (rc = dalloc(o, &d.qpos, 3 * P))) {
             d.P = 0;
             refresh_dev(s);
             return rc;

@@ -24,7 +24,8 @@ FIELD = {name: i for i, name in enumerate(FIELD_NAMES)}
 
 PREC_FP32 = 0
 FLAG_TIME_KERNELS = 1
-KT_PCG_DIR, KT_PCG_UPD = 0, 1
+FLAG_FUSED_CG = 2
+KT_PCG_DIR, KT_PCG_UPD, KT_PCG_FUSED = 0, 1, 2
 
 
 class Params(C.Structure):

@@ -192,17 +192,24 @@ class _Scene:
 class HeatflowMPM:
     """Eulerian air and temperature coupled to a particle material; substep on the GPU.
 
-    Same constructor as the reference (solver.py:107-108).  `body` (optional, not in the
-    reference) replaces the default StaticBody built from `particles`.
+    Same constructor as the reference (solver.py:107-108).  Extras, not in the reference: `body`
+    replaces the default StaticBody built from `particles`; `cg="fused"` runs the pressure PCG in
+    one launch per iteration (HFM_FLAG_FUSED_CG in include/hfm.h) instead of warp's two-pass form.
     """
+
+    CG_FORMS = ("warp", "fused")
 
     def __init__(self, particles: np.ndarray, mat: Material = Material(), heater: Heater = Heater(),
                  grid: GridConfig = GridConfig(), spacing: float = 2.5e-3, t_init: float = 20.0,
-                 kappa_air: float = KAPPA_AIR, device: str = "cuda:0", body=None):
+                 kappa_air: float = KAPPA_AIR, device: str = "cuda:0", body=None, cg: str = "warp"):
+        if cg not in self.CG_FORMS:
+            raise ValueError(f"cg must be one of {self.CG_FORMS}")
         self.mat, self.heater, self.dev, self.spacing, self.kappa_air = mat, heater, device, spacing, kappa_air
         particles = np.asarray(particles, np.float32).reshape(-1, 3)
         self.n = len(particles)
         inp = scene_inputs(self.n, mat, heater, grid, spacing, t_init, kappa_air)
+        if cg == "fused":
+            inp["params"]["flags"] |= _abi.FLAG_FUSED_CG
         self._scene = _Scene(inp["params"], parse_device(device))
         self.grid = AirGrid(grid, self._scene)
         self.heat = HeatField(self.grid.n, self.grid.h, self._scene)

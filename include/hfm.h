@@ -69,6 +69,12 @@ typedef struct hfm_params {
 #define HFM_PREC_FP32 0       /* the reference's arithmetic: fp32 storage and math */
 
 #define HFM_FLAG_TIME_KERNELS 1  /* record HIP events around every CG kernel launch */
+/* Pressure PCG in one launch per iteration instead of two (no counterpart in the reference).
+ * Same iterates as warp's PCG in exact arithmetic: p.Ap is formed from explicit dot products,
+ * p.Ap = z.Az + 2 beta z.Ap_old + beta^2 p.Ap_old, so one global reduction per iteration
+ * suffices, while r is still updated with the explicit A p.  Results agree with the default
+ * form to rounding; iteration counts within one.  Used where the pressure pass runs tiled. */
+#define HFM_FLAG_FUSED_CG 2
 
 /* Per-substep statistics of the last hfm_substep. */
 typedef struct hfm_stats {
@@ -168,7 +174,8 @@ int hfm_get_stats(hfm_scene* s, hfm_stats* out);
  * recorded around every launch on the scene's stream. */
 #define HFM_KT_PCG_DIR 0      /* pressure PCG pass A: direction update + SpMV + p.Ap */
 #define HFM_KT_PCG_UPD 1      /* pressure PCG pass B: x, r update + r.r, r.z */
-#define HFM_KT_COUNT 2
+#define HFM_KT_PCG_FUSED 2    /* pressure PCG, HFM_FLAG_FUSED_CG: one whole iteration */
+#define HFM_KT_COUNT 3
 int hfm_kernel_time(hfm_scene* s, int32_t kernel_class, double* total_ms, int64_t* launches);
 int hfm_kernel_time_reset(hfm_scene* s);
 
