@@ -99,11 +99,28 @@ __device__ __forceinline__ void backtrace(const float* __restrict__ u, const flo
     z = z - dt * vz;
 }
 
-// grid.py:46-59 — air fraction of an interior face between cells with solid fractions a, b
+// grid.py:46-59 — air fraction of an interior face between cells with solid fractions a, b:
+// clamp(1 - 0.5 (a + b), amin, 1).  0.5 x is exact, so RN(1 - RN(0.5 x)) = RN(1 - 0.5 x) is one
+// fused multiply-add, and for finite values the clamp is v_med3_f32: the same bits in 3
+// instructions instead of 5.
 __device__ __forceinline__ float face_fraction(float a_left, float a_right, float amin) {
-    float a = 0.5f * (a_left + a_right);
-    float v = 1.0f - a;
-    return v < amin ? amin : (v > 1.0f ? 1.0f : v);
+    return __builtin_amdgcn_fmed3f(__builtin_fmaf(-0.5f, a_left + a_right, 1.0f), amin, 1.0f);
+}
+
+// x / b rounded correctly, for a scene constant b with rb = RN(1/b): q = RN(x rb), then one
+// residual correction q + RN(x - q b) rb (Markstein's theorem: with rb the correctly rounded
+// reciprocal and q within an ulp, the corrected quotient is the correctly rounded one).  The
+// host enables it (fast = 1) only after checking every significand of one binade of x against
+// IEEE division for this b; all intermediate values stay normal for 2^-100 <= |x| <= 2^100, so
+// that check covers the whole range, and anything outside it (or fast = 0) divides.
+__device__ __forceinline__ float div_const(float x, float b, float rb, int fast) {
+    const float ax = fabsf(x);
+    if (fast && ((ax >= 0x1p-100f && ax <= 0x1p100f) || x == 0.0f)) {
+        const float q = x * rb;
+        const float e = __builtin_fmaf(-q, b, x);
+        return __builtin_fmaf(e, rb, q);
+    }
+    return x / b;
 }
 
 // heat.py:44-47
@@ -120,15 +137,50 @@ __device__ __forceinline__ float face_kappa(float ka, float kb) { return 2.0f * 
 // Deterministic grid reductions
 // ---------------------------------------------------------------------------------------------
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+// One DPP step on a double: each lane receives the value of the lane CTRL selects (rows outside
+// ROW_MASK receive 0).
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ double dpp_double(double x) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, ROW_MASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, ROW_MASK, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wave in a fixed tree with DPP moves (no LDS round trips): pairs and
+// quads (quad_perm), 8- and 16-lane rows (half mirror, mirror), then row 0 into row 1 and row 2
+// into row 3 (row_bcast:15), and rows 0-1 into row 3 (row_bcast:31).  Lane 63 holds the total,
+// which is returned to every lane.
+__device__ __forceinline__ double wave_sum(double x) {
+    x += dpp_double<0xB1>(x);        // quad_perm [1,0,3,2]
+    x += dpp_double<0x4E>(x);        // quad_perm [2,3,0,1]
+    x += dpp_double<0x141>(x);       // row_half_mirror
+    x += dpp_double<0x140>(x);       // row_mirror
+    x += dpp_double<0x142, 0xA>(x);  // row_bcast:15
+    x += dpp_double<0x143, 0xC>(x);  // row_bcast:31
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), 63);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum of the per-wave values w[0..NW) (NW <= 16, in LDS) in a fixed tree, returned to every lane:
+// lane l < NW reads w[l] and the first 16-lane row is reduced with DPP moves.
+template <int NW>
+__device__ __forceinline__ double sum_wave_values(const double* w) {
+    static_assert(NW <= 16, "at most 16 waves per workgroup");
+    const int lane = threadIdx.x & 63;
+    double x = lane < NW ? w[lane] : 0.0;
+    x += dpp_double<0xB1>(x);   // quad_perm [1,0,3,2]
+    x += dpp_double<0x4E>(x);   // quad_perm [2,3,0,1]
+    x += dpp_double<0x141>(x);  // row_half_mirror
+    x += dpp_double<0x140>(x);  // row_mirror: every lane of row 0 holds the sum
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), 0);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), 0);
+    return __hiloint2double(hi, lo);
 }
 
 // Sum of one double per thread over a workgroup of BS threads; the result is valid in thread 0.
-// The tree is fixed (lane butterfly, then waves in order), so the value does not depend on
-// timing.  lds must hold NV x (BS / 64) doubles.
+// The tree is fixed (wave tree, then waves in order), so the value does not depend on timing.
+// lds must hold NV x (BS / 64) doubles.
 template <int NV, int BS = kBlock>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double (*lds)[BS / 64]) {
     int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -138,14 +190,9 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double (*lds)[BS / 64
         if (lane == 0) lds[q][wv] = s;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 64) {
 #pragma unroll
-        for (int q = 0; q < NV; ++q) {
-            double s = lds[q][0];
-#pragma unroll
-            for (int w = 1; w < BS / 64; ++w) s += lds[q][w];
-            v[q] = s;
-        }
+        for (int q = 0; q < NV; ++q) v[q] = sum_wave_values<BS / 64>(lds[q]);
     }
     __syncthreads();
 }

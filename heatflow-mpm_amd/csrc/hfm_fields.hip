@@ -39,9 +39,9 @@ __device__ __forceinline__ void face_ijk(int axis, long f, int n, int& i, int& j
 // ---------------------------------------------------------------------------------------------
 // heat prep: T2 = T (previous temperature); Tw = blend(T); fixed; capped source
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_heat_prep(const SceneDev* __restrict__ tab,
-                                                      const MapEntry* __restrict__ map, float dt) {
-    HFM_BATCH_ENTRY(tab, map);
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_heat_prep(Src src, float dt) {
+    HFM_ENTRY(src);
     const long t = (long)xcd_tile(lb, nblocks(a.N)) * kBlock + threadIdx.x;
     if (t >= a.N) return;
     const float Tc = a.T[t];
@@ -64,9 +64,9 @@ __global__ __launch_bounds__(kBlock) void k_heat_prep(const SceneDev* __restrict
 // ---------------------------------------------------------------------------------------------
 // T advection (into T) + conductivity + heat rhs/diag
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_advect_T(const SceneDev* __restrict__ tab,
-                                                     const MapEntry* __restrict__ map, float dt) {
-    HFM_BATCH_ENTRY(tab, map);
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_advect_T(Src src, float dt) {
+    HFM_ENTRY(src);
     const long t = (long)xcd_tile(lb, nblocks(a.N)) * kBlock + threadIdx.x;
     if (t >= a.N) return;
     const int n = a.n;
@@ -108,8 +108,9 @@ __global__ __launch_bounds__(kBlock) void k_advect_T(const SceneDev* __restrict_
 // ---------------------------------------------------------------------------------------------
 // particle update (G2P): grid temperature before diffusion = Tw, after = T
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_g2p(const SceneDev* __restrict__ tab, const MapEntry* __restrict__ map) {
-    HFM_BATCH_ENTRY(tab, map);
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_g2p(Src src) {
+    HFM_ENTRY(src);
     if (lb == 0 && threadIdx.x == 0) a.ctl[0].geom_dirty = 0;  // this substep's gather rebuilt the geometry
     const long p = (long)lb * kBlock + threadIdx.x;
     if (p >= a.P) return;
@@ -130,9 +131,9 @@ __global__ __launch_bounds__(kBlock) void k_g2p(const SceneDev* __restrict__ tab
 // ---------------------------------------------------------------------------------------------
 // air face pass: advect (old u, v, w) -> buoyancy (new T) -> relax, into us; sub-range = axis
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_air_faces(const SceneDev* __restrict__ tab,
-                                                      const MapEntry* __restrict__ map, float dt) {
-    HFM_BATCH_ENTRY(tab, map);
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_air_faces(Src src, float dt) {
+    HFM_ENTRY(src);
     const int axis = sub;
     const long f = (long)xcd_tile(lb, nblocks(a.F)) * kBlock + threadIdx.x;
     if (f >= a.F) return;
@@ -176,12 +177,13 @@ __global__ __launch_bounds__(kBlock) void k_air_faces(const SceneDev* __restrict
 // ---------------------------------------------------------------------------------------------
 // divergence rhs (+ Jacobi diagonal)
 // ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ void divergence_cell(const SceneDev& a, const float* const* vel, long t, float scale,
-                                                float* rhs, float* pdiag) {
+template <class Scene_, class Vel, class Out>
+__device__ __forceinline__ void divergence_cell(const Scene_& a, const Vel& vel, long t, float scale, Out rhs,
+                                                Out pdiag) {
     const int n = a.n;
     const long nn = (long)n * n;
     const int i = (int)(t / nn), j = (int)((t / n) % n), k = (int)(t % n);
-    const float* as = a.alpha_s;
+    const auto as = a.alpha_s;
     const float amin = a.amin;
     const float ac = as[t];
     const float ax0 = i > 0 ? face_fraction(as[t - nn], ac, amin) : 0.0f;
@@ -193,8 +195,8 @@ __device__ __forceinline__ void divergence_cell(const SceneDev& a, const float* 
     const long fx0 = ((long)i * n + j) * n + k, fx1 = fx0 + nn;
     const long fy0 = ((long)i * (n + 1) + j) * n + k, fy1 = fy0 + n;
     const long fz0 = ((long)i * n + j) * (n + 1) + k, fz1 = fz0 + 1;
-    const float *u = vel[0], *v = vel[1], *w = vel[2];
-    const float *qx = a.q[0], *qy = a.q[1], *qz = a.q[2];
+    const auto u = vel[0], v = vel[1], w = vel[2];
+    const auto qx = a.q[0], qy = a.q[1], qz = a.q[2];
     float div = ax1 * u[fx1] + qx[fx1] - ax0 * u[fx0] - qx[fx0] + ay1 * v[fy1] + qy[fy1] - ay0 * v[fy0] - qy[fy0] +
                 az1 * w[fz1] + qz[fz1] - az0 * w[fz0] - qz[fz0];
     div = div / a.h;
@@ -212,9 +214,9 @@ __device__ __forceinline__ void divergence_cell(const SceneDev& a, const float* 
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_divergence(const SceneDev* __restrict__ tab,
-                                                       const MapEntry* __restrict__ map, double dt) {
-    HFM_BATCH_ENTRY(tab, map);
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_divergence(Src src, double dt) {
+    HFM_ENTRY(src);
     const long t = (long)xcd_tile(lb, nblocks(a.N)) * kBlock + threadIdx.x;
     if (t >= a.N) return;
     const float scale = (float)(a.rho_d / dt);  // rho/dt formed in double (grid.py:237)
@@ -224,7 +226,7 @@ __global__ __launch_bounds__(kBlock) void k_divergence(const SceneDev* __restric
 __global__ __launch_bounds__(kBlock) void k_divergence1(SceneDev a, float* out) {
     const long t = (long)blockIdx.x * kBlock + threadIdx.x;
     if (t >= a.N) return;
-    divergence_cell(a, a.u, t, 1.0f, out, nullptr);
+    divergence_cell(a, a.u, t, 1.0f, out, (float*)nullptr);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -255,9 +257,9 @@ __device__ float leaf_sum(const float* a, int len) {
     return res;
 }
 
-__global__ __launch_bounds__(kBlock) void k_rhs_mean(const SceneDev* __restrict__ tab,
-                                                     const MapEntry* __restrict__ map) {
-    HFM_BATCH_ENTRY(tab, map);
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_rhs_mean(Src src) {
+    HFM_ENTRY(src);
     __shared__ float leaves[kMaxLeaves];
     __shared__ int is_last;
     const int c = lb;
@@ -313,9 +315,9 @@ __global__ __launch_bounds__(kBlock) void k_rhs_mean(const SceneDev* __restrict_
 // ---------------------------------------------------------------------------------------------
 // projection: u = us - (dt/rho) grad p / h on interior faces, walls 0; sub-range = axis
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_project(const SceneDev* __restrict__ tab,
-                                                    const MapEntry* __restrict__ map, double dt) {
-    HFM_BATCH_ENTRY(tab, map);
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_project(Src src, double dt) {
+    HFM_ENTRY(src);
     const int axis = sub;
     const long f = (long)xcd_tile(lb, nblocks(a.F)) * kBlock + threadIdx.x;
     if (f >= a.F) return;
@@ -390,20 +392,13 @@ __global__ __launch_bounds__(kBlock) void k_apply_fixed(long N, const uint8_t* f
 
 }  // namespace
 
-#define HFM_BATCH_LAUNCH(KERNEL, L, ...)                                                              \
-    do {                                                                                              \
-        if ((L).blocks > 0)                                                                           \
-            hipLaunchKernelGGL(KERNEL, dim3((L).blocks), dim3(kBlock), 0, (L).stream, (L).tab, (L).map, \
-                               ##__VA_ARGS__);                                                        \
-    } while (0)
-
-void launch_heat_prep(const Launch& L, float dt) { HFM_BATCH_LAUNCH(k_heat_prep, L, dt); }
-void launch_advect_T(const Launch& L, float dt) { HFM_BATCH_LAUNCH(k_advect_T, L, dt); }
-void launch_g2p(const Launch& L) { HFM_BATCH_LAUNCH(k_g2p, L); }
-void launch_air_faces(const Launch& L, float dt) { HFM_BATCH_LAUNCH(k_air_faces, L, dt); }
-void launch_divergence(const Launch& L, double dt) { HFM_BATCH_LAUNCH(k_divergence, L, dt); }
-void launch_rhs_mean(const Launch& L) { HFM_BATCH_LAUNCH(k_rhs_mean, L); }
-void launch_project(const Launch& L, double dt) { HFM_BATCH_LAUNCH(k_project, L, dt); }
+void launch_heat_prep(const Launch& L, float dt) { HFM_LAUNCH(k_heat_prep, L, kBlock, nullptr, nullptr, dt); }
+void launch_advect_T(const Launch& L, float dt) { HFM_LAUNCH(k_advect_T, L, kBlock, nullptr, nullptr, dt); }
+void launch_g2p(const Launch& L) { HFM_LAUNCH(k_g2p, L, kBlock, nullptr, nullptr); }
+void launch_air_faces(const Launch& L, float dt) { HFM_LAUNCH(k_air_faces, L, kBlock, nullptr, nullptr, dt); }
+void launch_divergence(const Launch& L, double dt) { HFM_LAUNCH(k_divergence, L, kBlock, nullptr, nullptr, dt); }
+void launch_rhs_mean(const Launch& L) { HFM_LAUNCH(k_rhs_mean, L, kBlock, nullptr, nullptr); }
+void launch_project(const Launch& L, double dt) { HFM_LAUNCH(k_project, L, kBlock, nullptr, nullptr, dt); }
 
 void launch_divergence_scale1(const Scene* s, hipStream_t st, float* out) {
     hipLaunchKernelGGL(k_divergence1, dim3(nblocks(s->N)), dim3(kBlock), 0, st, s->dev, out);

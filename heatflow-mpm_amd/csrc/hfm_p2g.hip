@@ -175,14 +175,15 @@ __device__ __forceinline__ void node_terms(float qx, float qy, float qz, float o
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_p2g_gather(const SceneDev* __restrict__ tab,
-                                                       const MapEntry* __restrict__ map) {
-    HFM_BATCH_ENTRY(tab, map);
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_p2g_gather(Src src) {
+    HFM_ENTRY(src);
     const int lat = sub;  // 0 cell, 1 + axis for faces
     const bool full = a.ctl[0].geom_dirty != 0;
     if (!full && lat != 0) return;  // face lattices unchanged since the last gather
     const int n = a.n;
     const long count = lat == 0 ? a.N : a.F;
+    if (lb >= nblocks(count)) return;  // a batch of one launches the face lattices' workgroup count for all four
     const long node = (long)xcd_tile(lb, nblocks(count)) * kBlock + threadIdx.x;
     if (node >= count) return;
     int dx = n, dy = n, dz = n;
@@ -265,8 +266,6 @@ void launch_bin_particles(Scene* s, hipStream_t st) {
     s->bins_dirty = 0;
 }
 
-void launch_p2g(const Launch& L) {
-    if (L.blocks > 0) hipLaunchKernelGGL(k_p2g_gather, dim3(L.blocks), dim3(kBlock), 0, L.stream, L.tab, L.map);
-}
+void launch_p2g(const Launch& L) { HFM_LAUNCH(k_p2g_gather, L, kBlock, nullptr, nullptr); }
 
 }  // namespace hfm

@@ -46,6 +46,8 @@ struct PcgView {
     int n;
     long N;
     float h, hh, dt_hh, amin;
+    float rhh;             // RN(1 / h^2)
+    int fastdiv;           // division by h^2 through div_const's corrected product
     const float* alpha_s;  // pressure
     const float* kap;      // heat
     const uint8_t* flags;  // heat
@@ -69,12 +71,14 @@ struct PcgView {
 };
 
 template <int OP>
-__device__ __forceinline__ PcgView make_view(const SceneDev& a, int lb, int k, float dt) {
+__device__ __forceinline__ PcgView make_view(const SceneG& a, int lb, int k, float dt) {
     PcgView v;
     v.n = a.n;
     v.N = a.N;
     v.h = a.h;
     v.hh = a.hh;
+    v.rhh = a.rhh;
+    v.fastdiv = a.fastdiv;
     v.dt_hh = dt / (a.h * a.h);
     v.amin = a.amin;
     v.alpha_s = a.alpha_s;
@@ -134,7 +138,7 @@ struct Operator<1> {
     __device__ static float combine(const PcgView& a, float acc, float xc, bool fixed, float alpha) {
         (void)xc;
         (void)fixed;
-        return alpha * acc / a.hh;
+        return div_const(alpha * acc, a.hh, a.rhh, a.fastdiv);
     }
 };
 
@@ -197,37 +201,6 @@ __device__ __forceinline__ void post_pace(unsigned long long* pace, unsigned tag
                            __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// Whether the solve has stopped, read once per workgroup (thread 0, broadcast through LDS) so
-// that every wave of the workgroup takes the same branch: workgroup 0 of a pass A may set the
-// flag while other workgroups of the same launch start.
-__device__ __forceinline__ bool solve_stopped(const CgCtl* c) {
-    __shared__ int stopped;
-    if (threadIdx.x == 0) stopped = c->done;
-    __syncthreads();
-    return stopped != 0;
-}
-
-// Sum NV rows of per-workgroup partials (row q at P + q*nb) in a fixed order; every thread of the
-// BS-thread workgroup receives the totals.
-template <int NV, int BS = kBlock>
-__device__ __forceinline__ void sum_partials(const double* __restrict__ P, int nb, double (*lds)[BS / 64],
-                                             double (&out)[NV]) {
-    __shared__ double bcast[NV];
-    double acc[NV];
-#pragma unroll
-    for (int q = 0; q < NV; ++q) acc[q] = 0.0;
-    for (int b = threadIdx.x; b < nb; b += BS)
-#pragma unroll
-        for (int q = 0; q < NV; ++q) acc[q] += P[q * nb + b];
-    block_sum<NV, BS>(acc, lds);
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int q = 0; q < NV; ++q) bcast[q] = acc[q];
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NV; ++q) out[q] = bcast[q];
-}
-
 // Store this workgroup's partial sums (thread 0 after the block sum) in row slot lb.
 template <int NV, int BS = kBlock>
 __device__ __forceinline__ void store_partials(double (&v)[NV], double* __restrict__ P, int nb, int lb,
@@ -244,38 +217,82 @@ __device__ __forceinline__ void store_partials(double (&v)[NV], double* __restri
     for (int u_ = 0; u_ < (v).cpt; ++u_)                                     \
         if (long t = ((long)L_ * (v).cpt + u_) * kBlock + threadIdx.x; t < (v).N)
 
-// Prologue of every pass A: finish iteration k from pass B's partial sums (r.r, r.z; b.b too
-// at k = 0, where atol is formed), test convergence and the iteration cap, and derive beta.
-// Workgroup 0 of the scene records the state.  Returns true when the solve stops (in every
-// thread).
+// Reduce NV per-thread partial sums over the workgroup in a fixed order (wave trees, then waves
+// in order) into every thread's out[0..nv), with one barrier that also broadcasts thread 0's
+// `flag`; returns the flag in every thread.
+template <int NV, int BS>
+__device__ __forceinline__ int reduce_bcast(const double (&acc)[NV], int nv, int flag, double (&out)[NV]) {
+    __shared__ double wsum[NV][BS / 64];
+    __shared__ int flag_s;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        if (q < nv) {
+            const double w = wave_sum(acc[q]);
+            if ((threadIdx.x & 63) == 0) wsum[q][threadIdx.x >> 6] = w;
+        }
+    }
+    if (threadIdx.x == 0) flag_s = flag;
+    __syncthreads();
+    const int f = flag_s;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) out[q] = q < nv ? sum_wave_values<BS / 64>(wsum[q]) : 0.0;
+    return f;
+}
+
+// Operands of a pass A prologue, loaded together with the block's data: this workgroup's share
+// of the partial sums of pass B (r.r, r.z; b.b too from the init launch at k = 0) and the control
+// scalars written by earlier launches.
+struct PassAIn {
+    double acc[3];
+    int done, maxeff;
+    float rz_prev, atol_sq;
+    double atol, tol;
+};
+
+template <int BS>
+__device__ __forceinline__ void pass_a_load(const PcgView& a, PassAIn& p) {
+    const CgCtl* c = a.ctl;
+    const bool first = a.k == 0;
+    p.acc[0] = p.acc[1] = p.acc[2] = 0.0;
+#pragma unroll 1
+    for (int b = threadIdx.x; b < a.nb_upd; b += BS) {
+        p.acc[0] += a.part_upd[b];
+        p.acc[1] += a.part_upd[a.nb_upd + b];
+        if (first) p.acc[2] += a.part_upd[2 * a.nb_upd + b];
+    }
+    p.done = c->done;
+    p.maxeff = c->maxeff;
+    p.rz_prev = c->slot_rz[(a.k + 1) & 1];
+    p.atol_sq = c->atol_sq;
+    p.atol = c->atol;
+    p.tol = c->tol;
+}
+
+// Prologue of every pass A: finish iteration k from pass B's partial sums (r.r, r.z; b.b too at
+// k = 0, where atol is formed), test convergence and the iteration cap, and derive beta.
+// Workgroup 0 of the scene records the state.  Returns true (in every thread) when the
+// workgroup has nothing to do: the solve stopped in an earlier launch, or stops now.
 template <int BS = kBlock>
-__device__ bool finish_iteration(const PcgView& a, double (*lds)[BS / 64], float& beta) {
+__device__ bool finish_iteration(const PcgView& a, const PassAIn& p, float& beta) {
     CgCtl* c = a.ctl;
     const int k = a.k;
-    float rr, rz, atol_sq;
-    double atol;
+    double tot[3];
+    if (reduce_bcast<3, BS>(p.acc, k == 0 ? 3 : 2, p.done, tot)) return true;
+    const float rr = (float)tot[0];
+    const float rz = (float)tot[1];
+    float atol_sq = p.atol_sq;
+    double atol = p.atol;
     if (k == 0) {
-        double tot[3];
-        sum_partials<3, BS>(a.part_upd, a.nb_upd, lds, tot);
-        rr = (float)tot[0];
-        rz = (float)tot[1];
         const float bb = (float)tot[2];
         // warp: atol = max(tol * ||b||, tol)
-        atol = c->tol * sqrt((double)bb);
-        if (atol < c->tol) atol = c->tol;
+        atol = p.tol * sqrt((double)bb);
+        if (atol < p.tol) atol = p.tol;
         atol_sq = (float)(atol * atol);
-    } else {
-        double tot[2];
-        sum_partials<2, BS>(a.part_upd, a.nb_upd, lds, tot);
-        rr = (float)tot[0];
-        rz = (float)tot[1];
-        atol = c->atol;
-        atol_sq = c->atol_sq;
     }
     const bool conv = rr <= atol_sq;
-    const bool stop = conv || k >= c->maxeff;
+    const bool stop = conv || k >= p.maxeff;
     // beta of iteration k (warp: rz_new / rz_old while rr > atol^2)
-    beta = (k == 0) ? 0.0f : ((rr > atol_sq) ? rz / c->slot_rz[(k + 1) & 1] : 0.0f);
+    beta = (k == 0) ? 0.0f : ((rr > atol_sq) ? rz / p.rz_prev : 0.0f);
     if (a.lb == 0 && threadIdx.x == 0) {
         c->slot_rr[k & 1] = rr;
         c->slot_rz[k & 1] = rz;
@@ -294,10 +311,9 @@ __device__ bool finish_iteration(const PcgView& a, double (*lds)[BS / 64], float
 }
 
 // r = b - A x ; z = D r ; partial sums r.r, r.z, b.b
-template <int OP>
-__global__ __launch_bounds__(kBlock) void k_pcg_init(const SceneDev* __restrict__ tab,
-                                                     const MapEntry* __restrict__ map, float dt) {
-    HFM_BATCH_ENTRY(tab, map);
+template <int OP, class Src>
+__global__ __launch_bounds__(kBlock) void k_pcg_init(Src src, float dt) {
+    HFM_ENTRY(src);
     __shared__ double lds[3][kWaves];
     const PcgView v = make_view<OP>(a, lb, 0, dt);
     double s[3] = {0.0, 0.0, 0.0};
@@ -334,17 +350,17 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(const SceneDev* __restrict_
 
 // pass A of iteration k+1: finish iteration k (r.r, r.z, convergence, beta), then the direction
 // update at the cell and its neighbours, SpMV and partial p.Ap
-template <int OP>
-__global__ __launch_bounds__(kBlock) void k_pcg_dir(const SceneDev* __restrict__ tab,
-                                                    const MapEntry* __restrict__ map, float dt, int k,
+template <int OP, class Src>
+__global__ __launch_bounds__(kBlock) void k_pcg_dir(Src src, float dt, int k,
                                                     unsigned long long* pace, unsigned tag) {
     post_pace(pace, tag, k);
-    HFM_BATCH_ENTRY(tab, map);
+    HFM_ENTRY(src);
     __shared__ double lds[3][kWaves];
     const PcgView v = make_view<OP>(a, lb, k, dt);
-    if (solve_stopped(v.ctl)) return;
+    PassAIn pre;
+    pass_a_load<kBlock>(v, pre);
     float beta;
-    if (finish_iteration(v, lds, beta)) return;
+    if (finish_iteration(v, pre, beta)) return;
     double s[1] = {0.0};
     const bool first = k == 0;
     FOR_CELLS(v, v.nb_dir) {
@@ -365,28 +381,25 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(const SceneDev* __restrict__
 }
 
 // Pass A for the pressure operator, LDS-tiled.  A workgroup of 1024 threads owns a 16 x 16 x D
-// block of cells.  It first forms the new direction p = z + beta p_old and loads the solid
-// fraction on the block framed by a one-cell halo ((D+2) x 18 x 18 cells, all loads in flight at
-// once) into LDS, then every thread applies the stencil to D/4 cells from LDS.  Per cell the
-// arithmetic is exactly that of k_pcg_dir<1>.
+// block of cells and stages it framed by a one-cell halo ((D+2) x 18 x 18 cells) in LDS: the
+// solid fraction, and once the prologue has formed beta, the new direction p = z + beta p_old.
+// For the staging, 972 threads each own one (j, k) position of the 18 x 18 frame plane and step
+// through every third plane, so a load costs an address increment; all loads are in flight
+// before the prologue, whose latency they overlap.  Every thread then applies the stencil to
+// D/4 cells from LDS.  Per cell the arithmetic is exactly that of k_pcg_dir<1>.
 constexpr int kT3 = 16, kF3 = kT3 + 2, kBlock3 = 1024;
+constexpr int kPlane3 = kF3 * kF3;            // 324 positions of a frame plane
+constexpr int kGroups3 = kBlock3 / kPlane3;   // 3 plane groups
 
-template <int D>
-__global__ __launch_bounds__(kBlock3) void k_pcg_dir_tile3(const SceneDev* __restrict__ tab,
-                                                           const MapEntry* __restrict__ map, int kit,
-                                                           unsigned long long* pace, unsigned tag) {
+template <int D, class Src>
+__global__ __launch_bounds__(kBlock3) void k_pcg_dir_tile3(Src src, int kit, unsigned long long* pace,
+                                                           unsigned tag) {
     post_pace(pace, tag, kit);
-    HFM_BATCH_ENTRY(tab, map);
+    HFM_ENTRY(src);
     __shared__ float P[D + 2][kF3][kF3];
     __shared__ float S[D + 2][kF3][kF3];
-    __shared__ double lds[3][kBlock3 / 64];
+    __shared__ double lds[1][kBlock3 / 64];
     const PcgView v = make_view<1>(a, lb, kit, 0.0f);
-    {
-        __shared__ int stopped;
-        if (threadIdx.x == 0) stopped = v.ctl->done;
-        __syncthreads();
-        if (stopped) return;
-    }
     const int nb = v.nb_dir;
     const int n = v.n;
     const int L = xcd_tile(lb, nb);
@@ -397,84 +410,83 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_dir_tile3(const SceneDev* __res
     const float* __restrict__ zs = v.z;
     const float* __restrict__ ds = v.d_old;
     const float* __restrict__ as = v.alpha_s;
-    // frame operands are loaded before the prologue (they do not depend on beta), so the
-    // latency of the partial-sum prologue overlaps theirs
-    constexpr int kFrame = (D + 2) * kF3 * kF3;
-    constexpr int kPer = (kFrame + kBlock3 - 1) / kBlock3;
-    float fz[kPer], fd[kPer], fa[kPer];
+    const unsigned nn = (unsigned)n * n;
+    // staging: thread -> (plane group, frame position); out-of-box frame cells load from a clamped
+    // in-box address and hold 0
+    constexpr int kM = (D + 2 + kGroups3 - 1) / kGroups3;
+    const int grp = threadIdx.x / kPlane3, pos = threadIdx.x % kPlane3;
+    const int fj = pos / kF3, fk = pos % kF3;
+    const int gj = j0 + fj - 1, gk = k0 + fk - 1;
+    const bool ok_jk = grp < kGroups3 && gj >= 0 && gj < n && gk >= 0 && gk < n;
+    const unsigned t_jk = (unsigned)clampi(gj, 0, n - 1) * n + clampi(gk, 0, n - 1);
+    float fz[kM], fd[kM], fa[kM];
+    bool ok[kM];
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-        const int f = threadIdx.x + u * kBlock3;
-        fz[u] = 0.0f;
-        fd[u] = 0.0f;
-        fa[u] = 0.0f;
-        if (f < kFrame) {
-            const int fi = f / (kF3 * kF3), fr = f % (kF3 * kF3);
-            const int fj = fr / kF3, fk = fr % kF3;
-            const int gi = i0 + fi - 1, gj = j0 + fj - 1, gk = k0 + fk - 1;
-            if (gi >= 0 && gi < n && gj >= 0 && gj < n && gk >= 0 && gk < n) {
-                const long t = ((long)gi * n + gj) * n + gk;
-                fz[u] = zs[t];
-                if (!first) fd[u] = ds[t];
-                fa[u] = as[t];
+    for (int m = 0; m < kM; ++m) {
+        const int fi = grp + kGroups3 * m, gi = i0 + fi - 1;
+        ok[m] = ok_jk && fi < D + 2 && gi >= 0 && gi < n;
+        const unsigned t = (unsigned)clampi(gi, 0, n - 1) * nn + t_jk;
+        fz[m] = zs[t];
+        fd[m] = first ? 0.0f : ds[t];
+        fa[m] = as[t];
+    }
+    PassAIn pre;
+    pass_a_load<kBlock3>(v, pre);
+    float beta;
+    if (finish_iteration<kBlock3>(v, pre, beta)) return;
+    if (grp < kGroups3) {
+#pragma unroll
+        for (int m = 0; m < kM; ++m) {
+            const int fi = grp + kGroups3 * m;
+            if (fi < D + 2) {
+                P[fi][fj][fk] = ok[m] ? (first ? fz[m] : fz[m] + beta * fd[m]) : 0.0f;
+                S[fi][fj][fk] = ok[m] ? fa[m] : 0.0f;
             }
         }
     }
-    float beta;
-    if (finish_iteration<kBlock3>(v, lds, beta)) return;
-#pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-        const int f = threadIdx.x + u * kBlock3;
-        if (f < kFrame) {
-            const int fi = f / (kF3 * kF3), fr = f % (kF3 * kF3);
-            const int fj = fr / kF3, fk = fr % kF3;
-            P[fi][fj][fk] = first ? fz[u] : fz[u] + beta * fd[u];
-            S[fi][fj][fk] = fa[u];
-        }
-    }
     __syncthreads();
-    const float amin = v.amin;
+    const float amin = v.amin, hh = v.hh, rhh = a.rhh;
+    const int fastdiv = a.fastdiv;
     const int lk = threadIdx.x % kT3, lj = (threadIdx.x / kT3) % kT3, lq = threadIdx.x / (kT3 * kT3);
     const int j = j0 + lj, k = k0 + lk;
-    double s[1] = {0.0};
+    double sum[1] = {0.0};
     if (j < n && k < n) {
-        const int fj = lj + 1, fk = lk + 1;
+        const int sj = lj + 1, sk = lk + 1;
+        const unsigned tb = ((unsigned)i0 * n + j) * n + k;
 #pragma unroll
         for (int m = 0; m < D / 4; ++m) {
             const int li = lq + 4 * m, i = i0 + li;
             if (i >= n) break;
-            const int fi = li + 1;
-            const float pc = P[fi][fj][fk], ac = S[fi][fj][fk];
+            const int si = li + 1;
+            const float pc = P[si][sj][sk], ac = S[si][sj][sk];
             float acc = 0.0f;
-            if (i > 0) acc += face_fraction(S[fi - 1][fj][fk], ac, amin) * (pc - P[fi - 1][fj][fk]);
-            if (i < n - 1) acc += face_fraction(ac, S[fi + 1][fj][fk], amin) * (pc - P[fi + 1][fj][fk]);
-            if (j > 0) acc += face_fraction(S[fi][fj - 1][fk], ac, amin) * (pc - P[fi][fj - 1][fk]);
-            if (j < n - 1) acc += face_fraction(ac, S[fi][fj + 1][fk], amin) * (pc - P[fi][fj + 1][fk]);
-            if (k > 0) acc += face_fraction(S[fi][fj][fk - 1], ac, amin) * (pc - P[fi][fj][fk - 1]);
-            if (k < n - 1) acc += face_fraction(ac, S[fi][fj][fk + 1], amin) * (pc - P[fi][fj][fk + 1]);
-            const float apc = 1.0f * acc / v.hh;
-            const long t = ((long)i * n + j) * n + k;
+            if (i > 0) acc += face_fraction(S[si - 1][sj][sk], ac, amin) * (pc - P[si - 1][sj][sk]);
+            if (i < n - 1) acc += face_fraction(ac, S[si + 1][sj][sk], amin) * (pc - P[si + 1][sj][sk]);
+            if (j > 0) acc += face_fraction(S[si][sj - 1][sk], ac, amin) * (pc - P[si][sj - 1][sk]);
+            if (j < n - 1) acc += face_fraction(ac, S[si][sj + 1][sk], amin) * (pc - P[si][sj + 1][sk]);
+            if (k > 0) acc += face_fraction(S[si][sj][sk - 1], ac, amin) * (pc - P[si][sj][sk - 1]);
+            if (k < n - 1) acc += face_fraction(ac, S[si][sj][sk + 1], amin) * (pc - P[si][sj][sk + 1]);
+            const float apc = div_const(1.0f * acc, hh, rhh, fastdiv);
+            const unsigned t = tb + (unsigned)li * nn;
             v.d_new[t] = pc;
             v.Ap[t] = apc;
-            s[0] += (double)(pc * apc);
+            sum[0] += (double)(pc * apc);
         }
     }
-    store_partials<1, kBlock3>(s, v.part_dir, nb, lb, lds);
+    store_partials<1, kBlock3>(sum, v.part_dir, nb, lb, lds);
 }
 
 // pass B of iteration k+1: alpha from p.Ap, x and r update, z = D r, partial r.r and r.z
-template <int OP>
-__global__ __launch_bounds__(kBlock) void k_pcg_upd(const SceneDev* __restrict__ tab,
-                                                    const MapEntry* __restrict__ map, int kit) {
-    HFM_BATCH_ENTRY(tab, map);
+template <int OP, class Src>
+__global__ __launch_bounds__(kBlock) void k_pcg_upd(Src src, int kit) {
+    HFM_ENTRY(src);
     if (OP == 1 && a.cg_fused) return;  // the fused form has no pass B
     __shared__ double lds[2][kWaves];
     const PcgView v = make_view<OP>(a, lb, kit, 0.0f);
-    CgCtl* c = v.ctl;
-    if (solve_stopped(c)) return;
+    const CgCtl* c = v.ctl;
     const int L = xcd_tile(lb, v.nb_upd);
-    // the operands of the first kPre tiles are loaded before the p.Ap prologue, so its latency
-    // overlaps theirs
+    // the operands of the first kPre tiles, the partial sums of pass A and the scalars are all
+    // loaded before the p.Ap reduction, so their latencies overlap
     constexpr int kPre = 4;
     float px[kPre], pd[kPre], pr[kPre], pa[kPre], pD[kPre];
 #pragma unroll
@@ -488,11 +500,15 @@ __global__ __launch_bounds__(kBlock) void k_pcg_upd(const SceneDev* __restrict__
             pD[u] = v.D[t];
         }
     }
+    double acc[1] = {0.0};
+#pragma unroll 1
+    for (int b = threadIdx.x; b < v.nb_dir; b += kBlock) acc[0] += v.part_dir[b];
+    const int done = c->done;
+    const float rr = c->slot_rr[kit & 1], rz = c->slot_rz[kit & 1], atol_sq = c->atol_sq;
     double tot[1];
-    sum_partials<1>(v.part_dir, v.nb_dir, lds, tot);
+    if (reduce_bcast<1, kBlock>(acc, 1, done, tot)) return;
     const float pAp = (float)tot[0];
-    const float rr = c->slot_rr[kit & 1], rz = c->slot_rz[kit & 1];
-    const float alpha = (rr > c->atol_sq) ? rz / pAp : 0.0f;
+    const float alpha = (rr > atol_sq) ? rz / pAp : 0.0f;
     double s[2] = {0.0, 0.0};
 #pragma unroll
     for (int u = 0; u < kPre; ++u) {
@@ -543,23 +559,16 @@ __global__ __launch_bounds__(kBlock) void k_pcg_upd(const SceneDev* __restrict__
 // ---------------------------------------------------------------------------------------------
 constexpr int kF2 = kT3 + 4;
 
-template <int D>
-__global__ __launch_bounds__(kBlock3) void k_pcg_fused(const SceneDev* __restrict__ tab,
-                                                       const MapEntry* __restrict__ map, int kit,
-                                                       unsigned long long* pace, unsigned tag) {
+template <int D, class Src>
+__global__ __launch_bounds__(kBlock3) void k_pcg_fused(Src src, int kit, unsigned long long* pace, unsigned tag) {
     post_pace(pace, tag, kit);
-    HFM_BATCH_ENTRY(tab, map);
-    __shared__ float P2[D + 4][kF2][kF2];
-    __shared__ float S2[D + 4][kF2][kF2];
-    __shared__ float Z1[D + 2][kF3][kF3];
+    HFM_ENTRY(src);
+    __shared__ float P2[D + 4][kF2][kF2];   // z_k, then p_k (block + 2 halo)
+    __shared__ float S2[D + 4][kF2][kF2];   // alpha_s
+    __shared__ float Z1[D + 2][kF3][kF3];   // z_{k+1} (block + 1 halo)
+    __shared__ float A1[D + 2][kF3][kF3];   // A p_k (block + 1 halo)
     __shared__ double lds[kFusedRows][kBlock3 / 64];
     CgCtl* c = a.ctl + 1;
-    {
-        __shared__ int stopped;
-        if (threadIdx.x == 0) stopped = c->done;
-        __syncthreads();
-        if (stopped) return;
-    }
     const int n = a.n;
     const long N = a.N;
     const int nb = a.nb_dir;
@@ -577,79 +586,75 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_fused(const SceneDev* __restric
     float* __restrict__ x = a.p;
     const double* part_in = a.partials + (long)(kit & 1) * kFusedRows * kMaxDirBlocks;
     double* part_out = a.partials + (long)((kit + 1) & 1) * kFusedRows * kMaxDirBlocks;
-    // ---- operand loads, issued before the prologue (they do not depend on its scalars)
-    constexpr int kFr2 = (D + 4) * kF2 * kF2;
-    constexpr int kP2 = (kFr2 + kBlock3 - 1) / kBlock3;
-    float f_r[kP2], f_p[kP2], f_d[kP2], f_a[kP2];
+    const unsigned nn = (unsigned)n * n;
+    // ---- loads, all issued before the prologue: z_k = D r_k, p_{k-1} and alpha_s on block + 2
+    // halo (each of 800 threads owns one (j, k) position of the 20 x 20 plane and steps through
+    // planes), r_k and D on block + 1 halo, the partial sums of launch k-1 and the scalars
+    constexpr int kPl2 = kF2 * kF2, kG2 = kBlock3 / kPl2;   // 400 positions, 2 plane groups
+    constexpr int kM2 = (D + 4 + kG2 - 1) / kG2;
+    const int grp2 = threadIdx.x / kPl2, pos2 = threadIdx.x % kPl2;
+    const int fj2 = pos2 / kF2, fk2 = pos2 % kF2;
+    const int gj2 = j0 + fj2 - 2, gk2 = k0 + fk2 - 2;
+    const bool okjk2 = grp2 < kG2 && gj2 >= 0 && gj2 < n && gk2 >= 0 && gk2 < n;
+    const unsigned tjk2 = (unsigned)clampi(gj2, 0, n - 1) * n + clampi(gk2, 0, n - 1);
+    float f_z[kM2], f_q[kM2], f_a[kM2];
+    bool ok2[kM2];
 #pragma unroll
-    for (int u = 0; u < kP2; ++u) {
-        const int f = threadIdx.x + u * kBlock3;
-        f_r[u] = 0.0f;
-        f_p[u] = 0.0f;
-        f_d[u] = 0.0f;
-        f_a[u] = 0.0f;
-        if (f < kFr2) {
-            const int fi = f / (kF2 * kF2), fr = f % (kF2 * kF2);
-            const int gi = i0 + fi - 2, gj = j0 + fr / kF2 - 2, gk = k0 + fr % kF2 - 2;
-            if (gi >= 0 && gi < n && gj >= 0 && gj < n && gk >= 0 && gk < n) {
-                const long t = ((long)gi * n + gj) * n + gk;
-                f_r[u] = rk[t];
-                if (!first) f_p[u] = pold[t];
-                f_d[u] = Dg[t];
-                f_a[u] = as[t];
-            }
-        }
+    for (int m = 0; m < kM2; ++m) {
+        const int fi = grp2 + kG2 * m, gi = i0 + fi - 2;
+        ok2[m] = okjk2 && fi < D + 4 && gi >= 0 && gi < n;
+        const unsigned t = (unsigned)clampi(gi, 0, n - 1) * nn + tjk2;
+        f_z[m] = Dg[t] * rk[t];
+        f_q[m] = first ? 0.0f : pold[t];
+        f_a[m] = as[t];
     }
-    constexpr int kFr1 = (D + 2) * kF3 * kF3;
-    constexpr int kP1 = (kFr1 + kBlock3 - 1) / kBlock3;
-    float g_r[kP1], g_d[kP1], g_x[kP1];
+    constexpr int kPl1 = kF3 * kF3, kG1 = kBlock3 / kPl1;   // 324 positions, 3 plane groups
+    constexpr int kM1 = (D + 2 + kG1 - 1) / kG1;
+    const int grp1 = threadIdx.x / kPl1, pos1 = threadIdx.x % kPl1;
+    const int fj1 = pos1 / kF3, fk1 = pos1 % kF3;
+    const int j1 = j0 + fj1 - 1, k1 = k0 + fk1 - 1;
+    const bool okjk1 = grp1 < kG1 && j1 >= 0 && j1 < n && k1 >= 0 && k1 < n;
+    const unsigned tjk1 = (unsigned)clampi(j1, 0, n - 1) * n + clampi(k1, 0, n - 1);
+    float g_r[kM1], g_d[kM1];
 #pragma unroll
-    for (int u = 0; u < kP1; ++u) {
-        const int f = threadIdx.x + u * kBlock3;
-        g_r[u] = 0.0f;
-        g_d[u] = 0.0f;
-        g_x[u] = 0.0f;
-        if (f < kFr1) {
-            const int fi = f / (kF3 * kF3), fr = f % (kF3 * kF3);
-            const int fj = fr / kF3, fk = fr % kF3;
-            const int gi = i0 + fi - 1, gj = j0 + fj - 1, gk = k0 + fk - 1;
-            if (gi >= 0 && gi < n && gj >= 0 && gj < n && gk >= 0 && gk < n) {
-                const long t = ((long)gi * n + gj) * n + gk;
-                g_r[u] = rk[t];
-                g_d[u] = Dg[t];
-                if (fi >= 1 && fi <= D && fj >= 1 && fj <= kT3 && fk >= 1 && fk <= kT3) g_x[u] = x[t];
-            }
-        }
+    for (int m = 0; m < kM1; ++m) {
+        const int fi = grp1 + kG1 * m, gi = i0 + fi - 1;
+        const unsigned t = (unsigned)clampi(gi, 0, n - 1) * nn + tjk1;
+        g_r[m] = rk[t];  // out-of-box entries are never used
+        g_d[m] = Dg[t];
     }
-    // ---- prologue: scalars of iteration k from launch k-1's partial sums
-    float rr, rz, atol_sq;
-    double zaz, zap, atol;
+    double acc[kFusedRows] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    {
+        const int rows = first ? kFusedRows : 4;
+        const int nrows = first ? a.nb_upd : nb;  // the init launch wrote rows of its own count
+#pragma unroll 1
+        for (int b = threadIdx.x; b < nrows; b += kBlock3)
+#pragma unroll
+            for (int q = 0; q < kFusedRows; ++q)
+                if (q < rows) acc[q] += part_in[q * nrows + b];
+    }
+    const int done = c->done, maxeff = c->maxeff;
+    const float rz_prev = c->slot_rz[(kit + 1) & 1];
+    const double pap_prev = c->slot_pap[(kit + 1) & 1];
+    const double tol = c->tol;
+    double atol = c->atol;
+    float atol_sq = c->atol_sq;
+    // ---- prologue: scalars of iteration k
+    double tot[kFusedRows];
+    if (reduce_bcast<kFusedRows, kBlock3>(acc, first ? kFusedRows : 4, done, tot)) return;
+    const float rr = (float)tot[0];
+    const float rz = (float)tot[1];
+    const double zaz = tot[2], zap = tot[3];
     if (first) {
-        double tot[5];
-        sum_partials<5, kBlock3>(part_in, a.nb_upd, lds, tot);  // rows written by the init launch
-        rr = (float)tot[0];
-        rz = (float)tot[1];
-        zaz = tot[2];
-        zap = tot[3];
         const float bb = (float)tot[4];
-        atol = c->tol * sqrt((double)bb);  // warp: atol = max(tol * ||b||, tol)
-        if (atol < c->tol) atol = c->tol;
+        atol = tol * sqrt((double)bb);  // warp: atol = max(tol * ||b||, tol)
+        if (atol < tol) atol = tol;
         atol_sq = (float)(atol * atol);
-    } else {
-        double tot[4];
-        sum_partials<4, kBlock3>(part_in, nb, lds, tot);
-        rr = (float)tot[0];
-        rz = (float)tot[1];
-        zaz = tot[2];
-        zap = tot[3];
-        atol = c->atol;
-        atol_sq = c->atol_sq;
     }
     const bool conv = rr <= atol_sq;
-    const bool stop = conv || kit >= c->maxeff;
-    const float beta = first ? 0.0f : (rr > atol_sq ? rz / c->slot_rz[(kit + 1) & 1] : 0.0f);
-    const double pap = first ? zaz
-                             : zaz + 2.0 * (double)beta * zap + (double)beta * (double)beta * c->slot_pap[(kit + 1) & 1];
+    const bool stop = conv || kit >= maxeff;
+    const float beta = first ? 0.0f : (rr > atol_sq ? rz / rz_prev : 0.0f);
+    const double pap = first ? zaz : zaz + 2.0 * (double)beta * zap + (double)beta * (double)beta * pap_prev;
     const float alpha = rr > atol_sq ? rz / (float)pap : 0.0f;
     if (lb == 0 && threadIdx.x == 0) {
         c->slot_rr[kit & 1] = rr;
@@ -667,85 +672,81 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_fused(const SceneDev* __restric
         }
     }
     if (stop) return;
-    // ---- stage 1: p_k and alpha_s on block + 2 halo
+    // ---- stage 1: p_k = z_k + beta p_{k-1} and alpha_s on block + 2 halo
+    if (grp2 < kG2) {
 #pragma unroll
-    for (int u = 0; u < kP2; ++u) {
-        const int f = threadIdx.x + u * kBlock3;
-        if (f < kFr2) {
-            const int fi = f / (kF2 * kF2), fr = f % (kF2 * kF2);
-            const float z = f_d[u] * f_r[u];
-            P2[fi][fr / kF2][fr % kF2] = first ? z : z + beta * f_p[u];
-            S2[fi][fr / kF2][fr % kF2] = f_a[u];
+        for (int m = 0; m < kM2; ++m) {
+            const int fi = grp2 + kG2 * m;
+            if (fi < D + 4) {
+                P2[fi][fj2][fk2] = ok2[m] ? (first ? f_z[m] : f_z[m] + beta * f_q[m]) : 0.0f;
+                S2[fi][fj2][fk2] = ok2[m] ? f_a[m] : 0.0f;
+            }
         }
     }
     __syncthreads();
     const float amin = a.amin;
-    const float hh = a.hh;
+    const float hh = a.hh, rhh = a.rhh;
+    const int fastdiv = a.fastdiv;
     // ---- stage 2: A p_k, r_{k+1}, z_{k+1} on block + 1 halo
-    float k_ap[kP1], k_rn[kP1], k_zn[kP1];
+    if (grp1 < kG1) {
 #pragma unroll
-    for (int u = 0; u < kP1; ++u) {
-        const int f = threadIdx.x + u * kBlock3;
-        k_ap[u] = 0.0f;
-        k_rn[u] = 0.0f;
-        k_zn[u] = 0.0f;
-        if (f < kFr1) {
-            const int fi = f / (kF3 * kF3), fr = f % (kF3 * kF3);
-            const int fj = fr / kF3, fk = fr % kF3;
-            const int i = i0 + fi - 1, j = j0 + fj - 1, k = k0 + fk - 1;
-            float zn = 0.0f;
-            if (i >= 0 && i < n && j >= 0 && j < n && k >= 0 && k < n) {
-                const int ci = fi + 1, cj = fj + 1, ck = fk + 1;
-                const float pc = P2[ci][cj][ck], ac = S2[ci][cj][ck];
-                float acc = 0.0f;
-                if (i > 0) acc += face_fraction(S2[ci - 1][cj][ck], ac, amin) * (pc - P2[ci - 1][cj][ck]);
-                if (i < n - 1) acc += face_fraction(ac, S2[ci + 1][cj][ck], amin) * (pc - P2[ci + 1][cj][ck]);
-                if (j > 0) acc += face_fraction(S2[ci][cj - 1][ck], ac, amin) * (pc - P2[ci][cj - 1][ck]);
-                if (j < n - 1) acc += face_fraction(ac, S2[ci][cj + 1][ck], amin) * (pc - P2[ci][cj + 1][ck]);
-                if (k > 0) acc += face_fraction(S2[ci][cj][ck - 1], ac, amin) * (pc - P2[ci][cj][ck - 1]);
-                if (k < n - 1) acc += face_fraction(ac, S2[ci][cj][ck + 1], amin) * (pc - P2[ci][cj][ck + 1]);
-                const float ap = 1.0f * acc / hh;
-                const float rnew = g_r[u] - alpha * ap;
-                zn = g_d[u] * rnew;
-                k_ap[u] = ap;
-                k_rn[u] = rnew;
-                k_zn[u] = zn;
+        for (int m = 0; m < kM1; ++m) {
+            const int fi = grp1 + kG1 * m;
+            if (fi < D + 2) {
+                const int i = i0 + fi - 1;
+                float zn = 0.0f, ap = 0.0f;
+                if (okjk1 && i >= 0 && i < n) {
+                    const int ci = fi + 1, cj = fj1 + 1, ck = fk1 + 1;
+                    const float pc = P2[ci][cj][ck], ac = S2[ci][cj][ck];
+                    float sacc = 0.0f;
+                    if (i > 0) sacc += face_fraction(S2[ci - 1][cj][ck], ac, amin) * (pc - P2[ci - 1][cj][ck]);
+                    if (i < n - 1) sacc += face_fraction(ac, S2[ci + 1][cj][ck], amin) * (pc - P2[ci + 1][cj][ck]);
+                    if (j1 > 0) sacc += face_fraction(S2[ci][cj - 1][ck], ac, amin) * (pc - P2[ci][cj - 1][ck]);
+                    if (j1 < n - 1) sacc += face_fraction(ac, S2[ci][cj + 1][ck], amin) * (pc - P2[ci][cj + 1][ck]);
+                    if (k1 > 0) sacc += face_fraction(S2[ci][cj][ck - 1], ac, amin) * (pc - P2[ci][cj][ck - 1]);
+                    if (k1 < n - 1) sacc += face_fraction(ac, S2[ci][cj][ck + 1], amin) * (pc - P2[ci][cj][ck + 1]);
+                    ap = div_const(1.0f * sacc, hh, rhh, fastdiv);
+                    zn = g_d[m] * (g_r[m] - alpha * ap);
+                }
+                Z1[fi][fj1][fk1] = zn;
+                A1[fi][fj1][fk1] = ap;
             }
-            Z1[fi][fj][fk] = zn;
         }
     }
     __syncthreads();
-    // ---- stage 3: x_{k+1}, A z_{k+1} and the partial sums on the block
+    // ---- stage 3: on the block, r_{k+1}, x_{k+1}, A z_{k+1} and the partial sums
+    const int lk = threadIdx.x % kT3, lj = (threadIdx.x / kT3) % kT3, lq = threadIdx.x / (kT3 * kT3);
+    const int j = j0 + lj, k = k0 + lk;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (j < n && k < n) {
+        const int sj = lj + 1, sk = lk + 1;
+        const unsigned tb = ((unsigned)i0 * n + j) * n + k;
 #pragma unroll
-    for (int u = 0; u < kP1; ++u) {
-        const int f = threadIdx.x + u * kBlock3;
-        if (f < kFr1) {
-            const int fi = f / (kF3 * kF3), fr = f % (kF3 * kF3);
-            const int fj = fr / kF3, fk = fr % kF3;
-            const int i = i0 + fi - 1, j = j0 + fj - 1, k = k0 + fk - 1;
-            const bool inner = fi >= 1 && fi <= D && fj >= 1 && fj <= kT3 && fk >= 1 && fk <= kT3;
-            if (inner && i < n && j < n && k < n) {
-                const int ci = fi + 1, cj = fj + 1, ck = fk + 1;
-                const float zc = Z1[fi][fj][fk], ac = S2[ci][cj][ck];
-                float acc = 0.0f;
-                if (i > 0) acc += face_fraction(S2[ci - 1][cj][ck], ac, amin) * (zc - Z1[fi - 1][fj][fk]);
-                if (i < n - 1) acc += face_fraction(ac, S2[ci + 1][cj][ck], amin) * (zc - Z1[fi + 1][fj][fk]);
-                if (j > 0) acc += face_fraction(S2[ci][cj - 1][ck], ac, amin) * (zc - Z1[fi][fj - 1][fk]);
-                if (j < n - 1) acc += face_fraction(ac, S2[ci][cj + 1][ck], amin) * (zc - Z1[fi][fj + 1][fk]);
-                if (k > 0) acc += face_fraction(S2[ci][cj][ck - 1], ac, amin) * (zc - Z1[fi][fj][fk - 1]);
-                if (k < n - 1) acc += face_fraction(ac, S2[ci][cj][ck + 1], amin) * (zc - Z1[fi][fj][fk + 1]);
-                const float az = 1.0f * acc / hh;
-                const float pc = P2[ci][cj][ck];
-                const long t = ((long)i * n + j) * n + k;
-                x[t] = g_x[u] + alpha * pc;
-                pk[t] = pc;
-                rn[t] = k_rn[u];
-                v[0] += (double)(k_rn[u] * k_rn[u]);
-                v[1] += (double)(k_rn[u] * k_zn[u]);
-                v[2] += (double)(k_zn[u] * az);
-                v[3] += (double)(k_zn[u] * k_ap[u]);
-            }
+        for (int m = 0; m < D / 4; ++m) {
+            const int li = lq + 4 * m, i = i0 + li;
+            if (i >= n) break;
+            const int si = li + 1;
+            const unsigned t = tb + (unsigned)li * nn;
+            const float xo = x[t], ro = rk[t];
+            const float ap = A1[si][sj][sk];
+            const float rnew = ro - alpha * ap;
+            const float zc = Z1[si][sj][sk], ac = S2[si + 1][sj + 1][sk + 1];
+            float sacc = 0.0f;
+            if (i > 0) sacc += face_fraction(S2[si][sj + 1][sk + 1], ac, amin) * (zc - Z1[si - 1][sj][sk]);
+            if (i < n - 1) sacc += face_fraction(ac, S2[si + 2][sj + 1][sk + 1], amin) * (zc - Z1[si + 1][sj][sk]);
+            if (j > 0) sacc += face_fraction(S2[si + 1][sj][sk + 1], ac, amin) * (zc - Z1[si][sj - 1][sk]);
+            if (j < n - 1) sacc += face_fraction(ac, S2[si + 1][sj + 2][sk + 1], amin) * (zc - Z1[si][sj + 1][sk]);
+            if (k > 0) sacc += face_fraction(S2[si + 1][sj + 1][sk], ac, amin) * (zc - Z1[si][sj][sk - 1]);
+            if (k < n - 1) sacc += face_fraction(ac, S2[si + 1][sj + 1][sk + 2], amin) * (zc - Z1[si][sj][sk + 1]);
+            const float az = div_const(1.0f * sacc, hh, rhh, fastdiv);
+            const float pc = P2[si + 1][sj + 1][sk + 1];
+            x[t] = xo + alpha * pc;
+            pk[t] = pc;
+            rn[t] = rnew;
+            v[0] += (double)(rnew * rnew);
+            v[1] += (double)(rnew * zc);
+            v[2] += (double)(zc * az);
+            v[3] += (double)(zc * ap);
         }
     }
     store_partials<4, kBlock3>(v, part_out, nb, lb, lds);
@@ -753,9 +754,9 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_fused(const SceneDev* __restric
 
 // Scalars of the first fused iteration from r_0 (written by k_pcg_init<1>): r.r, r.z, z.Az,
 // z.Ap_{-1} = 0 and b.b, into the partial-sum buffer launch 0 reads.
-__global__ __launch_bounds__(kBlock) void k_pcg_init_fused(const SceneDev* __restrict__ tab,
-                                                           const MapEntry* __restrict__ map) {
-    HFM_BATCH_ENTRY(tab, map);
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_pcg_init_fused(Src src) {
+    HFM_ENTRY(src);
     if (!a.cg_fused) return;
     __shared__ double lds[kFusedRows][kWaves];
     const PcgView v = make_view<1>(a, lb, 0, 0.0f);
@@ -777,26 +778,17 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init_fused(const SceneDev* __res
     store_partials<kFusedRows>(s, a.partials, v.nb_upd, lb, lds);
 }
 
-// With ev0/ev1 the launch carries its own start/stop events (hipExtLaunchKernelGGL): the events
-// time the dispatch itself, with no separate event packets around it.
-template <class K, class... Args>
-void launch_timed(K kernel, int blocks, int threads, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1,
-                  Args... args) {
-    if (blocks <= 0) return;
-    if (ev0)
-        hipExtLaunchKernelGGL(kernel, dim3(blocks), dim3(threads), 0, st, ev0, ev1, 0, args...);
-    else
-        hipLaunchKernelGGL(kernel, dim3(blocks), dim3(threads), 0, st, args...);
-}
-
 }  // namespace
 
 // PCG geometry of a scene.  Init, pass B and the flat pass A cover cpt consecutive tiles of
 // kBlock cells per workgroup: at least the scene's pcg_cpt (HFM_PCG_CPT), and enough that at most
 // kMaxPcgBlocks workgroups run.  The pressure pass A is LDS-tiled (unless HFM_PCG_FLAT_DIR) with
-// planes D = 8, else 16, else 4, the first that keeps at most kMaxDirBlocks workgroups
-// (HFM_DIR_ICHUNK forces D); otherwise it is the flat kernel with the pass B geometry.
-void pcg_geometry(const Scene* s, int* cpt, int* nb_upd, int* nb_dir, int* tiles_j, int* tiles_k, int* dtile) {
+// planes D = 16, else 8, else 4, the first that keeps at most kMaxDirBlocks workgroups
+// (HFM_DIR_ICHUNK forces D; the fused form takes 8 or 4); otherwise it is the flat kernel with
+// the pass B geometry.  Deeper
+// blocks re-read less halo and launch fewer, longer workgroups, which measured fastest.
+void pcg_geometry(const Scene* s, bool fused, int* cpt, int* nb_upd, int* nb_dir, int* tiles_j, int* tiles_k,
+                  int* dtile) {
     const long tiles = (s->N + kBlock - 1) / kBlock;
     const long need = (tiles + kMaxPcgBlocks - 1) / kMaxPcgBlocks;
     const int c0 = s->pcg_cpt > 0 ? s->pcg_cpt : 1;
@@ -807,8 +799,8 @@ void pcg_geometry(const Scene* s, int* cpt, int* nb_upd, int* nb_dir, int* tiles
     const int tj = (n + kT3 - 1) / kT3, tk = tj;
     int D = 0;
     if (!s->flat_dir) {
-        const int cand[3] = {8, 16, 4};
-        for (int q = 0; q < 3 && !D; ++q) {
+        const int cand[3] = {16, 8, 4};
+        for (int q = fused ? 1 : 0; q < 3 && !D; ++q) {  // the fused iteration is built for 4 and 8 planes
             const int d = cand[q];
             if (s->dir_ichunk > 0 && d != s->dir_ichunk) continue;
             const long blocks = (long)tj * tk * ((n + d - 1) / d);
@@ -828,57 +820,42 @@ void pcg_geometry(const Scene* s, int* cpt, int* nb_upd, int* nb_dir, int* tiles
 
 void launch_pcg_init(const Launch& L, int op, float dt) {
     if (op == 1)
-        launch_timed(k_pcg_init<1>, L.blocks, kBlock, L.stream, nullptr, nullptr, L.tab, L.map, dt);
+        HFM_LAUNCH_T(k_pcg_init, 1, L, kBlock, nullptr, nullptr, dt);
     else
-        launch_timed(k_pcg_init<0>, L.blocks, kBlock, L.stream, nullptr, nullptr, L.tab, L.map, dt);
+        HFM_LAUNCH_T(k_pcg_init, 0, L, kBlock, nullptr, nullptr, dt);
 }
 
 void launch_pcg_dir(const Launch& L, Family fam, int op, float dt, int k, unsigned long long* pace, unsigned tag,
                     hipEvent_t ev0, hipEvent_t ev1) {
     switch (fam) {
-        case FAM_DIR4:
-            launch_timed(k_pcg_dir_tile3<4>, L.blocks, kBlock3, L.stream, ev0, ev1, L.tab, L.map, k, pace, tag);
-            break;
-        case FAM_DIR8:
-            launch_timed(k_pcg_dir_tile3<8>, L.blocks, kBlock3, L.stream, ev0, ev1, L.tab, L.map, k, pace, tag);
-            break;
-        case FAM_DIR16:
-            launch_timed(k_pcg_dir_tile3<16>, L.blocks, kBlock3, L.stream, ev0, ev1, L.tab, L.map, k, pace, tag);
-            break;
+        case FAM_DIR4: HFM_LAUNCH_T(k_pcg_dir_tile3, 4, L, kBlock3, ev0, ev1, k, pace, tag); break;
+        case FAM_DIR8: HFM_LAUNCH_T(k_pcg_dir_tile3, 8, L, kBlock3, ev0, ev1, k, pace, tag); break;
+        case FAM_DIR16: HFM_LAUNCH_T(k_pcg_dir_tile3, 16, L, kBlock3, ev0, ev1, k, pace, tag); break;
         default:
             if (op == 1)
-                launch_timed(k_pcg_dir<1>, L.blocks, kBlock, L.stream, ev0, ev1, L.tab, L.map, dt, k, pace, tag);
+                HFM_LAUNCH_T(k_pcg_dir, 1, L, kBlock, ev0, ev1, dt, k, pace, tag);
             else
-                launch_timed(k_pcg_dir<0>, L.blocks, kBlock, L.stream, ev0, ev1, L.tab, L.map, dt, k, pace, tag);
+                HFM_LAUNCH_T(k_pcg_dir, 0, L, kBlock, ev0, ev1, dt, k, pace, tag);
     }
 }
 
-void launch_pcg_init_fused(const Launch& L) {
-    launch_timed(k_pcg_init_fused, L.blocks, kBlock, L.stream, nullptr, nullptr, L.tab, L.map);
-}
+void launch_pcg_init_fused(const Launch& L) { HFM_LAUNCH(k_pcg_init_fused, L, kBlock, nullptr, nullptr); }
 
 void launch_pcg_fused(const Launch& L, Family fam, int k, unsigned long long* pace, unsigned tag, hipEvent_t ev0,
                       hipEvent_t ev1) {
     switch (fam) {
-        case FAM_FUSED4:
-            launch_timed(k_pcg_fused<4>, L.blocks, kBlock3, L.stream, ev0, ev1, L.tab, L.map, k, pace, tag);
-            break;
-        case FAM_FUSED8:
-            launch_timed(k_pcg_fused<8>, L.blocks, kBlock3, L.stream, ev0, ev1, L.tab, L.map, k, pace, tag);
-            break;
-        case FAM_FUSED16:
-            launch_timed(k_pcg_fused<16>, L.blocks, kBlock3, L.stream, ev0, ev1, L.tab, L.map, k, pace, tag);
-            break;
-        default:
-            break;
+        case FAM_FUSED4: HFM_LAUNCH_T(k_pcg_fused, 4, L, kBlock3, ev0, ev1, k, pace, tag); break;
+        case FAM_FUSED8: HFM_LAUNCH_T(k_pcg_fused, 8, L, kBlock3, ev0, ev1, k, pace, tag); break;
+        case FAM_FUSED16: HFM_LAUNCH_T(k_pcg_fused, 16, L, kBlock3, ev0, ev1, k, pace, tag); break;
+        default: break;
     }
 }
 
 void launch_pcg_upd(const Launch& L, int op, int k, hipEvent_t ev0, hipEvent_t ev1) {
     if (op == 1)
-        launch_timed(k_pcg_upd<1>, L.blocks, kBlock, L.stream, ev0, ev1, L.tab, L.map, k);
+        HFM_LAUNCH_T(k_pcg_upd, 1, L, kBlock, ev0, ev1, k);
     else
-        launch_timed(k_pcg_upd<0>, L.blocks, kBlock, L.stream, ev0, ev1, L.tab, L.map, k);
+        HFM_LAUNCH_T(k_pcg_upd, 0, L, kBlock, ev0, ev1, k);
 }
 
 }  // namespace hfm

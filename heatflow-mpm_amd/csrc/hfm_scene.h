@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <hip/hip_ext.h>
+
 #include <vector>
 
 #include "../../include/hfm.h"
@@ -46,36 +48,61 @@ struct CgMirror {
     double mean;
 };
 
+// Pointer members of the per-scene descriptor.  The kernels read the descriptor from device
+// memory, so the compiler cannot see where its pointers point; in the device pass the kernels'
+// view of it types them as global memory, which makes every access a global (not flat)
+// load/store with a scalar base.  Host code and the host pass see plain pointers.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define HFM_GLOBAL __attribute__((address_space(1)))
+#else
+#define HFM_GLOBAL
+#endif
+struct HostPtrs {
+    template <class T>
+    using p = T*;
+};
+struct DevPtrs {
+    template <class T>
+    using p = HFM_GLOBAL T*;
+};
+
 // Per-scene view read by the kernels (one entry per scene of a batch, in device memory).
 // Parameters are fp32 as the reference passes them to its kernels; rho is also kept in double
 // because rho/dt and dt/rho are formed in double on the reference's host (grid.py:237,251).
-struct SceneDev {
+template <class S>
+struct SceneView {
+    template <class T>
+    using ptr = typename S::template p<T>;
     int n, P;
     long N, F;
     float h, hh;
+    float rhh;               // RN(1 / hh)
+    int fastdiv;             // x / hh through div_const's corrected product (checked on the host)
     float rho, t_amb, g, amin, ka, km, frac, k0, t_sol, t_liq, ys, mus, mul, t_max, rate, t_cap;
     int rate_on, buoyancy;
     double rho_d;
-    float *u[3], *us[3], *q[3], *kf[3], *mf[3], *st_k[3], *st_m[3];
-    float *p, *rhs, *pdiag, *alpha_s, *st_alpha;
-    float *T, *T2, *Tw, *kap, *alpha_p, *tsum, *tfix, *hrhs, *hdiag;
-    uint8_t* flags;
-    float *cg_r, *cg_z, *cg_d, *cg_Ap;
-    double* partials;        // two-pass: [kMaxDirBlocks] pass A, then [3][nb_upd] pass B / init;
+    ptr<float> u[3], us[3], q[3], kf[3], mf[3], st_k[3], st_m[3];
+    ptr<float> p, rhs, pdiag, alpha_s, st_alpha;
+    ptr<float> T, T2, Tw, kap, alpha_p, tsum, tfix, hrhs, hdiag;
+    ptr<uint8_t> flags;
+    ptr<float> cg_r, cg_z, cg_d, cg_Ap;
+    ptr<double> partials;    // two-pass: [kMaxDirBlocks] pass A, then [3][nb_upd] pass B / init;
                              // fused: two parity buffers of [kFusedRows][kMaxDirBlocks]
-    CgCtl* ctl;              // [0] heat, [1] pressure
-    CgMirror* mirror;        // device view of the pinned mirrors, [0] heat, [1] pressure
+    ptr<CgCtl> ctl;          // [0] heat, [1] pressure
+    ptr<CgMirror> mirror;    // device view of the pinned mirrors, [0] heat, [1] pressure
     int cpt, nb_upd, nb_dir, tiles_j, tiles_k, dtile;
     int cg_fused;            // pressure PCG in one launch per iteration (tiled scenes only)
-    float* chunk_sums;
-    const int* mean_prog;
+    ptr<float> chunk_sums;
+    ptr<const int> mean_prog;
     int mean_tail;           // offset of the tail chunk's program in mean_prog, -1 if none
     int nchunks;
-    float *pos, *vel;
-    float *temp, *melt, *yield, *visc;
-    int *bin_start, *sorted;
-    float* qpos;             // pos / h per particle, in bin order (3 per particle)
+    ptr<float> pos, vel;
+    ptr<float> temp, melt, yield, visc;
+    ptr<int> bin_start, sorted;
+    ptr<float> qpos;         // pos / h per particle, in bin order (3 per particle)
 };
+using SceneDev = SceneView<HostPtrs>;  // host copy, written by the host
+using SceneG = SceneView<DevPtrs>;     // the kernels' view of the same bytes
 
 // Block-map entries: scene (24 bits) | sub-range (8 bits) | local workgroup (32 bits).
 typedef unsigned long long MapEntry;
@@ -85,15 +112,41 @@ __host__ __device__ inline MapEntry map_entry(int s, int sub, long l) {
     return ((MapEntry)s << 40) | ((MapEntry)sub << 32) | (MapEntry)(unsigned)l;
 }
 
-// Workgroup prologue of a batched kernel: `a` is the scene's descriptor, `sub` the sub-range
-// and `lb` the workgroup's index inside it.  Padding entries end the workgroup at once.
-#define HFM_BATCH_ENTRY(tab, map)                                 \
-    const ::hfm::MapEntry e_ = (map)[blockIdx.x];                 \
-    if (e_ == ::hfm::kNoBlock) return;                            \
-    const int sub = (int)((e_ >> 32) & 0xFFu);                    \
-    const int lb = (int)(e_ & 0xFFFFFFFFu);                       \
-    const ::hfm::SceneDev& a = (tab)[e_ >> 40];                   \
-    (void)sub;                                                    \
+// Where a launch finds its scenes.  A batch passes a block map and the descriptor table: workgroup
+// b reads map[b] (scene, sub-range, local workgroup), then the scene's descriptor.  A batch of one
+// passes the descriptor itself as a kernel argument (sub-range = blockIdx.y, local workgroup =
+// blockIdx.x), so no dependent loads stand between the launch and the first data load.
+struct BatchSrc {
+    const SceneG* __restrict__ tab;
+    const MapEntry* __restrict__ map;
+};
+struct SoloSrc {
+    SceneG s;
+};
+
+__device__ __forceinline__ bool resolve(const BatchSrc& src, int& sub, int& lb, const SceneG*& a) {
+    const MapEntry e = src.map[blockIdx.x];
+    if (e == kNoBlock) return false;
+    sub = (int)((e >> 32) & 0xFFu);
+    lb = (int)(e & 0xFFFFFFFFu);
+    a = &src.tab[e >> 40];
+    return true;
+}
+__device__ __forceinline__ bool resolve(const SoloSrc& src, int& sub, int& lb, const SceneG*& a) {
+    sub = (int)blockIdx.y;
+    lb = (int)blockIdx.x;
+    a = &src.s;
+    return true;
+}
+
+// Workgroup prologue of a scene kernel: `a` is the scene's descriptor, `sub` the sub-range and
+// `lb` the workgroup's index inside it.  Padding entries of a block map end the workgroup at once.
+#define HFM_ENTRY(src)                                   \
+    int sub, lb;                                         \
+    const ::hfm::SceneG* ap_;                            \
+    if (!::hfm::resolve(src, sub, lb, ap_)) return;      \
+    const ::hfm::SceneG& a = *ap_;                       \
+    (void)sub;                                           \
     (void)lb;
 
 // Launch families: how many workgroups each scene needs in each sub-range.
@@ -168,11 +221,38 @@ struct Batch {
 struct Launch {
     const SceneDev* tab;
     const MapEntry* map;
-    int blocks;
+    int blocks;             // workgroups of the batched launch (0: nothing to launch)
     hipStream_t stream;
+    const SceneDev* solo;   // batch of one: its descriptor, launched as a kernel argument
+    dim3 sgrid;             // batch of one: (workgroups per sub-range, sub-ranges)
 };
 
-inline Launch family(const Batch* b, Family f) { return Launch{b->table, b->maps[f], b->map_len[f], b->stream}; }
+Launch family(const Batch* b, Family f);
+
+// Launch a scene kernel over a family, as a batch (map + table) or as a batch of one (descriptor
+// argument); with ev0/ev1 the launch carries its own start/stop events (hipExtLaunchKernelGGL).
+template <class KB, class KS, class... Args>
+inline void launch_scene_kernel(KB kbatch, KS ksolo, const Launch& L, int threads, hipEvent_t ev0, hipEvent_t ev1,
+                                Args... args) {
+    if (L.blocks <= 0) return;
+    if (L.solo) {
+        const SoloSrc src{*reinterpret_cast<const SceneG*>(L.solo)};
+        if (ev0)
+            hipExtLaunchKernelGGL(ksolo, L.sgrid, dim3(threads), 0, L.stream, ev0, ev1, 0, src, args...);
+        else
+            hipLaunchKernelGGL(ksolo, L.sgrid, dim3(threads), 0, L.stream, src, args...);
+    } else {
+        const BatchSrc src{reinterpret_cast<const SceneG*>(L.tab), L.map};
+        if (ev0)
+            hipExtLaunchKernelGGL(kbatch, dim3(L.blocks), dim3(threads), 0, L.stream, ev0, ev1, 0, src, args...);
+        else
+            hipLaunchKernelGGL(kbatch, dim3(L.blocks), dim3(threads), 0, L.stream, src, args...);
+    }
+}
+#define HFM_LAUNCH(K, L, threads, ev0, ev1, ...) \
+    ::hfm::launch_scene_kernel(K<::hfm::BatchSrc>, K<::hfm::SoloSrc>, L, threads, ev0, ev1, ##__VA_ARGS__)
+#define HFM_LAUNCH_T(K, T, L, threads, ev0, ev1, ...) \
+    ::hfm::launch_scene_kernel(K<T, ::hfm::BatchSrc>, K<T, ::hfm::SoloSrc>, L, threads, ev0, ev1, ##__VA_ARGS__)
 
 // ---- batched kernel launchers (hfm_fields.hip) ----
 void launch_heat_prep(const Launch& L, float dt);
@@ -195,7 +275,8 @@ void launch_bin_particles(Scene* s, hipStream_t st);
 void launch_p2g(const Launch& L);
 
 // ---- PCG (hfm_pcg.hip); op 0 = heat, 1 = pressure ----
-void pcg_geometry(const Scene* s, int* cpt, int* nb_upd, int* nb_dir, int* tiles_j, int* tiles_k, int* dtile);
+void pcg_geometry(const Scene* s, bool fused, int* cpt, int* nb_upd, int* nb_dir, int* tiles_j, int* tiles_k,
+                  int* dtile);
 void launch_pcg_init(const Launch& L, int op, float dt);
 // pace: when non-null, workgroup 0 posts (tag << 32) | k to it as the launch starts
 void launch_pcg_dir(const Launch& L, Family fam, int op, float dt, int k, unsigned long long* pace, unsigned tag,

@@ -18,6 +18,9 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
+#include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -134,6 +137,35 @@ int warp_reported_count(int exact, int converged, int maxiter, int check_every) 
     return cur;
 }
 
+// Whether div_const's corrected product x * RN(1/b) + residual equals the IEEE quotient x / b for
+// every significand (x in [1, 2)); with all intermediate values normal that covers every x with
+// 2^-100 <= |x| <= 2^100 (the quotients scale exactly with x).  Results are cached per b.
+bool const_division_exact(float b) {
+    static std::mutex mu;
+    static std::map<uint32_t, bool> cache;
+    uint32_t key;
+    memcpy(&key, &b, 4);
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) return it->second;
+    }
+    bool exact = std::isfinite(b) && b != 0.0f && std::isfinite(1.0f / b);
+    const float rb = 1.0f / b;
+    for (uint32_t m = 0; exact && m < (1u << 23); ++m) {
+        const uint32_t bits = 0x3f800000u | m;
+        float x;
+        memcpy(&x, &bits, 4);
+        const float q = x * rb;
+        const float e = std::fma(-q, b, x);
+        const float r = std::fma(e, rb, q);
+        exact = r == x / b;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    cache[key] = exact;
+    return exact;
+}
+
 // Scalar part of the descriptor, from the scene's parameters and state (fp32 as the reference
 // passes them to its kernels).
 void refresh_dev(Scene* s) {
@@ -144,6 +176,8 @@ void refresh_dev(Scene* s) {
     d.F = s->F;
     d.h = (float)p.h;
     d.hh = d.h * d.h;
+    d.rhh = 1.0f / d.hh;
+    d.fastdiv = const_division_exact(d.hh) ? 1 : 0;
     d.rho = (float)p.rho;
     d.rho_d = p.rho;
     d.t_amb = (float)p.t_amb;
@@ -224,6 +258,32 @@ long family_blocks(const Scene* s, int f, int sub) {
         default: return 0;
     }
 }
+
+}  // namespace
+
+// Launch description of one family: the batched map, and for a batch of one its descriptor and
+// grid (workgroups per sub-range x sub-ranges).
+Launch hfm::family(const Batch* b, Family f) {
+    Launch L{b->table, b->maps[f], b->map_len[f], b->stream, nullptr, dim3(0, 0, 1)};
+    if (b->scenes.size() == 1) {
+        const Scene* s = b->scenes[0];
+        long x = 0;
+        int y = 0;
+        for (int sub = 0; sub < 4; ++sub) {
+            const long nb = family_blocks(s, f, sub);
+            if (nb > 0) {
+                x = std::max(x, nb);
+                y = sub + 1;
+            }
+        }
+        L.solo = &s->dev;
+        L.sgrid = dim3((unsigned)x, (unsigned)(y > 0 ? y : 1), 1);
+        L.blocks = x > 0 ? (int)(x * y) : 0;
+    }
+    return L;
+}
+
+namespace {
 
 // (Re)build the device descriptor table and block maps when a scene's descriptor changed.  Each
 // (scene, sub-range) run of workgroups starts at a multiple of 8, so a workgroup's index inside
@@ -723,9 +783,10 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
     A(prog_dev, (long)o->prog.size());
 #undef A
     d.mean_prog = prog_dev;
-    pcg_geometry(s, &d.cpt, &d.nb_upd, &d.nb_dir, &d.tiles_j, &d.tiles_k, &d.dtile);
+    const bool want_fused = (prm->flags & HFM_FLAG_FUSED_CG) != 0;
+    pcg_geometry(s, want_fused, &d.cpt, &d.nb_upd, &d.nb_dir, &d.tiles_j, &d.tiles_k, &d.dtile);
     // the fused iteration is built for 4- and 8-plane blocks (16 planes exceed its register budget)
-    d.cg_fused = (prm->flags & HFM_FLAG_FUSED_CG) && (d.dtile == 4 || d.dtile == 8);
+    d.cg_fused = want_fused && (d.dtile == 4 || d.dtile == 8);
     e = hipMemcpy(prog_dev, o->prog.data(), o->prog.size() * sizeof(int), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         set_err("hipMemcpy: %s", hipGetErrorString(e));
