@@ -1,6 +1,7 @@
 """Benchmark of the air/heat substep on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cg warp|fused]
+    python bench.py --workload batched [--scenes 64] [--steps 200]
 
 Workload: the default coupled scene at n=96 (0.48 m box, h = 5 mm, a static ellipsoid of ~26.8k
 particles at 20 C, 400 W heater capped at 120 C with a 2 m/s fan, pressure and heat PCG to a
@@ -29,7 +30,8 @@ sys.path.insert(0, os.path.join(REPO, "heatflow-mpm_amd"))
 
 import numpy as np  # noqa: E402
 
-from heatflow_mpm import HeatflowMPM, coupled_scene  # noqa: E402
+from heatflow_mpm import HeatflowBatch, HeatflowMPM, batched_scenes, coupled_scene  # noqa: E402
+from heatflow_mpm.serve import Server, poisson_requests, summarize  # noqa: E402
 from heatflow_mpm import _abi  # noqa: E402
 
 METRIC = "cell-substeps/s + HBM TB/s at 96^3; served p50/p99 vs plain at equal tolerance"
@@ -95,10 +97,117 @@ def cpu_baseline(kw, dt, state, budget_s: float = 15.0):
                        f"fp32 C oracle on one host thread, {el:.1f} s")
 
 
+def bench_batched(a) -> dict:
+    """Uneven independent scenes (n in {32..128}, random heaters, fans and particle blobs), all
+    advanced K substeps from rest: once one scene at a time (the plain form), once as one batch
+    (every stage one launch over all scenes).  Same work both ways; the final states are compared
+    scene by scene (they must be bit-identical)."""
+    scenes, dt, K0 = batched_scenes(a.scenes, seed=a.seed)
+    K = a.steps if a.steps_given else K0
+    cells = sum(kw["grid"].n ** 3 for _, kw in scenes)
+    # load the code objects of the solo and the batched kernels outside the timed regions
+    warm = [HeatflowMPM(**coupled_scene(32)[0], cg=a.cg) for _ in range(2)]
+    warm[0].run(dt, 2)
+    with HeatflowBatch(warm) as wb:
+        wb.run(dt, 2)
+    for w in warm:
+        w.close()
+    plain = [HeatflowMPM(**kw, cg=a.cg) for _, kw in scenes]
+    t_plain, p_iters_plain = 0.0, 0
+    for s in plain:
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.run(dt, K)
+        s.synchronize()
+        t_plain += time.perf_counter() - t0
+        p_iters_plain += s.stats()["p_iters_total"]
+    sims = [HeatflowMPM(**kw, cg=a.cg) for _, kw in scenes]
+    batch = HeatflowBatch(sims)
+    batch.synchronize()
+    t0 = time.perf_counter()
+    batch.run(dt, K)
+    batch.synchronize()
+    t_batch = time.perf_counter() - t0
+    same = True
+    for s, p in zip(sims, plain):
+        for f in ("T", "U", "V", "W", "P"):
+            if not np.array_equal(s._scene.get_field(f), p._scene.get_field(f)):
+                same = False
+        if not np.array_equal(s.temperatures(), p.temperatures()):
+            same = False
+    value, plain_value = cells * K / t_batch, cells * K / t_plain
+    ns = [kw["grid"].n for _, kw in scenes]
+    out = dict(metric="aggregate cell-substeps/s, batched uneven scenes", value=value, unit="cell-substeps/s",
+               n_gpus=1, steps=K, warmup=0, ms_per_step=t_batch * 1e3 / K, higher_is_better=True, scaling="weak",
+               vs_baseline=None, dtype="f32", data="synthetic",
+               config=dict(workload=f"{len(scenes)} independent scenes, n in {{32,48,64,96,128}} "
+                                    f"(counts {[ns.count(m) for m in (32, 48, 64, 96, 128)]}), heater 100-800 W, "
+                                    f"fan 0-3 m/s, 2k-50k particles, PCG tol 1e-6, dt 2 ms, {K} substeps from rest",
+                           scenes=len(scenes), cells=cells, seed=a.seed, pressure_cg=a.cg, parallelism="1 GPU"),
+               plain=dict(value=plain_value, seconds=round(t_plain, 3), pressure_iters_total=int(p_iters_plain)),
+               speedup_vs_plain=round(value / plain_value, 3), batched_seconds=round(t_batch, 3),
+               identical_to_plain=same)
+    batch.close()
+    for s in sims + plain:
+        s.close()
+    return out
+
+
+def bench_served(a) -> dict:
+    """A seeded Poisson stream of rollout requests (n from {48, 64, 96, 128, 192}, K uniform in
+    [kmin, kmax], heaters / fans / particle blobs as the batched scenes), served twice on the same
+    arrival times: with continuous batching (all admitted scenes advance in one batched substep;
+    admission in arrival order by device footprint) and in the plain form (one request at a time,
+    in arrival order).  Latency = completion - arrival."""
+    reqs = poisson_requests(a.rate, a.requests, seed=a.seed, k_range=(a.kmin, a.kmax))
+    warm = [HeatflowMPM(**coupled_scene(32)[0], cg=a.cg) for _ in range(2)]
+    warm[0].run(2e-3, 2)
+    with HeatflowBatch(warm) as wb:
+        wb.run(2e-3, 2)
+    for w in warm:
+        w.close()
+    from heatflow_mpm.serve import GpuBackend
+    budget = int(a.hbm_gib * (1 << 30))
+    served = Server(GpuBackend(cg=a.cg), hbm_budget=budget, max_active=a.slots)
+    t0 = time.perf_counter()
+    res = served.run(reqs)
+    span = time.perf_counter() - t0
+    s_b = summarize(res, span)
+    plain = Server(GpuBackend(cg=a.cg), hbm_budget=budget, max_active=1)
+    t0 = time.perf_counter()
+    res_p = plain.run(reqs)
+    span_p = time.perf_counter() - t0
+    s_p = summarize(res_p, span_p)
+    ns = [r.kwargs["grid"].n for r in reqs]
+    return dict(metric="served p50/p99 latency and aggregate cell-substeps/s, continuous batching",
+                value=s_b["cell_substeps_per_s"], unit="cell-substeps/s", n_gpus=1, steps=None, warmup=0,
+                ms_per_step=None, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32",
+                data="synthetic",
+                config=dict(workload=f"{len(reqs)} Poisson requests at {a.rate}/s, n in {{48,64,96,128,192}} "
+                                     f"(counts {[ns.count(m) for m in (48, 64, 96, 128, 192)]}), K uniform "
+                                     f"{a.kmin}-{a.kmax}, heater 100-800 W, fan 0-3 m/s, 2k-50k particles, "
+                                     f"PCG tol 1e-6, dt 2 ms", rate_per_s=a.rate, requests=len(reqs), seed=a.seed,
+                            hbm_budget_gib=a.hbm_gib, slots=a.slots, pressure_cg=a.cg, parallelism="1 GPU"),
+                served=s_b, plain=s_p, speedup_vs_plain=round(s_b["cell_substeps_per_s"] / s_p["cell_substeps_per_s"], 3),
+                p99_ratio_plain_over_served=round(s_p["p99_s"] / s_b["p99_s"], 3),
+                max_batch=max(e[3] for e in served.log))
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--workload", choices=("coupled", "batched", "served"), default="coupled",
+                    help="coupled: one n=96 coupled scene (default); batched: uneven scenes in one batch vs one by "
+                         "one; served: a Poisson request stream with continuous batching vs one at a time")
+    ap.add_argument("--rate", type=float, default=20.0, help="served: arrivals per second")
+    ap.add_argument("--requests", type=int, default=24, help="served: number of requests")
+    ap.add_argument("--kmin", type=int, default=20, help="served: fewest substeps of a request")
+    ap.add_argument("--kmax", type=int, default=100, help="served: most substeps of a request")
+    ap.add_argument("--hbm-gib", type=float, default=64.0, help="served: device memory budget of the scenes")
+    ap.add_argument("--slots", type=int, default=64, help="served: most scenes in one batch")
+    ap.add_argument("--scenes", type=int, default=64, help="batched: number of scenes")
+    ap.add_argument("--seed", type=int, default=0, help="batched: scene draw seed")
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=96)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
@@ -109,6 +218,15 @@ def main() -> int:
         print("bench.py measures one GPU: the substep runs on one device (scene-level sharding across GPUs is "
               "independent scenes, not one scene split); run with --gpus 1", file=sys.stderr)
         return 2
+    a.steps_given = a.steps is not None
+    if a.workload == "batched":
+        print(json.dumps(bench_batched(a)), flush=True)
+        return 0
+    if a.workload == "served":
+        print(json.dumps(bench_served(a)), flush=True)
+        return 0
+    if a.steps is None:
+        a.steps = 500
 
     kw, dt, _ = coupled_scene(a.n)
     sim = HeatflowMPM(**kw, device="cuda:0", cg=a.cg)

@@ -355,6 +355,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(Src src, float dt, int k,
                                                     unsigned long long* pace, unsigned tag) {
     post_pace(pace, tag, k);
     HFM_ENTRY(src);
+    if (batch_solve_done(src, a.ctl + OP)) return;
     __shared__ double lds[3][kWaves];
     const PcgView v = make_view<OP>(a, lb, k, dt);
     PassAIn pre;
@@ -396,6 +397,7 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_dir_tile3(Src src, int kit, uns
                                                            unsigned tag) {
     post_pace(pace, tag, kit);
     HFM_ENTRY(src);
+    if (batch_solve_done(src, a.ctl + 1)) return;
     __shared__ float P[D + 2][kF3][kF3];
     __shared__ float S[D + 2][kF3][kF3];
     __shared__ double lds[1][kBlock3 / 64];
@@ -481,6 +483,7 @@ template <int OP, class Src>
 __global__ __launch_bounds__(kBlock) void k_pcg_upd(Src src, int kit) {
     HFM_ENTRY(src);
     if (OP == 1 && a.cg_fused) return;  // the fused form has no pass B
+    if (batch_solve_done(src, a.ctl + OP)) return;
     __shared__ double lds[2][kWaves];
     const PcgView v = make_view<OP>(a, lb, kit, 0.0f);
     const CgCtl* c = v.ctl;
@@ -563,6 +566,7 @@ template <int D, class Src>
 __global__ __launch_bounds__(kBlock3) void k_pcg_fused(Src src, int kit, unsigned long long* pace, unsigned tag) {
     post_pace(pace, tag, kit);
     HFM_ENTRY(src);
+    if (batch_solve_done(src, a.ctl + 1)) return;
     __shared__ float P2[D + 4][kF2][kF2];   // z_k, then p_k (block + 2 halo)
     __shared__ float S2[D + 4][kF2][kF2];   // alpha_s
     __shared__ float Z1[D + 2][kF3][kF3];   // z_{k+1} (block + 1 halo)

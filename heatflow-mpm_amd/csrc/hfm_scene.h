@@ -116,15 +116,22 @@ __host__ __device__ inline MapEntry map_entry(int s, int sub, long l) {
 // b reads map[b] (scene, sub-range, local workgroup), then the scene's descriptor.  A batch of one
 // passes the descriptor itself as a kernel argument (sub-range = blockIdx.y, local workgroup =
 // blockIdx.x), so no dependent loads stand between the launch and the first data load.
+// The table and the map are read-only for the whole launch; in the device pass they are typed as
+// constant memory, so their uniform reads are scalar loads into SGPRs.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define HFM_CONST __attribute__((address_space(4)))
+#else
+#define HFM_CONST
+#endif
 struct BatchSrc {
-    const SceneG* __restrict__ tab;
-    const MapEntry* __restrict__ map;
+    const HFM_CONST SceneG* tab;
+    const HFM_CONST MapEntry* map;
 };
 struct SoloSrc {
     SceneG s;
 };
 
-__device__ __forceinline__ bool resolve(const BatchSrc& src, int& sub, int& lb, const SceneG*& a) {
+__device__ __forceinline__ bool resolve(const BatchSrc& src, int& sub, int& lb, const HFM_CONST SceneG*& a) {
     const MapEntry e = src.map[blockIdx.x];
     if (e == kNoBlock) return false;
     sub = (int)((e >> 32) & 0xFFu);
@@ -139,13 +146,39 @@ __device__ __forceinline__ bool resolve(const SoloSrc& src, int& sub, int& lb, c
     return true;
 }
 
+// In a batch, scenes whose solve has stopped are skipped before any of their data is loaded:
+// one uniform scalar load of the flag (a batch keeps launching until its slowest scene converges,
+// so most workgroups may belong to finished scenes).  A wave that leaves early is not waited for
+// at later barriers, and the workgroups that still run reach the same stop decision from the
+// partial sums, so the outcome does not depend on when the flag is seen.  A batch of one needs no
+// such check (its launches end with its solve) and saves the round trip.
+template <class Src>
+__device__ __forceinline__ bool batch_solve_done(const Src&, const CgCtl* c) {
+    return false;
+}
+template <>
+__device__ __forceinline__ bool batch_solve_done<BatchSrc>(const BatchSrc&, const CgCtl* c) {
+    return c->done != 0;
+}
+
+template <class S>
+struct SrcScene;
+template <>
+struct SrcScene<BatchSrc> {
+    using type = const HFM_CONST SceneG*;
+};
+template <>
+struct SrcScene<SoloSrc> {
+    using type = const SceneG*;
+};
+
 // Workgroup prologue of a scene kernel: `a` is the scene's descriptor, `sub` the sub-range and
 // `lb` the workgroup's index inside it.  Padding entries of a block map end the workgroup at once.
 #define HFM_ENTRY(src)                                   \
     int sub, lb;                                         \
-    const ::hfm::SceneG* ap_;                            \
+    typename ::hfm::SrcScene<decltype(src)>::type ap_;   \
     if (!::hfm::resolve(src, sub, lb, ap_)) return;      \
-    const ::hfm::SceneG& a = *ap_;                       \
+    const auto& a = *ap_;                                \
     (void)sub;                                           \
     (void)lb;
 
@@ -203,8 +236,10 @@ struct Batch {
     bool user = false;                       // created through the batch API
     std::vector<Scene*> scenes;
     SceneDev* table = nullptr;               // device copy of the descriptors
+    size_t table_cap = 0;                    // descriptors the table buffer holds
     MapEntry* maps[FAM_COUNT] = {};          // device block maps
     int map_len[FAM_COUNT] = {};
+    size_t map_cap[FAM_COUNT] = {};          // entries each map buffer holds (buffers only grow)
     std::vector<unsigned> versions;          // scene versions the maps and table were built for
     unsigned long long* pace = nullptr;      // pinned: (solve tag << 32) | iterations completed
     unsigned long long* pace_dev = nullptr;
@@ -242,7 +277,9 @@ inline void launch_scene_kernel(KB kbatch, KS ksolo, const Launch& L, int thread
         else
             hipLaunchKernelGGL(ksolo, L.sgrid, dim3(threads), 0, L.stream, src, args...);
     } else {
-        const BatchSrc src{reinterpret_cast<const SceneG*>(L.tab), L.map};
+        BatchSrc src;
+        src.tab = (const HFM_CONST SceneG*)(const void*)L.tab;
+        src.map = (const HFM_CONST MapEntry*)(const void*)L.map;
         if (ev0)
             hipExtLaunchKernelGGL(kbatch, dim3(L.blocks), dim3(threads), 0, L.stream, ev0, ev1, 0, src, args...);
         else

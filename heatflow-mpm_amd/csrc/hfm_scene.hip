@@ -74,6 +74,7 @@ void set_err(const char* fmt, ...) {
 struct Owner {
     Scene s;
     std::vector<void*> dev;
+    std::vector<size_t> dev_bytes;
     std::vector<int> prog;
 };
 
@@ -89,16 +90,18 @@ int dalloc(Owner* o, T** p, long count) {
         return e == hipErrorOutOfMemory ? HFM_ERR_OOM : HFM_ERR_HIP;
     }
     o->dev.push_back(q);
+    o->dev_bytes.push_back(bytes);
     HIP_OK(hipMemset(q, 0, bytes));
     *p = (T*)q;
     return HFM_OK;
 }
 
 void dfree(Owner* o, void* p) {
-    for (auto& q : o->dev)
-        if (q == p) {
-            (void)hipFree(q);
-            q = nullptr;
+    for (size_t i = 0; i < o->dev.size(); ++i)
+        if (o->dev[i] == p) {
+            (void)hipFree(o->dev[i]);
+            o->dev[i] = nullptr;
+            o->dev_bytes[i] = 0;
         }
 }
 
@@ -215,12 +218,29 @@ int batch_init(Batch* b, int device, hipStream_t stream, bool own_stream) {
 void batch_free_maps(Batch* b) {
     if (b->table) (void)hipFree(b->table);
     b->table = nullptr;
+    b->table_cap = 0;
     for (int f = 0; f < FAM_COUNT; ++f) {
         if (b->maps[f]) (void)hipFree(b->maps[f]);
         b->maps[f] = nullptr;
         b->map_len[f] = 0;
+        b->map_cap[f] = 0;
     }
     b->versions.clear();
+}
+
+// Device buffer of at least `count` elements, grown (never shrunk) as a batch changes.
+template <class T>
+int ensure_capacity(T** buf, size_t* cap, size_t count) {
+    if (count <= *cap && *buf) return HFM_OK;
+    if (*buf) HIP_OK(hipFree(*buf));
+    *buf = nullptr;
+    *cap = 0;
+    const size_t want = std::max(count, (size_t)64) + count / 4;
+    void* q = nullptr;
+    HIP_OK(hipMalloc(&q, want * sizeof(T)));
+    *buf = (T*)q;
+    *cap = want;
+    return HFM_OK;
 }
 
 void batch_release(Batch* b) {
@@ -294,12 +314,9 @@ int batch_prepare(Batch* b) {
     for (size_t i = 0; fresh && i < ns; ++i) fresh = b->versions[i] == b->scenes[i]->version;
     if (fresh) return HFM_OK;
     HIP_OK(hipStreamSynchronize(b->stream));
-    batch_free_maps(b);
     std::vector<SceneDev> tab(ns);
     for (size_t i = 0; i < ns; ++i) tab[i] = b->scenes[i]->dev;
-    void* q = nullptr;
-    HIP_OK(hipMalloc(&q, ns * sizeof(SceneDev)));
-    b->table = (SceneDev*)q;
+    RC(ensure_capacity(&b->table, &b->table_cap, ns));
     HIP_OK(hipMemcpy(b->table, tab.data(), ns * sizeof(SceneDev), hipMemcpyHostToDevice));
     std::vector<MapEntry> m;
     for (int f = 0; f < FAM_COUNT; ++f) {
@@ -316,8 +333,7 @@ int batch_prepare(Batch* b) {
         }
         b->map_len[f] = (int)m.size();
         if (m.empty()) continue;
-        HIP_OK(hipMalloc(&q, m.size() * sizeof(MapEntry)));
-        b->maps[f] = (MapEntry*)q;
+        RC(ensure_capacity(&b->maps[f], &b->map_cap[f], m.size()));
         HIP_OK(hipMemcpy(b->maps[f], m.data(), m.size() * sizeof(MapEntry), hipMemcpyHostToDevice));
     }
     b->versions.resize(ns);
@@ -1130,6 +1146,14 @@ int hfm_set_kernel_timing(hfm_scene* h, int32_t on) {
     return set_timing(s->solo, on);
 }
 
+int64_t hfm_scene_bytes(const hfm_scene* h) {
+    if (!h) return -1;
+    const Owner* o = reinterpret_cast<const Owner*>(h);
+    int64_t total = 0;
+    for (size_t b : o->dev_bytes) total += (int64_t)b;
+    return total;
+}
+
 int hfm_synchronize(hfm_scene* h) {
     SCENE_OR_FAIL(h);
     HIP_OK(hipStreamSynchronize(s->stream));
@@ -1184,6 +1208,40 @@ int hfm_batch_create(hfm_scene* const* scenes, int32_t count, hfm_batch** out) {
     return HFM_OK;
 }
 
+int hfm_batch_set_scenes(hfm_batch* h, hfm_scene* const* scenes, int32_t count) {
+    BATCH_OR_FAIL(h);
+    if ((count > 0 && !scenes) || count < 0 || count > kMaxBatch) {
+        set_err("bad batch arguments (count=%d, at most %d)", count, kMaxBatch);
+        return HFM_ERR_ARG;
+    }
+    std::vector<Scene*> list;
+    for (int i = 0; i < count; ++i) {
+        if (!scenes[i]) {
+            set_err("null scene at %d", i);
+            return HFM_ERR_ARG;
+        }
+        Scene* s = &owner(scenes[i])->s;
+        if (std::find(list.begin(), list.end(), s) != list.end()) {
+            set_err("scene %d appears twice in the batch", i);
+            return HFM_ERR_ARG;
+        }
+        if (s->device != b->device) {
+            set_err("scenes of a batch must be on one device");
+            return HFM_ERR_ARG;
+        }
+        list.push_back(s);
+    }
+    HIP_OK(hipStreamSynchronize(b->stream));
+    for (Scene* s : b->scenes) {
+        if (s->current == b) s->current = s->solo;
+        s->members -= 1;
+    }
+    for (Scene* s : list) s->members += 1;
+    b->scenes = list;
+    b->versions.clear();  // the next substep rebuilds the table and the maps (in the same buffers)
+    return HFM_OK;
+}
+
 int hfm_batch_destroy(hfm_batch* h) {
     if (!h) return HFM_OK;
     Batch* b = &h->b;
@@ -1200,6 +1258,7 @@ int hfm_batch_destroy(hfm_batch* h) {
 
 int hfm_batch_substep(hfm_batch* h, double dt) {
     BATCH_OR_FAIL(h);
+    if (b->scenes.empty()) return HFM_OK;
     if (!(dt > 0) || !(dt < 1e3)) {
         set_err("bad dt %g", dt);
         return HFM_ERR_ARG;
@@ -1209,6 +1268,10 @@ int hfm_batch_substep(hfm_batch* h, double dt) {
 
 int hfm_batch_run(hfm_batch* h, double dt, int32_t K, double* ms) {
     BATCH_OR_FAIL(h);
+    if (b->scenes.empty()) {
+        if (ms) *ms = 0.0;
+        return HFM_OK;
+    }
     if (K < 0 || !(dt > 0) || !(dt < 1e3)) {
         set_err("bad run arguments");
         return HFM_ERR_ARG;

@@ -78,8 +78,10 @@ SIGNATURES = {
     "hfm_kernel_time_reset": (C.c_int, [_VP]),
     "hfm_set_kernel_timing": (C.c_int, [_VP, C.c_int32]),
     "hfm_synchronize": (C.c_int, [_VP]),
+    "hfm_scene_bytes": (C.c_int64, [_VP]),
     "hfm_batch_create": (C.c_int, [C.POINTER(_VP), C.c_int32, C.POINTER(_VP)]),
     "hfm_batch_destroy": (C.c_int, [_VP]),
+    "hfm_batch_set_scenes": (C.c_int, [_VP, C.POINTER(_VP), C.c_int32]),
     "hfm_batch_substep": (C.c_int, [_VP, C.c_double]),
     "hfm_batch_run": (C.c_int, [_VP, C.c_double, C.c_int32, C.POINTER(C.c_double)]),
     "hfm_batch_synchronize": (C.c_int, [_VP]),

@@ -28,6 +28,13 @@ class HeatflowBatch:
     def __len__(self) -> int:
         return len(self.sims)
 
+    def set_sims(self, sims) -> None:
+        """Replace the simulations the batch steps (possibly none), keeping its stream and buffers."""
+        sims = list(sims)
+        arr = (C.c_void_p * max(len(sims), 1))(*[s._scene.handle.value for s in sims])
+        _abi.check(self.lib.hfm_batch_set_scenes(self.handle, arr, len(sims)), "hfm_batch_set_scenes")
+        self.sims = sims
+
     def substep(self, dt: float) -> None:
         """One substep of every simulation (solver.py:128-153 for each), moving bodies included."""
         for s in self.sims:

@@ -57,3 +57,36 @@ def coupled_scene(n: int = 96, particles_target: int | None = None, seed: int = 
     return dict(particles=P, mat=Material(t_sol=28.0, t_liq=34.0),
                 heater=Heater(power=400.0, t_cap=120.0, fan=2.0, dist=0.21),
                 grid=GridConfig(L=0.48, h=0.48 / n, cg_tol=1e-6), spacing=spacing, t_init=20.0), 2e-3, 500
+
+
+def random_scene(rng, n: int, name: str = "scene"):
+    """One scene of the batched and served workloads at n cells per side (0.48 m box): heater
+    power U(100, 800) W, fan U(0, 3) m/s, an ellipsoid of U{2000..50000} particles with semi-axes
+    U(3, 7) cm each (lattice spacing (4/3 pi a b c / P)^(1/3), jitter 12 % of it) at 20 C; melt
+    window 28-34 C, PCG tolerance 1e-6.  Draw order: power, fan, P, semi-axes, particle seed."""
+    power = float(rng.uniform(100.0, 800.0))
+    fan = float(rng.uniform(0.0, 3.0))
+    target = int(rng.integers(2000, 50001))
+    axes = tuple(float(x) for x in rng.uniform(0.03, 0.07, 3))
+    pseed = int(rng.integers(0, 2 ** 31))
+    spacing = (4.0 / 3.0 * np.pi * axes[0] * axes[1] * axes[2] / target) ** (1.0 / 3.0)
+    P = ellipsoid_particles(semi_axes=axes, spacing=spacing, jitter=0.12 * spacing, seed=pseed)
+    kw = dict(particles=P, mat=Material(t_sol=28.0, t_liq=34.0),
+              heater=Heater(power=power, t_cap=120.0, fan=fan, dist=0.21),
+              grid=GridConfig(L=0.48, h=0.48 / n, cg_tol=1e-6), spacing=spacing, t_init=20.0)
+    return f"{name}_n{n}_P{len(P)}", kw
+
+
+def batched_scenes(count: int = 64, seed: int = 0):
+    """Independent scenes of uneven size for batched stepping: n drawn from {32, 48, 64, 96, 128}
+    (0.48 m box), heater power U(100, 800) W, fan U(0, 3) m/s, an ellipsoid of U{2000..50000}
+    particles with semi-axes U(3, 7) cm each (lattice spacing derived from the count:
+    (4/3 pi a b c / P)^(1/3), jitter 12 % of it) at 20 C; melt window 28-34 C, PCG tolerance 1e-6.
+    Runs 200 substeps of 2 ms.  Draw order per scene: n, power, fan, P, semi-axes, then the
+    particle seed.  Returns a list of (name, kwargs)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(rng.choice([32, 48, 64, 96, 128]))
+        out.append(random_scene(rng, n, name=f"s{i:02d}"))
+    return out, 2e-3, 200

@@ -177,6 +177,9 @@ class _Scene:
     def synchronize(self) -> None:
         _abi.check(self.lib.hfm_synchronize(self.handle), "hfm_synchronize")
 
+    def device_bytes(self) -> int:
+        return int(self.lib.hfm_scene_bytes(self.handle))
+
     def set_kernel_timing(self, on: bool) -> None:
         _abi.check(self.lib.hfm_set_kernel_timing(self.handle, int(bool(on))), "hfm_set_kernel_timing")
 
@@ -201,13 +204,16 @@ class HeatflowMPM:
 
     def __init__(self, particles: np.ndarray, mat: Material = Material(), heater: Heater = Heater(),
                  grid: GridConfig = GridConfig(), spacing: float = 2.5e-3, t_init: float = 20.0,
-                 kappa_air: float = KAPPA_AIR, device: str = "cuda:0", body=None, cg: str = "warp"):
+                 kappa_air: float = KAPPA_AIR, device: str = "cuda:0", body=None, cg: str = "warp",
+                 inputs: dict | None = None):
         if cg not in self.CG_FORMS:
             raise ValueError(f"cg must be one of {self.CG_FORMS}")
         self.mat, self.heater, self.dev, self.spacing, self.kappa_air = mat, heater, device, spacing, kappa_air
         particles = np.asarray(particles, np.float32).reshape(-1, 3)
         self.n = len(particles)
-        inp = scene_inputs(self.n, mat, heater, grid, spacing, t_init, kappa_air)
+        # `inputs`: scene_inputs(...) of these same arguments, prepared ahead (e.g. by a server)
+        inp = inputs if inputs is not None else scene_inputs(self.n, mat, heater, grid, spacing, t_init, kappa_air)
+        inp = dict(inp, params=dict(inp["params"]))
         if cg == "fused":
             inp["params"]["flags"] |= _abi.FLAG_FUSED_CG
         self._scene = _Scene(inp["params"], parse_device(device))

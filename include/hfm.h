@@ -184,6 +184,10 @@ int hfm_set_kernel_timing(hfm_scene* s, int32_t on);
 
 int hfm_synchronize(hfm_scene* s);
 
+/* Device memory the scene holds, in bytes (fields, solver vectors, particle arrays, tables);
+ * no counterpart in the reference.  Used to admit scenes against an HBM budget. */
+int64_t hfm_scene_bytes(const hfm_scene* s);
+
 /* ---- Batches: many scenes stepped together ------------------------------------------------
  * No counterpart in the reference, which steps one HeatflowMPM at a time (solver.py:128-151):
  * a batch runs the same substep for every scene it holds, each stage one kernel launch over all
@@ -196,6 +200,9 @@ int hfm_synchronize(hfm_scene* s);
 typedef struct hfm_batch hfm_batch;
 int hfm_batch_create(hfm_scene* const* scenes, int32_t count, hfm_batch** out);
 int hfm_batch_destroy(hfm_batch* b);
+/* Replace the scenes a batch holds (count may be 0), keeping its stream and buffers: for a
+ * serving loop whose scenes join and leave between substeps. */
+int hfm_batch_set_scenes(hfm_batch* b, hfm_scene* const* scenes, int32_t count);
 /* One substep of every scene (same dt). */
 int hfm_batch_substep(hfm_batch* b, double dt);
 /* K substeps back to back; *ms (may be NULL) receives the HIP-event time of the batch. */
