@@ -386,8 +386,11 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(Src src, float dt, int k,
 // solid fraction, and once the prologue has formed beta, the new direction p = z + beta p_old.
 // For the staging, 972 threads each own one (j, k) position of the 18 x 18 frame plane and step
 // through every third plane, so a load costs an address increment; all loads are in flight
-// before the prologue, whose latency they overlap.  Every thread then applies the stencil to
-// D/4 cells from LDS.  Per cell the arithmetic is exactly that of k_pcg_dir<1>.
+// before the prologue, whose latency they overlap.  A frame cell outside the box takes the
+// values of the clamped in-box cell; for every frame cell a stencil reaches, that is the very cell
+// whose stencil reaches it, so the missing neighbour's term is a * (p - p) = +0 and the sum is
+// exactly the one k_pcg_dir<1> forms by skipping the term, with no branch.  Every thread then
+// applies the stencil to D/4 cells from LDS.
 constexpr int kT3 = 16, kF3 = kT3 + 2, kBlock3 = 1024;
 constexpr int kPlane3 = kF3 * kF3;            // 324 positions of a frame plane
 constexpr int kGroups3 = kBlock3 / kPlane3;   // 3 plane groups
@@ -419,14 +422,11 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_dir_tile3(Src src, int kit, uns
     const int grp = threadIdx.x / kPlane3, pos = threadIdx.x % kPlane3;
     const int fj = pos / kF3, fk = pos % kF3;
     const int gj = j0 + fj - 1, gk = k0 + fk - 1;
-    const bool ok_jk = grp < kGroups3 && gj >= 0 && gj < n && gk >= 0 && gk < n;
     const unsigned t_jk = (unsigned)clampi(gj, 0, n - 1) * n + clampi(gk, 0, n - 1);
     float fz[kM], fd[kM], fa[kM];
-    bool ok[kM];
 #pragma unroll
     for (int m = 0; m < kM; ++m) {
         const int fi = grp + kGroups3 * m, gi = i0 + fi - 1;
-        ok[m] = ok_jk && fi < D + 2 && gi >= 0 && gi < n;
         const unsigned t = (unsigned)clampi(gi, 0, n - 1) * nn + t_jk;
         fz[m] = zs[t];
         fd[m] = first ? 0.0f : ds[t];
@@ -441,8 +441,8 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_dir_tile3(Src src, int kit, uns
         for (int m = 0; m < kM; ++m) {
             const int fi = grp + kGroups3 * m;
             if (fi < D + 2) {
-                P[fi][fj][fk] = ok[m] ? (first ? fz[m] : fz[m] + beta * fd[m]) : 0.0f;
-                S[fi][fj][fk] = ok[m] ? fa[m] : 0.0f;
+                P[fi][fj][fk] = first ? fz[m] : fz[m] + beta * fd[m];
+                S[fi][fj][fk] = fa[m];
             }
         }
     }
@@ -462,12 +462,12 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_dir_tile3(Src src, int kit, uns
             const int si = li + 1;
             const float pc = P[si][sj][sk], ac = S[si][sj][sk];
             float acc = 0.0f;
-            if (i > 0) acc += face_fraction(S[si - 1][sj][sk], ac, amin) * (pc - P[si - 1][sj][sk]);
-            if (i < n - 1) acc += face_fraction(ac, S[si + 1][sj][sk], amin) * (pc - P[si + 1][sj][sk]);
-            if (j > 0) acc += face_fraction(S[si][sj - 1][sk], ac, amin) * (pc - P[si][sj - 1][sk]);
-            if (j < n - 1) acc += face_fraction(ac, S[si][sj + 1][sk], amin) * (pc - P[si][sj + 1][sk]);
-            if (k > 0) acc += face_fraction(S[si][sj][sk - 1], ac, amin) * (pc - P[si][sj][sk - 1]);
-            if (k < n - 1) acc += face_fraction(ac, S[si][sj][sk + 1], amin) * (pc - P[si][sj][sk + 1]);
+            acc += face_fraction(S[si - 1][sj][sk], ac, amin) * (pc - P[si - 1][sj][sk]);
+            acc += face_fraction(ac, S[si + 1][sj][sk], amin) * (pc - P[si + 1][sj][sk]);
+            acc += face_fraction(S[si][sj - 1][sk], ac, amin) * (pc - P[si][sj - 1][sk]);
+            acc += face_fraction(ac, S[si][sj + 1][sk], amin) * (pc - P[si][sj + 1][sk]);
+            acc += face_fraction(S[si][sj][sk - 1], ac, amin) * (pc - P[si][sj][sk - 1]);
+            acc += face_fraction(ac, S[si][sj][sk + 1], amin) * (pc - P[si][sj][sk + 1]);
             const float apc = div_const(1.0f * acc, hh, rhh, fastdiv);
             const unsigned t = tb + (unsigned)li * nn;
             v.d_new[t] = pc;

@@ -241,6 +241,15 @@ struct Batch {
     int map_len[FAM_COUNT] = {};
     size_t map_cap[FAM_COUNT] = {};          // entries each map buffer holds (buffers only grow)
     std::vector<unsigned> versions;          // scene versions the maps and table were built for
+    std::vector<MapEntry> hmap[FAM_COUNT];   // host copies of the maps
+    std::vector<std::pair<size_t, size_t>> ranges[FAM_COUNT];  // per scene: [begin, end) in the map
+    // maps restricted to the scenes whose solve still runs, in three rotating buffers (device and
+    // pinned staging), so a buffer is rewritten only after the launches that read it completed
+    MapEntry* fmap[3][FAM_COUNT] = {};
+    size_t fmap_cap[3][FAM_COUNT] = {};
+    MapEntry* fstage[3] = {};
+    size_t fstage_cap[3] = {};
+    int fslot = 0;
     unsigned long long* pace = nullptr;      // pinned: (solve tag << 32) | iterations completed
     unsigned long long* pace_dev = nullptr;
     unsigned solve_tag = 0;

@@ -246,6 +246,12 @@ int ensure_capacity(T** buf, size_t* cap, size_t count) {
 void batch_release(Batch* b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     batch_free_maps(b);
+    for (int r = 0; r < 3; ++r) {
+        for (int f = 0; f < FAM_COUNT; ++f)
+            if (b->fmap[r][f]) (void)hipFree(b->fmap[r][f]);
+        if (b->fstage[r]) (void)hipHostFree(b->fstage[r]);
+        b->fstage[r] = nullptr;
+    }
     if (b->pace) (void)hipHostFree(b->pace);
     b->pace = nullptr;
     if (b->ev_pool) {
@@ -318,15 +324,19 @@ int batch_prepare(Batch* b) {
     for (size_t i = 0; i < ns; ++i) tab[i] = b->scenes[i]->dev;
     RC(ensure_capacity(&b->table, &b->table_cap, ns));
     HIP_OK(hipMemcpy(b->table, tab.data(), ns * sizeof(SceneDev), hipMemcpyHostToDevice));
-    std::vector<MapEntry> m;
     for (int f = 0; f < FAM_COUNT; ++f) {
+        std::vector<MapEntry>& m = b->hmap[f];
         m.clear();
-        for (size_t i = 0; i < ns; ++i)
+        b->ranges[f].assign(ns, std::make_pair((size_t)0, (size_t)0));
+        for (size_t i = 0; i < ns; ++i) {
+            b->ranges[f][i].first = m.size();
             for (int sub = 0; sub < 4; ++sub) {
                 const long nb = family_blocks(b->scenes[i], f, sub);
                 for (long l = 0; l < nb; ++l) m.push_back(map_entry((int)i, sub, l));
                 while (m.size() % kXcds) m.push_back(kNoBlock);
             }
+            b->ranges[f][i].second = m.size();
+        }
         if (m.size() > (size_t)0x7fffffff) {
             set_err("batch too large: %zu workgroups in one launch", m.size());
             return HFM_ERR_ARG;
@@ -421,6 +431,45 @@ int kernel_time_reset(Batch* b) {
 
 // ---- PCG driver ----------------------------------------------------------------------------
 
+// Launches of families fams[0..nf) restricted to the scenes with active[i] set: the scenes' runs
+// of the host maps are concatenated in a pinned staging buffer and copied on the batch stream
+// into the next of three rotating device buffers.
+int filter_families(Batch* b, const Family* fams, int nf, const std::vector<char>& active, Launch* out) {
+    const int r = b->fslot;
+    b->fslot = (b->fslot + 1) % 3;
+    size_t total = 0;
+    for (int q = 0; q < nf; ++q) total += b->hmap[fams[q]].size();
+    if (total > b->fstage_cap[r]) {
+        if (b->fstage[r]) HIP_OK(hipHostFree(b->fstage[r]));
+        b->fstage[r] = nullptr;
+        b->fstage_cap[r] = 0;
+        HIP_OK(hipHostMalloc((void**)&b->fstage[r], total * sizeof(MapEntry), hipHostMallocDefault));
+        b->fstage_cap[r] = total;
+    }
+    size_t off = 0;
+    for (int q = 0; q < nf; ++q) {
+        const Family f = fams[q];
+        MapEntry* dst = b->fstage[r] + off;
+        size_t len = 0;
+        for (size_t i = 0; i < b->scenes.size(); ++i) {
+            if (!active[i]) continue;
+            const auto rg = b->ranges[f][i];
+            memcpy(dst + len, b->hmap[f].data() + rg.first, (rg.second - rg.first) * sizeof(MapEntry));
+            len += rg.second - rg.first;
+        }
+        Launch L = family(b, f);
+        L.blocks = (int)len;
+        if (len > 0) {
+            RC(ensure_capacity(&b->fmap[r][f], &b->fmap_cap[r][f], len));
+            HIP_OK(hipMemcpyAsync(b->fmap[r][f], dst, len * sizeof(MapEntry), hipMemcpyHostToDevice, b->stream));
+            L.map = b->fmap[r][f];
+        }
+        out[f] = L;
+        off += len;
+    }
+    return HFM_OK;
+}
+
 // Enqueue one PCG solve (op 0 heat, 1 pressure) of every scene of the batch and return when
 // the device reports all of them done.
 int run_pcg(Batch* b, int op, float dt) {
@@ -456,18 +505,45 @@ int run_pcg(Batch* b, int op, float dt) {
     }
     CHECK_LAUNCH();
     auto is_fused = [](Family f) { return f == FAM_FUSED4 || f == FAM_FUSED8 || f == FAM_FUSED16; };
-    // One launch of pass A (or of the fused iteration) per family for iteration k.
+    // Launches of the iteration families; in a batch they are restricted, between groups of
+    // iterations, to the scenes whose solve still runs.
+    Launch fl[FAM_COUNT];
+    for (int f = 0; f < FAM_COUNT; ++f) fl[f] = family(b, (Family)f);
+    Family iter_fams[8];
+    int niter = 0;
+    for (int d = 0; d < ndir; ++d) iter_fams[niter++] = dirs[d];
+    if (op == 1 && two_pass) iter_fams[niter++] = FAM_PCG;  // pressure pass B (heat: already dirs[0])
+    const size_t ns = b->scenes.size();
+    std::vector<char> active(ns, 1);
+    size_t n_active = ns;
+    auto refresh_active = [&]() -> int {
+        if (ns < 2) return HFM_OK;
+        bool changed = false;
+        for (size_t i = 0; i < ns; ++i)
+            if (active[i] && __atomic_load_n(&b->scenes[i]->mirror[op].done, __ATOMIC_ACQUIRE)) {
+                active[i] = 0;
+                --n_active;
+                changed = true;
+            }
+        if (changed && n_active > 0) return filter_families(b, iter_fams, niter, active, fl);
+        return HFM_OK;
+    };
+    // One launch of pass A (or of the fused iteration) per family for iteration k; the first
+    // non-empty launch posts the progress.
     auto pass_a = [&](int k, unsigned long long* pace, bool timed) -> int {
+        unsigned long long* pc = pace;
         for (int d = 0; d < ndir; ++d) {
+            const Launch& L = fl[dirs[d]];
+            if (L.blocks <= 0) continue;
             hipEvent_t e0 = nullptr, e1 = nullptr;
-            unsigned long long* pc = d == 0 ? pace : nullptr;
             if (is_fused(dirs[d])) {
                 if (timed) RC(timing_events(b, HFM_KT_PCG_FUSED, &e0, &e1));
-                launch_pcg_fused(family(b, dirs[d]), dirs[d], k, pc, tag, e0, e1);
+                launch_pcg_fused(L, dirs[d], k, pc, tag, e0, e1);
             } else {
                 if (timed) RC(timing_events(b, HFM_KT_PCG_DIR, &e0, &e1));
-                launch_pcg_dir(family(b, dirs[d]), dirs[d], op, dt, k, pc, tag, e0, e1);
+                launch_pcg_dir(L, dirs[d], op, dt, k, pc, tag, e0, e1);
             }
+            pc = nullptr;
         }
         return HFM_OK;
     };
@@ -499,6 +575,7 @@ int run_pcg(Batch* b, int op, float dt) {
             --inflight;
         }
         if (inflight < 2 && launched < maxeff) {
+            RC(refresh_active());
             int g = pred - launched;
             g = g < 2 ? 2 : (g > 16 ? 16 : g);
             if (g > maxeff - launched) g = maxeff - launched;
@@ -508,7 +585,7 @@ int run_pcg(Batch* b, int op, float dt) {
                 if (two_pass) {
                     hipEvent_t e0 = nullptr, e1 = nullptr;
                     if (timed) RC(timing_events(b, HFM_KT_PCG_UPD, &e0, &e1));
-                    launch_pcg_upd(family(b, FAM_PCG), op, k, e0, e1);
+                    launch_pcg_upd(fl[FAM_PCG], op, k, e0, e1);
                 }
             }
             CHECK_LAUNCH();
@@ -517,6 +594,7 @@ int run_pcg(Batch* b, int op, float dt) {
             continue;
         }
         if (launched >= maxeff && !closed) {
+            RC(refresh_active());
             RC(pass_a(maxeff, b->pace_dev, false));
             CHECK_LAUNCH();
             closed = true;
