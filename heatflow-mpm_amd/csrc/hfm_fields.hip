@@ -10,7 +10,7 @@
 //   project     k_project                                              (grid.py:140-153)
 //   g2p         k_particle_update                                      (heat.py:161-174)
 // Buffer roles are fixed, so the per-scene descriptors never change between substeps: heat_prep
-// keeps the previous T in T2 (what the reference's swap leaves there, solver.py:150) and
+// keeps the previous T in T2 (what the reference's swap leaves there, solver.py:85) and
 // advection writes the new T in place; the face pass writes the advected velocity to us and the
 // projection writes the projected velocity back to u.
 #include "hfm_device.h"

@@ -1,7 +1,7 @@
 // hfm_p2g.hip — particle-to-grid transfer without float atomics (gfx950).
 //
 // The reference scatters every particle into up to ten lattices with float atomics
-// (solver.py:84-103, heat.py:12-18 via fields.py:39-48), so its sums depend on timing.  Here the
+// (solver.py:19-38, heat.py:12-18 via fields.py:39-48), so its sums depend on timing.  Here the
 // particles are binned by cell (counting sort: integer counts, exclusive scan, then the ids of a
 // bin sorted ascending), and every lattice node gathers its own contributions from the bins that
 // can reach it, in bin order then particle order.  The result is the same sum of the same terms

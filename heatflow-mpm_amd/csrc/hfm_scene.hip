@@ -1,6 +1,6 @@
 // hfm_scene.hip — C ABI of libhfm.so and the host-side order of the substep.
 //
-// HeatflowMPM.substep (solver.py:128-151) as launched here, for a batch of scenes on one stream
+// HeatflowMPM.substep (solver.py:63-86) as launched here, for a batch of scenes on one stream
 // (a single scene is a batch of one), one launch per stage over all scenes:
 //   [particles] bin (when positions changed) + deterministic P2G gather
 //   heat_prep (T2 = T) -> advect_T (into T) -> heat PCG (x = T) -> [particles] G2P
@@ -1066,7 +1066,7 @@ int hfm_set_particles(hfm_scene* h, int64_t P, const float* pos, const float* ve
         HIP_OK(hipMemcpy(d.yield, init.data(), P * sizeof(float), hipMemcpyHostToDevice));  // mpm.py:42
         std::fill(init.begin(), init.end(), (float)s->prm.mu_solid);
         HIP_OK(hipMemcpy(d.visc, init.data(), P * sizeof(float), hipMemcpyHostToDevice));   // mpm.py:43
-        HIP_OK(hipMemset(d.melt, 0, P * sizeof(float)));                                     // solver.py:126
+        HIP_OK(hipMemset(d.melt, 0, P * sizeof(float)));                                     // solver.py:61
         s->bins_dirty = 1;
     } else {
         RC(reset_particle_fields(s));

@@ -1,6 +1,6 @@
 """HeatflowBatch: many HeatflowMPM simulations stepped together on one GPU.
 
-No counterpart in the reference, which steps one HeatflowMPM at a time (solver.py:128-151).
+No counterpart in the reference, which steps one HeatflowMPM at a time (solver.py:63-86).
 A batch runs every stage of the substep as one kernel launch over all of its scenes
 (hfm_batch_* in include/hfm.h), so grids too small to fill the GPU alone fill it together.
 Scenes may differ in size, particles, heater and solver settings; each keeps its own PCG solves
@@ -36,7 +36,7 @@ class HeatflowBatch:
         self.sims = sims
 
     def substep(self, dt: float) -> None:
-        """One substep of every simulation (solver.py:128-153 for each), moving bodies included."""
+        """One substep of every simulation (solver.py:63-88 for each), moving bodies included."""
         for s in self.sims:
             b = s.body
             if b is not None and not isinstance(b, StaticBody):

@@ -1,6 +1,6 @@
 """HeatflowMPM: the air/heat substep of Heatflow-MPM on an MI355X.
 
-Drop-in for heatflow_mpm.solver.HeatflowMPM of the reference (solver.py:106-175): same
+Drop-in for heatflow_mpm.solver.HeatflowMPM of the reference (solver.py:41-110): same
 constructor, substep(dt) and accessors.  The substep itself runs in libhfm.so (hand-written
 HIP kernels for gfx950); this module only prepares the scene on the host (heater masks, walls,
 fixed cells, parameters) and moves data across the C ABI.  The MPM constitutive solve stays
@@ -29,7 +29,7 @@ HEAT_CHECK_EVERY = 10       # heat.py:209
 
 def scene_inputs(n_particles: int, mat: Material, heater: Heater, grid: GridConfig, spacing: float, t_init: float,
                  kappa_air: float) -> dict:
-    """Everything HeatflowMPM.__init__ derives on the host (solver.py:107-126, heater.py:33-65).
+    """Everything HeatflowMPM.__init__ derives on the host (solver.py:42-61, heater.py:33-65).
 
     Returns the scene parameters (fields of hfm_params) and the static arrays: fixed cells with
     their values (rods at t_rod over room walls at t_amb on x = 0, n-1, y = 0, n-1 and the
@@ -195,7 +195,7 @@ class _Scene:
 class HeatflowMPM:
     """Eulerian air and temperature coupled to a particle material; substep on the GPU.
 
-    Same constructor as the reference (solver.py:107-108).  Extras, not in the reference: `body`
+    Same constructor as the reference (solver.py:42-43).  Extras, not in the reference: `body`
     replaces the default StaticBody built from `particles`; `cg="fused"` runs the pressure PCG in
     one launch per iteration (HFM_FLAG_FUSED_CG in include/hfm.h) instead of warp's two-pass form.
     """
@@ -228,7 +228,7 @@ class HeatflowMPM:
             self._scene.set_particles(self.body.pos, self.body.vel, np.full(self.n, t_init, np.float32))
 
     def substep(self, dt: float) -> None:
-        """One air/heat substep (solver.py:128-151), then the body's own step (solver.py:152-153)."""
+        """One air/heat substep (solver.py:63-86), then the body's own step (solver.py:87-88)."""
         b = self.body
         if b is not None and not isinstance(b, StaticBody):
             self._scene.update_particles(b.pos, b.vel)
@@ -242,7 +242,7 @@ class HeatflowMPM:
             raise ValueError("run() needs a static body; call substep() for a moving one")
         return self._scene.run(dt, substeps)
 
-    # ---- state (solver.py:155-161)
+    # ---- state (solver.py:90-96)
     def positions(self):
         return self.body.positions() if self.body is not None else np.zeros((0, 3), np.float32)
 
@@ -277,7 +277,7 @@ class HeatflowMPM:
     def close(self) -> None:
         self._scene.close()
 
-    # ---- output records (solver.py:163-175)
+    # ---- output records (solver.py:98-110)
     def static_dict(self) -> dict:
         hg = self.heater_geom
         return dict(L=self.grid.L, h=self.grid.h, n=self.grid.n, n_particles=self.n, spacing=self.spacing,

@@ -1,7 +1,7 @@
 /* hfm.h — C ABI of libhfm.so, the MI355X (gfx950) air/heat substep of Heatflow-MPM.
  *
  * The reference exposes this path as the Python class heatflow_mpm.solver.HeatflowMPM
- * (heatflow_mpm/solver.py:106-175) on top of AirGrid (heatflow_mpm/grid.py:177-263) and
+ * (heatflow_mpm/solver.py:41-110) on top of AirGrid (heatflow_mpm/grid.py:177-263) and
  * HeatField (heatflow_mpm/heat.py:177-210).  Every entry point below replaces one piece of
  * that class; the Python package heatflow_mpm (heatflow-mpm_amd/heatflow_mpm) binds them with
  * ctypes and keeps the reference's method names.
@@ -55,15 +55,15 @@ typedef struct hfm_params {
     double alpha_f_min;       /* floor of the air fraction (grid.py:20) */
     double cg_tol;            /* pressure PCG relative tolerance (grid.py:21) */
     double heat_tol;          /* heat PCG relative tolerance, 1e-6 (heat.py:198) */
-    double kappa_air;         /* KAPPA_AIR or the constructor argument (solver.py:15,108) */
+    double kappa_air;         /* KAPPA_AIR or the constructor argument (solver.py:15,43) */
     double kappa_mat;         /* Material.kappa (mpm.py:17) */
-    double vol;               /* particle volume spacing^3 (solver.py:130) */
+    double vol;               /* particle volume spacing^3 (solver.py:65) */
     double k_particle;        /* K_PARTICLE (solver.py:16) */
     double t_sol, t_liq;      /* melt window (mpm.py:12-13) */
     double yield_solid;       /* mpm.py:14 */
     double mu_solid;          /* viscosity_solid (mpm.py:15) */
     double mu_liquid;         /* viscosity_liquid (mpm.py:16) */
-    double t_max;             /* max(t_rod, t_cap, t_amb, t_init) (solver.py:122) */
+    double t_max;             /* max(t_rod, t_cap, t_amb, t_init) (solver.py:57) */
 } hfm_params;
 
 #define HFM_PREC_FP32 0       /* the reference's arithmetic: fp32 storage and math */
@@ -98,8 +98,8 @@ typedef struct hfm_stats {
 /* Field identifiers for hfm_get_field / hfm_set_field. */
 enum hfm_field {
     HFM_FIELD_T = 0,      /* cell temperature, HeatField.T (heat.py:181) */
-    HFM_FIELD_T2,         /* HeatField.T2: after a substep, the previous T (swap, solver.py:150) */
-    HFM_FIELD_TW,         /* blended/fixed/sourced T before advection (solver.py:139-142) */
+    HFM_FIELD_T2,         /* HeatField.T2: after a substep, the previous T (swap, solver.py:85) */
+    HFM_FIELD_TW,         /* blended/fixed/sourced T before advection (solver.py:74-77) */
     HFM_FIELD_U,          /* x-face velocity (grid.py:186) */
     HFM_FIELD_V,          /* y-face velocity */
     HFM_FIELD_W,          /* z-face velocity */
@@ -110,7 +110,7 @@ enum hfm_field {
     HFM_FIELD_TSUM,       /* particle heat sum, HeatField.tsum */
     HFM_FIELD_KAPPA,      /* cell diffusivity (heat.py:44-47) */
     HFM_FIELD_AX, HFM_FIELD_AY, HFM_FIELD_AZ,   /* face air fractions (grid.py:46-59) */
-    HFM_FIELD_QX, HFM_FIELD_QY, HFM_FIELD_QZ,   /* particle flux, zeroed on walls (solver.py:95-97) */
+    HFM_FIELD_QX, HFM_FIELD_QY, HFM_FIELD_QZ,   /* particle flux, zeroed on walls (solver.py:30-32) */
     HFM_FIELD_KX, HFM_FIELD_KY, HFM_FIELD_KZ,   /* relaxation weights, particle + static */
     HFM_FIELD_MX, HFM_FIELD_MY, HFM_FIELD_MZ,   /* relaxation momenta, particle + static */
     HFM_FIELD_PDIAG,      /* pressure Jacobi preconditioner (grid.py:118-137) */
@@ -138,14 +138,14 @@ int hfm_set_fixed(hfm_scene* s, const int32_t* fixed, const float* tfix);
 int hfm_set_source(hfm_scene* s, const int32_t* mask, double rate, double t_cap);
 
 /* The material enters as particle positions, velocities and temperatures (the state the
- * reference reads from ThermalMPM: mpm.py:54-60 and solver.py:125).  P may be 0. */
+ * reference reads from ThermalMPM: mpm.py:54-60 and solver.py:60).  P may be 0. */
 int hfm_set_particles(hfm_scene* s, int64_t P, const float* pos, const float* vel,
                       const float* temp);
 
 /* Positions/velocities only (a moving body between substeps); temperatures are kept. */
 int hfm_update_particles(hfm_scene* s, const float* pos, const float* vel);
 
-/* HeatflowMPM.substep(dt) without Newton's b.step (solver.py:128-151). */
+/* HeatflowMPM.substep(dt) without Newton's b.step (solver.py:63-86). */
 int hfm_substep(hfm_scene* s, double dt);
 
 /* K substeps back to back; *ms (may be NULL) receives the HIP-event time on the scene's
@@ -189,7 +189,7 @@ int hfm_synchronize(hfm_scene* s);
 int64_t hfm_scene_bytes(const hfm_scene* s);
 
 /* ---- Batches: many scenes stepped together ------------------------------------------------
- * No counterpart in the reference, which steps one HeatflowMPM at a time (solver.py:128-151):
+ * No counterpart in the reference, which steps one HeatflowMPM at a time (solver.py:63-86):
  * a batch runs the same substep for every scene it holds, each stage one kernel launch over all
  * of them, so small grids fill the GPU together.  Scenes may differ in size, particles, heater
  * and solver settings; each keeps its own PCG solves (own tolerances, iteration counts, stats)

@@ -204,7 +204,7 @@ ORC_API float orc_numpy_mean_f32(const float* a, long n) {
 }
 
 /* ------------------------------------------------------------------------------------------
- * Scene state (AirGrid grid.py:181-196, HeatField heat.py:178-187, HeatflowMPM solver.py:106-126)
+ * Scene state (AirGrid grid.py:181-196, HeatField heat.py:178-187, HeatflowMPM solver.py:41-61)
  * ------------------------------------------------------------------------------------------ */
 
 typedef struct orc_scene {
@@ -325,7 +325,7 @@ ORC_API void orc_set_source(orc_scene* s, const int32_t* mask, double rate, doub
     s->t_cap = t_cap;
 }
 
-/* solver.py:124-126 plus the body arrays of mpm.py:54-68 (static body) */
+/* solver.py:59-61 plus the body arrays of mpm.py:54-68 (static body) */
 ORC_API void orc_set_particles(orc_scene* s, long P, const float* pos, const float* vel, const float* temp) {
     free(s->pos); free(s->vel); free(s->temp); free(s->melt); free(s->yield); free(s->visc);
     s->P = P;
@@ -626,7 +626,7 @@ static void pcg(const opctx* op, const real* diag, const real* b, real* x, long 
 }
 
 /* ------------------------------------------------------------------------------------------
- * Substep stages (solver.py:128-151, grid.py:219-251, heat.py:12-47,147-174,198-210)
+ * Substep stages (solver.py:63-86, grid.py:219-251, heat.py:12-47,147-174,198-210)
  * ------------------------------------------------------------------------------------------ */
 
 enum {
@@ -653,7 +653,7 @@ enum {
     ORC_ST_COUNT
 };
 
-/* grid.py:202-204 and solver.py:132-133 */
+/* grid.py:202-204 and solver.py:67-68 */
 static void st_clear(orc_scene* s) {
     memset(s->alpha_s, 0, (size_t)s->N * sizeof(real));
     for (int a = 0; a < 3; ++a) {
@@ -665,7 +665,7 @@ static void st_clear(orc_scene* s) {
     memset(s->tsum, 0, (size_t)s->N * sizeof(real));
 }
 
-/* solver.py:84-103 (k_scatter_particles) then heat.py:12-18 (k_scatter_heat) */
+/* solver.py:19-38 (k_scatter_particles) then heat.py:12-18 (k_scatter_heat) */
 static void st_p2g(orc_scene* s) {
     int n = s->n;
     real h = s->h;
@@ -714,7 +714,7 @@ static void st_blend(orc_scene* s) {
     }
 }
 
-/* heat.py:154-158, launched only when rate > 0 (solver.py:141) */
+/* heat.py:154-158, launched only when rate > 0 (solver.py:76) */
 static void st_source(orc_scene* s, double dt) {
     if (!(s->rate > 0)) return;
     real rate = (real)s->rate, tcap = (real)s->t_cap, d = (real)dt;
@@ -795,7 +795,7 @@ static void st_particle_update(orc_scene* s) {
     }
 }
 
-/* solver.py:150 */
+/* solver.py:85 */
 static void st_swap_T(orc_scene* s) {
     real* tmp = s->T;
     s->T = s->T2;
@@ -1004,7 +1004,7 @@ ORC_API int orc_stage(orc_scene* s, int stage, double dt) {
     return 0;
 }
 
-/* solver.py:128-151: the whole substep without Newton's b.step */
+/* solver.py:63-86: the whole substep without Newton's b.step */
 ORC_API void orc_substep(orc_scene* s, double dt) {
     for (int st = 0; st < ORC_ST_COUNT; ++st) {
         if (st == ORC_ST_PARTICLE_UPDATE && s->P == 0) continue;

@@ -79,7 +79,7 @@ def test_heater_off_has_no_cells_and_no_rate():
     assert cell.max() == 0 and rate == 0.0
 
 
-# ---- scene inputs (solver.py:107-126) ---------------------------------------------------------
+# ---- scene inputs (solver.py:42-61) ---------------------------------------------------------
 
 def test_walls_and_rods_fixed_floor_adiabatic():
     grid = GridConfig(L=0.48, h=0.01)
