@@ -12,5 +12,6 @@ from .mpm import Material, StaticBody  # noqa: F401
 from .scenes import ellipsoid_particles, plume_scene, coupled_scene, batched_scenes  # noqa: F401
 from .batch import HeatflowBatch  # noqa: F401
 from .serve import Request, Server, poisson_requests  # noqa: F401
+from .frames import FrameWriter  # noqa: F401
 
 __version__ = "0.2.0"

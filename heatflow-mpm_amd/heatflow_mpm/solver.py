@@ -288,8 +288,12 @@ class HeatflowMPM:
                     **{f"mat_{k}": v for k, v in asdict(self.mat).items() if not isinstance(v, str)})
 
     def frame_dict(self, t: float, with_air: bool = True) -> dict:
+        if self.n:
+            temp, melt = (a[: self.n] for a in self._scene.particles()[:2])    # one read for both
+        else:
+            temp = melt = np.zeros(0, np.float32)
         d = dict(t=np.float32(t), pos=self.positions().astype(np.float32), vel=self.velocities().astype(np.float32),
-                 temp=self.temperatures().astype(np.float16), melt=self.melt_fraction().astype(np.float16),
+                 temp=temp.astype(np.float16), melt=melt.astype(np.float16),
                  grid_T=self.grid_T().reshape(-1).astype(np.float16))
         if with_air:
             d["air_u"] = self.air_velocity().astype(np.float16)
