@@ -1,7 +1,9 @@
 """Benchmark of the air/heat substep on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cg warp|fused]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cg warp|fused] [--precision fp32|mixed]
     python bench.py --workload batched [--scenes 64] [--steps 200]
+    python bench.py --workload served [--rate 20] [--requests 24] [--precision mixed]
+    python bench.py --workload mixed [--n 192] [--steps 100] [--fp64-n 48]
 
 Workload: the default coupled scene at n=96 (0.48 m box, h = 5 mm, a static ellipsoid of ~26.8k
 particles at 20 C, 400 W heater capped at 120 C with a 2 m/s fan, pressure and heat PCG to a
@@ -42,6 +44,9 @@ HBM_PEAK_GBS = 8000.0
 #   upd:   reads x, p, r, Ap, D; writes x, r, z                        -> 8 words
 #   fused: reads x, r, p_old, D, alpha_s; writes x, r, p              -> 8 words (whole iteration)
 KERNEL_WORDS = {_abi.KT_PCG_DIR: 5, _abi.KT_PCG_UPD: 8, _abi.KT_PCG_FUSED: 8}
+# Mixed precision, inner solve (bytes per cell): pass A reads z, p_old (FP16) and alpha_s, writes
+# p, Ap (FP16) -> 12 B; pass B reads x (fp32), p, r, Ap (FP16), D (fp32), writes x, r, z -> 22 B
+KERNEL_BYTES_MIXED = {_abi.KT_PCG_DIR: 12, _abi.KT_PCG_UPD: 22}
 KERNEL_NAMES = {_abi.KT_PCG_DIR: "k_pcg_dir_tile3", _abi.KT_PCG_UPD: "k_pcg_upd<1>",
                 _abi.KT_PCG_FUSED: "k_pcg_fused"}
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "pcg_pressure_n96_traffic.json")
@@ -112,7 +117,8 @@ def bench_batched(a) -> dict:
         wb.run(dt, 2)
     for w in warm:
         w.close()
-    plain = [HeatflowMPM(**kw, cg=a.cg) for _, kw in scenes]
+    prec = a.precision or "fp32"
+    plain = [HeatflowMPM(**kw, cg=a.cg, precision=prec) for _, kw in scenes]
     t_plain, p_iters_plain = 0.0, 0
     for s in plain:
         s.synchronize()
@@ -121,7 +127,7 @@ def bench_batched(a) -> dict:
         s.synchronize()
         t_plain += time.perf_counter() - t0
         p_iters_plain += s.stats()["p_iters_total"]
-    sims = [HeatflowMPM(**kw, cg=a.cg) for _, kw in scenes]
+    sims = [HeatflowMPM(**kw, cg=a.cg, precision=prec) for _, kw in scenes]
     batch = HeatflowBatch(sims)
     batch.synchronize()
     t0 = time.perf_counter()
@@ -143,7 +149,8 @@ def bench_batched(a) -> dict:
                config=dict(workload=f"{len(scenes)} independent scenes, n in {{32,48,64,96,128}} "
                                     f"(counts {[ns.count(m) for m in (32, 48, 64, 96, 128)]}), heater 100-800 W, "
                                     f"fan 0-3 m/s, 2k-50k particles, PCG tol 1e-6, dt 2 ms, {K} substeps from rest",
-                           scenes=len(scenes), cells=cells, seed=a.seed, pressure_cg=a.cg, parallelism="1 GPU"),
+                           scenes=len(scenes), cells=cells, seed=a.seed, pressure_cg=a.cg, precision=prec,
+                           parallelism="1 GPU"),
                plain=dict(value=plain_value, seconds=round(t_plain, 3), pressure_iters_total=int(p_iters_plain)),
                speedup_vs_plain=round(value / plain_value, 3), batched_seconds=round(t_batch, 3),
                identical_to_plain=same)
@@ -153,12 +160,20 @@ def bench_batched(a) -> dict:
     return out
 
 
+def rel_l2(a, b) -> float:
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    den = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / den) if den > 0 else float(np.linalg.norm(a - b))
+
+
 def bench_served(a) -> dict:
     """A seeded Poisson stream of rollout requests (n from {48, 64, 96, 128, 192}, K uniform in
     [kmin, kmax], heaters / fans / particle blobs as the batched scenes), served twice on the same
-    arrival times: with continuous batching (all admitted scenes advance in one batched substep;
-    admission in arrival order by device footprint) and in the plain form (one request at a time,
-    in arrival order).  Latency = completion - arrival."""
+    arrival times: with continuous batching in the chosen precision (all admitted scenes advance in
+    one batched substep; admission in arrival order by device footprint) and in the plain form
+    (fp32, one request at a time, in arrival order).  Latency = completion - arrival.  Each served
+    result is compared with the plain result of the same request (relative L2 of T and u)."""
     reqs = poisson_requests(a.rate, a.requests, seed=a.seed, k_range=(a.kmin, a.kmax))
     warm = [HeatflowMPM(**coupled_scene(32)[0], cg=a.cg) for _ in range(2)]
     warm[0].run(2e-3, 2)
@@ -168,37 +183,120 @@ def bench_served(a) -> dict:
         w.close()
     from heatflow_mpm.serve import GpuBackend
     budget = int(a.hbm_gib * (1 << 30))
-    served = Server(GpuBackend(cg=a.cg), hbm_budget=budget, max_active=a.slots)
+    prec = a.precision or "mixed"
+    served = Server(GpuBackend(cg=a.cg, precision=prec), hbm_budget=budget, max_active=a.slots, keep_results=True)
     t0 = time.perf_counter()
     res = served.run(reqs)
     span = time.perf_counter() - t0
     s_b = summarize(res, span)
-    plain = Server(GpuBackend(cg=a.cg), hbm_budget=budget, max_active=1)
+    plain = Server(GpuBackend(cg=a.cg), hbm_budget=budget, max_active=1, keep_results=True)
     t0 = time.perf_counter()
     res_p = plain.run(reqs)
     span_p = time.perf_counter() - t0
     s_p = summarize(res_p, span_p)
     ns = [r.kwargs["grid"].n for r in reqs]
+    err_T = err_u = 0.0
+    for rb, rp in zip(res, res_p):
+        err_T = max(err_T, rel_l2(rb.fields["T"], rp.fields["T"]))
+        ub = np.concatenate([rb.fields[f].ravel() for f in ("U", "V", "W")])
+        up = np.concatenate([rp.fields[f].ravel() for f in ("U", "V", "W")])
+        err_u = max(err_u, rel_l2(ub, up))
     return dict(metric="served p50/p99 latency and aggregate cell-substeps/s, continuous batching",
                 value=s_b["cell_substeps_per_s"], unit="cell-substeps/s", n_gpus=1, steps=None, warmup=0,
-                ms_per_step=None, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32",
+                ms_per_step=None, higher_is_better=True, scaling="weak", vs_baseline=None,
+                dtype="f32" if prec == "fp32" else "f32 state, f16 PCG vectors, bf16 departure velocity",
                 data="synthetic",
                 config=dict(workload=f"{len(reqs)} Poisson requests at {a.rate}/s, n in {{48,64,96,128,192}} "
                                      f"(counts {[ns.count(m) for m in (48, 64, 96, 128, 192)]}), K uniform "
                                      f"{a.kmin}-{a.kmax}, heater 100-800 W, fan 0-3 m/s, 2k-50k particles, "
                                      f"PCG tol 1e-6, dt 2 ms", rate_per_s=a.rate, requests=len(reqs), seed=a.seed,
-                            hbm_budget_gib=a.hbm_gib, slots=a.slots, pressure_cg=a.cg, parallelism="1 GPU"),
+                            hbm_budget_gib=a.hbm_gib, slots=a.slots, pressure_cg=a.cg, precision=prec,
+                            parallelism="1 GPU"),
+                max_rel_l2_vs_plain=dict(T=err_T, u=err_u),
                 served=s_b, plain=s_p, speedup_vs_plain=round(s_b["cell_substeps_per_s"] / s_p["cell_substeps_per_s"], 3),
                 p99_ratio_plain_over_served=round(s_p["p99_s"] / s_b["p99_s"], 3),
                 max_batch=max(e[3] for e in served.log))
 
 
+def bench_mixed(a) -> dict:
+    """The coupled scene at n=192 (h = 2.5 mm) with the ellipsoid seeded to about 50k particles,
+    K substeps from rest, once in fp32 and once in the mixed form (iterative refinement with FP16
+    PCG vectors, BF16 departure velocity) on the same GPU.  Reports both rates, the pressure-PCG
+    kernel times, and the relative L2 of the mixed T and u against fp32; with --fp64-n m also the
+    error of both forms against the fp64 CPU oracle after K substeps of the same scene at n = m."""
+    n = a.n if a.n_given else 192
+    K = a.steps if a.steps_given else 100
+    kw, dt, _ = coupled_scene(n, particles_target=50000)
+    out_forms = {}
+    fields = {}
+    for form in ("fp32", "mixed"):
+        sim = HeatflowMPM(**kw, precision=form)
+        sim._scene.set_kernel_timing(True)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.run(dt, K)
+        sim.synchronize()
+        wall = time.perf_counter() - t0
+        st = sim.stats()
+        kt = {c: sim._scene.kernel_time(c) for c in (_abi.KT_PCG_DIR, _abi.KT_PCG_UPD)}
+        dm, sm = sim._scene.diagnostics()
+        fields[form] = {f: sim._scene.get_field(f) for f in ("T", "U", "V", "W", "P")}
+        out_forms[form] = dict(value=n ** 3 * K / wall, seconds=round(wall, 3), ms_per_step=wall * 1e3 / K,
+                               pressure_iters_per_substep=st["p_iters_total"] / K,
+                               heat_iters_per_substep=st["h_iters_total"] / K,
+                               pass_a_us=round(kt[_abi.KT_PCG_DIR][0] * 1e3 / max(kt[_abi.KT_PCG_DIR][1], 1), 2),
+                               pass_b_us=round(kt[_abi.KT_PCG_UPD][0] * 1e3 / max(kt[_abi.KT_PCG_UPD][1], 1), 2),
+                               divergence_max=dm, speed_max=sm, p_converged=st["p_converged"],
+                               p_outer_last=st["p_outer"], h_outer_last=st["h_outer"], particles=sim.n)
+        sim.close()
+    fm, f32 = fields["mixed"], fields["fp32"]
+    u_m = np.concatenate([fm[f].ravel() for f in "UVW"])
+    u_32 = np.concatenate([f32[f].ravel() for f in "UVW"])
+    vs_fp32 = dict(T=rel_l2(fm["T"], f32["T"]), u=rel_l2(u_m, u_32),
+                   p_centered=rel_l2(fm["P"] - fm["P"].mean(), f32["P"] - f32["P"].mean()))
+    vs_fp64 = None
+    if a.fp64_n:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle as O
+        kw_s, _, _ = coupled_scene(a.fp64_n, particles_target=int(50000 * (a.fp64_n / 192) ** 3) or None)
+        ref = O.OracleSim(**kw_s, precision="f64")
+        ref.run(dt, K)
+        uref = np.concatenate([ref.field(f).ravel() for f in "UVW"])
+        vs_fp64 = dict(n=a.fp64_n, substeps=K)
+        for form in ("fp32", "mixed"):
+            sim = HeatflowMPM(**kw_s, precision=form)
+            sim.run(dt, K)
+            sim.synchronize()
+            ug = np.concatenate([sim._scene.get_field(f).ravel() for f in "UVW"])
+            dm, _ = sim._scene.diagnostics()
+            vs_fp64[form] = dict(T=rel_l2(sim._scene.get_field("T"), ref.field("T")), u=rel_l2(ug, uref),
+                                 divergence_max=dm)
+            sim.close()
+        vs_fp64["fp64_divergence_max"] = ref.diagnostics()[0]
+    mixed = out_forms["mixed"]
+    return dict(metric="cell-substeps/s, mixed precision (iterative refinement, FP16 PCG vectors)",
+                value=mixed["value"], unit="cell-substeps/s", n_gpus=1, steps=K, warmup=0,
+                ms_per_step=mixed["ms_per_step"], higher_is_better=True, scaling="weak", vs_baseline=None,
+                dtype="f32 state, f16 PCG vectors, bf16 departure velocity", data="synthetic",
+                config=dict(workload=f"coupled scene n={n} (h={0.48 / n * 1e3:.2f} mm), static ellipsoid "
+                                     f"P={mixed['particles']}, heater 400 W cap 120 C fan 2 m/s, PCG tol 1e-6, "
+                                     f"dt 2 ms, {K} substeps from rest", n=n, parallelism="1 GPU"),
+                fp32=out_forms["fp32"], mixed=mixed,
+                speedup_vs_fp32=round(mixed["value"] / out_forms["fp32"]["value"], 3),
+                rel_l2_mixed_vs_fp32=vs_fp32, error_vs_fp64=vs_fp64)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("coupled", "batched", "served"), default="coupled",
+    ap.add_argument("--workload", choices=("coupled", "batched", "served", "mixed"), default="coupled",
                     help="coupled: one n=96 coupled scene (default); batched: uneven scenes in one batch vs one by "
-                         "one; served: a Poisson request stream with continuous batching vs one at a time")
+                         "one; served: a Poisson request stream with continuous batching vs one at a time; "
+                         "mixed: the n=192 scene in fp32 and in mixed precision")
+    ap.add_argument("--precision", choices=tuple(HeatflowMPM.PRECISIONS), default=None,
+                    help="coupled/batched: precision of the substep (fp32 default); served: precision of the "
+                         "batched server (mixed default; the plain form is fp32)")
+    ap.add_argument("--fp64-n", type=int, default=0, help="mixed: grid size of the fp64-oracle error check (0: off)")
     ap.add_argument("--rate", type=float, default=20.0, help="served: arrivals per second")
     ap.add_argument("--requests", type=int, default=24, help="served: number of requests")
     ap.add_argument("--kmin", type=int, default=20, help="served: fewest substeps of a request")
@@ -209,7 +307,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0, help="batched: scene draw seed")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--n", type=int, default=96)
+    ap.add_argument("--n", type=int, default=None, help="grid size (coupled: 96, mixed: 192)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cg", choices=HeatflowMPM.CG_FORMS, default="warp",
                     help="pressure PCG form: warp's two-pass iteration, or one fused launch per iteration")
@@ -219,6 +317,12 @@ def main() -> int:
               "independent scenes, not one scene split); run with --gpus 1", file=sys.stderr)
         return 2
     a.steps_given = a.steps is not None
+    a.n_given = a.n is not None
+    if a.n is None:
+        a.n = 96
+    if a.workload == "mixed":
+        print(json.dumps(bench_mixed(a)), flush=True)
+        return 0
     if a.workload == "batched":
         print(json.dumps(bench_batched(a)), flush=True)
         return 0
@@ -228,8 +332,9 @@ def main() -> int:
     if a.steps is None:
         a.steps = 500
 
+    prec = a.precision or "fp32"
     kw, dt, _ = coupled_scene(a.n)
-    sim = HeatflowMPM(**kw, device="cuda:0", cg=a.cg)
+    sim = HeatflowMPM(**kw, device="cuda:0", cg=a.cg, precision=prec)
     n, N, P = sim.grid.n, sim.grid.n ** 3, sim.n
     if a.warmup:
         sim.run(dt, a.warmup)
@@ -248,7 +353,7 @@ def main() -> int:
     I_p = (st1["p_iters_total"] - st0["p_iters_total"]) / a.steps
     I_h = (st1["h_iters_total"] - st0["h_iters_total"]) / a.steps
     value = N * a.steps / wall
-    hbm_tbs = substep_bytes(N, P, I_p, I_h, a.cg == "fused") * a.steps / wall / 1e12
+    hbm_tbs = substep_bytes(N, P, I_p, I_h, a.cg == "fused") * a.steps / wall / 1e12 if prec == "fp32" else None
 
     # second timed region with HIP events around every pressure-PCG launch
     sim._scene.set_kernel_timing(True)
@@ -262,9 +367,9 @@ def main() -> int:
     sim._scene.set_kernel_timing(False)
     dom = max(kt, key=lambda c: kt[c][0])
     total_ms, launches = kt[dom]
-    bytes_per_launch = KERNEL_WORDS[dom] * 4.0 * N
+    bytes_per_launch = (KERNEL_WORDS[dom] * 4.0 if prec == "fp32" else KERNEL_BYTES_MIXED[dom]) * N
     achieved = bytes_per_launch * iters / (total_ms * 1e-3) / 1e9
-    traffic = measured_traffic(KERNEL_NAMES[dom], n)
+    traffic = measured_traffic(KERNEL_NAMES[dom], n) if prec == "fp32" else None
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4),
                     traffic=None if traffic is None else round(traffic), kernel=KERNEL_NAMES[dom],
@@ -275,10 +380,11 @@ def main() -> int:
 
     out = dict(metric=METRIC, value=value, unit="cell-substeps/s", n_gpus=1, steps=a.steps, warmup=a.warmup,
                ms_per_step=wall * 1e3 / a.steps, higher_is_better=True, scaling="weak", vs_baseline=None,
-               dtype="f32", data="synthetic",
+               dtype="f32" if prec == "fp32" else "f32 state, f16 PCG vectors, bf16 departure velocity",
+               data="synthetic",
                config=dict(workload=f"coupled scene n={n} (h=5 mm), static ellipsoid P={P}, heater 400 W cap 120 C fan "
                                     f"2 m/s, PCG tol 1e-6, dt 2 ms", n=n, particles=P, dt=dt, cg_tol=1e-6,
-                           pressure_cg=a.cg,
+                           pressure_cg=a.cg, precision=prec,
                            pressure_iters_per_substep=round(I_p, 2), heat_iters_per_substep=round(I_h, 2),
                            parallelism="1 GPU"),
                hbm_tb_s=round(hbm_tbs, 3), roofline=roofline, cpu_baseline=cpu)

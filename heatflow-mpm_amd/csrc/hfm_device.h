@@ -44,9 +44,21 @@ __device__ __forceinline__ void base_index1(float pos, float off, float h, int& 
     f = g - (float)i;
 }
 
+// Lattice values are fp32, or BF16 bit patterns (the upper half of the fp32 pattern).
+__device__ __forceinline__ float load_elem(const float* a, long i) { return a[i]; }
+__device__ __forceinline__ float load_elem(const uint16_t* a, long i) { return __uint_as_float((unsigned)a[i] << 16); }
+
+// fp32 -> BF16 rounded to nearest even (finite values)
+__device__ __forceinline__ uint16_t to_bf16(float x) {
+    unsigned u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
 // fields.py:25-36 — trilinear sample on a lattice of dims (nx, ny, nz), clamped indices, weights
 // from the unclamped fractions, accumulation order dx, dy, dz.
-__device__ __forceinline__ float sample(const float* __restrict__ a, float px, float py, float pz, float ox,
+template <class E = float>
+__device__ __forceinline__ float sample(const E* __restrict__ a, float px, float py, float pz, float ox,
                                         float oy, float oz, float h, int nx, int ny, int nz) {
     int bi, bj, bk;
     float fx, fy, fz;
@@ -62,14 +74,14 @@ __device__ __forceinline__ float sample(const float* __restrict__ a, float px, f
         for (int dy = 0; dy < 2; ++dy) {
             int iy = clampi(bj + dy, 0, ny - 1);
             float wy = dy ? fy : 1.0f - fy;
-            const float* row = a + ((long)ix * ny + iy) * nz;
+            const E* row = a + ((long)ix * ny + iy) * nz;
 #pragma unroll
             for (int dz = 0; dz < 2; ++dz) {
                 int iz = clampi(bk + dz, 0, nz - 1);
                 float wz = dz ? fz : 1.0f - fz;
                 float w = wx * wy;
                 w = w * wz;
-                s += w * row[iz];
+                s += w * load_elem(row, iz);
             }
         }
     }
@@ -77,23 +89,25 @@ __device__ __forceinline__ float sample(const float* __restrict__ a, float px, f
 }
 
 // fields.py:68-72
-__device__ __forceinline__ void sample_velocity(const float* __restrict__ u, const float* __restrict__ v,
-                                                const float* __restrict__ w, float px, float py, float pz, float h,
+template <class E = float>
+__device__ __forceinline__ void sample_velocity(const E* __restrict__ u, const E* __restrict__ v,
+                                                const E* __restrict__ w, float px, float py, float pz, float h,
                                                 int n, float& vx, float& vy, float& vz) {
-    vx = sample(u, px, py, pz, 0.0f, 0.5f, 0.5f, h, n + 1, n, n);
-    vy = sample(v, px, py, pz, 0.5f, 0.0f, 0.5f, h, n, n + 1, n);
-    vz = sample(w, px, py, pz, 0.5f, 0.5f, 0.0f, h, n, n, n + 1);
+    vx = sample<E>(u, px, py, pz, 0.0f, 0.5f, 0.5f, h, n + 1, n, n);
+    vy = sample<E>(v, px, py, pz, 0.5f, 0.0f, 0.5f, h, n, n + 1, n);
+    vz = sample<E>(w, px, py, pz, 0.5f, 0.5f, 0.0f, h, n, n, n + 1);
 }
 
-// fields.py:111-116 — midpoint RK2 departure point
-__device__ __forceinline__ void backtrace(const float* __restrict__ u, const float* __restrict__ v,
-                                          const float* __restrict__ w, float& x, float& y, float& z, float h, int n,
+// fields.py:111-116 — midpoint RK2 departure point (E = uint16_t: through BF16 copies of u, v, w)
+template <class E = float>
+__device__ __forceinline__ void backtrace(const E* __restrict__ u, const E* __restrict__ v,
+                                          const E* __restrict__ w, float& x, float& y, float& z, float h, int n,
                                           float dt) {
     float vx, vy, vz;
-    sample_velocity(u, v, w, x, y, z, h, n, vx, vy, vz);
+    sample_velocity<E>(u, v, w, x, y, z, h, n, vx, vy, vz);
     float hd = 0.5f * dt;
     float x1 = x - hd * vx, y1 = y - hd * vy, z1 = z - hd * vz;
-    sample_velocity(u, v, w, x1, y1, z1, h, n, vx, vy, vz);
+    sample_velocity<E>(u, v, w, x1, y1, z1, h, n, vx, vy, vz);
     x = x - dt * vx;
     y = y - dt * vy;
     z = z - dt * vz;

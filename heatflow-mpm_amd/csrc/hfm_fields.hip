@@ -79,7 +79,10 @@ __global__ __launch_bounds__(kBlock) void k_advect_T(Src src, float dt) {
         t2 = a.Tw[t];
     } else {
         float x = ((float)i + 0.5f) * h, y = ((float)j + 0.5f) * h, z = ((float)k + 0.5f) * h;
-        backtrace(a.u[0], a.u[1], a.u[2], x, y, z, h, n, dt);
+        if (a.mixed)
+            backtrace<uint16_t>(a.ub[0], a.ub[1], a.ub[2], x, y, z, h, n, dt);
+        else
+            backtrace<float>(a.u[0], a.u[1], a.u[2], x, y, z, h, n, dt);
         t2 = sample(a.Tw, x, y, z, 0.5f, 0.5f, 0.5f, h, n, n, n);
     }
     a.T[t] = t2;
@@ -159,7 +162,10 @@ __global__ __launch_bounds__(kBlock) void k_air_faces(Src src, float dt) {
         x = ((float)i + 0.5f) * h; y = ((float)j + 0.5f) * h; z = (float)k * h;
         ox = 0.5f; oy = 0.5f; oz = 0.0f; dz = n + 1;
     }
-    backtrace(a.u[0], a.u[1], a.u[2], x, y, z, h, n, dt);
+    if (a.mixed)
+        backtrace<uint16_t>(a.ub[0], a.ub[1], a.ub[2], x, y, z, h, n, dt);
+    else
+        backtrace<float>(a.u[0], a.u[1], a.u[2], x, y, z, h, n, dt);
     float val = sample(a.u[axis], x, y, z, ox, oy, oz, h, dx, dy, dz);
     const long c = ((long)i * n + j) * n + k;
     if (axis == 2 && a.buoyancy) {
@@ -327,13 +333,23 @@ __global__ __launch_bounds__(kBlock) void k_project(Src src, double dt) {
     const int idx = axis == 0 ? i : (axis == 1 ? j : k);
     if (idx == 0 || idx == n) {
         a.u[axis][f] = 0.0f;
+        if (a.mixed) a.ub[axis][f] = 0;
         return;
     }
     const float dor = (float)(dt / a.rho_d);  // dt/rho formed in double (grid.py:251)
     const long c = ((long)i * n + j) * n + k;
     const long cl = axis == 0 ? c - (long)n * n : (axis == 1 ? c - n : c - 1);
     const float gp = a.p[c] - a.p[cl];
-    a.u[axis][f] = a.us[axis][f] - dor * gp / a.h;
+    const float val = a.us[axis][f] - dor * gp / a.h;
+    a.u[axis][f] = val;
+    if (a.mixed) a.ub[axis][f] = to_bf16(val);  // departure points of the next substep's advection
+}
+
+// BF16 copies of u, v, w (after the host wrote the velocity)
+__global__ __launch_bounds__(kBlock) void k_bf16_copy(const float* __restrict__ u, uint16_t* __restrict__ ub,
+                                                      long F) {
+    const long f = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (f < F) ub[f] = to_bf16(u[f]);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -399,6 +415,12 @@ void launch_air_faces(const Launch& L, float dt) { HFM_LAUNCH(k_air_faces, L, kB
 void launch_divergence(const Launch& L, double dt) { HFM_LAUNCH(k_divergence, L, kBlock, nullptr, nullptr, dt); }
 void launch_rhs_mean(const Launch& L) { HFM_LAUNCH(k_rhs_mean, L, kBlock, nullptr, nullptr); }
 void launch_project(const Launch& L, double dt) { HFM_LAUNCH(k_project, L, kBlock, nullptr, nullptr, dt); }
+
+void launch_bf16_copy(const Scene* s, hipStream_t st) {
+    if (!s->dev.mixed) return;
+    for (int a = 0; a < 3; ++a)
+        hipLaunchKernelGGL(k_bf16_copy, dim3(nblocks(s->F)), dim3(kBlock), 0, st, s->dev.u[a], s->dev.ub[a], s->F);
+}
 
 void launch_divergence_scale1(const Scene* s, hipStream_t st, float* out) {
     hipLaunchKernelGGL(k_divergence1, dim3(nblocks(s->N)), dim3(kBlock), 0, st, s->dev, out);

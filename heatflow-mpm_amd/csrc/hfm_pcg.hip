@@ -31,6 +31,18 @@
 //   pressure (grid.py:94-115): sum over in-range faces a_f (p_c - p_nb) / h^2, a_f recomputed
 //                              from the solid fraction (grid.py:46-59)
 //   heat (heat.py:55-105):     fixed cells identity; free cells x + dt/h^2 sum kappa_f terms
+//
+// Mixed precision (HFM_PREC_MIXED) runs each solve as iterative refinement.  A refinement step
+// forms the true residual r = b - A x (fp32 arithmetic, ||r||^2 and ||b||^2 in fp64) and stops
+// the solve when ||r|| <= atol = max(tol ||b||, tol); otherwise it scales r to unit RMS
+// (sigma = ||r|| / sqrt(N)) and an inner PCG solves A^ d = r / sigma with A^ = c A (c = h^2 for
+// the pressure operator, whose A^ is then sum a_f (p_c - p_nb) with entries of order one; c = 1
+// for the heat operator) and Jacobi D^ = D / c.  The inner solve keeps r, z, p and A p in FP16
+// (all arithmetic fp32, dot products fp64), runs the same two passes as the fp32 solve, stops
+// when its ||r||^2 falls below max(eta^2 ||r_0||^2, min((atol / sigma)^2, ||r_0||^2 / 4))
+// (eta = 1e-2), and adds its
+// update to x directly: x += sigma c alpha p.  The FP16 vectors hold values of order one: |r^|
+// <= sqrt(N), |z^| <= |r^| / sum a_f with sum a_f >= 3 amin.
 #include <hip/hip_ext.h>
 
 #include "hfm_device.h"
@@ -41,8 +53,10 @@ namespace hfm {
 
 namespace {
 
-// One scene's view of one solve (op 0 heat, 1 pressure) at iteration k.
-struct PcgView {
+// One scene's view of one solve (op 0 heat, 1 pressure) at iteration k; V is the storage type of
+// the iteration vectors (float, or _Float16 in the inner solve of a refinement).
+template <class V>
+struct PcgViewT {
     int n;
     long N;
     float h, hh, dt_hh, amin;
@@ -53,11 +67,11 @@ struct PcgView {
     const uint8_t* flags;  // heat
     float* x;
     float* b;
-    float* r;
-    float* z;
-    const float* d_old;
-    float* d_new;
-    float* Ap;
+    V* r;
+    V* z;
+    const V* d_old;
+    V* d_new;
+    V* Ap;
     const float* D;
     CgCtl* ctl;
     CgMirror* mirror;
@@ -69,10 +83,23 @@ struct PcgView {
     int k;                 // completed iterations before this launch
     int lb;                // workgroup index inside the scene
 };
+using PcgView = PcgViewT<float>;
 
-template <int OP>
-__device__ __forceinline__ PcgView make_view(const SceneG& a, int lb, int k, float dt) {
-    PcgView v;
+template <bool IR>
+struct VecOf {
+    using type = float;
+};
+template <>
+struct VecOf<true> {
+    using type = _Float16;
+};
+
+constexpr float kIrEta2 = 1.0e-4f;  // inner solve: reduce ||r||^2 by eta^2, eta = 1e-2
+constexpr int kIrPart = kMaxDirBlocks + 3 * kMaxPcgBlocks;  // residual partials of a refinement step
+
+template <int OP, bool IR = false>
+__device__ __forceinline__ PcgViewT<typename VecOf<IR>::type> make_view(const SceneG& a, int lb, int k, float dt) {
+    PcgViewT<typename VecOf<IR>::type> v;
     v.n = a.n;
     v.N = a.N;
     v.h = a.h;
@@ -93,11 +120,19 @@ __device__ __forceinline__ PcgView make_view(const SceneG& a, int lb, int k, flo
         v.b = a.hrhs;
         v.D = a.hdiag;
     }
-    v.r = a.cg_r;
-    v.z = a.cg_z;
-    v.Ap = a.cg_Ap;
-    v.d_new = a.cg_d + (long)(k & 1) * a.N;
-    v.d_old = a.cg_d + (long)((k & 1) ^ 1) * a.N;
+    if constexpr (IR) {
+        v.r = a.ir_r;
+        v.z = a.ir_z;
+        v.Ap = a.ir_q;
+        v.d_new = a.ir_d + (long)(k & 1) * a.N;
+        v.d_old = a.ir_d + (long)((k & 1) ^ 1) * a.N;
+    } else {
+        v.r = a.cg_r;
+        v.z = a.cg_z;
+        v.Ap = a.cg_Ap;
+        v.d_new = a.cg_d + (long)(k & 1) * a.N;
+        v.d_old = a.cg_d + (long)((k & 1) ^ 1) * a.N;
+    }
     v.ctl = a.ctl + OP;
     v.mirror = a.mirror + OP;
     v.part_dir = a.partials;
@@ -118,8 +153,8 @@ struct Operator;
 // grid.py:94-115 with alpha/beta applied by the caller
 template <>
 struct Operator<1> {
-    template <class Dir>
-    __device__ static float apply(const PcgView& a, long t, int i, int j, int k, const Dir& dir) {
+    template <class View, class Dir>
+    __device__ static float apply(const View& a, long t, int i, int j, int k, const Dir& dir) {
         const int n = a.n;
         const long nn = (long)n * n;
         const float* as = a.alpha_s;
@@ -134,19 +169,23 @@ struct Operator<1> {
         if (k < n - 1) acc += face_fraction(ac, as[t + 1], a.amin) * (pc - dir(t + 1));
         return acc;
     }
-    // alpha * acc / h^2
-    __device__ static float combine(const PcgView& a, float acc, float xc, bool fixed, float alpha) {
+    // alpha * acc / h^2; the inner solve of a refinement applies A^ = h^2 A: alpha * acc
+    template <bool IR = false, class View>
+    __device__ static float combine(const View& a, float acc, float xc, bool fixed, float alpha) {
         (void)xc;
         (void)fixed;
+        if constexpr (IR) return alpha * acc;
         return div_const(alpha * acc, a.hh, a.rhh, a.fastdiv);
     }
+    // Jacobi of A^ = h^2 A: D / h^2
+    __device__ static float inner_diag_scale(float rhh) { return rhh; }
 };
 
 // heat.py:55-105
 template <>
 struct Operator<0> {
-    template <class Dir>
-    __device__ static float apply(const PcgView& a, long t, int i, int j, int k, const Dir& dir) {
+    template <class View, class Dir>
+    __device__ static float apply(const View& a, long t, int i, int j, int k, const Dir& dir) {
         const int n = a.n;
         const long nn = (long)n * n;
         const float kc = a.kap[t];
@@ -165,11 +204,13 @@ struct Operator<0> {
         }
         return acc;
     }
-    // fixed: alpha * x ; free: alpha * (x + dt/h^2 * acc)
-    __device__ static float combine(const PcgView& a, float acc, float xc, bool fixed, float alpha) {
+    // fixed: alpha * x ; free: alpha * (x + dt/h^2 * acc) (also the inner operator: c = 1)
+    template <bool IR = false, class View>
+    __device__ static float combine(const View& a, float acc, float xc, bool fixed, float alpha) {
         if (fixed) return alpha * xc;
         return alpha * (xc + a.dt_hh * acc);
     }
+    __device__ static float inner_diag_scale(float) { return 1.0f; }
 };
 
 __device__ __forceinline__ void cell_ijk(long t, int n, int& i, int& j, int& k) {
@@ -244,15 +285,17 @@ __device__ __forceinline__ int reduce_bcast(const double (&acc)[NV], int nv, int
 // scalars written by earlier launches.
 struct PassAIn {
     double acc[3];
-    int done, maxeff;
-    float rz_prev, atol_sq;
+    int done, maxeff, total;
+    float rz_prev, atol_sq, gl;
     double atol, tol;
 };
 
-template <int BS>
-__device__ __forceinline__ void pass_a_load(const PcgView& a, PassAIn& p) {
+// IR: the inner solve of a refinement step, whose first launch takes r.r and r.z from the scale
+// kernel (no b.b)
+template <int BS, bool IR = false, class View>
+__device__ __forceinline__ void pass_a_load(const View& a, PassAIn& p) {
     const CgCtl* c = a.ctl;
-    const bool first = a.k == 0;
+    const bool first = a.k == 0 && !IR;
     p.acc[0] = p.acc[1] = p.acc[2] = 0.0;
 #pragma unroll 1
     for (int b = threadIdx.x; b < a.nb_upd; b += BS) {
@@ -266,23 +309,33 @@ __device__ __forceinline__ void pass_a_load(const PcgView& a, PassAIn& p) {
     p.atol_sq = c->atol_sq;
     p.atol = c->atol;
     p.tol = c->tol;
+    p.total = c->ir_total;
+    p.gl = c->ir_gl;
 }
 
 // Prologue of every pass A: finish iteration k from pass B's partial sums (r.r, r.z; b.b too at
 // k = 0, where atol is formed), test convergence and the iteration cap, and derive beta.
 // Workgroup 0 of the scene records the state.  Returns true (in every thread) when the
 // workgroup has nothing to do: the solve stopped in an earlier launch, or stops now.
-template <int BS = kBlock>
-__device__ bool finish_iteration(const PcgView& a, const PassAIn& p, float& beta) {
+// In the inner solve of a refinement (IR) the threshold is set at k = 0 from the scale kernel's
+// r.r, and a stop ends only the inner solve: the host then starts the next refinement step.
+template <int BS = kBlock, bool IR = false, class View>
+__device__ bool finish_iteration(const View& a, const PassAIn& p, float& beta) {
     CgCtl* c = a.ctl;
     const int k = a.k;
     double tot[3];
-    if (reduce_bcast<3, BS>(p.acc, k == 0 ? 3 : 2, p.done, tot)) return true;
+    if (reduce_bcast<3, BS>(p.acc, k == 0 && !IR ? 3 : 2, p.done, tot)) return true;
     const float rr = (float)tot[0];
     const float rz = (float)tot[1];
     float atol_sq = p.atol_sq;
     double atol = p.atol;
-    if (k == 0) {
+    if (IR && k == 0) {
+        // max(eta^2 ||r_0||^2, (atol / sigma)^2), the latter capped at ||r_0||^2 / 4 so that every
+        // inner solve reduces the residual (FP16 rounding of r_0 could otherwise leave it above a
+        // threshold that the true residual has nearly reached)
+        const float rel = kIrEta2 * rr, gl = fminf(p.gl, 0.25f * rr);
+        atol_sq = rel > gl ? rel : gl;
+    } else if (k == 0) {
         const float bb = (float)tot[2];
         // warp: atol = max(tol * ||b||, tol)
         atol = p.tol * sqrt((double)bb);
@@ -297,14 +350,21 @@ __device__ bool finish_iteration(const PcgView& a, const PassAIn& p, float& beta
         c->slot_rr[k & 1] = rr;
         c->slot_rz[k & 1] = rz;
         if (k == 0) {
-            c->atol = atol;
+            if (!IR) c->atol = atol;
             c->atol_sq = atol_sq;
         }
         if (stop) {
             c->done = 1;
             c->converged = conv;
             c->it = k;
-            mirror_final(a.mirror, k, rr, atol, conv);
+            if (IR) {
+                c->ir_total = p.total + k;
+                __hip_atomic_store(&a.mirror->it, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                wait_stores();
+                __hip_atomic_store(&a.mirror->done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            } else {
+                mirror_final(a.mirror, k, rr, atol, conv);
+            }
         }
     }
     return stop;
@@ -350,33 +410,37 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(Src src, float dt) {
 
 // pass A of iteration k+1: finish iteration k (r.r, r.z, convergence, beta), then the direction
 // update at the cell and its neighbours, SpMV and partial p.Ap
-template <int OP, class Src>
+template <int OP, bool IR, class Src>
 __global__ __launch_bounds__(kBlock) void k_pcg_dir(Src src, float dt, int k,
                                                     unsigned long long* pace, unsigned tag) {
+    using V = typename VecOf<IR>::type;
     post_pace(pace, tag, k);
     HFM_ENTRY(src);
     if (batch_solve_done(src, a.ctl + OP)) return;
     __shared__ double lds[3][kWaves];
-    const PcgView v = make_view<OP>(a, lb, k, dt);
+    const auto v = make_view<OP, IR>(a, lb, k, dt);
     PassAIn pre;
-    pass_a_load<kBlock>(v, pre);
+    pass_a_load<kBlock, IR>(v, pre);
     float beta;
-    if (finish_iteration(v, pre, beta)) return;
+    if (finish_iteration<kBlock, IR>(v, pre, beta)) return;
     double s[1] = {0.0};
     const bool first = k == 0;
     FOR_CELLS(v, v.nb_dir) {
         int i, j, kk;
         cell_ijk(t, v.n, i, j, kk);
-        const float* zs = v.z;
-        const float* ds = v.d_old;
-        auto dir = [zs, ds, beta, first](long q) { return first ? zs[q] : zs[q] + beta * ds[q]; };
+        const V* zs = v.z;
+        const V* ds = v.d_old;
+        auto dir = [zs, ds, beta, first](long q) {
+            return first ? (float)zs[q] : (float)zs[q] + beta * (float)ds[q];
+        };
         const float pc = dir(t);
         const float acc = Operator<OP>::apply(v, t, i, j, kk, dir);
         const bool fixed = OP == 0 && (v.flags[t] & FLAG_FIXED);
-        const float apc = Operator<OP>::combine(v, acc, pc, fixed, 1.0f);
-        v.d_new[t] = pc;
-        v.Ap[t] = apc;
-        s[0] += (double)(pc * apc);
+        const float apc = Operator<OP>::template combine<IR>(v, acc, pc, fixed, 1.0f);
+        const V pv = (V)pc, av = (V)apc;
+        v.d_new[t] = pv;
+        v.Ap[t] = av;
+        s[0] += (double)((float)pv * (float)av);
     }
     store_partials<1>(s, v.part_dir, v.nb_dir, lb, lds);
 }
@@ -395,16 +459,17 @@ constexpr int kT3 = 16, kF3 = kT3 + 2, kBlock3 = 1024;
 constexpr int kPlane3 = kF3 * kF3;            // 324 positions of a frame plane
 constexpr int kGroups3 = kBlock3 / kPlane3;   // 3 plane groups
 
-template <int D, class Src>
+template <int D, bool IR, class Src>
 __global__ __launch_bounds__(kBlock3) void k_pcg_dir_tile3(Src src, int kit, unsigned long long* pace,
                                                            unsigned tag) {
+    using V = typename VecOf<IR>::type;
     post_pace(pace, tag, kit);
     HFM_ENTRY(src);
     if (batch_solve_done(src, a.ctl + 1)) return;
     __shared__ float P[D + 2][kF3][kF3];
     __shared__ float S[D + 2][kF3][kF3];
     __shared__ double lds[1][kBlock3 / 64];
-    const PcgView v = make_view<1>(a, lb, kit, 0.0f);
+    const auto v = make_view<1, IR>(a, lb, kit, 0.0f);
     const int nb = v.nb_dir;
     const int n = v.n;
     const int L = xcd_tile(lb, nb);
@@ -412,8 +477,8 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_dir_tile3(Src src, int kit, uns
     const int ti = L / tjk, rem = L % tjk;
     const int i0 = ti * D, j0 = (rem / v.tiles_k) * kT3, k0 = (rem % v.tiles_k) * kT3;
     const bool first = kit == 0;
-    const float* __restrict__ zs = v.z;
-    const float* __restrict__ ds = v.d_old;
+    const V* __restrict__ zs = v.z;
+    const V* __restrict__ ds = v.d_old;
     const float* __restrict__ as = v.alpha_s;
     const unsigned nn = (unsigned)n * n;
     // staging: thread -> (plane group, frame position); out-of-box frame cells load from a clamped
@@ -428,14 +493,14 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_dir_tile3(Src src, int kit, uns
     for (int m = 0; m < kM; ++m) {
         const int fi = grp + kGroups3 * m, gi = i0 + fi - 1;
         const unsigned t = (unsigned)clampi(gi, 0, n - 1) * nn + t_jk;
-        fz[m] = zs[t];
-        fd[m] = first ? 0.0f : ds[t];
+        fz[m] = (float)zs[t];
+        fd[m] = first ? 0.0f : (float)ds[t];
         fa[m] = as[t];
     }
     PassAIn pre;
-    pass_a_load<kBlock3>(v, pre);
+    pass_a_load<kBlock3, IR>(v, pre);
     float beta;
-    if (finish_iteration<kBlock3>(v, pre, beta)) return;
+    if (finish_iteration<kBlock3, IR>(v, pre, beta)) return;
     if (grp < kGroups3) {
 #pragma unroll
         for (int m = 0; m < kM; ++m) {
@@ -468,26 +533,35 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_dir_tile3(Src src, int kit, uns
             acc += face_fraction(ac, S[si][sj + 1][sk], amin) * (pc - P[si][sj + 1][sk]);
             acc += face_fraction(S[si][sj][sk - 1], ac, amin) * (pc - P[si][sj][sk - 1]);
             acc += face_fraction(ac, S[si][sj][sk + 1], amin) * (pc - P[si][sj][sk + 1]);
-            const float apc = div_const(1.0f * acc, hh, rhh, fastdiv);
+            float apc;
+            if constexpr (IR)
+                apc = acc;  // A^ = h^2 A
+            else
+                apc = div_const(1.0f * acc, hh, rhh, fastdiv);
             const unsigned t = tb + (unsigned)li * nn;
-            v.d_new[t] = pc;
-            v.Ap[t] = apc;
-            sum[0] += (double)(pc * apc);
+            const V pv = (V)pc, av = (V)apc;
+            v.d_new[t] = pv;
+            v.Ap[t] = av;
+            sum[0] += (double)((float)pv * (float)av);
         }
     }
     store_partials<1, kBlock3>(sum, v.part_dir, nb, lb, lds);
 }
 
 // pass B of iteration k+1: alpha from p.Ap, x and r update, z = D r, partial r.r and r.z
-template <int OP, class Src>
+// In the inner solve of a refinement (IR) x is the fp32 solution itself: x += (sigma c alpha) p,
+// r and z are FP16 and z = (D / c) r.
+template <int OP, bool IR, class Src>
 __global__ __launch_bounds__(kBlock) void k_pcg_upd(Src src, int kit) {
+    using V = typename VecOf<IR>::type;
     HFM_ENTRY(src);
     if (OP == 1 && a.cg_fused) return;  // the fused form has no pass B
     if (batch_solve_done(src, a.ctl + OP)) return;
     __shared__ double lds[2][kWaves];
-    const PcgView v = make_view<OP>(a, lb, kit, 0.0f);
+    const auto v = make_view<OP, IR>(a, lb, kit, 0.0f);
     const CgCtl* c = v.ctl;
     const int L = xcd_tile(lb, v.nb_upd);
+    const float dsc = IR ? Operator<OP>::inner_diag_scale(v.rhh) : 1.0f;
     // the operands of the first kPre tiles, the partial sums of pass A and the scalars are all
     // loaded before the p.Ap reduction, so their latencies overlap
     constexpr int kPre = 4;
@@ -497,9 +571,9 @@ __global__ __launch_bounds__(kBlock) void k_pcg_upd(Src src, int kit) {
         const long t = ((long)L * v.cpt + u) * kBlock + threadIdx.x;
         if (u < v.cpt && t < v.N) {
             px[u] = v.x[t];
-            pd[u] = v.d_new[t];
-            pr[u] = v.r[t];
-            pa[u] = v.Ap[t];
+            pd[u] = (float)v.d_new[t];
+            pr[u] = (float)v.r[t];
+            pa[u] = (float)v.Ap[t];
             pD[u] = v.D[t];
         }
     }
@@ -508,38 +582,162 @@ __global__ __launch_bounds__(kBlock) void k_pcg_upd(Src src, int kit) {
     for (int b = threadIdx.x; b < v.nb_dir; b += kBlock) acc[0] += v.part_dir[b];
     const int done = c->done;
     const float rr = c->slot_rr[kit & 1], rz = c->slot_rz[kit & 1], atol_sq = c->atol_sq;
+    const float step = IR ? c->ir_step : 1.0f;
     double tot[1];
     if (reduce_bcast<1, kBlock>(acc, 1, done, tot)) return;
     const float pAp = (float)tot[0];
     const float alpha = (rr > atol_sq) ? rz / pAp : 0.0f;
+    const float xa = IR ? step * alpha : alpha;
     double s[2] = {0.0, 0.0};
+    auto update = [&](long t, float x0, float d0, float r0, float a0, float D0) {
+        const float xc = x0 + xa * d0;
+        const V rv = (V)(r0 - alpha * a0);
+        const float rc = (float)rv;
+        const V zv = (V)((IR ? D0 * dsc : D0) * rc);
+        const float zc = (float)zv;
+        v.x[t] = xc;
+        v.r[t] = rv;
+        v.z[t] = zv;
+        s[0] += (double)(rc * rc);
+        s[1] += (double)(rc * zc);
+    };
 #pragma unroll
     for (int u = 0; u < kPre; ++u) {
         const long t = ((long)L * v.cpt + u) * kBlock + threadIdx.x;
-        if (u < v.cpt && t < v.N) {
-            const float xc = px[u] + alpha * pd[u];
-            const float rc = pr[u] - alpha * pa[u];
-            const float zc = pD[u] * rc;
-            v.x[t] = xc;
-            v.r[t] = rc;
-            v.z[t] = zc;
-            s[0] += (double)(rc * rc);
-            s[1] += (double)(rc * zc);
-        }
+        if (u < v.cpt && t < v.N) update(t, px[u], pd[u], pr[u], pa[u], pD[u]);
     }
     for (int u = kPre; u < v.cpt; ++u) {
         const long t = ((long)L * v.cpt + u) * kBlock + threadIdx.x;
         if (t >= v.N) break;
-        const float xc = v.x[t] + alpha * v.d_new[t];
-        const float rc = v.r[t] - alpha * v.Ap[t];
-        const float zc = v.D[t] * rc;
-        v.x[t] = xc;
+        update(t, v.x[t], (float)v.d_new[t], (float)v.r[t], (float)v.Ap[t], v.D[t]);
+    }
+    store_partials<2>(s, v.part_upd, v.nb_upd, lb, lds);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Mixed precision: the refinement step (HFM_PREC_MIXED).
+// ---------------------------------------------------------------------------------------------
+
+// True residual of x: r = b - A x (fp32, as the fp32 solve's first residual), ||r||^2 in fp64
+// (products and sums in double); the first step of a solve also forms ||b||^2 and removes the
+// pressure rhs mean (grid.py:240-241), and clears the solve's state.
+template <int OP, class Src>
+__global__ __launch_bounds__(kBlock) void k_ir_residual(Src src, float dt, int first) {
+    HFM_ENTRY(src);
+    CgCtl* c = a.ctl + OP;
+    if (!first && c->ir_end) return;  // set by an earlier launch: this scene's solve has ended
+    __shared__ double lds[2][kWaves];
+    const PcgView v = make_view<OP>(a, lb, 0, dt);
+    double s[2] = {0.0, 0.0};
+    FOR_CELLS(v, v.nb_upd) {
+        int i, j, k;
+        cell_ijk(t, v.n, i, j, k);
+        float bc = v.b[t];
+        if (OP == 1 && first) {
+            const float mean = c->mean;
+            if (mean != 0.0f) {
+                bc = bc - mean;
+                v.b[t] = bc;
+            }
+        }
+        const float* xs = v.x;
+        auto dir = [xs](long q) { return xs[q]; };
+        const float acc = Operator<OP>::apply(v, t, i, j, k, dir);
+        const bool fixed = OP == 0 && (v.flags[t] & FLAG_FIXED);
+        const float rc = Operator<OP>::combine(v, acc, xs[t], fixed, -1.0f) + 1.0f * bc;
         v.r[t] = rc;
-        v.z[t] = zc;
+        s[0] += (double)rc * (double)rc;
+        s[1] += (double)bc * (double)bc;
+    }
+    store_partials<2>(s, a.partials + kIrPart, v.nb_upd, lb, lds);
+    if (first && lb == 0 && threadIdx.x == 0) {
+        c->done = 0;
+        c->converged = 0;
+        c->it = 0;
+        c->ir_total = 0;
+        c->ir_outer = 0;
+        c->ir_end = 0;
+    }
+}
+
+// Decide the refinement step from the residual's partial sums: stop the solve when
+// ||r|| <= atol (atol = max(tol ||b||, tol), formed at the first step) or at the iteration cap
+// (or on a non-finite residual); otherwise write r^ = r / sigma and z^ = (D / c) r^ in FP16 with
+// their partial sums, which the inner solve's first pass A reads, and set the inner solve's
+// state.
+template <int OP, class Src>
+__global__ __launch_bounds__(kBlock) void k_ir_scale(Src src, int first) {
+    HFM_ENTRY(src);
+    CgCtl* c = a.ctl + OP;
+    if (c->ir_end) return;
+    __shared__ double lds[2][kWaves];
+    const auto v = make_view<OP, true>(a, lb, 0, 0.0f);
+    const double* part = a.partials + kIrPart;
+    double acc[2] = {0.0, 0.0};
+#pragma unroll 1
+    for (int b = threadIdx.x; b < v.nb_upd; b += kBlock) {
+        acc[0] += part[b];
+        if (first) acc[1] += part[v.nb_upd + b];
+    }
+    const double tol = c->tol;
+    double atol = c->atol;
+    const int total = c->ir_total, outer = c->ir_outer, maxiter = c->ir_maxiter;
+    double tot[2];
+    reduce_bcast<2, kBlock>(acc, first ? 2 : 1, 0, tot);
+    const double rr = tot[0];
+    if (first) {
+        atol = tol * sqrt(tot[1]);
+        if (atol < tol) atol = tol;
+    }
+    const bool finite = rr <= 1.0e300;   // false for NaN and infinity
+    const bool conv = finite && rr <= atol * atol;
+    if (conv || !finite || total >= maxiter) {
+        if (lb == 0 && threadIdx.x == 0) {
+            c->ir_end = 1;
+            c->done = 1;
+            c->converged = conv;
+            c->it = total;
+            c->atol = atol;
+            CgMirror* m = v.mirror;
+            __hip_atomic_store(&m->rr, rr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&m->atol, atol, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&m->converged, conv ? 1 : (finite ? 0 : -1), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&m->it, total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&m->outer, outer, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&m->end, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            wait_stores();
+            __hip_atomic_store(&m->done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        return;
+    }
+    const double sigma = sqrt(rr / (double)v.N);
+    const float inv = (float)(1.0 / sigma);
+    const float dsc = Operator<OP>::inner_diag_scale(v.rhh);
+    const float* __restrict__ r32 = a.cg_r;  // the residual kernel's fp32 r
+    double s[2] = {0.0, 0.0};
+    FOR_CELLS(v, v.nb_upd) {
+        const _Float16 rv = (_Float16)(r32[t] * inv);
+        const float rc = (float)rv;
+        const _Float16 zv = (_Float16)((v.D[t] * dsc) * rc);
+        const float zc = (float)zv;
+        v.r[t] = rv;
+        v.z[t] = zv;
         s[0] += (double)(rc * rc);
         s[1] += (double)(rc * zc);
     }
     store_partials<2>(s, v.part_upd, v.nb_upd, lb, lds);
+    if (lb == 0 && threadIdx.x == 0) {
+        const double cop = OP == 1 ? (double)a.hh : 1.0;
+        c->ir_outer = outer + 1;
+        c->ir_gl = (float)(atol * atol / rr * (double)v.N);   // (atol / sigma)^2
+        c->ir_step = (float)(sigma * cop);
+        c->atol = atol;
+        c->done = 0;
+        c->converged = 0;
+        c->it = 0;
+        c->maxeff = maxiter - total;
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -829,18 +1027,27 @@ void launch_pcg_init(const Launch& L, int op, float dt) {
         HFM_LAUNCH_T(k_pcg_init, 0, L, kBlock, nullptr, nullptr, dt);
 }
 
-void launch_pcg_dir(const Launch& L, Family fam, int op, float dt, int k, unsigned long long* pace, unsigned tag,
-                    hipEvent_t ev0, hipEvent_t ev1) {
+template <bool IR>
+void launch_dir(const Launch& L, Family fam, int op, float dt, int k, unsigned long long* pace, unsigned tag,
+                hipEvent_t ev0, hipEvent_t ev1) {
     switch (fam) {
-        case FAM_DIR4: HFM_LAUNCH_T(k_pcg_dir_tile3, 4, L, kBlock3, ev0, ev1, k, pace, tag); break;
-        case FAM_DIR8: HFM_LAUNCH_T(k_pcg_dir_tile3, 8, L, kBlock3, ev0, ev1, k, pace, tag); break;
-        case FAM_DIR16: HFM_LAUNCH_T(k_pcg_dir_tile3, 16, L, kBlock3, ev0, ev1, k, pace, tag); break;
+        case FAM_DIR4: HFM_LAUNCH_T2(k_pcg_dir_tile3, 4, IR, L, kBlock3, ev0, ev1, k, pace, tag); break;
+        case FAM_DIR8: HFM_LAUNCH_T2(k_pcg_dir_tile3, 8, IR, L, kBlock3, ev0, ev1, k, pace, tag); break;
+        case FAM_DIR16: HFM_LAUNCH_T2(k_pcg_dir_tile3, 16, IR, L, kBlock3, ev0, ev1, k, pace, tag); break;
         default:
             if (op == 1)
-                HFM_LAUNCH_T(k_pcg_dir, 1, L, kBlock, ev0, ev1, dt, k, pace, tag);
+                HFM_LAUNCH_T2(k_pcg_dir, 1, IR, L, kBlock, ev0, ev1, dt, k, pace, tag);
             else
-                HFM_LAUNCH_T(k_pcg_dir, 0, L, kBlock, ev0, ev1, dt, k, pace, tag);
+                HFM_LAUNCH_T2(k_pcg_dir, 0, IR, L, kBlock, ev0, ev1, dt, k, pace, tag);
     }
+}
+
+void launch_pcg_dir(const Launch& L, Family fam, int op, float dt, int k, unsigned long long* pace, unsigned tag,
+                    hipEvent_t ev0, hipEvent_t ev1, bool ir) {
+    if (ir)
+        launch_dir<true>(L, fam, op, dt, k, pace, tag, ev0, ev1);
+    else
+        launch_dir<false>(L, fam, op, dt, k, pace, tag, ev0, ev1);
 }
 
 void launch_pcg_init_fused(const Launch& L) { HFM_LAUNCH(k_pcg_init_fused, L, kBlock, nullptr, nullptr); }
@@ -855,11 +1062,29 @@ void launch_pcg_fused(const Launch& L, Family fam, int k, unsigned long long* pa
     }
 }
 
-void launch_pcg_upd(const Launch& L, int op, int k, hipEvent_t ev0, hipEvent_t ev1) {
-    if (op == 1)
-        HFM_LAUNCH_T(k_pcg_upd, 1, L, kBlock, ev0, ev1, k);
+void launch_pcg_upd(const Launch& L, int op, int k, hipEvent_t ev0, hipEvent_t ev1, bool ir) {
+    if (op == 1 && ir)
+        HFM_LAUNCH_T2(k_pcg_upd, 1, true, L, kBlock, ev0, ev1, k);
+    else if (op == 1)
+        HFM_LAUNCH_T2(k_pcg_upd, 1, false, L, kBlock, ev0, ev1, k);
+    else if (ir)
+        HFM_LAUNCH_T2(k_pcg_upd, 0, true, L, kBlock, ev0, ev1, k);
     else
-        HFM_LAUNCH_T(k_pcg_upd, 0, L, kBlock, ev0, ev1, k);
+        HFM_LAUNCH_T2(k_pcg_upd, 0, false, L, kBlock, ev0, ev1, k);
+}
+
+void launch_ir_residual(const Launch& L, int op, float dt, bool first) {
+    if (op == 1)
+        HFM_LAUNCH_T(k_ir_residual, 1, L, kBlock, nullptr, nullptr, dt, (int)first);
+    else
+        HFM_LAUNCH_T(k_ir_residual, 0, L, kBlock, nullptr, nullptr, dt, (int)first);
+}
+
+void launch_ir_scale(const Launch& L, int op, bool first) {
+    if (op == 1)
+        HFM_LAUNCH_T(k_ir_scale, 1, L, kBlock, nullptr, nullptr, (int)first);
+    else
+        HFM_LAUNCH_T(k_ir_scale, 0, L, kBlock, nullptr, nullptr, (int)first);
 }
 
 }  // namespace hfm

@@ -34,6 +34,13 @@ struct CgCtl {
     double atol;       // max(tol * ||b||, tol)
     float mean;        // pressure rhs mean to remove (numpy float32 mean)
     unsigned ticket2;  // arrival counter of the rhs-mean chunks
+    // mixed precision (iterative refinement): written by the scale kernel of each refinement step
+    float ir_gl;       // (atol / sigma)^2: the global threshold in the inner solve's units
+    float ir_step;     // sigma * c: x += ir_step * alpha * p of the inner solve
+    int ir_total;      // inner iterations of this solve so far (written when an inner solve stops)
+    int ir_outer;      // refinement steps of this solve so far
+    int ir_end;        // 1: the refinement has converged or reached the cap
+    int ir_maxiter;    // iteration cap (inner iterations over the whole solve)
 };
 
 // Pinned host memory written by the device with system-scope stores (results of a solve),
@@ -42,10 +49,12 @@ struct CgMirror {
     int it;
     int done;
     int converged;
-    int pad;
+    int outer;         // mixed precision: refinement steps
     double rr;
     double atol;
     double mean;
+    int end;           // mixed precision: 1 when `done` ends the whole solve, 0 when one inner solve
+    int pad;
 };
 
 // Pointer members of the per-scene descriptor.  The kernels read the descriptor from device
@@ -100,6 +109,9 @@ struct SceneView {
     ptr<float> temp, melt, yield, visc;
     ptr<int> bin_start, sorted;
     ptr<float> qpos;         // pos / h per particle, in bin order (3 per particle)
+    int mixed;               // HFM_PREC_MIXED
+    ptr<_Float16> ir_r, ir_z, ir_d, ir_q;  // inner solve: r, z, p (two parity buffers), A p
+    ptr<uint16_t> ub[3];     // BF16 copies of u, v, w (departure points of the advection)
 };
 using SceneDev = SceneView<HostPtrs>;  // host copy, written by the host
 using SceneG = SceneView<DevPtrs>;     // the kernels' view of the same bytes
@@ -299,6 +311,9 @@ inline void launch_scene_kernel(KB kbatch, KS ksolo, const Launch& L, int thread
     ::hfm::launch_scene_kernel(K<::hfm::BatchSrc>, K<::hfm::SoloSrc>, L, threads, ev0, ev1, ##__VA_ARGS__)
 #define HFM_LAUNCH_T(K, T, L, threads, ev0, ev1, ...) \
     ::hfm::launch_scene_kernel(K<T, ::hfm::BatchSrc>, K<T, ::hfm::SoloSrc>, L, threads, ev0, ev1, ##__VA_ARGS__)
+#define HFM_LAUNCH_T2(K, T1, T2, L, threads, ev0, ev1, ...)                                                    \
+    ::hfm::launch_scene_kernel(K<T1, T2, ::hfm::BatchSrc>, K<T1, T2, ::hfm::SoloSrc>, L, threads, ev0, ev1, \
+                               ##__VA_ARGS__)
 
 // ---- batched kernel launchers (hfm_fields.hip) ----
 void launch_heat_prep(const Launch& L, float dt);
@@ -325,9 +340,16 @@ void pcg_geometry(const Scene* s, bool fused, int* cpt, int* nb_upd, int* nb_dir
                   int* dtile);
 void launch_pcg_init(const Launch& L, int op, float dt);
 // pace: when non-null, workgroup 0 posts (tag << 32) | k to it as the launch starts
+// ir: the inner iteration of a mixed-precision refinement (FP16 vectors, scaled operator)
 void launch_pcg_dir(const Launch& L, Family fam, int op, float dt, int k, unsigned long long* pace, unsigned tag,
-                    hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
-void launch_pcg_upd(const Launch& L, int op, int k, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+                    hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, bool ir = false);
+void launch_pcg_upd(const Launch& L, int op, int k, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
+                    bool ir = false);
+// mixed precision: true residual of a refinement step (first: also b.b and the rhs mean removal),
+// then its scaling into the inner solve's FP16 vectors (or the end of the solve)
+void launch_ir_residual(const Launch& L, int op, float dt, bool first);
+void launch_ir_scale(const Launch& L, int op, bool first);
+void launch_bf16_copy(const Scene* s, hipStream_t st);
 // fused pressure form: first-iteration scalars after launch_pcg_init, then one launch per iteration
 constexpr int kFusedRows = 5;
 void launch_pcg_init_fused(const Launch& L);

@@ -470,20 +470,11 @@ int filter_families(Batch* b, const Family* fams, int nf, const std::vector<char
     return HFM_OK;
 }
 
-// Enqueue one PCG solve (op 0 heat, 1 pressure) of every scene of the batch and return when
-// the device reports all of them done.
-int run_pcg(Batch* b, int op, float dt) {
+// Run PCG iterations from iteration 0 for every scene of the batch whose mirror is not done
+// (the init launches are enqueued), and return when the device reports all of them done.
+// ir: the inner solves of a mixed-precision refinement step (FP16 kernels).
+int pcg_iterate(Batch* b, int op, float dt, bool ir, int maxeff, int pred) {
     const unsigned tag = ++b->solve_tag;
-    int maxeff = 0, pred = 2;
-    for (Scene* s : b->scenes) {
-        CgMirror* m = &s->mirror[op];
-        __atomic_store_n(&m->it, 0, __ATOMIC_RELAXED);
-        __atomic_store_n(&m->done, 0, __ATOMIC_RELEASE);
-        const int mi = op ? s->prm.cg_max_iter : s->prm.heat_max_iter;
-        maxeff = std::max(maxeff, mi + (mi & 1));
-        pred = std::max(pred, op ? s->pres_pred : s->heat_pred);
-    }
-    launch_pcg_init(family(b, FAM_PCG), op, dt);
     // pass A launches: the heat operator uses the flat kernel with the PCG geometry; the
     // pressure operator one launch per tiling present in the batch, two-pass or fused
     Family dirs[8];
@@ -501,9 +492,8 @@ int run_pcg(Batch* b, int op, float dt) {
             else
                 two_pass = true;
         }
-        if (any_fused) launch_pcg_init_fused(family(b, FAM_PCG));
+        (void)any_fused;
     }
-    CHECK_LAUNCH();
     auto is_fused = [](Family f) { return f == FAM_FUSED4 || f == FAM_FUSED8 || f == FAM_FUSED16; };
     // Launches of the iteration families; in a batch they are restricted, between groups of
     // iterations, to the scenes whose solve still runs.
@@ -516,6 +506,16 @@ int run_pcg(Batch* b, int op, float dt) {
     const size_t ns = b->scenes.size();
     std::vector<char> active(ns, 1);
     size_t n_active = ns;
+    if (ir && ns > 1) {   // scenes whose solve has ended take no part in this inner solve
+        bool changed = false;
+        for (size_t i = 0; i < ns; ++i)
+            if (__atomic_load_n(&b->scenes[i]->mirror[op].done, __ATOMIC_ACQUIRE)) {
+                active[i] = 0;
+                --n_active;
+                changed = true;
+            }
+        if (changed && n_active > 0) RC(filter_families(b, iter_fams, niter, active, fl));
+    }
     auto refresh_active = [&]() -> int {
         if (ns < 2) return HFM_OK;
         bool changed = false;
@@ -541,7 +541,7 @@ int run_pcg(Batch* b, int op, float dt) {
                 launch_pcg_fused(L, dirs[d], k, pc, tag, e0, e1);
             } else {
                 if (timed) RC(timing_events(b, HFM_KT_PCG_DIR, &e0, &e1));
-                launch_pcg_dir(L, dirs[d], op, dt, k, pc, tag, e0, e1);
+                launch_pcg_dir(L, dirs[d], op, dt, k, pc, tag, e0, e1, ir);
             }
             pc = nullptr;
         }
@@ -585,7 +585,7 @@ int run_pcg(Batch* b, int op, float dt) {
                 if (two_pass) {
                     hipEvent_t e0 = nullptr, e1 = nullptr;
                     if (timed) RC(timing_events(b, HFM_KT_PCG_UPD, &e0, &e1));
-                    launch_pcg_upd(fl[FAM_PCG], op, k, e0, e1);
+                    launch_pcg_upd(fl[FAM_PCG], op, k, e0, e1, ir);
                 }
             }
             CHECK_LAUNCH();
@@ -615,6 +615,27 @@ int run_pcg(Batch* b, int op, float dt) {
             std::this_thread::yield();
         }
     }
+    return HFM_OK;
+}
+
+// Enqueue one PCG solve (op 0 heat, 1 pressure) of every scene of the batch and return when
+// the device reports all of them done.
+int run_pcg(Batch* b, int op, float dt) {
+    int maxeff = 0, pred = 2;
+    bool any_fused = false;
+    for (Scene* s : b->scenes) {
+        CgMirror* m = &s->mirror[op];
+        __atomic_store_n(&m->it, 0, __ATOMIC_RELAXED);
+        __atomic_store_n(&m->done, 0, __ATOMIC_RELEASE);
+        const int mi = op ? s->prm.cg_max_iter : s->prm.heat_max_iter;
+        maxeff = std::max(maxeff, mi + (mi & 1));
+        pred = std::max(pred, op ? s->pres_pred : s->heat_pred);
+        if (op == 1 && s->dev.cg_fused) any_fused = true;
+    }
+    launch_pcg_init(family(b, FAM_PCG), op, dt);
+    if (any_fused) launch_pcg_init_fused(family(b, FAM_PCG));
+    CHECK_LAUNCH();
+    RC(pcg_iterate(b, op, dt, false, maxeff, pred));
     for (Scene* s : b->scenes) {
         CgMirror* m = &s->mirror[op];
         const int it = __atomic_load_n(&m->it, __ATOMIC_ACQUIRE);
@@ -644,6 +665,78 @@ int run_pcg(Batch* b, int op, float dt) {
     return HFM_OK;
 }
 
+// One solve of every (mixed-precision) scene of the batch as iterative refinement: the true
+// residual and its scaling (k_ir_residual, k_ir_scale), then an inner FP16 PCG, repeated until
+// the scale kernel ends each scene's solve.  The host learns of both kinds of end through the
+// mirror's `done`; `end` tells them apart.
+int run_ir(Batch* b, int op, float dt) {
+    int maxeff = 0, pred = 2;
+    for (Scene* s : b->scenes) {
+        CgMirror* m = &s->mirror[op];
+        __atomic_store_n(&m->end, 0, __ATOMIC_RELAXED);
+        __atomic_store_n(&m->done, 0, __ATOMIC_RELEASE);
+        maxeff = std::max(maxeff, op ? s->prm.cg_max_iter : s->prm.heat_max_iter);
+        pred = std::max(pred, op ? s->pres_pred : s->heat_pred);
+    }
+    const Launch fam = family(b, FAM_PCG);
+    launch_ir_residual(fam, op, dt, true);
+    launch_ir_scale(fam, op, true);
+    CHECK_LAUNCH();
+    for (int step = 0;; ++step) {
+        RC(pcg_iterate(b, op, dt, true, maxeff, pred));
+        bool all_end = true;
+        int longest = 2;
+        for (Scene* s : b->scenes) {
+            CgMirror* m = &s->mirror[op];
+            if (__atomic_load_n(&m->end, __ATOMIC_ACQUIRE)) continue;
+            all_end = false;
+            longest = std::max(longest, __atomic_load_n(&m->it, __ATOMIC_ACQUIRE));
+            __atomic_store_n(&m->done, 0, __ATOMIC_RELEASE);  // before the launches that may set it
+        }
+        if (all_end) break;
+        pred = longest;
+        if (step > 4 * maxeff + 8) {
+            set_err("%s refinement did not end", op ? "pressure" : "heat");
+            return HFM_ERR_STATE;
+        }
+        launch_ir_residual(fam, op, dt, false);
+        launch_ir_scale(fam, op, false);
+        CHECK_LAUNCH();
+    }
+    for (Scene* s : b->scenes) {
+        CgMirror* m = &s->mirror[op];
+        const int it = __atomic_load_n(&m->it, __ATOMIC_ACQUIRE);
+        const int conv = __atomic_load_n(&m->converged, __ATOMIC_ACQUIRE);
+        const int outer = __atomic_load_n(&m->outer, __ATOMIC_ACQUIRE);
+        if (conv < 0) {
+            set_err("%s solve: non-finite residual", op ? "pressure" : "heat");
+            return HFM_ERR_NONFINITE;
+        }
+        const int inner = outer > 0 ? (it + outer - 1) / outer : 2;
+        if (op == 1) {
+            s->pres_pred = inner > 2 ? inner : 2;
+            s->stats.p_iters_total += it;
+            s->stats.p_iters_exact = it;
+            s->stats.p_iters = it;
+            s->stats.p_converged = conv;
+            s->stats.p_rr = m->rr;
+            s->stats.p_atol = m->atol;
+            s->stats.p_outer = outer;
+            s->stats.rhs_mean = m->mean;
+        } else {
+            s->heat_pred = inner > 2 ? inner : 2;
+            s->stats.h_iters_total += it;
+            s->stats.h_iters_exact = it;
+            s->stats.h_iters = it;
+            s->stats.h_converged = conv;
+            s->stats.h_rr = m->rr;
+            s->stats.h_atol = m->atol;
+            s->stats.h_outer = outer;
+        }
+    }
+    return HFM_OK;
+}
+
 int batch_substep(Batch* b, double dt) {
     RC(batch_prepare(b));
     RC(batch_adopt(b));
@@ -654,13 +747,14 @@ int batch_substep(Batch* b, double dt) {
     launch_heat_prep(family(b, FAM_CELL), dt32);
     launch_advect_T(family(b, FAM_CELL), dt32);
     CHECK_LAUNCH();
-    RC(run_pcg(b, 0, dt32));
+    const bool mixed = !b->scenes.empty() && b->scenes[0]->dev.mixed != 0;  // one precision per batch
+    RC(mixed ? run_ir(b, 0, dt32) : run_pcg(b, 0, dt32));
     launch_g2p(family(b, FAM_G2P));
     launch_air_faces(family(b, FAM_FACE3), dt32);
     launch_divergence(family(b, FAM_CELL), dt);
     launch_rhs_mean(family(b, FAM_MEAN));
     CHECK_LAUNCH();
-    RC(run_pcg(b, 1, dt32));
+    RC(mixed ? run_ir(b, 1, dt32) : run_pcg(b, 1, dt32));
     launch_project(family(b, FAM_FACE3), dt);
     CHECK_LAUNCH();
     for (Scene* s : b->scenes) s->stats.substeps += 1;
@@ -784,8 +878,12 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
         set_err("bad parameters (n=%d h=%g)", prm->n, prm->h);
         return HFM_ERR_ARG;
     }
-    if (prm->precision != HFM_PREC_FP32) {
+    if (prm->precision != HFM_PREC_FP32 && prm->precision != HFM_PREC_MIXED) {
         set_err("precision mode %d is not available", prm->precision);
+        return HFM_ERR_ARG;
+    }
+    if (prm->precision == HFM_PREC_MIXED && (prm->flags & HFM_FLAG_FUSED_CG)) {
+        set_err("HFM_FLAG_FUSED_CG is not available with HFM_PREC_MIXED");
         return HFM_ERR_ARG;
     }
     const int ndev = hfm_device_count();
@@ -858,7 +956,15 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
     A(d.cg_z, N);
     A(d.cg_d, 2 * N);
     A(d.cg_Ap, N);
-    A(d.partials, std::max(kMaxDirBlocks + 3 * kMaxPcgBlocks, 2 * kFusedRows * kMaxDirBlocks));
+    A(d.partials, std::max(kMaxDirBlocks + 5 * kMaxPcgBlocks, 2 * kFusedRows * kMaxDirBlocks));
+    if (prm->precision == HFM_PREC_MIXED) {
+        d.mixed = 1;
+        A(d.ir_r, N);
+        A(d.ir_z, N);
+        A(d.ir_d, 2 * N);
+        A(d.ir_q, N);
+        for (int a = 0; a < 3; ++a) A(d.ub[a], F);  // zero: the BF16 pattern of u = 0
+    }
     A(d.ctl, 2);
     d.nchunks = (int)((N + 8191) / 8192);
     A(d.chunk_sums, d.nchunks);
@@ -892,8 +998,10 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
     memset(ctl, 0, sizeof(ctl));
     ctl[0].tol = prm->heat_tol;
     ctl[0].maxeff = prm->heat_max_iter + (prm->heat_max_iter & 1);
+    ctl[0].ir_maxiter = prm->heat_max_iter;
     ctl[1].tol = prm->cg_tol;
     ctl[1].maxeff = prm->cg_max_iter + (prm->cg_max_iter & 1);
+    ctl[1].ir_maxiter = prm->cg_max_iter;
     e = hipMemcpy(d.ctl, ctl, sizeof(ctl), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipHostMalloc((void**)&s->mirror, 2 * sizeof(CgMirror), hipHostMallocMapped | hipHostMallocCoherent);
@@ -1150,6 +1258,11 @@ int hfm_set_field(hfm_scene* h, int32_t field, const float* in) {
     RC(check_finite(in, size, "field"));
     HIP_OK(hipStreamSynchronize(s->stream));
     HIP_OK(hipMemcpy(p, in, size * sizeof(float), hipMemcpyHostToDevice));
+    if (field == HFM_FIELD_U || field == HFM_FIELD_V || field == HFM_FIELD_W) {
+        launch_bf16_copy(s, s->stream);   // mixed precision: the advection's copy of the velocity
+        CHECK_LAUNCH();
+        HIP_OK(hipStreamSynchronize(s->stream));
+    }
     return HFM_OK;
 }
 
@@ -1261,6 +1374,10 @@ int hfm_batch_create(hfm_scene* const* scenes, int32_t count, hfm_batch** out) {
             set_err("scenes of a batch must be on one device");
             return HFM_ERR_ARG;
         }
+        if (!list.empty() && s->prm.precision != list[0]->prm.precision) {
+            set_err("scenes of a batch must have one precision mode");
+            return HFM_ERR_ARG;
+        }
         list.push_back(s);
     }
     HIP_OK(hipSetDevice(list[0]->device));
@@ -1305,6 +1422,10 @@ int hfm_batch_set_scenes(hfm_batch* h, hfm_scene* const* scenes, int32_t count) 
         }
         if (s->device != b->device) {
             set_err("scenes of a batch must be on one device");
+            return HFM_ERR_ARG;
+        }
+        if (!list.empty() && s->prm.precision != list[0]->prm.precision) {
+            set_err("scenes of a batch must have one precision mode");
             return HFM_ERR_ARG;
         }
         list.push_back(s);

@@ -23,6 +23,7 @@ FIELD_NAMES = ["T", "T2", "TW", "U", "V", "W", "P", "RHS", "ALPHA_S", "ALPHA_P",
 FIELD = {name: i for i, name in enumerate(FIELD_NAMES)}
 
 PREC_FP32 = 0
+PREC_MIXED = 1
 FLAG_TIME_KERNELS = 1
 FLAG_FUSED_CG = 2
 KT_PCG_DIR, KT_PCG_UPD, KT_PCG_FUSED = 0, 1, 2
@@ -46,7 +47,7 @@ class Stats(C.Structure):
                 ("h_iters_exact", C.c_int32), ("p_converged", C.c_int32), ("h_converged", C.c_int32),
                 ("substeps", C.c_int64), ("p_rr", C.c_double), ("p_atol", C.c_double), ("h_rr", C.c_double),
                 ("h_atol", C.c_double), ("rhs_mean", C.c_double), ("p_iters_total", C.c_int64),
-                ("h_iters_total", C.c_int64)]
+                ("h_iters_total", C.c_int64), ("p_outer", C.c_int32), ("h_outer", C.c_int32)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}

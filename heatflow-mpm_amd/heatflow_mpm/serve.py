@@ -40,7 +40,7 @@ class Result:
     admitted: float
     finished: float
     device_bytes: int
-    fields: dict = field(default_factory=dict)   # final T (and particle temperatures) if kept
+    fields: dict = field(default_factory=dict)   # final T, U, V, W and particle temperatures if kept
     stats: dict = field(default_factory=dict)
 
     @property
@@ -48,20 +48,22 @@ class Result:
         return self.finished - self.arrival
 
 
-def footprint_estimate(n: int, particles: int) -> int:
+def footprint_estimate(n: int, particles: int, precision: str = "fp32") -> int:
     """Device bytes of a scene before it is created: 21 face-sized and 27 cell-sized fp32
     arrays (fields, static parts, solver vectors, bins), 1 byte of flags per cell, 12 fp32
-    words and 4 ints per particle, and fixed tables (partial sums, programs)."""
+    words and 4 ints per particle, and fixed tables (partial sums, programs); the mixed form
+    adds 5 FP16 cell arrays (inner solve) and 3 BF16 face arrays (velocity copy)."""
     N = n ** 3
     F = (n + 1) * n * n
-    return 4 * (21 * F + 27 * N) + N + 64 * particles + 256 * 1024
+    extra = 2 * (5 * N + 3 * F) if precision == "mixed" else 0
+    return 4 * (21 * F + 27 * N) + N + 64 * particles + 256 * 1024 + extra
 
 
 class GpuBackend:
     """HeatflowMPM scenes stepped by HeatflowBatch on one device."""
 
-    def __init__(self, cg: str = "warp", device: str = "cuda:0"):
-        self.cg, self.device = cg, device
+    def __init__(self, cg: str = "warp", device: str = "cuda:0", precision: str = "fp32"):
+        self.cg, self.device, self.precision = cg, device, precision
 
     def prepare(self, req: Request):
         from .solver import scene_inputs, KAPPA_AIR
@@ -76,7 +78,7 @@ class GpuBackend:
 
     def create(self, req: Request, prepared):
         from .solver import HeatflowMPM
-        return HeatflowMPM(**req.kwargs, cg=self.cg, device=self.device, inputs=prepared)
+        return HeatflowMPM(**req.kwargs, cg=self.cg, device=self.device, inputs=prepared, precision=self.precision)
 
     def n_of(self, sim) -> int:
         return sim.grid.n
@@ -98,6 +100,8 @@ class GpuBackend:
         out = {}
         if keep:
             out["T"] = sim.grid_T()
+            for f in ("U", "V", "W"):
+                out[f] = sim._scene.get_field(f)
             out["temperatures"] = sim.temperatures().copy()
         return out, sim.stats()
 
@@ -132,11 +136,12 @@ class Server:
 
     def run(self, requests) -> list:
         reqs = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        prec = getattr(self.backend, "precision", "fp32")
         dts = {r.dt for r in reqs}
         if len(dts) > 1:
             raise ValueError("all requests of one run share dt (they advance in the same batched substeps)")
         for r in reqs:
-            if footprint_estimate(r.kwargs["grid"].n, len(r.kwargs["particles"])) > self.budget:
+            if footprint_estimate(r.kwargs["grid"].n, len(r.kwargs["particles"]), prec) > self.budget:
                 raise ValueError(f"request {r.rid} does not fit the device budget on its own")
         dt = dts.pop() if dts else 2e-3
         pending = collections.deque(reqs)
@@ -156,7 +161,7 @@ class Server:
                 # admission: arrival order, while the head of the queue fits
                 while waiting and len(active) < self.max_active:
                     r = waiting[0]
-                    est = footprint_estimate(r.kwargs["grid"].n, len(r.kwargs["particles"]))
+                    est = footprint_estimate(r.kwargs["grid"].n, len(r.kwargs["particles"]), prec)
                     if used + est > self.budget:
                         break
                     waiting.popleft()

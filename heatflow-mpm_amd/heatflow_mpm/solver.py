@@ -197,17 +197,25 @@ class HeatflowMPM:
 
     Same constructor as the reference (solver.py:42-43).  Extras, not in the reference: `body`
     replaces the default StaticBody built from `particles`; `cg="fused"` runs the pressure PCG in
-    one launch per iteration (HFM_FLAG_FUSED_CG in include/hfm.h) instead of warp's two-pass form.
+    one launch per iteration (HFM_FLAG_FUSED_CG in include/hfm.h) instead of warp's two-pass form;
+    `precision="mixed"` solves both systems by iterative refinement with FP16 inner vectors and
+    traces advection through a BF16 copy of the velocity (HFM_PREC_MIXED); the state stays fp32.
     """
 
     CG_FORMS = ("warp", "fused")
+    PRECISIONS = {"fp32": _abi.PREC_FP32, "mixed": _abi.PREC_MIXED}
 
     def __init__(self, particles: np.ndarray, mat: Material = Material(), heater: Heater = Heater(),
                  grid: GridConfig = GridConfig(), spacing: float = 2.5e-3, t_init: float = 20.0,
                  kappa_air: float = KAPPA_AIR, device: str = "cuda:0", body=None, cg: str = "warp",
-                 inputs: dict | None = None):
+                 inputs: dict | None = None, precision: str = "fp32"):
         if cg not in self.CG_FORMS:
             raise ValueError(f"cg must be one of {self.CG_FORMS}")
+        if precision not in self.PRECISIONS:
+            raise ValueError(f"precision must be one of {tuple(self.PRECISIONS)}")
+        if precision == "mixed" and cg == "fused":
+            raise ValueError("the fused PCG form is not available with precision='mixed'")
+        self.precision = precision
         self.mat, self.heater, self.dev, self.spacing, self.kappa_air = mat, heater, device, spacing, kappa_air
         particles = np.asarray(particles, np.float32).reshape(-1, 3)
         self.n = len(particles)
@@ -216,6 +224,7 @@ class HeatflowMPM:
         inp = dict(inp, params=dict(inp["params"]))
         if cg == "fused":
             inp["params"]["flags"] |= _abi.FLAG_FUSED_CG
+        inp["params"]["precision"] = self.PRECISIONS[precision]
         self._scene = _Scene(inp["params"], parse_device(device))
         self.grid = AirGrid(grid, self._scene)
         self.heat = HeatField(self.grid.n, self.grid.h, self._scene)

@@ -67,6 +67,15 @@ typedef struct hfm_params {
 } hfm_params;
 
 #define HFM_PREC_FP32 0       /* the reference's arithmetic: fp32 storage and math */
+/* Mixed precision (no counterpart in the reference).  State (T, u, p) stays fp32.  Both PCG
+ * solves become iterative refinement: an outer step forms the true residual r = b - A x in fp32
+ * arithmetic with its norm in fp64 and stops when ||r|| <= max(tol * ||b||, tol) (the
+ * reference's threshold, now on the true residual); otherwise an inner Jacobi-PCG on r scaled to
+ * unit RMS, with r, z, p and A p stored as FP16 (fp32 arithmetic, fp64 dot products), reduces it
+ * by 1e-2 and adds its correction to x.  The departure points of semi-Lagrangian advection are
+ * traced through a BF16 copy of u, v, w.  The iteration cap counts inner iterations; counts are
+ * reported exactly (no check_every quantisation).  Not combinable with HFM_FLAG_FUSED_CG. */
+#define HFM_PREC_MIXED 1
 
 #define HFM_FLAG_TIME_KERNELS 1  /* record HIP events around every CG kernel launch */
 /* Pressure PCG in one launch per iteration instead of two (no counterpart in the reference).
@@ -93,6 +102,8 @@ typedef struct hfm_stats {
     double rhs_mean;          /* mean removed from the pressure rhs (grid.py:239) */
     int64_t p_iters_total;    /* sum of p_iters_exact over all substeps */
     int64_t h_iters_total;    /* sum of h_iters_exact over all substeps */
+    int32_t p_outer;          /* HFM_PREC_MIXED: refinement steps of the pressure solve (0 in fp32) */
+    int32_t h_outer;          /* HFM_PREC_MIXED: refinement steps of the heat solve */
 } hfm_stats;
 
 /* Field identifiers for hfm_get_field / hfm_set_field. */

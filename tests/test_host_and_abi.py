@@ -145,10 +145,32 @@ def test_library_exports_every_declared_symbol():
     assert set(declared) == set(_abi.SIGNATURES), "ctypes signatures and header disagree"
 
 
-def test_struct_layouts_match_header():
-    # hfm_params: 8 int32 then 17 doubles; hfm_stats: 6 int32, 1 int64, 5 doubles, 2 int64
-    assert C.sizeof(_abi.Params) == 8 * 4 + 17 * 8
-    assert C.sizeof(_abi.Stats) == 6 * 4 + 8 + 5 * 8 + 2 * 8
+def test_struct_layouts_match_header(tmp_path):
+    """Every field offset and the size of hfm_params and hfm_stats as gcc lays them out from
+    include/hfm.h equal the ctypes mirrors."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{REPO}/include/hfm.h"', "int main(void) {"]
+    for cname, cls in (("hfm_params", _abi.Params), ("hfm_stats", _abi.Stats)):
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in cls._fields_:
+            lines.append(f'printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n"):
+        if line:
+            sname, field, val = line.split()
+            got[(sname, field)] = int(val)
+    for cname, cls in (("hfm_params", _abi.Params), ("hfm_stats", _abi.Stats)):
+        assert got[(cname, "size")] == C.sizeof(cls), cname
+        for field, _ in cls._fields_:
+            assert got[(cname, field)] == getattr(cls, field).offset, (cname, field)
 
 
 def test_errors_surface_without_a_device():
@@ -162,3 +184,21 @@ def test_errors_surface_without_a_device():
     assert rc == -1 and b"device" in lib.hfm_last_error()
     with pytest.raises(RuntimeError):
         HeatflowMPM(np.zeros((0, 3), np.float32), grid=GridConfig(L=0.08, h=0.01))
+
+
+def test_mixed_precision_argument_checks():
+    """The mixed form excludes the fused PCG form (Python and C checks; no device needed)."""
+    with pytest.raises(ValueError, match="fused"):
+        HeatflowMPM(np.zeros((0, 3), np.float32), grid=GridConfig(L=0.08, h=0.01), cg="fused", precision="mixed")
+    with pytest.raises(ValueError, match="precision"):
+        HeatflowMPM(np.zeros((0, 3), np.float32), grid=GridConfig(L=0.08, h=0.01), precision="fp16")
+    lib = _abi.load()
+    p = _abi.Params(n=8, cg_max_iter=10, cg_check_every=2, heat_max_iter=10, heat_check_every=2, h=0.01,
+                    precision=_abi.PREC_MIXED, flags=_abi.FLAG_FUSED_CG)
+    h = C.c_void_p()
+    assert lib.hfm_scene_create(C.byref(p), 0, C.byref(h)) == -1
+    assert b"FUSED" in lib.hfm_last_error()
+    p = _abi.Params(n=8, cg_max_iter=10, cg_check_every=2, heat_max_iter=10, heat_check_every=2, h=0.01,
+                    precision=7)
+    assert lib.hfm_scene_create(C.byref(p), 0, C.byref(h)) == -1
+    assert b"precision" in lib.hfm_last_error()
