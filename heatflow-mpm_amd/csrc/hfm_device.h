@@ -194,7 +194,7 @@ __device__ __forceinline__ double sum_wave_values(const double* w) {
 
 // Sum of one double per thread over a workgroup of BS threads; the result is valid in thread 0.
 // The tree is fixed (wave tree, then waves in order), so the value does not depend on timing.
-// lds must hold NV x (BS / 64) doubles.
+// lds must hold NV x (BS / 64) doubles and is not reused afterwards (the caller ends the kernel).
 template <int NV, int BS = kBlock>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double (*lds)[BS / 64]) {
     int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -208,7 +208,18 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double (*lds)[BS / 64
 #pragma unroll
         for (int q = 0; q < NV; ++q) v[q] = sum_wave_values<BS / 64>(lds[q]);
     }
-    __syncthreads();
+}
+
+// Element i of an array through a 32-bit byte offset (i * sizeof(T) < 2^32: arrays of at most
+// 2^30 four-byte elements), so the access is a global load/store with a scalar base and a 32-bit
+// vector offset instead of a 64-bit address computed per lane.
+template <class T>
+__device__ __forceinline__ T ld32(const T* p, unsigned i) {
+    return *(const T*)((const char*)p + i * (unsigned)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ void st32(T* p, unsigned i, T v) {
+    *(T*)((char*)p + i * (unsigned)sizeof(T)) = v;
 }
 
 // Cross-workgroup traffic of the reductions uses relaxed agent-scope atomics only: they are

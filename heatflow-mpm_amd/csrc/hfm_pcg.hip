@@ -258,9 +258,22 @@ __device__ __forceinline__ void store_partials(double (&v)[NV], double* __restri
     for (int u_ = 0; u_ < (v).cpt; ++u_)                                     \
         if (long t = ((long)L_ * (v).cpt + u_) * kBlock + threadIdx.x; t < (v).N)
 
-// Reduce NV per-thread partial sums over the workgroup in a fixed order (wave trees, then waves
-// in order) into every thread's out[0..nv), with one barrier that also broadcasts thread 0's
-// `flag`; returns the flag in every thread.
+// Partial sums of the previous launch: every thread adds entries threadIdx.x, + blockDim.x, ...
+// of each row, then reduce_bcast reduces them over the workgroup in a fixed order (wave trees,
+// then waves in order) into every thread's out[0..nv), with one barrier that also broadcasts
+// thread 0's `flag`; it returns the flag in every thread.  (Letting wave 0 alone load and reduce
+// the rows measured slower: the other waves then idle at the barrier.)
+template <int NV>
+__device__ __forceinline__ void load_rows(const double* P, int nb, int nv, double (&acc)[NV]) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+#pragma unroll 1
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+#pragma unroll
+        for (int q = 0; q < NV; ++q)
+            if (q < nv) acc[q] += P[q * nb + b];
+}
+
 template <int NV, int BS>
 __device__ __forceinline__ int reduce_bcast(const double (&acc)[NV], int nv, int flag, double (&out)[NV]) {
     __shared__ double wsum[NV][BS / 64];
@@ -296,13 +309,7 @@ template <int BS, bool IR = false, class View>
 __device__ __forceinline__ void pass_a_load(const View& a, PassAIn& p) {
     const CgCtl* c = a.ctl;
     const bool first = a.k == 0 && !IR;
-    p.acc[0] = p.acc[1] = p.acc[2] = 0.0;
-#pragma unroll 1
-    for (int b = threadIdx.x; b < a.nb_upd; b += BS) {
-        p.acc[0] += a.part_upd[b];
-        p.acc[1] += a.part_upd[a.nb_upd + b];
-        if (first) p.acc[2] += a.part_upd[2 * a.nb_upd + b];
-    }
+    load_rows<3>(a.part_upd, a.nb_upd, first ? 3 : 2, p.acc);
     p.done = c->done;
     p.maxeff = c->maxeff;
     p.rz_prev = c->slot_rz[(a.k + 1) & 1];
@@ -493,9 +500,9 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_dir_tile3(Src src, int kit, uns
     for (int m = 0; m < kM; ++m) {
         const int fi = grp + kGroups3 * m, gi = i0 + fi - 1;
         const unsigned t = (unsigned)clampi(gi, 0, n - 1) * nn + t_jk;
-        fz[m] = (float)zs[t];
-        fd[m] = first ? 0.0f : (float)ds[t];
-        fa[m] = as[t];
+        fz[m] = (float)ld32(zs, t);
+        fd[m] = first ? 0.0f : (float)ld32(ds, t);
+        fa[m] = ld32(as, t);
     }
     PassAIn pre;
     pass_a_load<kBlock3, IR>(v, pre);
@@ -540,8 +547,8 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_dir_tile3(Src src, int kit, uns
                 apc = div_const(1.0f * acc, hh, rhh, fastdiv);
             const unsigned t = tb + (unsigned)li * nn;
             const V pv = (V)pc, av = (V)apc;
-            v.d_new[t] = pv;
-            v.Ap[t] = av;
+            st32(v.d_new, t, pv);
+            st32(v.Ap, t, av);
             sum[0] += (double)((float)pv * (float)av);
         }
     }
@@ -566,20 +573,20 @@ __global__ __launch_bounds__(kBlock) void k_pcg_upd(Src src, int kit) {
     // loaded before the p.Ap reduction, so their latencies overlap
     constexpr int kPre = 4;
     float px[kPre], pd[kPre], pr[kPre], pa[kPre], pD[kPre];
+    const unsigned N32 = (unsigned)v.N;
 #pragma unroll
     for (int u = 0; u < kPre; ++u) {
-        const long t = ((long)L * v.cpt + u) * kBlock + threadIdx.x;
-        if (u < v.cpt && t < v.N) {
-            px[u] = v.x[t];
-            pd[u] = (float)v.d_new[t];
-            pr[u] = (float)v.r[t];
-            pa[u] = (float)v.Ap[t];
-            pD[u] = v.D[t];
+        const unsigned t = ((unsigned)L * v.cpt + u) * kBlock + threadIdx.x;
+        if (u < v.cpt && t < N32) {
+            px[u] = ld32(v.x, t);
+            pd[u] = (float)ld32(v.d_new, t);
+            pr[u] = (float)ld32(v.r, t);
+            pa[u] = (float)ld32(v.Ap, t);
+            pD[u] = ld32(v.D, t);
         }
     }
-    double acc[1] = {0.0};
-#pragma unroll 1
-    for (int b = threadIdx.x; b < v.nb_dir; b += kBlock) acc[0] += v.part_dir[b];
+    double acc[1];
+    load_rows<1>(v.part_dir, v.nb_dir, 1, acc);
     const int done = c->done;
     const float rr = c->slot_rr[kit & 1], rz = c->slot_rz[kit & 1], atol_sq = c->atol_sq;
     const float step = IR ? c->ir_step : 1.0f;
@@ -589,27 +596,27 @@ __global__ __launch_bounds__(kBlock) void k_pcg_upd(Src src, int kit) {
     const float alpha = (rr > atol_sq) ? rz / pAp : 0.0f;
     const float xa = IR ? step * alpha : alpha;
     double s[2] = {0.0, 0.0};
-    auto update = [&](long t, float x0, float d0, float r0, float a0, float D0) {
+    auto update = [&](unsigned t, float x0, float d0, float r0, float a0, float D0) {
         const float xc = x0 + xa * d0;
         const V rv = (V)(r0 - alpha * a0);
         const float rc = (float)rv;
         const V zv = (V)((IR ? D0 * dsc : D0) * rc);
         const float zc = (float)zv;
-        v.x[t] = xc;
-        v.r[t] = rv;
-        v.z[t] = zv;
+        st32(v.x, t, xc);
+        st32(v.r, t, rv);
+        st32(v.z, t, zv);
         s[0] += (double)(rc * rc);
         s[1] += (double)(rc * zc);
     };
 #pragma unroll
     for (int u = 0; u < kPre; ++u) {
-        const long t = ((long)L * v.cpt + u) * kBlock + threadIdx.x;
-        if (u < v.cpt && t < v.N) update(t, px[u], pd[u], pr[u], pa[u], pD[u]);
+        const unsigned t = ((unsigned)L * v.cpt + u) * kBlock + threadIdx.x;
+        if (u < v.cpt && t < N32) update(t, px[u], pd[u], pr[u], pa[u], pD[u]);
     }
     for (int u = kPre; u < v.cpt; ++u) {
-        const long t = ((long)L * v.cpt + u) * kBlock + threadIdx.x;
-        if (t >= v.N) break;
-        update(t, v.x[t], (float)v.d_new[t], (float)v.r[t], (float)v.Ap[t], v.D[t]);
+        const unsigned t = ((unsigned)L * v.cpt + u) * kBlock + threadIdx.x;
+        if (t >= N32) break;
+        update(t, ld32(v.x, t), (float)ld32(v.d_new, t), (float)ld32(v.r, t), (float)ld32(v.Ap, t), ld32(v.D, t));
     }
     store_partials<2>(s, v.part_upd, v.nb_upd, lb, lds);
 }
@@ -673,12 +680,8 @@ __global__ __launch_bounds__(kBlock) void k_ir_scale(Src src, int first) {
     __shared__ double lds[2][kWaves];
     const auto v = make_view<OP, true>(a, lb, 0, 0.0f);
     const double* part = a.partials + kIrPart;
-    double acc[2] = {0.0, 0.0};
-#pragma unroll 1
-    for (int b = threadIdx.x; b < v.nb_upd; b += kBlock) {
-        acc[0] += part[b];
-        if (first) acc[1] += part[v.nb_upd + b];
-    }
+    double acc[2];
+    load_rows<2>(part, v.nb_upd, first ? 2 : 1, acc);
     const double tol = c->tol;
     double atol = c->atol;
     const int total = c->ir_total, outer = c->ir_outer, maxiter = c->ir_maxiter;
@@ -825,16 +828,9 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_fused(Src src, int kit, unsigne
         g_r[m] = rk[t];  // out-of-box entries are never used
         g_d[m] = Dg[t];
     }
-    double acc[kFusedRows] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    {
-        const int rows = first ? kFusedRows : 4;
-        const int nrows = first ? a.nb_upd : nb;  // the init launch wrote rows of its own count
-#pragma unroll 1
-        for (int b = threadIdx.x; b < nrows; b += kBlock3)
-#pragma unroll
-            for (int q = 0; q < kFusedRows; ++q)
-                if (q < rows) acc[q] += part_in[q * nrows + b];
-    }
+    double acc[kFusedRows];
+    // the init launch wrote rows of its own count
+    load_rows<kFusedRows>(part_in, first ? a.nb_upd : nb, first ? kFusedRows : 4, acc);
     const int done = c->done, maxeff = c->maxeff;
     const float rz_prev = c->slot_rz[(kit + 1) & 1];
     const double pap_prev = c->slot_pap[(kit + 1) & 1];
