@@ -196,11 +196,15 @@ def bench_served(a) -> dict:
     s_p = summarize(res_p, span_p)
     ns = [r.kwargs["grid"].n for r in reqs]
     err_T = err_u = 0.0
+    per = []
     for rb, rp in zip(res, res_p):
-        err_T = max(err_T, rel_l2(rb.fields["T"], rp.fields["T"]))
+        eT = rel_l2(rb.fields["T"], rp.fields["T"])
         ub = np.concatenate([rb.fields[f].ravel() for f in ("U", "V", "W")])
         up = np.concatenate([rp.fields[f].ravel() for f in ("U", "V", "W")])
-        err_u = max(err_u, rel_l2(ub, up))
+        eu = rel_l2(ub, up)
+        err_T, err_u = max(err_T, eT), max(err_u, eu)
+        per.append(dict(rid=rb.rid, n=rb.n, substeps=rb.substeps, T=eT, u=eu, p_converged=rb.stats.get("p_converged")))
+    worst = sorted(per, key=lambda d: -d["u"])[:4]
     return dict(metric="served p50/p99 latency and aggregate cell-substeps/s, continuous batching",
                 value=s_b["cell_substeps_per_s"], unit="cell-substeps/s", n_gpus=1, steps=None, warmup=0,
                 ms_per_step=None, higher_is_better=True, scaling="weak", vs_baseline=None,
@@ -212,7 +216,7 @@ def bench_served(a) -> dict:
                                      f"PCG tol 1e-6, dt 2 ms", rate_per_s=a.rate, requests=len(reqs), seed=a.seed,
                             hbm_budget_gib=a.hbm_gib, slots=a.slots, pressure_cg=a.cg, precision=prec,
                             parallelism="1 GPU"),
-                max_rel_l2_vs_plain=dict(T=err_T, u=err_u),
+                max_rel_l2_vs_plain=dict(T=err_T, u=err_u), worst_requests_vs_plain=worst,
                 served=s_b, plain=s_p, speedup_vs_plain=round(s_b["cell_substeps_per_s"] / s_p["cell_substeps_per_s"], 3),
                 p99_ratio_plain_over_served=round(s_p["p99_s"] / s_b["p99_s"], 3),
                 max_batch=max(e[3] for e in served.log))

@@ -668,10 +668,12 @@ __global__ __launch_bounds__(kBlock) void k_ir_residual(Src src, float dt, int f
 }
 
 // Decide the refinement step from the residual's partial sums: stop the solve when
-// ||r|| <= atol (atol = max(tol ||b||, tol), formed at the first step) or at the iteration cap
-// (or on a non-finite residual); otherwise write r^ = r / sigma and z^ = (D / c) r^ in FP16 with
-// their partial sums, which the inner solve's first pass A reads, and set the inner solve's
-// state.
+// ||r|| <= atol (atol = max(tol ||b||, tol), formed at the first step), when the last step cut
+// ||r|| by less than sqrt(2) (the true residual has reached the floor of its fp32 evaluation,
+// which can lie above atol: the fp32 solve, which tests its recursively updated residual, stops
+// there too), at the iteration cap, or on a non-finite residual; otherwise write r^ = r / sigma
+// and z^ = (D / c) r^ in FP16 with their partial sums, which the inner solve's first pass A
+// reads, and set the inner solve's state.
 template <int OP, class Src>
 __global__ __launch_bounds__(kBlock) void k_ir_scale(Src src, int first) {
     HFM_ENTRY(src);
@@ -685,6 +687,7 @@ __global__ __launch_bounds__(kBlock) void k_ir_scale(Src src, int first) {
     const double tol = c->tol;
     double atol = c->atol;
     const int total = c->ir_total, outer = c->ir_outer, maxiter = c->ir_maxiter;
+    const double rr_prev = c->ir_rr;
     double tot[2];
     reduce_bcast<2, kBlock>(acc, first ? 2 : 1, 0, tot);
     const double rr = tot[0];
@@ -694,7 +697,8 @@ __global__ __launch_bounds__(kBlock) void k_ir_scale(Src src, int first) {
     }
     const bool finite = rr <= 1.0e300;   // false for NaN and infinity
     const bool conv = finite && rr <= atol * atol;
-    if (conv || !finite || total >= maxiter) {
+    const bool stalled = !first && finite && !conv && rr > 0.5 * rr_prev;
+    if (conv || stalled || !finite || total >= maxiter) {
         if (lb == 0 && threadIdx.x == 0) {
             c->ir_end = 1;
             c->done = 1;
@@ -704,7 +708,7 @@ __global__ __launch_bounds__(kBlock) void k_ir_scale(Src src, int first) {
             CgMirror* m = v.mirror;
             __hip_atomic_store(&m->rr, rr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             __hip_atomic_store(&m->atol, atol, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-            __hip_atomic_store(&m->converged, conv ? 1 : (finite ? 0 : -1), __ATOMIC_RELAXED,
+            __hip_atomic_store(&m->converged, conv ? 1 : (stalled ? 2 : (finite ? 0 : -1)), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_SYSTEM);
             __hip_atomic_store(&m->it, total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             __hip_atomic_store(&m->outer, outer, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -735,6 +739,7 @@ __global__ __launch_bounds__(kBlock) void k_ir_scale(Src src, int first) {
         c->ir_outer = outer + 1;
         c->ir_gl = (float)(atol * atol / rr * (double)v.N);   // (atol / sigma)^2
         c->ir_step = (float)(sigma * cop);
+        c->ir_rr = rr;
         c->atol = atol;
         c->done = 0;
         c->converged = 0;

@@ -41,6 +41,7 @@ struct CgCtl {
     int ir_outer;      // refinement steps of this solve so far
     int ir_end;        // 1: the refinement has converged or reached the cap
     int ir_maxiter;    // iteration cap (inner iterations over the whole solve)
+    double ir_rr;      // ||r||^2 (true residual) at the previous refinement step
 };
 
 // Pinned host memory written by the device with system-scope stores (results of a solve),

@@ -92,7 +92,8 @@ typedef struct hfm_stats {
     int32_t p_iters_exact;    /* iterations that changed the pressure */
     int32_t h_iters;          /* heat PCG count as warp returns it: HeatField.cg_iters */
     int32_t h_iters_exact;
-    int32_t p_converged;      /* 1 when ||r||^2 <= atol^2 at exit */
+    int32_t p_converged;      /* 1 when ||r||^2 <= atol^2 at exit; HFM_PREC_MIXED also 2: stopped when a
+                                 refinement step no longer cut the true residual (its fp32 floor) */
     int32_t h_converged;
     int64_t substeps;         /* substeps run since creation */
     double p_rr;              /* final ||r||^2 of the pressure solve */

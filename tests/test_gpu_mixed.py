@@ -61,7 +61,7 @@ def test_mixed_plume_tracks_fp64(gpu):
           f"fp32 T {eT32:.2e} u {eU32:.2e} div {dp:.2e}", flush=True)
     assert eT <= 1e-3 and eU <= 1e-3 and eP <= 1e-3
     assert dm <= 1e-6
-    assert st["p_converged"] == 1 and st["h_converged"] == 1 and st["p_outer"] >= 1 and st["h_outer"] >= 1
+    assert st["p_converged"] in (1, 2) and st["h_converged"] in (1, 2) and st["p_outer"] >= 1 and st["h_outer"] >= 1
     assert its_m <= 1.15 * its_p
 
 
