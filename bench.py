@@ -174,7 +174,7 @@ def bench_served(a) -> dict:
     one batched substep; admission in arrival order by device footprint) and in the plain form
     (fp32, one request at a time, in arrival order).  Latency = completion - arrival.  Each served
     result is compared with the plain result of the same request (relative L2 of T and u)."""
-    reqs = poisson_requests(a.rate, a.requests, seed=a.seed, k_range=(a.kmin, a.kmax))
+    reqs = poisson_requests(a.rate, a.requests, seed=a.seed, k_range=(a.kmin, a.kmax), cg_max_iter=a.cg_max_iter)
     warm = [HeatflowMPM(**coupled_scene(32)[0], cg=a.cg) for _ in range(2)]
     warm[0].run(2e-3, 2)
     with HeatflowBatch(warm) as wb:
@@ -213,7 +213,8 @@ def bench_served(a) -> dict:
                 config=dict(workload=f"{len(reqs)} Poisson requests at {a.rate}/s, n in {{48,64,96,128,192}} "
                                      f"(counts {[ns.count(m) for m in (48, 64, 96, 128, 192)]}), K uniform "
                                      f"{a.kmin}-{a.kmax}, heater 100-800 W, fan 0-3 m/s, 2k-50k particles, "
-                                     f"PCG tol 1e-6, dt 2 ms", rate_per_s=a.rate, requests=len(reqs), seed=a.seed,
+                                     f"PCG tol 1e-6 (pressure cap {a.cg_max_iter}), dt 2 ms", rate_per_s=a.rate,
+                            requests=len(reqs), seed=a.seed, cg_max_iter=a.cg_max_iter,
                             hbm_budget_gib=a.hbm_gib, slots=a.slots, pressure_cg=a.cg, precision=prec,
                             parallelism="1 GPU"),
                 max_rel_l2_vs_plain=dict(T=err_T, u=err_u), worst_requests_vs_plain=worst,
@@ -238,16 +239,20 @@ def bench_mixed(a) -> dict:
     fields = {}
     for form in ("fp32", "mixed"):
         sim = HeatflowMPM(**kw, precision=form)
-        sim._scene.set_kernel_timing(True)
         sim.synchronize()
         t0 = time.perf_counter()
         sim.run(dt, K)
         sim.synchronize()
         wall = time.perf_counter() - t0
         st = sim.stats()
-        kt = {c: sim._scene.kernel_time(c) for c in (_abi.KT_PCG_DIR, _abi.KT_PCG_UPD)}
         dm, sm = sim._scene.diagnostics()
         fields[form] = {f: sim._scene.get_field(f) for f in ("T", "U", "V", "W", "P")}
+        # kernel times from HIP events over a few more substeps (events are kept out of the timed run)
+        sim._scene.set_kernel_timing(True)
+        sim.run(dt, min(K, 10))
+        sim.synchronize()
+        kt = {c: sim._scene.kernel_time(c) for c in (_abi.KT_PCG_DIR, _abi.KT_PCG_UPD)}
+        sim._scene.set_kernel_timing(False)
         out_forms[form] = dict(value=n ** 3 * K / wall, seconds=round(wall, 3), ms_per_step=wall * 1e3 / K,
                                pressure_iters_per_substep=st["p_iters_total"] / K,
                                heat_iters_per_substep=st["h_iters_total"] / K,
@@ -306,7 +311,8 @@ def main() -> int:
                     help="coupled/batched: precision of the substep (fp32 default); served: precision of the "
                          "batched server (mixed default; the plain form is fp32)")
     ap.add_argument("--fp64-n", type=int, default=0, help="mixed: grid size of the fp64-oracle error check (0: off)")
-    ap.add_argument("--cg-max-iter", type=int, default=2000, help="mixed: pressure PCG iteration cap")
+    ap.add_argument("--cg-max-iter", type=int, default=2000,
+                    help="mixed and served: pressure PCG iteration cap (n=192 needs more than the default 400)")
     ap.add_argument("--rate", type=float, default=20.0, help="served: arrivals per second")
     ap.add_argument("--requests", type=int, default=24, help="served: number of requests")
     ap.add_argument("--kmin", type=int, default=20, help="served: fewest substeps of a request")

@@ -59,11 +59,12 @@ def coupled_scene(n: int = 96, particles_target: int | None = None, seed: int = 
                 grid=GridConfig(L=0.48, h=0.48 / n, cg_tol=1e-6), spacing=spacing, t_init=20.0), 2e-3, 500
 
 
-def random_scene(rng, n: int, name: str = "scene"):
+def random_scene(rng, n: int, name: str = "scene", cg_max_iter: int = 400):
     """One scene of the batched and served workloads at n cells per side (0.48 m box): heater
     power U(100, 800) W, fan U(0, 3) m/s, an ellipsoid of U{2000..50000} particles with semi-axes
     U(3, 7) cm each (lattice spacing (4/3 pi a b c / P)^(1/3), jitter 12 % of it) at 20 C; melt
-    window 28-34 C, PCG tolerance 1e-6.  Draw order: power, fan, P, semi-axes, particle seed."""
+    window 28-34 C, PCG tolerance 1e-6 with at most cg_max_iter pressure iterations (400, the
+    reference's default).  Draw order: power, fan, P, semi-axes, particle seed."""
     power = float(rng.uniform(100.0, 800.0))
     fan = float(rng.uniform(0.0, 3.0))
     target = int(rng.integers(2000, 50001))
@@ -73,7 +74,8 @@ def random_scene(rng, n: int, name: str = "scene"):
     P = ellipsoid_particles(semi_axes=axes, spacing=spacing, jitter=0.12 * spacing, seed=pseed)
     kw = dict(particles=P, mat=Material(t_sol=28.0, t_liq=34.0),
               heater=Heater(power=power, t_cap=120.0, fan=fan, dist=0.21),
-              grid=GridConfig(L=0.48, h=0.48 / n, cg_tol=1e-6), spacing=spacing, t_init=20.0)
+              grid=GridConfig(L=0.48, h=0.48 / n, cg_tol=1e-6, cg_max_iter=cg_max_iter), spacing=spacing,
+              t_init=20.0)
     return f"{name}_n{n}_P{len(P)}", kw
 
 

@@ -52,10 +52,11 @@ def footprint_estimate(n: int, particles: int, precision: str = "fp32") -> int:
     """Device bytes of a scene before it is created: 21 face-sized and 27 cell-sized fp32
     arrays (fields, static parts, solver vectors, bins), 1 byte of flags per cell, 12 fp32
     words and 4 ints per particle, and fixed tables (partial sums, programs); the mixed form
-    adds 5 FP16 cell arrays (inner solve) and 3 BF16 face arrays (velocity copy)."""
+    adds 5 FP16 cell arrays (inner solve), 4 FP16 cell arrays (face coefficients) and 3 BF16 face
+    arrays (velocity copy)."""
     N = n ** 3
     F = (n + 1) * n * n
-    extra = 2 * (5 * N + 3 * F) if precision == "mixed" else 0
+    extra = 2 * (9 * N + 3 * F) if precision == "mixed" else 0
     return 4 * (21 * F + 27 * N) + N + 64 * particles + 256 * 1024 + extra
 
 
@@ -208,10 +209,12 @@ class Server:
         return sorted(results, key=lambda r: r.rid)
 
 
-def poisson_requests(rate: float, count: int, seed: int = 0, sizes=(48, 64, 96, 128, 192), k_range=(50, 1000)):
+def poisson_requests(rate: float, count: int, seed: int = 0, sizes=(48, 64, 96, 128, 192), k_range=(50, 1000),
+                     cg_max_iter: int = 400):
     """Seeded Poisson arrivals at `rate` per second; each request draws n from `sizes`, K
     uniformly from k_range, and heater / fan / particle-blob parameters as the batched scenes
-    (heatflow_mpm.scenes.batched_scenes).  Draw order: inter-arrival gap, n, K, then the scene."""
+    (heatflow_mpm.scenes.batched_scenes), with at most cg_max_iter pressure iterations.  Draw
+    order: inter-arrival gap, n, K, then the scene."""
     from .scenes import random_scene
     rng = np.random.default_rng(seed)
     t, out = 0.0, []
@@ -219,7 +222,7 @@ def poisson_requests(rate: float, count: int, seed: int = 0, sizes=(48, 64, 96, 
         t += float(rng.exponential(1.0 / rate))
         n = int(rng.choice(list(sizes)))
         K = int(rng.integers(k_range[0], k_range[1] + 1))
-        _, kw = random_scene(rng, n, name=f"r{i:03d}")
+        _, kw = random_scene(rng, n, name=f"r{i:03d}", cg_max_iter=cg_max_iter)
         out.append(Request(i, kw, K, t))
     return out
 
