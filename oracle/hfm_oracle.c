@@ -18,6 +18,11 @@
  *
  * The pressure rhs mean of grid.py:239 is numpy's float32 mean (pairwise float32 sum, then
  * float32(sum / intp(N))); it is restated exactly for the fp32 build.
+ *
+ * Storage rounding (orc_set_storage_rounding, off by default) is not part of the restatement:
+ * it rounds the stored velocity and/or temperature to BF16 or FP16 after every write, or traces
+ * departure points through a BF16 copy of the velocity, so tests can measure which
+ * low-precision storage keeps a run within a tolerance of the fp64 truth.
  */
 #include <math.h>
 #include <stdint.h>
@@ -227,7 +232,40 @@ typedef struct orc_scene {
     /* CG scratch */
     real *cr, *cz, *cp, *cAp;
     hfm_stats st;
+    /* storage rounding of the velocity and the temperature (0 off, 1 BF16, 2 FP16) and departure
+       points traced through a BF16 copy of the velocity (orc_set_storage_rounding) */
+    int round_u, round_t, bt_bf16;
 } orc_scene;
+
+/* x rounded to BF16 (round to nearest even on the fp32 bit pattern) or FP16 (11 significant
+   bits, subnormal spacing 2^-24, overflow to infinity), returned as a double */
+static double store_round(double v, int mode) {
+    if (mode == 1) {
+        float f = (float)v;
+        uint32_t u;
+        memcpy(&u, &f, 4);
+        u += 0x7FFFu + ((u >> 16) & 1u);
+        u &= 0xFFFF0000u;
+        memcpy(&f, &u, 4);
+        return f;
+    }
+    if (mode == 2) {
+        double a = fabs((double)(float)v);
+        if (a == 0.0) return 0.0;
+        int e;
+        frexp(a, &e);
+        double q = (e - 1) >= -14 ? ldexp(1.0, e - 11) : ldexp(1.0, -24);
+        double r = nearbyint(a / q) * q;
+        if (r > 65504.0) r = INFINITY;
+        return v < 0 ? -r : r;
+    }
+    return v;
+}
+
+static void store_round_array(real* a, long n, int mode) {
+    if (!mode) return;
+    for (long t = 0; t < n; ++t) a[t] = (real)store_round((double)a[t], mode);
+}
 
 static real* zalloc(long n) { return (real*)calloc((size_t)(n > 0 ? n : 1), sizeof(real)); }
 
@@ -728,8 +766,17 @@ static void st_source(orc_scene* s, double dt) {
     }
 }
 
+static real* bf16_copy(const real* a, long n) {
+    real* c = zalloc(n);
+    for (long t = 0; t < n; ++t) c[t] = (real)store_round((double)a[t], 1);
+    return c;
+}
+
 /* heat.py:31-41 */
 static void st_advect_T(orc_scene* s, double dtd) {
+    real* bu[3] = {s->u[0], s->u[1], s->u[2]};
+    if (s->bt_bf16)
+        for (int a = 0; a < 3; ++a) bu[a] = bf16_copy(s->u[a], s->F[a]);
     int n = s->n;
     real h = s->h, dt = (real)dtd;
     for (int i = 0; i < n; ++i)
@@ -742,9 +789,12 @@ static void st_advect_T(orc_scene* s, double dtd) {
                 }
                 real x[3] = {((real)i + R(0.5)) * h, ((real)j + R(0.5)) * h, ((real)k + R(0.5)) * h};
                 real d[3];
-                backtrace(s->u[0], s->u[1], s->u[2], x, h, n, dt, d);
+                backtrace(bu[0], bu[1], bu[2], x, h, n, dt, d);
                 s->T2[t] = sample(s->Tw, d, OFF_CELL, h, n, n, n);
             }
+    if (s->bt_bf16)
+        for (int a = 0; a < 3; ++a) free(bu[a]);
+    store_round_array(s->T2, s->N, s->round_t);
 }
 
 /* heat.py:44-47 */
@@ -773,6 +823,7 @@ static void st_diffuse(orc_scene* s, double dtd) {
     s->st.h_converged = r.converged;
     s->st.h_rr = r.rr;
     s->st.h_atol = r.atol;
+    store_round_array(s->T2, s->N, s->round_t);
 }
 
 /* heat.py:161-174 */
@@ -836,6 +887,9 @@ static void st_face_fraction(orc_scene* s) {
 
 /* grid.py:25-33 and the swap of grid.py:224-227: all three axes read the old u, v, w */
 static void st_advect_faces(orc_scene* s, double dtd) {
+    real* bu[3] = {s->u[0], s->u[1], s->u[2]};
+    if (s->bt_bf16)
+        for (int a = 0; a < 3; ++a) bu[a] = bf16_copy(s->u[a], s->F[a]);
     int n = s->n;
     real h = s->h, dt = (real)dtd;
     for (int axis = 0; axis < 3; ++axis) {
@@ -853,14 +907,17 @@ static void st_advect_faces(orc_scene* s, double dtd) {
                     }
                     real x[3], dp[3];
                     face_position(axis, i, j, k, h, x);
-                    backtrace(s->u[0], s->u[1], s->u[2], x, h, n, dt, dp);
+                    backtrace(bu[0], bu[1], bu[2], x, h, n, dt, dp);
                     s->scr[axis][f] = sample(s->u[axis], dp, off, h, d[0], d[1], d[2]);
                 }
     }
+    if (s->bt_bf16)
+        for (int a = 0; a < 3; ++a) free(bu[a]);
     for (int a = 0; a < 3; ++a) {
         real* tmp = s->u[a];
         s->u[a] = s->scr[a];
         s->scr[a] = tmp;
+        store_round_array(s->u[a], s->F[a], s->round_u);
     }
 }
 
@@ -975,6 +1032,7 @@ static void st_project(orc_scene* s, double dtd) {
                     s->u[axis][f] = s->u[axis][f] - dor * gp / h;
                 }
     }
+    for (int a = 0; a < 3; ++a) store_round_array(s->u[a], s->F[a], s->round_u);
 }
 
 ORC_API int orc_stage(orc_scene* s, int stage, double dt) {
@@ -1012,6 +1070,12 @@ ORC_API void orc_substep(orc_scene* s, double dt) {
         orc_stage(s, st, dt);
     }
     s->st.substeps += 1;
+}
+
+ORC_API void orc_set_storage_rounding(orc_scene* s, int round_u, int round_t, int bt_bf16) {
+    s->round_u = round_u;
+    s->round_t = round_t;
+    s->bt_bf16 = bt_bf16;
 }
 
 ORC_API void orc_run(orc_scene* s, double dt, int K) {

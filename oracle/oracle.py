@@ -51,6 +51,7 @@ _SIGS = {
     "orc_stage": (C.c_int, [_VP, C.c_int, C.c_double]),
     "orc_substep": (None, [_VP, C.c_double]),
     "orc_run": (None, [_VP, C.c_double, C.c_int]),
+    "orc_set_storage_rounding": (None, [_VP, C.c_int, C.c_int, C.c_int]),
     "orc_cell_velocity": (None, [_VP, _D]),
     "orc_diagnostics": (None, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "orc_numpy_mean_f32": (C.c_float, [_F, C.c_long]),
@@ -147,6 +148,13 @@ class OracleSim:
         self.L.orc_set_particles(self.s, self.n_particles, pos.ctypes.data_as(C.c_void_p),
                                  vel.ctypes.data_as(C.c_void_p), temp.ctypes.data_as(C.c_void_p))
         self._keep = (pos, vel, temp)
+
+    def set_storage_rounding(self, u: str = "fp32", T: str = "fp32", backtrace_bf16: bool = False) -> None:
+        """Round the stored velocity / temperature to 'bf16' or 'fp16' after every write, and/or trace
+        departure points through a BF16 copy of the velocity (not part of the restatement: a probe of
+        which low-precision storage a tolerance allows)."""
+        modes = {"fp32": 0, "bf16": 1, "fp16": 2}
+        self.L.orc_set_storage_rounding(self.s, modes[u], modes[T], int(bool(backtrace_bf16)))
 
     def substep(self, dt: float) -> None:
         self.L.orc_substep(self.s, float(dt))

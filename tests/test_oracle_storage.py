@@ -1,0 +1,49 @@
+"""Which low-precision storage keeps the substep within the served tolerance (1e-3 relative L2
+against fp64)?  The oracle rounds the stored velocity or temperature to BF16 / FP16 after every
+write (or traces departure points through a BF16 velocity copy) and runs the heater plume
+against the fp64 oracle.  These are the measurements that fix the mixed form's design: BF16 or
+FP16 state storage fails the tolerance, the BF16 departure-point velocity does not."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import rel_l2
+from heatflow_mpm import GridConfig, Heater
+
+K = 60
+BAR = 1e-3
+
+
+@pytest.fixture(scope="module")
+def runs():
+    kw = dict(particles=np.zeros((0, 3), np.float32), heater=Heater(power=400.0, t_cap=120.0, t_rod=300.0, fan=0.0),
+              grid=GridConfig(L=0.48, h=0.02, cg_tol=1e-6), t_init=25.0)   # n = 24, heater inside the box
+    ref = O.OracleSim(**kw, precision="f64")
+    ref.run(2e-3, K)
+    uref = np.concatenate([ref.field(f).ravel() for f in "UVW"])
+    out = {}
+    for name, cfg in (("fp32", {}), ("T_bf16", dict(T="bf16")), ("T_fp16", dict(T="fp16")),
+                      ("u_bf16", dict(u="bf16")), ("backtrace_bf16", dict(backtrace_bf16=True))):
+        o = O.OracleSim(**kw)
+        o.set_storage_rounding(**cfg)
+        o.run(2e-3, K)
+        u = np.concatenate([o.field(f).ravel() for f in "UVW"])
+        out[name] = (rel_l2(o.field("T"), ref.field("T")), rel_l2(u, uref))
+        print(name, "T %.2e u %.2e" % out[name], flush=True)
+    return out
+
+
+def test_fp32_tracks_fp64(runs):
+    eT, eu = runs["fp32"]
+    assert eT < 1e-4 and eu < 1e-4
+
+
+@pytest.mark.parametrize("name", ["T_bf16", "T_fp16", "u_bf16"])
+def test_low_precision_state_fails_the_bar(runs, name):
+    eT, eu = runs[name]
+    assert max(eT, eu) > BAR
+
+
+def test_bf16_departure_velocity_keeps_the_bar(runs):
+    eT, eu = runs["backtrace_bf16"]
+    assert eT < 2e-4 and eu < 2e-4
