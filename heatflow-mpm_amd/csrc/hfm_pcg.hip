@@ -452,16 +452,6 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(Src src, float dt, int k,
     store_partials<1>(s, v.part_dir, v.nb_dir, lb, lds);
 }
 
-// FP16 pairs in 32-bit words
-__device__ __forceinline__ unsigned pack_half2(float lo, float hi) {
-    return (unsigned)__builtin_bit_cast(unsigned short, (_Float16)lo) |
-           ((unsigned)__builtin_bit_cast(unsigned short, (_Float16)hi) << 16);
-}
-__device__ __forceinline__ float half_lo(unsigned u) {
-    return (float)__builtin_bit_cast(_Float16, (unsigned short)(u & 0xffffu));
-}
-__device__ __forceinline__ float half_hi(unsigned u) { return (float)__builtin_bit_cast(_Float16, (unsigned short)(u >> 16)); }
-
 // Pass A for the pressure operator, LDS-tiled.  A workgroup of 1024 threads owns a 16 x 16 x D
 // block of cells and stages it framed by a one-cell halo ((D+2) x 18 x 18 cells) in LDS: the
 // solid fraction, and once the prologue has formed beta, the new direction p = z + beta p_old.
@@ -557,95 +547,6 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_dir_tile3(Src src, int kit, uns
                 apc = div_const(1.0f * acc, hh, rhh, fastdiv);
             const unsigned t = tb + (unsigned)li * nn;
             const V pv = (V)pc, av = (V)apc;
-            st32(v.d_new, t, pv);
-            st32(v.Ap, t, av);
-            sum[0] += (double)((float)pv * (float)av);
-        }
-    }
-    store_partials<1, kBlock3>(sum, v.part_dir, nb, lb, lds);
-}
-
-// Pass A of the inner solve of a refinement (FP16 vectors), LDS-tiled like k_pcg_dir_tile3, with
-// the inner operator A^ p = sum a_f (p_c - p_nb) taken from the FP16 face coefficients of
-// k_ir_residual (a cell's -x, -y, -z faces; its +x face is the -x face of the next cell) instead
-// of recomputing the six face fractions from the solid fraction: 6 subtractions and 6 mixed
-// FP16 x fp32 fused multiply-adds per cell.  The inner operator needs no bit-exact match with A,
-// the refinement's true residual (fp32 A) corrects for it.  Out-of-box frame cells hold the
-// clamped in-box values, so their terms are a * 0.
-template <int D, class Src>
-__global__ __launch_bounds__(kBlock3) void k_pcg_dir_tile3_ir(Src src, int kit, unsigned long long* pace,
-                                                              unsigned tag) {
-    using V = _Float16;
-    post_pace(pace, tag, kit);
-    HFM_ENTRY(src);
-    if (batch_solve_done(src, a.ctl + 1)) return;
-    __shared__ float P[D + 2][kF3][kF3];
-    __shared__ uint2 AF[D + 2][kF3][kF3];
-    __shared__ double lds[1][kBlock3 / 64];
-    const auto v = make_view<1, true>(a, lb, kit, 0.0f);
-    const int nb = v.nb_dir;
-    const int n = v.n;
-    const int L = xcd_tile(lb, nb);
-    const int tjk = v.tiles_j * v.tiles_k;
-    const int ti = L / tjk, rem = L % tjk;
-    const int i0 = ti * D, j0 = (rem / v.tiles_k) * kT3, k0 = (rem % v.tiles_k) * kT3;
-    const bool first = kit == 0;
-    const V* __restrict__ zs = v.z;
-    const V* __restrict__ ds = v.d_old;
-    const uint2* __restrict__ afs = a.ir_af;
-    const unsigned nn = (unsigned)n * n;
-    constexpr int kM = (D + 2 + kGroups3 - 1) / kGroups3;
-    const int grp = threadIdx.x / kPlane3, pos = threadIdx.x % kPlane3;
-    const int fj = pos / kF3, fk = pos % kF3;
-    const int gj = j0 + fj - 1, gk = k0 + fk - 1;
-    const unsigned t_jk = (unsigned)clampi(gj, 0, n - 1) * n + clampi(gk, 0, n - 1);
-    float fz[kM], fd[kM];
-    uint2 fa[kM];
-#pragma unroll
-    for (int m = 0; m < kM; ++m) {
-        const int fi = grp + kGroups3 * m, gi = i0 + fi - 1;
-        const unsigned t = (unsigned)clampi(gi, 0, n - 1) * nn + t_jk;
-        fz[m] = (float)ld32(zs, t);
-        fd[m] = first ? 0.0f : (float)ld32(ds, t);
-        fa[m] = ld32(afs, t);
-    }
-    PassAIn pre;
-    pass_a_load<kBlock3, true>(v, pre);
-    float beta;
-    if (finish_iteration<kBlock3, true>(v, pre, beta)) return;
-    if (grp < kGroups3) {
-#pragma unroll
-        for (int m = 0; m < kM; ++m) {
-            const int fi = grp + kGroups3 * m;
-            if (fi < D + 2) {
-                P[fi][fj][fk] = first ? fz[m] : fz[m] + beta * fd[m];
-                AF[fi][fj][fk] = fa[m];
-            }
-        }
-    }
-    __syncthreads();
-    const int lk = threadIdx.x % kT3, lj = (threadIdx.x / kT3) % kT3, lq = threadIdx.x / (kT3 * kT3);
-    const int j = j0 + lj, k = k0 + lk;
-    double sum[1] = {0.0};
-    if (j < n && k < n) {
-        const int sj = lj + 1, sk = lk + 1;
-        const unsigned tb = ((unsigned)i0 * n + j) * n + k;
-#pragma unroll
-        for (int m = 0; m < D / 4; ++m) {
-            const int li = lq + 4 * m, i = i0 + li;
-            if (i >= n) break;
-            const int si = li + 1;
-            const float pc = P[si][sj][sk];
-            const uint2 own = AF[si][sj][sk];
-            float acc = 0.0f;
-            acc = fmaf(half_lo(own.x), pc - P[si - 1][sj][sk], acc);
-            acc = fmaf(half_lo(AF[si + 1][sj][sk].x), pc - P[si + 1][sj][sk], acc);
-            acc = fmaf(half_hi(own.x), pc - P[si][sj - 1][sk], acc);
-            acc = fmaf(half_hi(AF[si][sj + 1][sk].x), pc - P[si][sj + 1][sk], acc);
-            acc = fmaf(half_lo(own.y), pc - P[si][sj][sk - 1], acc);
-            acc = fmaf(half_lo(AF[si][sj][sk + 1].y), pc - P[si][sj][sk + 1], acc);
-            const unsigned t = tb + (unsigned)li * nn;
-            const V pv = (V)pc, av = (V)acc;
             st32(v.d_new, t, pv);
             st32(v.Ap, t, av);
             sum[0] += (double)((float)pv * (float)av);
@@ -754,15 +655,6 @@ __global__ __launch_bounds__(kBlock) void k_ir_residual(Src src, float dt, int f
         v.r[t] = rc;
         s[0] += (double)rc * (double)rc;
         s[1] += (double)bc * (double)bc;
-        if (OP == 1 && first) {   // the inner operator's face coefficients, constant over the solve
-            const float* as = v.alpha_s;
-            const long nn = (long)v.n * v.n;
-            const float ac = as[t], amin = v.amin;
-            const float fx = i > 0 ? face_fraction(as[t - nn], ac, amin) : 0.0f;
-            const float fy = j > 0 ? face_fraction(as[t - v.n], ac, amin) : 0.0f;
-            const float fz = k > 0 ? face_fraction(as[t - 1], ac, amin) : 0.0f;
-            a.ir_af[t] = make_uint2(pack_half2(fx, fy), pack_half2(fz, 0.0f));
-        }
     }
     store_partials<2>(s, a.partials + kIrPart, v.nb_upd, lb, lds);
     if (first && lb == 0 && threadIdx.x == 0) {
@@ -1139,14 +1031,6 @@ void launch_pcg_init(const Launch& L, int op, float dt) {
 template <bool IR>
 void launch_dir(const Launch& L, Family fam, int op, float dt, int k, unsigned long long* pace, unsigned tag,
                 hipEvent_t ev0, hipEvent_t ev1) {
-    if (IR) {
-        switch (fam) {
-            case FAM_DIR4: HFM_LAUNCH_T(k_pcg_dir_tile3_ir, 4, L, kBlock3, ev0, ev1, k, pace, tag); return;
-            case FAM_DIR8: HFM_LAUNCH_T(k_pcg_dir_tile3_ir, 8, L, kBlock3, ev0, ev1, k, pace, tag); return;
-            case FAM_DIR16: HFM_LAUNCH_T(k_pcg_dir_tile3_ir, 16, L, kBlock3, ev0, ev1, k, pace, tag); return;
-            default: break;
-        }
-    }
     switch (fam) {
         case FAM_DIR4: HFM_LAUNCH_T2(k_pcg_dir_tile3, 4, IR, L, kBlock3, ev0, ev1, k, pace, tag); break;
         case FAM_DIR8: HFM_LAUNCH_T2(k_pcg_dir_tile3, 8, IR, L, kBlock3, ev0, ev1, k, pace, tag); break;

@@ -112,7 +112,6 @@ struct SceneView {
     ptr<float> qpos;         // pos / h per particle, in bin order (3 per particle)
     int mixed;               // HFM_PREC_MIXED
     ptr<_Float16> ir_r, ir_z, ir_d, ir_q;  // inner solve: r, z, p (two parity buffers), A p
-    ptr<uint2> ir_af;        // inner pressure operator: FP16 air fractions of each cell's -x, -y, -z faces
     ptr<uint16_t> ub[3];     // BF16 copies of u, v, w (departure points of the advection)
 };
 using SceneDev = SceneView<HostPtrs>;  // host copy, written by the host

@@ -963,7 +963,6 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
         A(d.ir_z, N);
         A(d.ir_d, 2 * N);
         A(d.ir_q, N);
-        A(d.ir_af, N);
         for (int a = 0; a < 3; ++a) A(d.ub[a], F);  // zero: the BF16 pattern of u = 0
     }
     A(d.ctl, 2);

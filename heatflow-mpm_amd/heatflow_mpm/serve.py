@@ -52,11 +52,10 @@ def footprint_estimate(n: int, particles: int, precision: str = "fp32") -> int:
     """Device bytes of a scene before it is created: 21 face-sized and 27 cell-sized fp32
     arrays (fields, static parts, solver vectors, bins), 1 byte of flags per cell, 12 fp32
     words and 4 ints per particle, and fixed tables (partial sums, programs); the mixed form
-    adds 5 FP16 cell arrays (inner solve), 4 FP16 cell arrays (face coefficients) and 3 BF16 face
-    arrays (velocity copy)."""
+    adds 5 FP16 cell arrays (inner solve) and 3 BF16 face arrays (velocity copy)."""
     N = n ** 3
     F = (n + 1) * n * n
-    extra = 2 * (9 * N + 3 * F) if precision == "mixed" else 0
+    extra = 2 * (5 * N + 3 * F) if precision == "mixed" else 0
     return 4 * (21 * F + 27 * N) + N + 64 * particles + 256 * 1024 + extra
 
 

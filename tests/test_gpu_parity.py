@@ -149,3 +149,4 @@ def test_cg_cap_is_reported_not_raised(gpu):
         sim.substep(dt)
     st = sim.stats()
     assert st["p_iters_exact"] == 4 and st["p_iters"] == 4 and st["p_converged"] == 0
+
