@@ -15,9 +15,10 @@ Beside the throughput the JSON line carries:
   roofline     the pressure-PCG kernel that takes most time, from HIP events around each of its
                launches over a second timed region (same K): algorithmic bytes of one PCG iteration
                of that kernel x real iterations / total kernel time, against 8 TB/s.
-  cpu_baseline the fp32 CPU oracle (C restatement of the reference's substep, one thread) on a
-               bounded sample of the same workload: it starts from the GPU state after warmup and
-               runs substeps for about 15 s.
+  cpu_baseline the fp32 CPU oracle (C restatement of the reference's substep) on a bounded sample
+               of the same workload: it starts from the GPU state after warmup and runs substeps
+               for about 15 s on one host thread (value, cores = 1) and about 8 s on all OpenMP
+               threads (parallel).
 """
 from __future__ import annotations
 
@@ -81,25 +82,38 @@ def substep_bytes(N: int, P: int, I_p: float, I_h: float, fused: bool) -> float:
 
 
 def cpu_baseline(kw, dt, state, budget_s: float = 15.0):
-    """fp32 CPU oracle from the GPU state after warmup; substeps until ~budget_s elapsed."""
+    """fp32 CPU oracle from the GPU state after warmup, substeps until ~budget_s elapsed: once on
+    one host thread (the reported value) and once on every thread OpenMP is given (the oracle's
+    per-cell loops in parallel; sums stay serial, results are the same bits)."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O
-    orc = O.OracleSim(**kw)
-    for name in ("T", "U", "V", "W", "P"):
-        orc.set_field(name, state[name])
-    if orc.n_particles:
-        orc.set_particle_temperatures(state["temp"])
-    n = orc.n
-    done, t0 = 0, time.perf_counter()
-    while True:
-        orc.substep(dt)
-        done += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or done >= 50:
-            break
+    lib = O.lib("f32")
+    threads_all = int(lib.orc_max_threads())
+
+    def run(threads, budget):
+        lib.orc_set_threads(threads)
+        orc = O.OracleSim(**kw)
+        for name in ("T", "U", "V", "W", "P"):
+            orc.set_field(name, state[name])
+        if orc.n_particles:
+            orc.set_particle_temperatures(state["temp"])
+        done, t0 = 0, time.perf_counter()
+        while True:
+            orc.substep(dt)
+            done += 1
+            el = time.perf_counter() - t0
+            if el >= budget or done >= 50:
+                break
+        return orc.n, done, el
+
+    n, done, el = run(1, budget_s)
+    _, done_p, el_p = run(threads_all, budget_s / 2)
+    lib.orc_set_threads(threads_all)
     return dict(value=n ** 3 * done / el, unit="cell-substeps/s", cores=1, kind="port",
                 sample=f"n={n} coupled scene, {done} substeps starting from the GPU state after warmup, "
-                       f"fp32 C oracle on one host thread, {el:.1f} s")
+                       f"fp32 C oracle on one host thread, {el:.1f} s",
+                parallel=dict(value=n ** 3 * done_p / el_p, cores=threads_all,
+                              sample=f"same, {done_p} substeps on {threads_all} OpenMP threads, {el_p:.1f} s"))
 
 
 def bench_batched(a) -> dict:

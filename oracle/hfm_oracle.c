@@ -19,12 +19,18 @@
  * The pressure rhs mean of grid.py:239 is numpy's float32 mean (pairwise float32 sum, then
  * float32(sum / intp(N))); it is restated exactly for the fp32 build.
  *
+ * Loops over cells, faces and vector entries run with OpenMP (every iteration writes its own
+ * element, so the result does not depend on the thread count); sums (dot products, the numpy
+ * mean) and the particle scatter stay serial in their fixed order.  orc_set_threads sets the
+ * thread count (1 for the single-core baseline).
+ *
  * Storage rounding (orc_set_storage_rounding, off by default) is not part of the restatement:
  * it rounds the stored velocity and/or temperature to BF16 or FP16 after every write, or traces
  * departure points through a BF16 copy of the velocity, so tests can measure which
  * low-precision storage keeps a run within a tolerance of the fp64 truth.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -343,6 +349,7 @@ ORC_API void orc_set_static(orc_scene* s, const float* alpha, const float* kx, c
 
 /* heat.py:147-151 */
 static void apply_fixed(orc_scene* s, real* T) {
+    #pragma omp parallel for schedule(static)
     for (long t = 0; t < s->N; ++t)
         if (s->fixed[t] == 1) T[t] = s->tfix[t];
 }
@@ -456,6 +463,7 @@ ORC_API void orc_get_stats(orc_scene* s, hfm_stats* out) { *out = s->st; }
 static void laplacian_apply(int n, real h, real* const af[3], const real* p, const real* y, real* out, real alpha,
                             real beta) {
     long nn = (long)n * n;
+    #pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j)
             for (int k = 0; k < n; ++k) {
@@ -475,6 +483,7 @@ static void laplacian_apply(int n, real h, real* const af[3], const real* p, con
 
 /* grid.py:118-137 */
 static void jacobi_diag(int n, real h, real* const af[3], real* d) {
+    #pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j)
             for (int k = 0; k < n; ++k) {
@@ -525,6 +534,7 @@ static real neighbour_terms(const real* x, const real* kap, const int32_t* fixed
 /* heat.py:94-105 */
 static void heat_apply(int n, real h, real dt, const real* kap, const int32_t* fixed, const real* tfix, const real* x,
                        const real* y, real* out, real alpha, real beta) {
+    #pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j)
             for (int k = 0; k < n; ++k) {
@@ -542,6 +552,7 @@ static void heat_apply(int n, real h, real dt, const real* kap, const int32_t* f
 /* heat.py:108-118 */
 static void heat_rhs(int n, real h, real dt, const real* T, const real* kap, const int32_t* fixed, const real* tfix,
                      real* rhs) {
+    #pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j)
             for (int k = 0; k < n; ++k) {
@@ -557,6 +568,7 @@ static void heat_rhs(int n, real h, real dt, const real* T, const real* kap, con
 /* heat.py:121-144 */
 static void heat_diag(int n, real h, real dt, const real* kap, const int32_t* fixed, real* d) {
     long nn = (long)n * n;
+    #pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j)
             for (int k = 0; k < n; ++k) {
@@ -631,6 +643,7 @@ static void pcg(const opctx* op, const real* diag, const real* b, real* x, long 
         res->rr = rr;
         return;
     }
+    #pragma omp parallel for schedule(static)
     for (long t = 0; t < N; ++t) z[t] = diag[t] * r[t];
     real rz_new = dotr(r, z, N);
     memcpy(p, z, (size_t)N * sizeof(real));
@@ -641,14 +654,17 @@ static void pcg(const opctx* op, const real* diag, const real* b, real* x, long 
         op_apply(op, p, Ap, Ap, R(1.0), R(0.0));
         real pAp = dotr(p, Ap, N);
         real alpha = (rr > atol_sq) ? rz_old / pAp : R(0.0);
+        #pragma omp parallel for schedule(static)
         for (long t = 0; t < N; ++t) {
             x[t] = x[t] + alpha * p[t];
             r[t] = r[t] - alpha * Ap[t];
         }
         rr = dotr(r, r, N);
+        #pragma omp parallel for schedule(static)
         for (long t = 0; t < N; ++t) z[t] = diag[t] * r[t];
         rz_new = dotr(r, z, N);
         real beta = (rr > atol_sq) ? rz_new / rz_old : R(0.0);
+        #pragma omp parallel for schedule(static)
         for (long t = 0; t < N; ++t) p[t] = z[t] + beta * p[t];
         ++it;
         /* past this point alpha = beta = 0: x and r are frozen, so stopping is exact */
@@ -734,8 +750,10 @@ static void st_p2g(orc_scene* s) {
 /* grid.py:206-213 */
 static void st_add_static(orc_scene* s) {
     if (!s->has_static) return;
+    #pragma omp parallel for schedule(static)
     for (long t = 0; t < s->N; ++t) s->alpha_s[t] = s->alpha_s[t] + s->st_alpha[t];
     for (int a = 0; a < 3; ++a)
+        #pragma omp parallel for schedule(static)
         for (long t = 0; t < s->F[a]; ++t) {
             s->kf[a][t] = s->kf[a][t] + s->st_k[a][t];
             s->mf[a][t] = s->mf[a][t] + s->st_m[a][t];
@@ -744,6 +762,7 @@ static void st_add_static(orc_scene* s) {
 
 /* heat.py:21-28 */
 static void st_blend(orc_scene* s) {
+    #pragma omp parallel for schedule(static)
     for (long t = 0; t < s->N; ++t) {
         if (s->halpha[t] > R(0.05))
             s->Tw[t] = s->tsum[t] / s->halpha[t];
@@ -756,6 +775,7 @@ static void st_blend(orc_scene* s) {
 static void st_source(orc_scene* s, double dt) {
     if (!(s->rate > 0)) return;
     real rate = (real)s->rate, tcap = (real)s->t_cap, d = (real)dt;
+    #pragma omp parallel for schedule(static)
     for (long t = 0; t < s->N; ++t) {
         if (s->src[t] == 1) {
             real T = s->Tw[t];
@@ -779,6 +799,7 @@ static void st_advect_T(orc_scene* s, double dtd) {
         for (int a = 0; a < 3; ++a) bu[a] = bf16_copy(s->u[a], s->F[a]);
     int n = s->n;
     real h = s->h, dt = (real)dtd;
+    #pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j)
             for (int k = 0; k < n; ++k) {
@@ -800,6 +821,7 @@ static void st_advect_T(orc_scene* s, double dtd) {
 /* heat.py:44-47 */
 static void st_conductivity(orc_scene* s) {
     real ka = (real)s->prm.kappa_air, km = (real)s->prm.kappa_mat;
+    #pragma omp parallel for schedule(static)
     for (long t = 0; t < s->N; ++t) {
         real c = s->alpha_s[t] / R(0.6);
         c = c < R(0.0) ? R(0.0) : (c > R(1.0) ? R(1.0) : c);
@@ -860,6 +882,7 @@ static void st_face_fraction(orc_scene* s) {
     for (int axis = 0; axis < 3; ++axis) {
         int d[3];
         face_dims(axis, n, d);
+        #pragma omp parallel for schedule(static)
         for (int i = 0; i < d[0]; ++i)
             for (int j = 0; j < d[1]; ++j)
                 for (int k = 0; k < d[2]; ++k) {
@@ -876,6 +899,7 @@ static void st_face_fraction(orc_scene* s) {
                     v = v < amin ? amin : (v > R(1.0) ? R(1.0) : v);
                     s->af[axis][f] = v;
                 }
+        #pragma omp parallel for schedule(static)
         for (int i = 0; i < d[0]; ++i)
             for (int j = 0; j < d[1]; ++j)
                 for (int k = 0; k < d[2]; ++k) {
@@ -896,6 +920,7 @@ static void st_advect_faces(orc_scene* s, double dtd) {
         int d[3];
         face_dims(axis, n, d);
         const real* off = face_offset(axis);
+        #pragma omp parallel for schedule(static)
         for (int i = 0; i < d[0]; ++i)
             for (int j = 0; j < d[1]; ++j)
                 for (int k = 0; k < d[2]; ++k) {
@@ -927,6 +952,7 @@ static void st_buoyancy(orc_scene* s, double dtd) {
     int n = s->n;
     real dt = (real)dtd, g = (real)s->prm.gravity, tamb = (real)s->prm.t_amb;
     real* w = s->u[2];
+    #pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j)
             for (int k = 1; k < n; ++k) {
@@ -944,6 +970,7 @@ static void st_relax(orc_scene* s, double dtd) {
     for (int axis = 0; axis < 3; ++axis) {
         int d[3];
         face_dims(axis, n, d);
+        #pragma omp parallel for schedule(static)
         for (int i = 0; i < d[0]; ++i)
             for (int j = 0; j < d[1]; ++j)
                 for (int k = 0; k < d[2]; ++k) {
@@ -965,6 +992,7 @@ static void divergence(orc_scene* s, real scale, real* rhs) {
     real *u = s->u[0], *v = s->u[1], *w = s->u[2];
     real *ax = s->af[0], *ay = s->af[1], *az = s->af[2];
     real *qx = s->q[0], *qy = s->q[1], *qz = s->q[2];
+    #pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j)
             for (int k = 0; k < n; ++k) {
@@ -1017,6 +1045,7 @@ static void st_project(orc_scene* s, double dtd) {
     for (int axis = 0; axis < 3; ++axis) {
         int d[3];
         face_dims(axis, n, d);
+        #pragma omp parallel for schedule(static)
         for (int i = 0; i < d[0]; ++i)
             for (int j = 0; j < d[1]; ++j)
                 for (int k = 0; k < d[2]; ++k) {
@@ -1077,6 +1106,10 @@ ORC_API void orc_set_storage_rounding(orc_scene* s, int round_u, int round_t, in
     s->round_t = round_t;
     s->bt_bf16 = bt_bf16;
 }
+
+ORC_API void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+
+ORC_API int orc_max_threads(void) { return omp_get_max_threads(); }
 
 ORC_API void orc_run(orc_scene* s, double dt, int K) {
     for (int i = 0; i < K; ++i) orc_substep(s, dt);

@@ -52,6 +52,8 @@ _SIGS = {
     "orc_substep": (None, [_VP, C.c_double]),
     "orc_run": (None, [_VP, C.c_double, C.c_int]),
     "orc_set_storage_rounding": (None, [_VP, C.c_int, C.c_int, C.c_int]),
+    "orc_set_threads": (None, [C.c_int]),
+    "orc_max_threads": (C.c_int, []),
     "orc_cell_velocity": (None, [_VP, _D]),
     "orc_diagnostics": (None, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "orc_numpy_mean_f32": (C.c_float, [_F, C.c_long]),
