@@ -132,7 +132,8 @@ def bench_batched(a) -> dict:
     for w in warm:
         w.close()
     prec = a.precision or "fp32"
-    plain = [HeatflowMPM(**kw, cg=a.cg, precision=prec) for _, kw in scenes]
+    lay = lambda ctx: None if a.layout == "rule" else ("auto-batch" if ctx == "batch" else "auto")  # noqa: E731
+    plain = [HeatflowMPM(**kw, cg=a.cg, precision=prec, layout=lay("solo")) for _, kw in scenes]
     t_plain, p_iters_plain = 0.0, 0
     for s in plain:
         s.synchronize()
@@ -141,7 +142,7 @@ def bench_batched(a) -> dict:
         s.synchronize()
         t_plain += time.perf_counter() - t0
         p_iters_plain += s.stats()["p_iters_total"]
-    sims = [HeatflowMPM(**kw, cg=a.cg, precision=prec) for _, kw in scenes]
+    sims = [HeatflowMPM(**kw, cg=a.cg, precision=prec, layout=lay("batch")) for _, kw in scenes]
     batch = HeatflowBatch(sims)
     batch.synchronize()
     t0 = time.perf_counter()
@@ -164,6 +165,7 @@ def bench_batched(a) -> dict:
                                     f"(counts {[ns.count(m) for m in (32, 48, 64, 96, 128)]}), heater 100-800 W, "
                                     f"fan 0-3 m/s, 2k-50k particles, PCG tol 1e-6, dt 2 ms, {K} substeps from rest",
                            scenes=len(scenes), cells=cells, seed=a.seed, pressure_cg=a.cg, precision=prec,
+                           layout=a.layout,
                            parallelism="1 GPU"),
                plain=dict(value=plain_value, seconds=round(t_plain, 3), pressure_iters_total=int(p_iters_plain)),
                speedup_vs_plain=round(value / plain_value, 3), batched_seconds=round(t_batch, 3),
@@ -198,12 +200,14 @@ def bench_served(a) -> dict:
     from heatflow_mpm.serve import GpuBackend
     budget = int(a.hbm_gib * (1 << 30))
     prec = a.precision or "mixed"
-    served = Server(GpuBackend(cg=a.cg, precision=prec), hbm_budget=budget, max_active=a.slots, keep_results=True)
+    served = Server(GpuBackend(cg=a.cg, precision=prec, layout=None if a.layout == "rule" else "auto-batch"),
+                    hbm_budget=budget, max_active=a.slots, keep_results=True)
     t0 = time.perf_counter()
     res = served.run(reqs)
     span = time.perf_counter() - t0
     s_b = summarize(res, span)
-    plain = Server(GpuBackend(cg=a.cg), hbm_budget=budget, max_active=1, keep_results=True)
+    plain = Server(GpuBackend(cg=a.cg, layout=None if a.layout == "rule" else "auto"), hbm_budget=budget,
+                   max_active=1, keep_results=True)
     t0 = time.perf_counter()
     res_p = plain.run(reqs)
     span_p = time.perf_counter() - t0
@@ -229,7 +233,7 @@ def bench_served(a) -> dict:
                                      f"{a.kmin}-{a.kmax}, heater 100-800 W, fan 0-3 m/s, 2k-50k particles, "
                                      f"PCG tol 1e-6 (pressure cap {a.cg_max_iter}), dt 2 ms", rate_per_s=a.rate,
                             requests=len(reqs), seed=a.seed, cg_max_iter=a.cg_max_iter,
-                            hbm_budget_gib=a.hbm_gib, slots=a.slots, pressure_cg=a.cg, precision=prec,
+                            hbm_budget_gib=a.hbm_gib, slots=a.slots, pressure_cg=a.cg, precision=prec, layout=a.layout,
                             parallelism="1 GPU"),
                 max_rel_l2_vs_plain=dict(T=err_T, u=err_u), worst_requests_vs_plain=worst,
                 served=s_b, plain=s_p, speedup_vs_plain=round(s_b["cell_substeps_per_s"] / s_p["cell_substeps_per_s"], 3),
@@ -324,6 +328,8 @@ def main() -> int:
     ap.add_argument("--precision", choices=tuple(HeatflowMPM.PRECISIONS), default=None,
                     help="coupled/batched: precision of the substep (fp32 default); served: precision of the "
                          "batched server (mixed default; the plain form is fp32)")
+    ap.add_argument("--layout", choices=("rule", "auto"), default="auto",
+                    help="pressure PCG layout: chosen by grid size, or the autotuned table (heatflow_mpm.autotune)")
     ap.add_argument("--fp64-n", type=int, default=0, help="mixed: grid size of the fp64-oracle error check (0: off)")
     ap.add_argument("--cg-max-iter", type=int, default=2000,
                     help="mixed and served: pressure PCG iteration cap (n=192 needs more than the default 400)")
@@ -364,7 +370,7 @@ def main() -> int:
 
     prec = a.precision or "fp32"
     kw, dt, _ = coupled_scene(a.n)
-    sim = HeatflowMPM(**kw, device="cuda:0", cg=a.cg, precision=prec)
+    sim = HeatflowMPM(**kw, device="cuda:0", cg=a.cg, precision=prec, layout=None if a.layout == "rule" else "auto")
     n, N, P = sim.grid.n, sim.grid.n ** 3, sim.n
     if a.warmup:
         sim.run(dt, a.warmup)
@@ -383,7 +389,8 @@ def main() -> int:
     I_p = (st1["p_iters_total"] - st0["p_iters_total"]) / a.steps
     I_h = (st1["h_iters_total"] - st0["h_iters_total"]) / a.steps
     value = N * a.steps / wall
-    hbm_tbs = substep_bytes(N, P, I_p, I_h, a.cg == "fused") * a.steps / wall / 1e12 if prec == "fp32" else None
+    hbm_tbs = (substep_bytes(N, P, I_p, I_h, sim.layout["cg"] == "fused") * a.steps / wall / 1e12
+               if prec == "fp32" else None)
 
     # second timed region with HIP events around every pressure-PCG launch
     sim._scene.set_kernel_timing(True)
@@ -414,7 +421,8 @@ def main() -> int:
                data="synthetic",
                config=dict(workload=f"coupled scene n={n} (h=5 mm), static ellipsoid P={P}, heater 400 W cap 120 C fan "
                                     f"2 m/s, PCG tol 1e-6, dt 2 ms", n=n, particles=P, dt=dt, cg_tol=1e-6,
-                           pressure_cg=a.cg, precision=prec,
+                           pressure_cg=sim.layout["cg"], precision=prec, layout=a.layout,
+                           dir_planes=sim.layout["dir_planes"],
                            pressure_iters_per_substep=round(I_p, 2), heat_iters_per_substep=round(I_h, 2),
                            parallelism="1 GPU"),
                hbm_tb_s=round(hbm_tbs, 3), roofline=roofline, cpu_baseline=cpu)

@@ -882,6 +882,11 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
         set_err("precision mode %d is not available", prm->precision);
         return HFM_ERR_ARG;
     }
+    if (prm->dir_planes != 0 && prm->dir_planes != -1 && prm->dir_planes != 4 && prm->dir_planes != 8 &&
+        prm->dir_planes != 16) {
+        set_err("dir_planes %d: use 0, -1, 4, 8 or 16", prm->dir_planes);
+        return HFM_ERR_ARG;
+    }
     if (prm->precision == HFM_PREC_MIXED && (prm->flags & HFM_FLAG_FUSED_CG)) {
         set_err("HFM_FLAG_FUSED_CG is not available with HFM_PREC_MIXED");
         return HFM_ERR_ARG;
@@ -912,6 +917,13 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
         e = getenv("HFM_DIR_ICHUNK");
         const int ic = e ? atoi(e) : 0;
         s->dir_ichunk = ic >= 1 && ic <= 1024 ? ic : 0;
+        // the layout asked for in the parameters (an autotuned choice) takes precedence
+        if (prm->dir_planes == 4 || prm->dir_planes == 8 || prm->dir_planes == 16) {
+            s->dir_ichunk = prm->dir_planes;
+            s->flat_dir = 0;
+        } else if (prm->dir_planes == -1) {
+            s->flat_dir = 1;
+        }
     }
     SceneDev& d = s->dev;
     d.mean_tail = -1;

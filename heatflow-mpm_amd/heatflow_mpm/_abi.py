@@ -38,7 +38,7 @@ class Params(C.Structure):
                 ("alpha_f_min", C.c_double), ("cg_tol", C.c_double), ("heat_tol", C.c_double),
                 ("kappa_air", C.c_double), ("kappa_mat", C.c_double), ("vol", C.c_double), ("k_particle", C.c_double),
                 ("t_sol", C.c_double), ("t_liq", C.c_double), ("yield_solid", C.c_double), ("mu_solid", C.c_double),
-                ("mu_liquid", C.c_double), ("t_max", C.c_double)]
+                ("mu_liquid", C.c_double), ("t_max", C.c_double), ("dir_planes", C.c_int32)]
 
 
 class Stats(C.Structure):

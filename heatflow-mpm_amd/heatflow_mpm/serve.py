@@ -62,8 +62,8 @@ def footprint_estimate(n: int, particles: int, precision: str = "fp32") -> int:
 class GpuBackend:
     """HeatflowMPM scenes stepped by HeatflowBatch on one device."""
 
-    def __init__(self, cg: str = "warp", device: str = "cuda:0", precision: str = "fp32"):
-        self.cg, self.device, self.precision = cg, device, precision
+    def __init__(self, cg: str = "warp", device: str = "cuda:0", precision: str = "fp32", layout=None):
+        self.cg, self.device, self.precision, self.layout = cg, device, precision, layout
 
     def prepare(self, req: Request):
         from .solver import scene_inputs, KAPPA_AIR
@@ -78,7 +78,8 @@ class GpuBackend:
 
     def create(self, req: Request, prepared):
         from .solver import HeatflowMPM
-        return HeatflowMPM(**req.kwargs, cg=self.cg, device=self.device, inputs=prepared, precision=self.precision)
+        return HeatflowMPM(**req.kwargs, cg=self.cg, device=self.device, inputs=prepared, precision=self.precision,
+                           layout=self.layout)
 
     def n_of(self, sim) -> int:
         return sim.grid.n

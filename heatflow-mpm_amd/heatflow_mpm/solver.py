@@ -54,7 +54,7 @@ def scene_inputs(n_particles: int, mat: Material, heater: Heater, grid: GridConf
                   kappa_air=float(kappa_air), kappa_mat=float(mat.kappa), vol=float(spacing) ** 3,
                   k_particle=K_PARTICLE, t_sol=float(mat.t_sol), t_liq=float(mat.t_liq),
                   yield_solid=float(mat.yield_solid), mu_solid=float(mat.viscosity_solid),
-                  mu_liquid=float(mat.viscosity_liquid), t_max=float(t_max))
+                  mu_liquid=float(mat.viscosity_liquid), t_max=float(t_max), dir_planes=0)
     return dict(params=params, cell_type=cell, fixed=fixed, tfix=tfix, source=source.astype(np.int32),
                 rate=float(rate), t_cap=float(heater.t_cap), solid=solid.astype(np.float32),
                 k_faces=[np.ascontiguousarray(k, np.float32) for k in k_faces],
@@ -199,7 +199,10 @@ class HeatflowMPM:
     replaces the default StaticBody built from `particles`; `cg="fused"` runs the pressure PCG in
     one launch per iteration (HFM_FLAG_FUSED_CG in include/hfm.h) instead of warp's two-pass form;
     `precision="mixed"` solves both systems by iterative refinement with FP16 inner vectors and
-    traces advection through a BF16 copy of the velocity (HFM_PREC_MIXED); the state stays fp32.
+    traces advection through a BF16 copy of the velocity (HFM_PREC_MIXED); the state stays fp32;
+    `layout` picks the pressure PCG layout: None (chosen by grid size), "auto" / "auto-batch"
+    (the autotuned table of heatflow_mpm.autotune for a scene stepped alone / in a batch), or
+    dict(dir_planes=0|-1|4|8|16, cg="warp"|"fused").
     """
 
     CG_FORMS = ("warp", "fused")
@@ -208,9 +211,23 @@ class HeatflowMPM:
     def __init__(self, particles: np.ndarray, mat: Material = Material(), heater: Heater = Heater(),
                  grid: GridConfig = GridConfig(), spacing: float = 2.5e-3, t_init: float = 20.0,
                  kappa_air: float = KAPPA_AIR, device: str = "cuda:0", body=None, cg: str = "warp",
-                 inputs: dict | None = None, precision: str = "fp32"):
+                 inputs: dict | None = None, precision: str = "fp32", layout=None):
         if cg not in self.CG_FORMS:
             raise ValueError(f"cg must be one of {self.CG_FORMS}")
+        dir_planes = 0
+        if layout is not None:
+            if isinstance(layout, str):
+                from .autotune import lookup
+                if layout not in ("auto", "auto-batch"):
+                    raise ValueError("layout must be None, 'auto', 'auto-batch' or a dict")
+                choice = lookup(grid.n, "batch" if layout == "auto-batch" else "solo")
+            else:
+                choice = dict(layout)
+            if choice:
+                dir_planes = int(choice.get("dir_planes", 0))
+                if choice.get("cg") in self.CG_FORMS and precision == "fp32":
+                    cg = choice["cg"]
+        self.layout = dict(dir_planes=dir_planes, cg=cg)
         if precision not in self.PRECISIONS:
             raise ValueError(f"precision must be one of {tuple(self.PRECISIONS)}")
         if precision == "mixed" and cg == "fused":
@@ -225,6 +242,7 @@ class HeatflowMPM:
         if cg == "fused":
             inp["params"]["flags"] |= _abi.FLAG_FUSED_CG
         inp["params"]["precision"] = self.PRECISIONS[precision]
+        inp["params"]["dir_planes"] = dir_planes
         self._scene = _Scene(inp["params"], parse_device(device))
         self.grid = AirGrid(grid, self._scene)
         self.heat = HeatField(self.grid.n, self.grid.h, self._scene)

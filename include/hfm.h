@@ -64,6 +64,8 @@ typedef struct hfm_params {
     double mu_solid;          /* viscosity_solid (mpm.py:15) */
     double mu_liquid;         /* viscosity_liquid (mpm.py:16) */
     double t_max;             /* max(t_rod, t_cap, t_amb, t_init) (solver.py:57) */
+    int32_t dir_planes;       /* pressure pass A layout (no counterpart in the reference): 0 chosen by
+                                 grid size, 4 / 8 / 16 planes per LDS tile, -1 untiled */
 } hfm_params;
 
 #define HFM_PREC_FP32 0       /* the reference's arithmetic: fp32 storage and math */

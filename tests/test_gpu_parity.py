@@ -150,3 +150,19 @@ def test_cg_cap_is_reported_not_raised(gpu):
     st = sim.stats()
     assert st["p_iters_exact"] == 4 and st["p_iters"] == 4 and st["p_converged"] == 0
 
+
+
+@pytest.mark.parametrize("layout", [dict(dir_planes=-1), dict(dir_planes=4), dict(dir_planes=8),
+                                    dict(dir_planes=16), dict(dir_planes=8, cg="fused")],
+                         ids=["untiled", "4 planes", "8 planes", "16 planes", "fused on 8 planes"])
+def test_layouts_match_the_oracle(gpu, layout):
+    """Every layout the autotuner may pick gives the oracle's substep (n = 40: tiles cut by the box)."""
+    kw, dt, _ = coupled_scene(40)
+    sim = HeatflowMPM(**kw, layout=layout)
+    assert sim.layout["dir_planes"] == layout["dir_planes"]
+    orc = O.OracleSim(**kw)
+    sim.substep(dt)
+    orc.substep(dt)
+    assert abs(sim.stats()["p_iters_exact"] - orc.stats()["p_iters_exact"]) <= 1
+    for f in ("T", "U", "V", "W", "P"):
+        assert rel_l2(sim._scene.get_field(f), orc.field(f)) <= 1e-5, f

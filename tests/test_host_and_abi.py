@@ -202,3 +202,29 @@ def test_mixed_precision_argument_checks():
                     precision=7)
     assert lib.hfm_scene_create(C.byref(p), 0, C.byref(h)) == -1
     assert b"precision" in lib.hfm_last_error()
+
+
+def test_autotune_table_lookup_and_candidates():
+    """The autotuner's table lookup takes the nearest measured size (in log n); candidates respect
+    the pass-A workgroup cap (16-plane tiles up to 2048 workgroups) and the fused form's tiles."""
+    from heatflow_mpm.autotune import candidates, lookup
+    t = {"solo": {"64": {"choice": {"dir_planes": 8, "cg": "fused"}},
+                  "128": {"choice": {"dir_planes": 16, "cg": "warp"}}}}
+    assert lookup(96, "solo", t) == {"dir_planes": 16, "cg": "warp"}
+    assert lookup(70, "solo", t) == {"dir_planes": 8, "cg": "fused"}
+    assert lookup(96, "batch", t) is None
+    c = candidates(192)
+    assert {"dir_planes": 16, "cg": "warp"} in c and {"dir_planes": -1, "cg": "warp"} in c
+    assert all(x["dir_planes"] != 8 for x in c)          # 12 x 12 x 24 tiles exceed the cap
+    assert all(x["cg"] == "warp" or x["dir_planes"] in (4, 8) for x in candidates(64))
+
+
+def test_layout_argument_checks():
+    lib = _abi.load()
+    p = _abi.Params(n=8, cg_max_iter=10, cg_check_every=2, heat_max_iter=10, heat_check_every=2, h=0.01,
+                    dir_planes=5)
+    h = C.c_void_p()
+    assert lib.hfm_scene_create(C.byref(p), 0, C.byref(h)) == -1
+    assert b"dir_planes" in lib.hfm_last_error()
+    with pytest.raises(ValueError, match="layout"):
+        HeatflowMPM(np.zeros((0, 3), np.float32), grid=GridConfig(L=0.08, h=0.01), layout="fastest")
