@@ -95,6 +95,7 @@ struct VecOf<true> {
 };
 
 constexpr float kIrEta2 = 1.0e-4f;  // inner solve: reduce ||r||^2 by eta^2, eta = 1e-2
+constexpr int kIrInnerMax = 256;    // iterations of one inner solve at most (then a refinement step)
 constexpr int kIrPart = kMaxDirBlocks + 3 * kMaxPcgBlocks;  // residual partials of a refinement step
 
 template <int OP, bool IR = false>
@@ -351,8 +352,9 @@ __device__ bool finish_iteration(const View& a, const PassAIn& p, float& beta) {
     }
     const bool conv = rr <= atol_sq;
     const bool stop = conv || k >= p.maxeff;
-    // beta of iteration k (warp: rz_new / rz_old while rr > atol^2)
-    beta = (k == 0) ? 0.0f : ((rr > atol_sq) ? rz / p.rz_prev : 0.0f);
+    // beta of iteration k (warp: rz_new / rz_old while rr > atol^2); the inner FP16 solve of a
+    // refinement also takes beta = 0 on a breakdown (r.z of the previous iteration not positive)
+    beta = (k == 0) ? 0.0f : ((rr > atol_sq && (!IR || p.rz_prev > 0.0f)) ? rz / p.rz_prev : 0.0f);
     if (a.lb == 0 && threadIdx.x == 0) {
         c->slot_rr[k & 1] = rr;
         c->slot_rz[k & 1] = rz;
@@ -593,7 +595,9 @@ __global__ __launch_bounds__(kBlock) void k_pcg_upd(Src src, int kit) {
     double tot[1];
     if (reduce_bcast<1, kBlock>(acc, 1, done, tot)) return;
     const float pAp = (float)tot[0];
-    const float alpha = (rr > atol_sq) ? rz / pAp : 0.0f;
+    // the inner FP16 solve of a refinement takes alpha = 0 on a breakdown (p.Ap not positive, or not
+    // a number): x and r stay, the solve stops at its cap and the refinement step takes over
+    const float alpha = (rr > atol_sq && (!IR || pAp > 0.0f)) ? rz / pAp : 0.0f;
     const float xa = IR ? step * alpha : alpha;
     double s[2] = {0.0, 0.0};
     auto update = [&](unsigned t, float x0, float d0, float r0, float a0, float D0) {
@@ -744,7 +748,7 @@ __global__ __launch_bounds__(kBlock) void k_ir_scale(Src src, int first) {
         c->done = 0;
         c->converged = 0;
         c->it = 0;
-        c->maxeff = maxiter - total;
+        c->maxeff = maxiter - total < kIrInnerMax ? maxiter - total : kIrInnerMax;
     }
 }
 
