@@ -8,7 +8,7 @@ from .solver import HeatflowMPM, KAPPA_AIR, K_PARTICLE, scene_inputs  # noqa: F4
 from .grid import GridConfig, AirGrid  # noqa: F401
 from .heat import HeatField  # noqa: F401
 from .heater import Heater  # noqa: F401
-from .mpm import Material, StaticBody  # noqa: F401
+from .mpm import KinematicBody, Material, StaticBody  # noqa: F401
 from .scenes import ellipsoid_particles, plume_scene, coupled_scene, batched_scenes  # noqa: F401
 from .batch import HeatflowBatch  # noqa: F401
 from .serve import Request, Server, poisson_requests  # noqa: F401

@@ -11,7 +11,6 @@ from __future__ import annotations
 import ctypes as C
 
 from . import _abi
-from .mpm import StaticBody
 
 
 class HeatflowBatch:
@@ -39,7 +38,7 @@ class HeatflowBatch:
         """One substep of every simulation (solver.py:63-88 for each), moving bodies included."""
         for s in self.sims:
             b = s.body
-            if b is not None and not isinstance(b, StaticBody):
+            if b is not None and not getattr(b, "static", False):
                 s._scene.update_particles(b.pos, b.vel)
         _abi.check(self.lib.hfm_batch_substep(self.handle, float(dt)), "hfm_batch_substep")
         for s in self.sims:
@@ -48,7 +47,7 @@ class HeatflowBatch:
 
     def run(self, dt: float, substeps: int) -> float:
         """`substeps` substeps of static scenes back to back; returns the GPU time in ms."""
-        if any(s.body is not None and not isinstance(s.body, StaticBody) for s in self.sims):
+        if any(s.body is not None and not getattr(s.body, "static", False) for s in self.sims):
             raise ValueError("run() needs static bodies; call substep() for moving ones")
         ms = C.c_double(0.0)
         _abi.check(self.lib.hfm_batch_run(self.handle, float(dt), int(substeps), C.byref(ms)), "hfm_batch_run")

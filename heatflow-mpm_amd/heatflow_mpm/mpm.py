@@ -38,6 +38,8 @@ class StaticBody:
     positions(), velocities(), step(dt).
     """
 
+    static = True   # the substep uploads particle positions only for bodies without this flag
+
     def __init__(self, particles: np.ndarray, velocities: np.ndarray | None = None):
         self.pos = np.ascontiguousarray(particles, dtype=np.float32).reshape(-1, 3)
         self.vel = (np.zeros_like(self.pos) if velocities is None
@@ -49,6 +51,34 @@ class StaticBody:
 
     def step(self, dt: float) -> None:
         """A static body does not move."""
+
+    def positions(self) -> np.ndarray:
+        return self.pos.copy()
+
+    def velocities(self) -> np.ndarray:
+        return self.vel.copy()
+
+
+class KinematicBody:
+    """Particles moved by a prescribed velocity field: each step(dt) advances positions by
+    dt * velocity(pos, t) and stores that velocity.  A stand-in for ThermalMPM (mpm.py:25-80)
+    where the material moves but its motion is given rather than solved, so the substep's
+    moving-material paths (particle flux q and relaxation target m from nonzero velocities,
+    re-binning after every move) run on known motion.  `velocity(pos, t)` returns a (P, 3)
+    float32 array; the initial velocities are velocity(pos, 0)."""
+
+    static = False
+
+    def __init__(self, particles: np.ndarray, velocity):
+        self.pos = np.ascontiguousarray(particles, dtype=np.float32).reshape(-1, 3)
+        self.vel = np.ascontiguousarray(velocity(self.pos, 0.0), dtype=np.float32).reshape(-1, 3)
+        self.n = len(self.pos)
+        self.velocity, self.t = velocity, 0.0
+
+    def step(self, dt: float) -> None:
+        self.pos = np.ascontiguousarray(self.pos + np.float32(dt) * self.vel, dtype=np.float32)
+        self.t += dt
+        self.vel = np.ascontiguousarray(self.velocity(self.pos, self.t), dtype=np.float32).reshape(-1, 3)
 
     def positions(self) -> np.ndarray:
         return self.pos.copy()

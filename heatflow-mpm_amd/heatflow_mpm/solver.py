@@ -257,7 +257,7 @@ class HeatflowMPM:
     def substep(self, dt: float) -> None:
         """One air/heat substep (solver.py:63-86), then the body's own step (solver.py:87-88)."""
         b = self.body
-        if b is not None and not isinstance(b, StaticBody):
+        if b is not None and not getattr(b, "static", False):
             self._scene.update_particles(b.pos, b.vel)
         self._scene.substep(dt)
         if b is not None:
@@ -265,7 +265,7 @@ class HeatflowMPM:
 
     def run(self, dt: float, substeps: int) -> float:
         """`substeps` substeps of a static scene back to back; returns the GPU time in ms."""
-        if self.body is not None and not isinstance(self.body, StaticBody):
+        if self.body is not None and not getattr(self.body, "static", False):
             raise ValueError("run() needs a static body; call substep() for a moving one")
         return self._scene.run(dt, substeps)
 

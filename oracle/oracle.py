@@ -158,6 +158,14 @@ class OracleSim:
         modes = {"fp32": 0, "bf16": 1, "fp16": 2}
         self.L.orc_set_storage_rounding(self.s, modes[u], modes[T], int(bool(backtrace_bf16)))
 
+    def update_particles(self, pos, vel) -> None:
+        """Replace positions and velocities (same count; temperatures kept), as a moving body does."""
+        _, _, temp = self._keep
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        vel = np.ascontiguousarray(vel, np.float32).reshape(-1, 3)
+        self.L.orc_update_particles(self.s, pos.ctypes.data_as(C.c_void_p), vel.ctypes.data_as(C.c_void_p))
+        self._keep = (pos, vel, temp)
+
     def substep(self, dt: float) -> None:
         self.L.orc_substep(self.s, float(dt))
 
