@@ -111,6 +111,13 @@ struct SceneView {
     ptr<int> bin_start, sorted;
     ptr<float> qpos;         // pos / h per particle, in bin order (3 per particle)
     int mixed;               // HFM_PREC_MIXED
+    int cg_mg;               // pressure PCG preconditioned by a multigrid V-cycle (HFM_FLAG_MG_PCG)
+    int mg_levels;           // grid levels, level 0 the scene's grid, each next one halved
+    int mg_tail;             // first level of at most kMgTailMaxN cells per side (solved in LDS)
+    int mg_n[8];             // cells per side of each level
+    long mg_off[8];          // first cell of level l >= 1 in the packed coarse-level arrays
+    ptr<float> mg_ax, mg_ay, mg_az, mg_w;   // coarse levels: -x, -y, -z face coefficients, omega / sum
+    ptr<float> mg_s, mg_e;   // coarse levels: right-hand side and correction of the V-cycle
     ptr<_Float16> ir_r, ir_z, ir_d, ir_q;  // inner solve: r, z, p (two parity buffers), A p
     ptr<uint16_t> ub[3];     // BF16 copies of u, v, w (departure points of the advection)
 };
@@ -210,6 +217,15 @@ enum Family {
     FAM_FUSED4,     // fused pressure iteration, 16x16x4 blocks (scenes with cg_fused)
     FAM_FUSED8,
     FAM_FUSED16,
+    FAM_MG0,        // multigrid level l = 0..7 cells, one workgroup per 256 (scenes with cg_mg)
+    FAM_MG1,
+    FAM_MG2,
+    FAM_MG3,
+    FAM_MG4,
+    FAM_MG5,
+    FAM_MG6,
+    FAM_MG7,
+    FAM_MGC,        // the multigrid tail (levels of at most 12^3 cells), one workgroup per scene
     FAM_COUNT
 };
 
@@ -351,6 +367,12 @@ void launch_pcg_upd(const Launch& L, int op, int k, hipEvent_t ev0 = nullptr, hi
 void launch_ir_residual(const Launch& L, int op, float dt, bool first);
 void launch_ir_scale(const Launch& L, int op, bool first);
 void launch_bf16_copy(const Scene* s, hipStream_t st);
+
+// ---- multigrid preconditioner of the pressure PCG (hfm_mg.hip) ----
+constexpr int kMgMaxLevels = 8;
+constexpr int kMgTailMaxN = 12;   // levels of at most 12^3 cells are solved in one workgroup
+void launch_mg_coefficients(const Batch* b, int levels);
+void launch_mg_vcycle(const Batch* b, int tail);
 // fused pressure form: first-iteration scalars after launch_pcg_init, then one launch per iteration
 constexpr int kFusedRows = 5;
 void launch_pcg_init_fused(const Launch& L);

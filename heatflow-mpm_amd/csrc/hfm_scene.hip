@@ -281,7 +281,15 @@ long family_blocks(const Scene* s, int f, int sub) {
         case FAM_FUSED4: return sub == 0 && d.dtile == 4 && d.cg_fused ? d.nb_dir : 0;
         case FAM_FUSED8: return sub == 0 && d.dtile == 8 && d.cg_fused ? d.nb_dir : 0;
         case FAM_FUSED16: return sub == 0 && d.dtile == 16 && d.cg_fused ? d.nb_dir : 0;
-        default: return 0;
+        case FAM_MGC: return sub == 0 && d.cg_mg ? 1 : 0;
+        default:
+            if (f >= FAM_MG0 && f <= FAM_MG7) {
+                const int l = f - FAM_MG0;
+                if (sub != 0 || !d.cg_mg || l >= d.mg_levels) return 0;
+                const long m = d.mg_n[l];
+                return nblocks(m * m * m);
+            }
+            return 0;
     }
 }
 
@@ -470,11 +478,21 @@ int filter_families(Batch* b, const Family* fams, int nf, const std::vector<char
     return HFM_OK;
 }
 
+// Most multigrid levels (tail: the deepest tail level) among the batch's scenes with the
+// multigrid-preconditioned pressure PCG (0: none).
+int mg_depth(const Batch* b, bool tail) {
+    int L = 0;
+    for (const Scene* s : b->scenes)
+        if (s->dev.cg_mg) L = std::max(L, tail ? s->dev.mg_tail : s->dev.mg_levels);
+    return L;
+}
+
 // Run PCG iterations from iteration 0 for every scene of the batch whose mirror is not done
 // (the init launches are enqueued), and return when the device reports all of them done.
 // ir: the inner solves of a mixed-precision refinement step (FP16 kernels).
 int pcg_iterate(Batch* b, int op, float dt, bool ir, int maxeff, int pred) {
     const unsigned tag = ++b->solve_tag;
+    const int mg = op == 1 && !ir ? mg_depth(b, true) : 0;
     // pass A launches: the heat operator uses the flat kernel with the PCG geometry; the
     // pressure operator one launch per tiling present in the batch, two-pass or fused
     Family dirs[8];
@@ -586,6 +604,7 @@ int pcg_iterate(Batch* b, int op, float dt, bool ir, int maxeff, int pred) {
                     hipEvent_t e0 = nullptr, e1 = nullptr;
                     if (timed) RC(timing_events(b, HFM_KT_PCG_UPD, &e0, &e1));
                     launch_pcg_upd(fl[FAM_PCG], op, k, e0, e1, ir);
+                    if (mg) launch_mg_vcycle(b, mg);
                 }
             }
             CHECK_LAUNCH();
@@ -634,6 +653,10 @@ int run_pcg(Batch* b, int op, float dt) {
     }
     launch_pcg_init(family(b, FAM_PCG), op, dt);
     if (any_fused) launch_pcg_init_fused(family(b, FAM_PCG));
+    if (op == 1 && mg_depth(b, false)) {   // multigrid: this solve's coarse levels, then z = M r
+        launch_mg_coefficients(b, mg_depth(b, false));
+        launch_mg_vcycle(b, mg_depth(b, true));
+    }
     CHECK_LAUNCH();
     RC(pcg_iterate(b, op, dt, false, maxeff, pred));
     for (Scene* s : b->scenes) {
@@ -887,6 +910,20 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
         set_err("dir_planes %d: use 0, -1, 4, 8 or 16", prm->dir_planes);
         return HFM_ERR_ARG;
     }
+    if ((prm->flags & HFM_FLAG_MG_PCG) &&
+        (prm->precision != HFM_PREC_FP32 || (prm->flags & HFM_FLAG_FUSED_CG))) {
+        set_err("HFM_FLAG_MG_PCG is available with HFM_PREC_FP32 and the two-pass PCG only");
+        return HFM_ERR_ARG;
+    }
+    if (prm->flags & HFM_FLAG_MG_PCG) {
+        int m = prm->n, levels = 1;
+        for (; m > 8 && m % 2 == 0; ++levels) m /= 2;
+        if (m > 8 || prm->n < 16 || levels > kMgMaxLevels) {
+            set_err("HFM_FLAG_MG_PCG needs n = m 2^k with m <= 8, k <= %d and n >= 16 (n = %d)", kMgMaxLevels - 1,
+                    prm->n);
+            return HFM_ERR_ARG;
+        }
+    }
     if (prm->precision == HFM_PREC_MIXED && (prm->flags & HFM_FLAG_FUSED_CG)) {
         set_err("HFM_FLAG_FUSED_CG is not available with HFM_PREC_MIXED");
         return HFM_ERR_ARG;
@@ -969,6 +1006,28 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
     A(d.cg_d, 2 * N);
     A(d.cg_Ap, N);
     A(d.partials, std::max(kMaxDirBlocks + 5 * kMaxPcgBlocks, 2 * kFusedRows * kMaxDirBlocks));
+    if (prm->flags & HFM_FLAG_MG_PCG) {   // multigrid levels: halve n down to at most 8 cells per side
+        d.cg_mg = 1;
+        int L = 1;
+        d.mg_n[0] = prm->n;
+        d.mg_off[0] = 0;
+        long coarse = 0;
+        while (d.mg_n[L - 1] > 8 && d.mg_n[L - 1] % 2 == 0 && L < kMgMaxLevels) {
+            d.mg_n[L] = d.mg_n[L - 1] / 2;
+            d.mg_off[L] = coarse;
+            coarse += (long)d.mg_n[L] * d.mg_n[L] * d.mg_n[L];
+            ++L;
+        }
+        d.mg_levels = L;
+        d.mg_tail = 1;
+        while (d.mg_n[d.mg_tail] > kMgTailMaxN) ++d.mg_tail;   // the last level has at most 8
+        A(d.mg_ax, coarse);
+        A(d.mg_ay, coarse);
+        A(d.mg_az, coarse);
+        A(d.mg_w, coarse);
+        A(d.mg_s, coarse);
+        A(d.mg_e, coarse);
+    }
     if (prm->precision == HFM_PREC_MIXED) {
         d.mixed = 1;
         A(d.ir_r, N);
