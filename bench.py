@@ -191,12 +191,15 @@ def rel_l2(a, b) -> float:
 def bench_served(a) -> dict:
     """A seeded Poisson stream of rollout requests (n from {48, 64, 96, 128, 192}, K uniform in
     [kmin, kmax], heaters / fans / particle blobs as the batched scenes), served twice on the same
-    arrival times: with continuous batching in the chosen precision (all admitted scenes advance in
-    one batched substep; admission in arrival order by device footprint) and in the plain form
-    (fp32, one request at a time, in arrival order).  Latency = completion - arrival.  Each served
-    result is compared with the plain result of the same request (relative L2 of T and u)."""
+    arrival times: with continuous batching (all admitted scenes advance in one batched substep;
+    admission in arrival order by device footprint), the multigrid-preconditioned pressure PCG and
+    fp32 state by default, and in the plain form (fp32, the reference's Jacobi PCG, one request at
+    a time, in arrival order).  Latency = completion - arrival.  Each served result is compared with
+    the plain result of the same request (relative L2 of T and u): both solve to the same relative
+    residual, so they agree to the solve tolerance, not in bits."""
     reqs = poisson_requests(a.rate, a.requests, seed=a.seed, k_range=(a.kmin, a.kmax), cg_max_iter=a.cg_max_iter)
-    warm = [HeatflowMPM(**coupled_scene(32)[0], cg=a.cg) for _ in range(2)]
+    scg = a.cg or "mg"
+    warm = [HeatflowMPM(**coupled_scene(32)[0], cg=scg) for _ in range(2)]
     warm[0].run(2e-3, 2)
     with HeatflowBatch(warm) as wb:
         wb.run(2e-3, 2)
@@ -204,14 +207,14 @@ def bench_served(a) -> dict:
         w.close()
     from heatflow_mpm.serve import GpuBackend
     budget = int(a.hbm_gib * (1 << 30))
-    prec = a.precision or "mixed"
-    served = Server(GpuBackend(cg=a.cg, precision=prec, layout=None if a.layout == "rule" else "auto-batch"),
+    prec = a.precision or ("mixed" if scg == "warp" else "fp32")
+    served = Server(GpuBackend(cg=scg, precision=prec, layout=None if a.layout == "rule" else "auto-batch"),
                     hbm_budget=budget, max_active=a.slots, keep_results=True)
     t0 = time.perf_counter()
     res = served.run(reqs)
     span = time.perf_counter() - t0
     s_b = summarize(res, span)
-    plain = Server(GpuBackend(cg=a.cg, layout=None if a.layout == "rule" else "auto"), hbm_budget=budget,
+    plain = Server(GpuBackend(cg=a.plain_cg, layout=None if a.layout == "rule" else "auto"), hbm_budget=budget,
                    max_active=1, keep_results=True)
     t0 = time.perf_counter()
     res_p = plain.run(reqs)
@@ -238,7 +241,8 @@ def bench_served(a) -> dict:
                                      f"{a.kmin}-{a.kmax}, heater 100-800 W, fan 0-3 m/s, 2k-50k particles, "
                                      f"PCG tol 1e-6 (pressure cap {a.cg_max_iter}), dt 2 ms", rate_per_s=a.rate,
                             requests=len(reqs), seed=a.seed, cg_max_iter=a.cg_max_iter,
-                            hbm_budget_gib=a.hbm_gib, slots=a.slots, pressure_cg=a.cg, precision=prec, layout=a.layout,
+                            hbm_budget_gib=a.hbm_gib, slots=a.slots, pressure_cg=scg, plain_cg=a.plain_cg, precision=prec,
+                            layout=a.layout,
                             parallelism="1 GPU"),
                 max_rel_l2_vs_plain=dict(T=err_T, u=err_u), worst_requests_vs_plain=worst,
                 served=s_b, plain=s_p, speedup_vs_plain=round(s_b["cell_substeps_per_s"] / s_p["cell_substeps_per_s"], 3),
@@ -323,6 +327,33 @@ def bench_mixed(a) -> dict:
                 rel_l2_mixed_vs_fp32=vs_fp32, error_vs_fp64=vs_fp64)
 
 
+def bench_mg_form(a, kw, dt, end) -> dict:
+    """The same scene and substeps with the multigrid-preconditioned pressure PCG (fp32): warmup,
+    the timed substeps, then as many untimed ones again, so that its T and u stand beside the plain
+    run's after the same substeps (the plain run takes a second, kernel-timed pass)."""
+    sim = HeatflowMPM(**kw, device="cuda:0", cg="mg", layout=None if a.layout == "rule" else "auto")
+    N = sim.grid.n ** 3
+    if a.warmup:
+        sim.run(dt, a.warmup)
+    st0 = sim.stats()
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.run(dt, a.steps)
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    st1 = sim.stats()
+    sim.run(dt, a.steps)
+    sim.synchronize()
+    u = np.concatenate([sim._scene.get_field(f).ravel() for f in ("U", "V", "W")])
+    u_plain = np.concatenate([end[f].ravel() for f in ("U", "V", "W")])
+    out = dict(value=N * a.steps / wall, ms_per_step=wall * 1e3 / a.steps,
+               pressure_iters_per_substep=round((st1["p_iters_total"] - st0["p_iters_total"]) / a.steps, 2),
+               rel_l2_vs_plain=dict(T=rel_l2(sim._scene.get_field("T"), end["T"]), u=rel_l2(u, u_plain)),
+               substeps_compared=a.warmup + 2 * a.steps)
+    sim.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -350,13 +381,20 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=None, help="grid size (coupled: 96, mixed: 192)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cg", choices=HeatflowMPM.CG_FORMS, default="warp",
-                    help="pressure PCG form: warp's two-pass iteration, or one fused launch per iteration")
+    ap.add_argument("--no-mg", action="store_true",
+                    help="coupled: skip the multigrid-preconditioned form timed beside the plain one")
+    ap.add_argument("--cg", choices=HeatflowMPM.CG_FORMS, default=None,
+                    help="pressure PCG form: warp's two-pass Jacobi iteration (default), one fused launch per "
+                         "iteration, or the multigrid-preconditioned iteration (mg; the served form's default)")
+    ap.add_argument("--plain-cg", choices=HeatflowMPM.CG_FORMS, default="warp",
+                    help="served: pressure PCG form of the plain baseline (the reference's Jacobi PCG)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the substep runs on one device (scene-level sharding across GPUs is "
               "independent scenes, not one scene split); run with --gpus 1", file=sys.stderr)
         return 2
+    if a.workload != "served" and a.cg is None:
+        a.cg = "warp"
     a.steps_given = a.steps is not None
     a.n_given = a.n is not None
     if a.n is None:
@@ -419,6 +457,8 @@ def main() -> int:
                     iterations=iters, bytes_per_launch=bytes_per_launch)
 
     cpu = None if a.no_cpu else cpu_baseline(kw, dt, state)
+    end = {name: sim._scene.get_field(name) for name in ("T", "U", "V", "W")}
+    mg = None if a.no_mg or a.cg != "warp" or prec != "fp32" else bench_mg_form(a, kw, dt, end)
 
     out = dict(metric=METRIC, value=value, unit="cell-substeps/s", n_gpus=1, steps=a.steps, warmup=a.warmup,
                ms_per_step=wall * 1e3 / a.steps, higher_is_better=True, scaling="weak", vs_baseline=None,
@@ -430,7 +470,7 @@ def main() -> int:
                            dir_planes=sim.layout["dir_planes"],
                            pressure_iters_per_substep=round(I_p, 2), heat_iters_per_substep=round(I_h, 2),
                            parallelism="1 GPU"),
-               hbm_tb_s=round(hbm_tbs, 3), roofline=roofline, cpu_baseline=cpu)
+               hbm_tb_s=round(hbm_tbs, 3), roofline=roofline, cpu_baseline=cpu, mg_form=mg)
     print(json.dumps(out), flush=True)
     sim.close()
     return 0

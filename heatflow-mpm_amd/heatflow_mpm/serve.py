@@ -48,14 +48,17 @@ class Result:
         return self.finished - self.arrival
 
 
-def footprint_estimate(n: int, particles: int, precision: str = "fp32") -> int:
+def footprint_estimate(n: int, particles: int, precision: str = "fp32", cg: str = "warp") -> int:
     """Device bytes of a scene before it is created: 21 face-sized and 27 cell-sized fp32
     arrays (fields, static parts, solver vectors, bins), 1 byte of flags per cell, 12 fp32
     words and 4 ints per particle, and fixed tables (partial sums, programs); the mixed form
-    adds 5 FP16 cell arrays (inner solve) and 3 BF16 face arrays (velocity copy)."""
+    adds 5 FP16 cell arrays (inner solve) and 3 BF16 face arrays (velocity copy), the multigrid
+    preconditioner 6 fp32 arrays over its coarse levels (fewer than N / 7 cells)."""
     N = n ** 3
     F = (n + 1) * n * n
     extra = 2 * (5 * N + 3 * F) if precision == "mixed" else 0
+    if cg == "mg":
+        extra += 24 * (N // 7 + 1)
     return 4 * (21 * F + 27 * N) + N + 64 * particles + 256 * 1024 + extra
 
 
@@ -138,11 +141,12 @@ class Server:
     def run(self, requests) -> list:
         reqs = sorted(requests, key=lambda r: (r.arrival, r.rid))
         prec = getattr(self.backend, "precision", "fp32")
+        cg = getattr(self.backend, "cg", "warp")
         dts = {r.dt for r in reqs}
         if len(dts) > 1:
             raise ValueError("all requests of one run share dt (they advance in the same batched substeps)")
         for r in reqs:
-            if footprint_estimate(r.kwargs["grid"].n, len(r.kwargs["particles"]), prec) > self.budget:
+            if footprint_estimate(r.kwargs["grid"].n, len(r.kwargs["particles"]), prec, cg) > self.budget:
                 raise ValueError(f"request {r.rid} does not fit the device budget on its own")
         dt = dts.pop() if dts else 2e-3
         pending = collections.deque(reqs)
@@ -162,7 +166,7 @@ class Server:
                 # admission: arrival order, while the head of the queue fits
                 while waiting and len(active) < self.max_active:
                     r = waiting[0]
-                    est = footprint_estimate(r.kwargs["grid"].n, len(r.kwargs["particles"]), prec)
+                    est = footprint_estimate(r.kwargs["grid"].n, len(r.kwargs["particles"]), prec, cg)
                     if used + est > self.budget:
                         break
                     waiting.popleft()

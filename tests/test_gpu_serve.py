@@ -1,29 +1,30 @@
 """Serving with continuous batching on the GPU: requests of different sizes arrive while others
 run, join and leave the batch between substeps, and each result is bit-identical to the same
-request run alone."""
+request run alone (with the Jacobi or the multigrid-preconditioned pressure PCG)."""
 import numpy as np
 import pytest
 
 from heatflow_mpm import HeatflowMPM
 from heatflow_mpm.scenes import random_scene
-from heatflow_mpm.serve import Request, Server, summarize
+from heatflow_mpm.serve import GpuBackend, Request, Server, summarize
 
 pytestmark = pytest.mark.gpu
 
 
-def test_served_results_equal_the_request_run_alone(gpu):
+@pytest.mark.parametrize("cg", ["warp", "mg"])
+def test_served_results_equal_the_request_run_alone(gpu, cg):
     rng = np.random.default_rng(5)
     reqs = []
     for i in range(6):
         _, kw = random_scene(rng, [24, 32, 40][i % 3], name=f"q{i}")
         reqs.append(Request(i, kw, int(rng.integers(3, 9)), 0.02 * i))
-    srv = Server(keep_results=True, max_active=8)
+    srv = Server(GpuBackend(cg=cg), keep_results=True, max_active=8)
     res = srv.run(reqs)
     joined = [e for e in srv.log if e[1] == "admit" and e[3] > 1]
     print("admissions into a running batch:", len(joined), summarize(res), flush=True)
     assert len(res) == len(reqs)
     for r, q in zip(res, reqs):
-        alone = HeatflowMPM(**q.kwargs)
+        alone = HeatflowMPM(**q.kwargs, cg=cg)
         alone.run(q.dt, q.substeps)
         assert np.array_equal(alone.grid_T(), r.fields["T"]), r.rid
         assert np.array_equal(alone.temperatures(), r.fields["temperatures"]), r.rid

@@ -128,3 +128,13 @@ def test_a_request_larger_than_the_budget_is_refused():
     reqs = _reqs(3)
     with pytest.raises(ValueError, match="does not fit"):
         _serve(reqs, hbm_budget=1000)
+
+
+def test_multigrid_footprint_covers_the_coarse_levels():
+    """Six fp32 arrays over the coarse levels (n/2, n/4, ... down to at most 8 per side)."""
+    for n in (16, 24, 48, 96, 128, 192):
+        coarse, m = 0, n
+        while m > 8 and m % 2 == 0:
+            m //= 2
+            coarse += m ** 3
+        assert footprint_estimate(n, 1000, cg="mg") - footprint_estimate(n, 1000) >= 24 * coarse
