@@ -10,7 +10,7 @@
 // pressure operator's own), a coarse face the average of the 4 fine faces it covers.
 //   down, level l -> l+1:  e = omega s / d (one weighted-Jacobi sweep from zero, formed where it is
 //                          read), t = s - A^ e, s_{l+1} = 0.5 * (sum of t over the 8 children)
-//   coarsest level:        30 weighted-Jacobi sweeps from zero
+//   coarsest level:        16 weighted-Jacobi sweeps from zero
 //   up, level l+1 -> l:    e = omega s / d + e_{l+1}(parent) (injection), then one sweep
 //                          e += omega (s - A^ e) / d; at level 0 that is z, with r.z partials
 // Pre- and post-smoothing are the same symmetric sweep and injection is 2 x the transpose of the
@@ -36,7 +36,7 @@ namespace {
 
 constexpr float kMgOmega = 0.8f;
 constexpr float kMgRestrict = 0.5f;
-constexpr int kMgCoarseSweeps = 30;
+constexpr int kMgCoarseSweeps = 16;
 constexpr int kTailBlock = 1024;
 constexpr int kTailFine = kMgTailMaxN * kMgTailMaxN * kMgTailMaxN;   // cells of the tail level
 constexpr int kTailCoarse = (kMgTailMaxN / 2) * (kMgTailMaxN / 2) * (kMgTailMaxN / 2);
@@ -195,11 +195,15 @@ __global__ __launch_bounds__(kBlock) void k_mg_coef(Src src, int l) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Down-sweep l -> l + 1 (l below the tail level): one thread per fine cell (kCells of them,
-// kBlock apart), 8 consecutive threads per coarse cell;
+// Down-sweep l -> l + 1 (l below the tail level): one thread per fine cell (kCells passes of the
+// workgroup), 8 threads per coarse cell;
 // s_{l+1} = 0.5 * sum over the 8 children of (s - A^ (w s)).
 // ---------------------------------------------------------------------------------------------
-constexpr int kCells = 2;   // cells per thread of the level passes: two sets of loads in flight
+// Cells per thread of the level passes and of the final up-sweep.  One is fastest on MI355X: at
+// 96^3 and 192^3, 2 or 4 cells per thread (their loads issued together) lowered the occupancy
+// more than they overlapped latency (3 % and 10 % slower substeps).
+constexpr int kCells = 1;
+constexpr int kUp0Cells = 1;
 
 // s - A^ (w s) at fine cell (i, j, k) of level l (level 0: s = h^2 r, w s = omega D r)
 template <class A>
@@ -440,8 +444,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_up(Src src, int l) {
 }
 
 // 1 -> 0: z = e + omega D (r - A e) with e = omega D r + e_1(parent); z and the partial sums of
-// r.z (row 1 of the PCG partials, in the PCG geometry, as pass B writes them).  Cells are taken
-// kCells at a time with their loads issued together.
+// r.z (row 1 of the PCG partials, in the PCG geometry, as pass B writes them).
 template <class Src>
 __global__ __launch_bounds__(kBlock) void k_mg_up0(Src src) {
     HFM_ENTRY(src);
@@ -455,10 +458,10 @@ __global__ __launch_bounds__(kBlock) void k_mg_up0(Src src) {
     const float* P = a.mg_e + a.mg_off[1];
     double acc[1] = {0.0};
     const int L_ = xcd_tile(lb, a.nb_upd);
-    for (int u0 = 0; u0 < a.cpt; u0 += kCells) {
-        float z[kCells], rz[kCells];
+    for (int u0 = 0; u0 < a.cpt; u0 += kUp0Cells) {
+        float z[kUp0Cells], rz[kUp0Cells];
 #pragma unroll
-        for (int u = 0; u < kCells; ++u) {
+        for (int u = 0; u < kUp0Cells; ++u) {
             const unsigned t0 = ((unsigned)L_ * a.cpt + u0 + u) * kBlock + threadIdx.x;
             const unsigned t = t0 < N ? t0 : N - 1;
             int i, j, k;
@@ -475,7 +478,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_up0(Src src) {
             rz[u] = rc * z[u];
         }
 #pragma unroll
-        for (int u = 0; u < kCells; ++u) {
+        for (int u = 0; u < kUp0Cells; ++u) {
             const unsigned t = ((unsigned)L_ * a.cpt + u0 + u) * kBlock + threadIdx.x;
             if (u0 + u < a.cpt && t < N) {
                 a.cg_z[t] = z[u];
