@@ -124,15 +124,19 @@ __global__ __launch_bounds__(kBlock) void k_bin_sort(long N, const int* start, i
     }
 }
 
-// pos / h in bin order; marks the particle geometry as changed for the next gather
+// pos / h in bin order; marks the particle geometry as changed for the next gather and the
+// cell lattice's contribution list as stale
 __global__ __launch_bounds__(kBlock) void k_bin_pack(long P, float h, const float* pos, const int* sorted, float* qpos,
-                                                     int* geom_dirty) {
+                                                     CgCtl* ctl0) {
     long s = (long)blockIdx.x * kBlock + threadIdx.x;
     if (s >= P) return;
     const long p = sorted[s];
 #pragma unroll
     for (int d = 0; d < 3; ++d) qpos[3 * s + d] = pos[3 * p + d] / h;
-    if (s == 0) *geom_dirty = 1;
+    if (s == 0) {
+        ctl0->geom_dirty = 1;
+        ctl0->csr_ready = 0;
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -175,12 +179,49 @@ __device__ __forceinline__ void node_terms(float qx, float qy, float qz, float o
     }
 }
 
+// The particles that can reach node (ni, nj, nk) of a lattice (offsets ox, oy, oz; dims dx, dy,
+// dz), in bin order: f(slot, w, cnt) with the node's corner terms w[0..cnt) of each particle that
+// has any (bins of the 3 x 3 x 3 neighbourhood, 2 along a face axis; fields.py:41-48).
+template <class F>
+__device__ __forceinline__ void walk_node(const int* bin_start, const float* qpos, int n, float ox, float oy, float oz,
+                                          int dx, int dy, int dz, int ni, int nj, int nk, F&& f) {
+    int lo_x = ni - 1, hi_x = ox == 0.0f ? ni : ni + 1;
+    int lo_y = nj - 1, hi_y = oy == 0.0f ? nj : nj + 1;
+    int lo_z = nk - 1, hi_z = oz == 0.0f ? nk : nk + 1;
+    lo_x = lo_x < 0 ? 0 : lo_x; hi_x = hi_x > n - 1 ? n - 1 : hi_x;
+    lo_y = lo_y < 0 ? 0 : lo_y; hi_y = hi_y > n - 1 ? n - 1 : hi_y;
+    lo_z = lo_z < 0 ? 0 : lo_z; hi_z = hi_z > n - 1 ? n - 1 : hi_z;
+    for (int bx = lo_x; bx <= hi_x; ++bx)
+        for (int by = lo_y; by <= hi_y; ++by) {
+            long row = ((long)bx * n + by) * n;
+            int b = bin_start[row + lo_z], e = bin_start[row + hi_z + 1];
+            for (int s = b; s < e; ++s) {
+                const float* qp = qpos + 3 * (long)s;
+                float w[8];
+                int cnt;
+                node_terms(qp[0], qp[1], qp[2], ox, oy, oz, dx, dy, dz, ni, nj, nk, w, cnt);
+                if (cnt == 0) continue;
+                f(s, w, cnt);
+            }
+        }
+}
+
+__device__ __forceinline__ void cell_of(long node, int n, int& i, int& j, int& k) {
+    i = (int)(node / ((long)n * n));
+    j = (int)((node / n) % n);
+    k = (int)(node % n);
+}
+
 template <class Src>
 __global__ __launch_bounds__(kBlock) void k_p2g_gather(Src src) {
     HFM_ENTRY(src);
     const int lat = sub;  // 0 cell, 1 + axis for faces
     const bool full = a.ctl[0].geom_dirty != 0;
     if (!full && lat != 0) return;  // face lattices unchanged since the last gather
+    // the cell lattice of unchanged particles: summed from its contribution list once that exists;
+    // in the first substep without it, this gather also counts each node's terms for the list
+    if (!full && a.ctl[0].csr_ready) return;
+    const bool count_terms = !full && lat == 0;
     const int n = a.n;
     const long count = lat == 0 ? a.N : a.F;
     if (lb >= nblocks(count)) return;  // a batch of one launches the face lattices' workgroup count for all four
@@ -193,43 +234,29 @@ __global__ __launch_bounds__(kBlock) void k_p2g_gather(Src src) {
     if (axis == 1) { dy = n + 1; oy = 0.0f; }
     if (axis == 2) { dz = n + 1; oz = 0.0f; }
     int ni = (int)(node / ((long)dy * dz)), nj = (int)((node / dz) % dy), nk = (int)(node % dz);
-    // candidate bins per axis
-    int lo_x = ni - 1, hi_x = ox == 0.0f ? ni : ni + 1;
-    int lo_y = nj - 1, hi_y = oy == 0.0f ? nj : nj + 1;
-    int lo_z = nk - 1, hi_z = oz == 0.0f ? nk : nk + 1;
-    lo_x = lo_x < 0 ? 0 : lo_x; hi_x = hi_x > n - 1 ? n - 1 : hi_x;
-    lo_y = lo_y < 0 ? 0 : lo_y; hi_y = hi_y > n - 1 ? n - 1 : hi_y;
-    lo_z = lo_z < 0 ? 0 : lo_z; hi_z = hi_z > n - 1 ? n - 1 : hi_z;
     float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f;  // cell: alpha, tsum ; face: q, K, m
-    for (int bx = lo_x; bx <= hi_x; ++bx)
-        for (int by = lo_y; by <= hi_y; ++by) {
-            long row = ((long)bx * n + by) * n;
-            int b = a.bin_start[row + lo_z], e = a.bin_start[row + hi_z + 1];
-            for (int s = b; s < e; ++s) {
-                const float* qp = a.qpos + 3 * (long)s;
-                float w[8];
-                int cnt;
-                node_terms(qp[0], qp[1], qp[2], ox, oy, oz, dx, dy, dz, ni, nj, nk, w, cnt);
-                if (cnt == 0) continue;
-                const int p = a.sorted[s];
-                if (lat == 0) {
-                    const float vt = a.frac * a.temp[p];
-                    if (full)
-                        for (int q = 0; q < cnt; ++q) acc0 += w[q] * a.frac;
-                    for (int q = 0; q < cnt; ++q) acc1 += w[q] * vt;
-                } else {
-                    float vq = a.frac * a.vel[3 * p + axis];
-                    float vm = a.k0 * a.vel[3 * p + axis];
-                    for (int q = 0; q < cnt; ++q) {
-                        acc0 += w[q] * vq;
-                        acc1 += w[q] * a.k0;
-                        acc2 += w[q] * vm;
-                    }
-                }
+    int terms = 0;
+    walk_node(a.bin_start, a.qpos, n, ox, oy, oz, dx, dy, dz, ni, nj, nk, [&](int s, const float* w, int cnt) {
+        const int p = a.sorted[s];
+        terms += cnt;
+        if (lat == 0) {
+            const float vt = a.frac * a.temp[p];
+            if (full)
+                for (int q = 0; q < cnt; ++q) acc0 += w[q] * a.frac;
+            for (int q = 0; q < cnt; ++q) acc1 += w[q] * vt;
+        } else {
+            float vq = a.frac * a.vel[3 * p + axis];
+            float vm = a.k0 * a.vel[3 * p + axis];
+            for (int q = 0; q < cnt; ++q) {
+                acc0 += w[q] * vq;
+                acc1 += w[q] * a.k0;
+                acc2 += w[q] * vm;
             }
         }
+    });
     if (lat == 0) {
         a.tsum[node] = acc1;
+        if (count_terms) a.csr_start[node] = terms;  // scanned in place into the list's starts
         if (full) {
             a.alpha_p[node] = acc0;
             a.alpha_s[node] = acc0 + a.st_alpha[node];
@@ -240,6 +267,51 @@ __global__ __launch_bounds__(kBlock) void k_p2g_gather(Src src) {
         a.kf[axis][node] = acc1 + a.st_k[axis][node];
         a.mf[axis][node] = acc2 + a.st_m[axis][node];
     }
+}
+
+// The contribution list of the cell lattice: each node's terms in the gather's order (bins,
+// particles, corners) at the node's start; then the list is marked ready.
+// (Every particle has 8 corner terms, each on one node within a bin of its own: 8 P terms in all;
+// the bound below only keeps a stray count from writing past the list.)
+__global__ __launch_bounds__(kBlock) void k_csr_fill(long N, int n, const int* bin_start, const float* qpos,
+                                                     const int* sorted, const int* start, int* csr_p, float* csr_w,
+                                                     long cap, CgCtl* ctl0) {
+    const long node = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (node == 0 && threadIdx.x == 0) ctl0->csr_ready = 1;   // read by later launches only
+    if (node >= N) return;
+    int ni, nj, nk;
+    cell_of(node, n, ni, nj, nk);
+    int e = start[node];
+    walk_node(bin_start, qpos, n, 0.5f, 0.5f, 0.5f, n, n, n, ni, nj, nk, [&](int s, const float* w, int cnt) {
+        const int p = sorted[s];
+        for (int q = 0; q < cnt; ++q) {
+            if (e < cap) {
+                csr_p[e] = p;
+                csr_w[e] = w[q];
+            }
+            ++e;
+        }
+    });
+}
+
+// The cell lattice's particle temperature sum from the contribution list: the gather's terms
+// w * (frac * temp) in the gather's order, so the same bits.
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_p2g_listed(Src src) {
+    HFM_ENTRY(src);
+    if (a.P == 0 || !a.ctl[0].csr_ready || a.ctl[0].geom_dirty) return;
+    const long N = a.N;
+    const long node = (long)xcd_tile(lb, nblocks(N)) * kBlock + threadIdx.x;
+    if (node >= N) return;
+    const long cap = 8L * a.P;
+    const int b = a.csr_start[node];
+    const int e = a.csr_start[node + 1] < cap ? a.csr_start[node + 1] : (int)cap;
+    float acc1 = 0.0f;
+    for (int i = b; i < e; ++i) {
+        const float vt = a.frac * a.temp[a.csr_p[i]];
+        acc1 += a.csr_w[i] * vt;
+    }
+    a.tsum[node] = acc1;
 }
 
 }  // namespace
@@ -261,11 +333,26 @@ void launch_bin_particles(Scene* s, hipStream_t st) {
                            s->dev.bin_start, s->dev.sorted);
         hipLaunchKernelGGL(k_bin_sort, dim3(nblocks(N)), dim3(kBlock), 0, st, N, s->dev.bin_start, s->dev.sorted);
         hipLaunchKernelGGL(k_bin_pack, dim3(nblocks(P)), dim3(kBlock), 0, st, P, s->dev.h, s->dev.pos,
-                           s->dev.sorted, s->dev.qpos, &s->dev.ctl[0].geom_dirty);
+                           s->dev.sorted, s->dev.qpos, &s->dev.ctl[0]);
     }
     s->bins_dirty = 0;
 }
 
 void launch_p2g(const Launch& L) { HFM_LAUNCH(k_p2g_gather, L, kBlock, nullptr, nullptr); }
+
+// After the gather that counted each node's terms into csr_start: scan the counts in place into
+// starts (N + 1), then fill the list.
+void launch_csr_build(Scene* s, hipStream_t st) {
+    const long N = s->N;
+    const int ntiles = (int)((N + 1 + kScanTile - 1) / kScanTile);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(kBlock), 0, st, s->dev.csr_start, N, s->scan_tmp);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, st, s->scan_tmp, ntiles);
+    hipLaunchKernelGGL(k_scan_final, dim3(ntiles), dim3(kBlock), 0, st, s->dev.csr_start, N, s->scan_tmp,
+                       s->dev.csr_start);
+    hipLaunchKernelGGL(k_csr_fill, dim3(nblocks(N)), dim3(kBlock), 0, st, N, s->n, s->dev.bin_start, s->dev.qpos,
+                       s->dev.sorted, s->dev.csr_start, s->dev.csr_p, s->dev.csr_w, 8L * s->dev.P, &s->dev.ctl[0]);
+}
+
+void launch_p2g_listed(const Launch& L) { HFM_LAUNCH(k_p2g_listed, L, kBlock, nullptr, nullptr); }
 
 }  // namespace hfm

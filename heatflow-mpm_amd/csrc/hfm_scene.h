@@ -30,6 +30,7 @@ struct CgCtl {
     int converged;
     int maxeff;        // iteration cap rounded up to the 2-iteration cycle
     int geom_dirty;    // ctl[0] only: particles were re-binned, the next gather rebuilds every lattice
+    int csr_ready;     // ctl[0] only: the cell lattice's contribution list matches the binned particles
     double tol;        // relative tolerance
     double atol;       // max(tol * ||b||, tol)
     float mean;        // pressure rhs mean to remove (numpy float32 mean)
@@ -110,6 +111,11 @@ struct SceneView {
     ptr<float> temp, melt, yield, visc;
     ptr<int> bin_start, sorted;
     ptr<float> qpos;         // pos / h per particle, in bin order (3 per particle)
+    // contribution list of the cell lattice while the particles stay put: the terms w of node c
+    // are csr_w[csr_start[c] .. csr_start[c + 1]) of particles csr_p[...], in the gather's order
+    ptr<int> csr_start;      // [N + 1]
+    ptr<int> csr_p;          // [8 P]
+    ptr<float> csr_w;        // [8 P]
     int mixed;               // HFM_PREC_MIXED
     int cg_mg;               // pressure PCG preconditioned by a multigrid V-cycle (HFM_FLAG_MG_PCG)
     int mg_levels;           // grid levels, level 0 the scene's grid, each next one halved
@@ -250,6 +256,7 @@ struct Scene {
     int* bin_cnt;
     int* scan_tmp;
     int bins_dirty;
+    int csr_state;           // contribution list: 0 none, 1 built in this substep, 2 in use
     unsigned version;       // bumped whenever the descriptor changes
     hfm_stats stats;
     int heat_pred, pres_pred;
@@ -351,6 +358,8 @@ void launch_apply_fixed(const Scene* s, hipStream_t st, float* T);
 // ---- particles (hfm_p2g.hip) ----
 void launch_bin_particles(Scene* s, hipStream_t st);
 void launch_p2g(const Launch& L);
+void launch_csr_build(Scene* s, hipStream_t st);
+void launch_p2g_listed(const Launch& L);
 
 // ---- PCG (hfm_pcg.hip); op 0 = heat, 1 = pressure ----
 void pcg_geometry(const Scene* s, bool fused, int* cpt, int* nb_upd, int* nb_dir, int* tiles_j, int* tiles_k,

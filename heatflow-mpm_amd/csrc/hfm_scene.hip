@@ -764,9 +764,26 @@ int batch_substep(Batch* b, double dt) {
     RC(batch_prepare(b));
     RC(batch_adopt(b));
     const float dt32 = (float)dt;
+    // particle-grid gather: every lattice after a re-binning; while the particles stay put, the cell
+    // lattice is counted once more by the general gather, its contribution list built from the
+    // counts, and from then on summed from the list (host and device step this state together:
+    // k_bin_pack marks the list stale, k_csr_fill ready)
+    bool general = false, listed = false;
+    for (Scene* s : b->scenes) {
+        if (s->dev.P == 0) continue;
+        if (s->bins_dirty) {
+            launch_bin_particles(s, b->stream);
+            s->csr_state = 0;
+        } else {
+            s->csr_state = s->csr_state == 0 ? 1 : 2;
+        }
+        general |= s->csr_state < 2;
+        listed |= s->csr_state == 2;
+    }
+    if (general) launch_p2g(family(b, FAM_P2G));
     for (Scene* s : b->scenes)
-        if (s->dev.P > 0 && s->bins_dirty) launch_bin_particles(s, b->stream);
-    launch_p2g(family(b, FAM_P2G));
+        if (s->dev.P > 0 && s->csr_state == 1) launch_csr_build(s, b->stream);
+    if (listed) launch_p2g_listed(family(b, FAM_CELL));
     launch_heat_prep(family(b, FAM_CELL), dt32);
     launch_advect_T(family(b, FAM_CELL), dt32);
     CHECK_LAUNCH();
@@ -1043,6 +1060,7 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
     A(s->umax, 4);
     A(s->bin_cnt, N + 1);
     A(d.bin_start, N + 1);
+    A(d.csr_start, N + 1);
     A(s->scan_tmp, (N + 1) / 2048 + 16);
     // pairwise-sum programs for the numpy mean
     append_program(o->prog, (int)(N >= 8192 ? 8192 : N));
@@ -1219,16 +1237,18 @@ int hfm_set_particles(hfm_scene* h, int64_t P, const float* pos, const float* ve
     HIP_OK(hipStreamSynchronize(s->stream));
     SceneDev& d = s->dev;
     if (P > s->P_cap) {
-        void* olds[] = {d.pos, d.vel, d.temp, d.melt, d.yield, d.visc, s->pkey, s->pslot, d.sorted, d.qpos};
+        void* olds[] = {d.pos, d.vel, d.temp, d.melt, d.yield, d.visc, s->pkey, s->pslot, d.sorted, d.qpos,
+                        d.csr_p, d.csr_w};
         for (void* q : olds)
             if (q) dfree(o, q);
-        d.pos = d.vel = d.temp = d.melt = d.yield = d.visc = d.qpos = nullptr;
-        s->pkey = s->pslot = d.sorted = nullptr;
+        d.pos = d.vel = d.temp = d.melt = d.yield = d.visc = d.qpos = d.csr_w = nullptr;
+        s->pkey = s->pslot = d.sorted = d.csr_p = nullptr;
         s->P_cap = 0;
         if ((rc = dalloc(o, &d.pos, 3 * P)) || (rc = dalloc(o, &d.vel, 3 * P)) || (rc = dalloc(o, &d.temp, P)) ||
             (rc = dalloc(o, &d.melt, P)) || (rc = dalloc(o, &d.yield, P)) || (rc = dalloc(o, &d.visc, P)) ||
             (rc = dalloc(o, &s->pkey, P)) || (rc = dalloc(o, &s->pslot, P)) || (rc = dalloc(o, &d.sorted, P)) ||
-            (rc = dalloc(o, &d.qpos, 3 * P))) {
+            (rc = dalloc(o, &d.qpos, 3 * P)) || (rc = dalloc(o, &d.csr_p, 8 * P)) ||
+            (rc = dalloc(o, &d.csr_w, 8 * P))) {
             d.P = 0;
             refresh_dev(s);
             return rc;
