@@ -306,8 +306,22 @@ __global__ __launch_bounds__(kBlock) void k_p2g_listed(Src src) {
     const long cap = 8L * a.P;
     const int b = a.csr_start[node];
     const int e = a.csr_start[node + 1] < cap ? a.csr_start[node + 1] : (int)cap;
+    // the loads of kLoads terms are issued together (the list is read twice-indirect); the sum
+    // still adds the terms one by one in list order
+    constexpr int kLoads = 8;
     float acc1 = 0.0f;
-    for (int i = b; i < e; ++i) {
+    int i = b;
+    for (; i + kLoads <= e; i += kLoads) {
+        float vt[kLoads], w[kLoads];
+#pragma unroll
+        for (int u = 0; u < kLoads; ++u) {
+            vt[u] = a.temp[a.csr_p[i + u]];
+            w[u] = a.csr_w[i + u];
+        }
+#pragma unroll
+        for (int u = 0; u < kLoads; ++u) acc1 += w[u] * (a.frac * vt[u]);
+    }
+    for (; i < e; ++i) {
         const float vt = a.frac * a.temp[a.csr_p[i]];
         acc1 += a.csr_w[i] * vt;
     }
