@@ -19,7 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "heatflow_mpm", "libhfm.so")
 BUILD = os.path.join(HERE, "_build")
-SOURCES = ["hfm_fields.hip", "hfm_p2g.hip", "hfm_pcg.hip", "hfm_mg.hip", "hfm_scene.hip"]
+SOURCES = ["hfm_fields.hip", "hfm_p2g.hip", "hfm_pcg.hip", "hfm_mg.hip", "hfm_mesh.hip", "hfm_scene.hip"]
 ARCH = os.environ.get("HFM_OFFLOAD_ARCH", "gfx950")
 
 CFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",

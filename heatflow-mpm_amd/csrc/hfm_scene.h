@@ -359,6 +359,8 @@ void launch_apply_fixed(const Scene* s, hipStream_t st, float* T);
 void launch_bin_particles(Scene* s, hipStream_t st);
 void launch_p2g(const Launch& L);
 void launch_csr_build(Scene* s, hipStream_t st);
+void launch_winding(const float* tri, long nt, const float* pts, long np, float threshold, int* flags, float* wn,
+                    hipStream_t st);
 void launch_p2g_listed(const Launch& L);
 
 // ---- PCG (hfm_pcg.hip); op 0 = heat, 1 = pressure ----

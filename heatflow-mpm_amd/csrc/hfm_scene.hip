@@ -907,6 +907,51 @@ int hfm_device_count(void) {
     return c;
 }
 
+int hfm_mesh_inside(const float* tri, int64_t nt, const float* points, int64_t np, float threshold, int32_t* flags,
+                    float* winding, int device) {
+    if (nt < 0 || np < 0 || nt > ((int64_t)1 << 26) || np > ((int64_t)1 << 30) || (nt > 0 && !tri) ||
+        (np > 0 && (!points || !flags)) || !(threshold == threshold)) {
+        set_err("bad mesh-inside arguments (triangles %lld, points %lld)", (long long)nt, (long long)np);
+        return HFM_ERR_ARG;
+    }
+    if (np == 0) return HFM_OK;
+    RC(check_finite(tri, 9 * nt, "triangles"));
+    RC(check_finite(points, 3 * np, "points"));
+    const int ndev = hfm_device_count();
+    if (device < 0 || device >= ndev) {
+        set_err("device %d not available (%d HIP devices)", device, ndev);
+        return HFM_ERR_ARG;
+    }
+    HIP_OK(hipSetDevice(device));
+    float *d_tri = nullptr, *d_pts = nullptr, *d_wn = nullptr;
+    int* d_flags = nullptr;
+    int rc = HFM_OK;
+    auto fail = [&](hipError_t e, const char* what) {
+        set_err("%s failed: %s", what, hipGetErrorString(e));
+        rc = e == hipErrorOutOfMemory ? HFM_ERR_OOM : HFM_ERR_HIP;
+    };
+    hipError_t e = hipSuccess;
+    if (nt > 0 && (e = hipMalloc(&d_tri, 9 * nt * sizeof(float))) != hipSuccess) fail(e, "hipMalloc");
+    if (!rc && (e = hipMalloc(&d_pts, 3 * np * sizeof(float))) != hipSuccess) fail(e, "hipMalloc");
+    if (!rc && (e = hipMalloc(&d_flags, np * sizeof(int))) != hipSuccess) fail(e, "hipMalloc");
+    if (!rc && winding && (e = hipMalloc(&d_wn, np * sizeof(float))) != hipSuccess) fail(e, "hipMalloc");
+    if (!rc && nt > 0 && (e = hipMemcpy(d_tri, tri, 9 * nt * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess)
+        fail(e, "hipMemcpy");
+    if (!rc && (e = hipMemcpy(d_pts, points, 3 * np * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess)
+        fail(e, "hipMemcpy");
+    if (!rc) {
+        launch_winding(d_tri, nt, d_pts, np, threshold, d_flags, d_wn, nullptr);
+        if ((e = hipGetLastError()) != hipSuccess) fail(e, "kernel launch");
+    }
+    if (!rc && (e = hipMemcpy(flags, d_flags, np * sizeof(int), hipMemcpyDeviceToHost)) != hipSuccess)
+        fail(e, "hipMemcpy");
+    if (!rc && winding && (e = hipMemcpy(winding, d_wn, np * sizeof(float), hipMemcpyDeviceToHost)) != hipSuccess)
+        fail(e, "hipMemcpy");
+    for (void* q : {(void*)d_tri, (void*)d_pts, (void*)d_flags, (void*)d_wn})
+        if (q) (void)hipFree(q);
+    return rc;
+}
+
 int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
     if (!prm || !out) {
         set_err("null argument");

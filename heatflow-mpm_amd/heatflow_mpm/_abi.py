@@ -91,6 +91,7 @@ SIGNATURES = {
     "hfm_batch_kernel_time": (C.c_int, [_VP, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "hfm_batch_kernel_time_reset": (C.c_int, [_VP]),
     "hfm_device_count": (C.c_int, []),
+    "hfm_mesh_inside": (C.c_int, [_VP, C.c_int64, _VP, C.c_int64, C.c_float, _VP, _VP, C.c_int]),
     "hfm_last_error": (C.c_char_p, []),
 }
 

@@ -238,6 +238,17 @@ int hfm_batch_kernel_time_reset(hfm_batch* b);
 /* Number of HIP devices visible (0 without a GPU). */
 int hfm_device_count(void);
 
+/* ---- Seeding material in a mesh ------------------------------------------------------------
+ * Replaces the Warp kernel `_inside` of shapes.py:13-17 as launched by sample_interior
+ * (shapes.py:35-44): flags[i] = 1 where the winding number of the closed triangle mesh at
+ * points[i] exceeds `threshold` (the reference uses 0.5), else 0.  tri: nt triangles as 9 floats
+ * each (three xyz vertices, counter-clockwise seen from outside); points: np xyz.  The winding
+ * number is the exact sum of the triangles' solid angles over 4 pi (Warp's query approximates the
+ * same sum); `winding` (may be NULL) receives it per point.  Host buffers in and out; the call
+ * copies, runs on `device` and returns when the results are written. */
+int hfm_mesh_inside(const float* tri, int64_t nt, const float* points, int64_t np, float threshold, int32_t* flags,
+                    float* winding, int device);
+
 const char* hfm_last_error(void);
 
 #ifdef __cplusplus
