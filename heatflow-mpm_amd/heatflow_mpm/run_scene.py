@@ -1,19 +1,20 @@
-"""Heat a block of material in front of the heater and write frames (scripts/run_scene.py of the reference).
+"""Heat a mesh of material in front of the heater and write frames (scripts/run_scene.py of the reference).
 
-    python -m heatflow_mpm.run_scene --out runs/ellipsoid_fan --fan 2.0 --power 400
+    python -m heatflow_mpm.run_scene --out runs/bunny_fan --mesh path/to/bunny.obj --fan 2.0 --power 400
+    python -m heatflow_mpm.run_scene --out runs/ellipsoid_fan --ellipsoid --fan 2.0
 
-Same arguments and output as the reference script, except the material: the reference seeds
-particles from a mesh (shapes.py, Warp BVH winding queries), which is not part of this library;
-here an ellipsoid of --semi-axes is seeded on the same jittered lattice (jitter 0.2 spacing, the
-reference's default, shapes.py:38).  The particles are held static (Newton's MPM step is outside
-this library).
+Same arguments and output as the reference script.  The mesh is seeded by mesh_particles (jittered
+lattice, winding-number inside test on the GPU, shapes.py:47-51); no mesh ships with this library,
+so --mesh names a file (default: the HFM_BUNNY environment variable, or the reference's
+assets/bunny.obj copied next to the package).  --ellipsoid seeds an ellipsoid of --semi-axes on the
+same lattice instead.  The particles are held static (Newton's MPM step is outside this library).
 """
 from __future__ import annotations
 
 import argparse
 import sys
 
-from . import GridConfig, Heater, HeatflowMPM, Material, ellipsoid_particles
+from . import BUNNY, GridConfig, Heater, HeatflowMPM, Material, ellipsoid_particles, mesh_particles
 from .frames import run_scene
 
 
@@ -23,6 +24,10 @@ def parser() -> argparse.ArgumentParser:
     ap.add_argument("--seconds", type=float, default=20.0)
     ap.add_argument("--frame-dt", type=float, default=0.02)
     ap.add_argument("--dt", type=float, default=2e-3)
+    ap.add_argument("--mesh", default=BUNNY)
+    ap.add_argument("--height", type=float, default=0.10)
+    ap.add_argument("--yaw", type=float, default=0.0)
+    ap.add_argument("--ellipsoid", action="store_true", help="seed an ellipsoid of --semi-axes instead of the mesh")
     ap.add_argument("--semi-axes", type=float, nargs=3, default=(0.05, 0.04, 0.05))
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--spacing", type=float, default=2.5e-3)
@@ -54,8 +59,11 @@ def build(a) -> HeatflowMPM:
                    viscosity_liquid=a.viscosity_liquid, kappa=a.kappa)
     heater = Heater(dist=a.heater_dist, z=a.heater_z, t_rod=a.t_rod, power=a.power, t_cap=a.t_cap, fan=a.fan,
                     on=not a.no_heater)
-    P = ellipsoid_particles(semi_axes=tuple(a.semi_axes), spacing=a.spacing, jitter=0.2 * a.spacing, seed=a.seed,
-                            L=a.L)
+    if a.ellipsoid:
+        P = ellipsoid_particles(semi_axes=tuple(a.semi_axes), spacing=a.spacing, jitter=0.2 * a.spacing, seed=a.seed,
+                                L=a.L)
+    else:
+        P = mesh_particles(a.mesh, a.height, a.spacing, a.L, a.seed, a.yaw, a.device)
     grid = GridConfig(L=a.L, h=a.h, t_amb=a.t_amb, cg_tol=a.cg_tol)
     return HeatflowMPM(P, mat, heater, grid, a.spacing, a.t_init, device=a.device)
 
