@@ -88,6 +88,35 @@ __device__ __forceinline__ float sample(const E* __restrict__ a, float px, float
     return s;
 }
 
+// fields.py:51-65 — min and max of the eight lattice values around a point (the MacCormack
+// limiter), in the loop order of sample.
+__device__ __forceinline__ void value_range(const float* __restrict__ a, float px, float py, float pz, float ox,
+                                            float oy, float oz, float h, int nx, int ny, int nz, float& lo,
+                                            float& hi) {
+    int bi, bj, bk;
+    float fx, fy, fz;
+    base_index1(px, ox, h, bi, fx);
+    base_index1(py, oy, h, bj, fy);
+    base_index1(pz, oz, h, bk, fz);
+    lo = 1.0e30f;
+    hi = -1.0e30f;
+#pragma unroll
+    for (int dx = 0; dx < 2; ++dx) {
+        int ix = clampi(bi + dx, 0, nx - 1);
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy) {
+            int iy = clampi(bj + dy, 0, ny - 1);
+            const float* row = a + ((long)ix * ny + iy) * nz;
+#pragma unroll
+            for (int dz = 0; dz < 2; ++dz) {
+                const float v = row[clampi(bk + dz, 0, nz - 1)];
+                lo = v < lo ? v : lo;
+                hi = v > hi ? v : hi;
+            }
+        }
+    }
+}
+
 // fields.py:68-72
 template <class E = float>
 __device__ __forceinline__ void sample_velocity(const E* __restrict__ u, const E* __restrict__ v,

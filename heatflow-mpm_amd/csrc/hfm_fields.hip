@@ -64,27 +64,13 @@ __global__ __launch_bounds__(kBlock) void k_heat_prep(Src src, float dt) {
 // ---------------------------------------------------------------------------------------------
 // T advection (into T) + conductivity + heat rhs/diag
 // ---------------------------------------------------------------------------------------------
-template <class Src>
-__global__ __launch_bounds__(kBlock) void k_advect_T(Src src, float dt) {
-    HFM_ENTRY(src);
-    const long t = (long)xcd_tile(lb, nblocks(a.N)) * kBlock + threadIdx.x;
-    if (t >= a.N) return;
+// The rest of the T pass once the advected value t2 of cell t is known: conductivity
+// (heat.py:44-47) and the heat system's rhs and Jacobi diagonal (heat.py:116-144).
+template <class A>
+__device__ __forceinline__ void heat_tail(const A& a, long t, int i, int j, int k, float t2, float ac, float dt) {
     const int n = a.n;
     const long nn = (long)n * n;
-    const int i = (int)(t / nn), j = (int)((t / n) % n), k = (int)(t % n);
     const float h = a.h;
-    const float ac = a.alpha_s[t];
-    float t2;
-    if (ac > 0.5f) {
-        t2 = a.Tw[t];
-    } else {
-        float x = ((float)i + 0.5f) * h, y = ((float)j + 0.5f) * h, z = ((float)k + 0.5f) * h;
-        if (a.mixed)
-            backtrace<uint16_t>(a.ub[0], a.ub[1], a.ub[2], x, y, z, h, n, dt);
-        else
-            backtrace<float>(a.u[0], a.u[1], a.u[2], x, y, z, h, n, dt);
-        t2 = sample(a.Tw, x, y, z, 0.5f, 0.5f, 0.5f, h, n, n, n);
-    }
     a.T[t] = t2;
     const float kc = conductivity(ac, a.ka, a.km);
     a.kap[t] = kc;
@@ -106,6 +92,66 @@ __global__ __launch_bounds__(kBlock) void k_advect_T(Src src, float dt) {
     }
     a.hrhs[t] = t2 + dt_hh * ntf;
     a.hdiag[t] = 1.0f / (1.0f + dt_hh * sd);
+}
+
+// heat.py:31-41 (T advection) fused with the conductivity and heat rhs/diagonal.  With
+// HFM_FLAG_MACCORMACK this pass only stores the semi-Lagrangian value; k_mc_T finishes the cell.
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_advect_T(Src src, float dt) {
+    HFM_ENTRY(src);
+    const long t = (long)xcd_tile(lb, nblocks(a.N)) * kBlock + threadIdx.x;
+    if (t >= a.N) return;
+    const int n = a.n;
+    const long nn = (long)n * n;
+    const int i = (int)(t / nn), j = (int)((t / n) % n), k = (int)(t % n);
+    const float h = a.h;
+    const float ac = a.alpha_s[t];
+    float t2;
+    if (ac > 0.5f) {
+        t2 = a.Tw[t];
+    } else {
+        float x = ((float)i + 0.5f) * h, y = ((float)j + 0.5f) * h, z = ((float)k + 0.5f) * h;
+        if (a.mixed)
+            backtrace<uint16_t>(a.ub[0], a.ub[1], a.ub[2], x, y, z, h, n, dt);
+        else
+            backtrace<float>(a.u[0], a.u[1], a.u[2], x, y, z, h, n, dt);
+        t2 = sample(a.Tw, x, y, z, 0.5f, 0.5f, 0.5f, h, n, n, n);
+    }
+    if (a.maccormack) {
+        a.mc_T[t] = t2;
+        return;
+    }
+    heat_tail(a, t, i, j, k, t2, ac, dt);
+}
+
+// MacCormack step of T (HFM_FLAG_MACCORMACK): with T^ the semi-Lagrangian field of k_advect_T,
+// T~ = T^ sampled at the point traced forward (RK2 with -dt) and d the departure point,
+//   T = clamp(T^ + (Tw - T~) / 2, value_range(Tw, d)),
+// then the same rest of the pass.  Cells kept by the solid fraction keep Tw.
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_mc_T(Src src, float dt) {
+    HFM_ENTRY(src);
+    if (!a.maccormack) return;
+    const long t = (long)xcd_tile(lb, nblocks(a.N)) * kBlock + threadIdx.x;
+    if (t >= a.N) return;
+    const int n = a.n;
+    const long nn = (long)n * n;
+    const int i = (int)(t / nn), j = (int)((t / n) % n), k = (int)(t % n);
+    const float h = a.h;
+    const float ac = a.alpha_s[t];
+    float t2 = a.mc_T[t];
+    if (!(ac > 0.5f)) {
+        const float xc = ((float)i + 0.5f) * h, yc = ((float)j + 0.5f) * h, zc = ((float)k + 0.5f) * h;
+        float dx = xc, dy = yc, dz = zc, fx = xc, fy = yc, fz = zc;
+        backtrace<float>(a.u[0], a.u[1], a.u[2], dx, dy, dz, h, n, dt);
+        backtrace<float>(a.u[0], a.u[1], a.u[2], fx, fy, fz, h, n, -dt);
+        const float back = sample(a.mc_T, fx, fy, fz, 0.5f, 0.5f, 0.5f, h, n, n, n);
+        float v = t2 + 0.5f * (a.Tw[t] - back);
+        float lo, hi;
+        value_range(a.Tw, dx, dy, dz, 0.5f, 0.5f, 0.5f, h, n, n, n, lo, hi);
+        t2 = v < lo ? lo : (v > hi ? hi : v);
+    }
+    heat_tail(a, t, i, j, k, t2, ac, dt);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -134,9 +180,76 @@ __global__ __launch_bounds__(kBlock) void k_g2p(Src src) {
 // ---------------------------------------------------------------------------------------------
 // air face pass: advect (old u, v, w) -> buoyancy (new T) -> relax, into us; sub-range = axis
 // ---------------------------------------------------------------------------------------------
+// Face position, lattice offsets and dims of face f of an axis (fields.py:75-108).
+__device__ __forceinline__ void face_geometry(int axis, int i, int j, int k, float h, int n, float& x, float& y,
+                                              float& z, float& ox, float& oy, float& oz, int& dx, int& dy, int& dz) {
+    dx = dy = dz = n;
+    if (axis == 0) {
+        x = (float)i * h; y = ((float)j + 0.5f) * h; z = ((float)k + 0.5f) * h;
+        ox = 0.0f; oy = 0.5f; oz = 0.5f; dx = n + 1;
+    } else if (axis == 1) {
+        x = ((float)i + 0.5f) * h; y = (float)j * h; z = ((float)k + 0.5f) * h;
+        ox = 0.5f; oy = 0.0f; oz = 0.5f; dy = n + 1;
+    } else {
+        x = ((float)i + 0.5f) * h; y = ((float)j + 0.5f) * h; z = (float)k * h;
+        ox = 0.5f; oy = 0.5f; oz = 0.0f; dz = n + 1;
+    }
+}
+
+// The rest of the face pass once the advected value of an interior face is known: buoyancy
+// (grid.py:36-43, new T) and implicit relaxation (grid.py:70-79).
+template <class A>
+__device__ __forceinline__ float face_tail(const A& a, int axis, long f, int i, int j, int k, float val, float dt) {
+    const int n = a.n;
+    const float h = a.h;
+    const long c = ((long)i * n + j) * n + k;
+    if (axis == 2 && a.buoyancy) {
+        float tf = 0.5f * (a.T[c - 1] + a.T[c]);
+        val = val + dt * a.g * (tf - a.t_amb) / (a.t_amb + 273.15f);
+    }
+    const long cl = axis == 0 ? c - (long)n * n : (axis == 1 ? c - n : c - 1);
+    const float af = face_fraction(a.alpha_s[cl], a.alpha_s[c], a.amin);
+    const float am = af > a.amin ? af : a.amin;
+    const float cc = 1.0f / (a.rho * am * h * h * h);
+    return (val + dt * cc * a.mf[axis][f]) / (1.0f + dt * cc * a.kf[axis][f]);
+}
+
+// grid.py:25-33 (face advection of u, v, w from the old fields) fused with buoyancy and
+// relaxation.  With HFM_FLAG_MACCORMACK this pass only stores the semi-Lagrangian value;
+// k_mc_faces finishes the face.
 template <class Src>
 __global__ __launch_bounds__(kBlock) void k_air_faces(Src src, float dt) {
     HFM_ENTRY(src);
+    const int axis = sub;
+    const long f = (long)xcd_tile(lb, nblocks(a.F)) * kBlock + threadIdx.x;
+    if (f >= a.F) return;
+    const int n = a.n;
+    int i, j, k;
+    face_ijk(axis, f, n, i, j, k);
+    const int idx = axis == 0 ? i : (axis == 1 ? j : k);
+    float* out = a.maccormack ? a.mc_u[axis] : a.us[axis];
+    if (idx == 0 || idx == n) {
+        out[f] = 0.0f;
+        return;
+    }
+    const float h = a.h;
+    float x, y, z, ox, oy, oz;
+    int dx, dy, dz;
+    face_geometry(axis, i, j, k, h, n, x, y, z, ox, oy, oz, dx, dy, dz);
+    if (a.mixed)
+        backtrace<uint16_t>(a.ub[0], a.ub[1], a.ub[2], x, y, z, h, n, dt);
+    else
+        backtrace<float>(a.u[0], a.u[1], a.u[2], x, y, z, h, n, dt);
+    const float val = sample(a.u[axis], x, y, z, ox, oy, oz, h, dx, dy, dz);
+    out[f] = a.maccormack ? val : face_tail(a, axis, f, i, j, k, val, dt);
+}
+
+// MacCormack step of the faces (HFM_FLAG_MACCORMACK), as k_mc_T with the old component u[axis]
+// as the start field and both traces through the old u, v, w; then buoyancy and relaxation.
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_mc_faces(Src src, float dt) {
+    HFM_ENTRY(src);
+    if (!a.maccormack) return;
     const int axis = sub;
     const long f = (long)xcd_tile(lb, nblocks(a.F)) * kBlock + threadIdx.x;
     if (f >= a.F) return;
@@ -151,33 +264,18 @@ __global__ __launch_bounds__(kBlock) void k_air_faces(Src src, float dt) {
     }
     const float h = a.h;
     float x, y, z, ox, oy, oz;
-    int dx = n, dy = n, dz = n;
-    if (axis == 0) {
-        x = (float)i * h; y = ((float)j + 0.5f) * h; z = ((float)k + 0.5f) * h;
-        ox = 0.0f; oy = 0.5f; oz = 0.5f; dx = n + 1;
-    } else if (axis == 1) {
-        x = ((float)i + 0.5f) * h; y = (float)j * h; z = ((float)k + 0.5f) * h;
-        ox = 0.5f; oy = 0.0f; oz = 0.5f; dy = n + 1;
-    } else {
-        x = ((float)i + 0.5f) * h; y = ((float)j + 0.5f) * h; z = (float)k * h;
-        ox = 0.5f; oy = 0.5f; oz = 0.0f; dz = n + 1;
-    }
-    if (a.mixed)
-        backtrace<uint16_t>(a.ub[0], a.ub[1], a.ub[2], x, y, z, h, n, dt);
-    else
-        backtrace<float>(a.u[0], a.u[1], a.u[2], x, y, z, h, n, dt);
-    float val = sample(a.u[axis], x, y, z, ox, oy, oz, h, dx, dy, dz);
-    const long c = ((long)i * n + j) * n + k;
-    if (axis == 2 && a.buoyancy) {
-        float tf = 0.5f * (a.T[c - 1] + a.T[c]);
-        val = val + dt * a.g * (tf - a.t_amb) / (a.t_amb + 273.15f);
-    }
-    const long cl = axis == 0 ? c - (long)n * n : (axis == 1 ? c - n : c - 1);
-    const float af = face_fraction(a.alpha_s[cl], a.alpha_s[c], a.amin);
-    const float am = af > a.amin ? af : a.amin;
-    const float cc = 1.0f / (a.rho * am * h * h * h);
-    val = (val + dt * cc * a.mf[axis][f]) / (1.0f + dt * cc * a.kf[axis][f]);
-    out[f] = val;
+    int dx, dy, dz;
+    face_geometry(axis, i, j, k, h, n, x, y, z, ox, oy, oz, dx, dy, dz);
+    float px = x, py = y, pz = z, fx = x, fy = y, fz = z;
+    backtrace<float>(a.u[0], a.u[1], a.u[2], px, py, pz, h, n, dt);
+    backtrace<float>(a.u[0], a.u[1], a.u[2], fx, fy, fz, h, n, -dt);
+    const float fwd = a.mc_u[axis][f];
+    const float back = sample(a.mc_u[axis], fx, fy, fz, ox, oy, oz, h, dx, dy, dz);
+    float v = fwd + 0.5f * (a.u[axis][f] - back);
+    float lo, hi;
+    value_range(a.u[axis], px, py, pz, ox, oy, oz, h, dx, dy, dz, lo, hi);
+    v = v < lo ? lo : (v > hi ? hi : v);
+    out[f] = face_tail(a, axis, f, i, j, k, v, dt);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -412,6 +510,10 @@ void launch_heat_prep(const Launch& L, float dt) { HFM_LAUNCH(k_heat_prep, L, kB
 void launch_advect_T(const Launch& L, float dt) { HFM_LAUNCH(k_advect_T, L, kBlock, nullptr, nullptr, dt); }
 void launch_g2p(const Launch& L) { HFM_LAUNCH(k_g2p, L, kBlock, nullptr, nullptr); }
 void launch_air_faces(const Launch& L, float dt) { HFM_LAUNCH(k_air_faces, L, kBlock, nullptr, nullptr, dt); }
+
+void launch_mc_T(const Launch& L, float dt) { HFM_LAUNCH(k_mc_T, L, kBlock, nullptr, nullptr, dt); }
+
+void launch_mc_faces(const Launch& L, float dt) { HFM_LAUNCH(k_mc_faces, L, kBlock, nullptr, nullptr, dt); }
 void launch_divergence(const Launch& L, double dt) { HFM_LAUNCH(k_divergence, L, kBlock, nullptr, nullptr, dt); }
 void launch_rhs_mean(const Launch& L) { HFM_LAUNCH(k_rhs_mean, L, kBlock, nullptr, nullptr); }
 void launch_project(const Launch& L, double dt) { HFM_LAUNCH(k_project, L, kBlock, nullptr, nullptr, dt); }

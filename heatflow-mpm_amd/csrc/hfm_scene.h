@@ -116,7 +116,10 @@ struct SceneView {
     ptr<int> csr_start;      // [N + 1]
     ptr<int> csr_p;          // [8 P]
     ptr<float> csr_w;        // [8 P]
-    int mixed;               // HFM_PREC_MIXED
+    int mixed;
+    int maccormack;          // HFM_FLAG_MACCORMACK: limited MacCormack advection of T and u, v, w
+    ptr<float> mc_T;         // [N] forward semi-Lagrangian T of this substep (MacCormack only)
+    ptr<float> mc_u[3];      // [F] forward semi-Lagrangian u, v, w of this substep (MacCormack only)               // HFM_PREC_MIXED
     int cg_mg;               // pressure PCG preconditioned by a multigrid V-cycle (HFM_FLAG_MG_PCG)
     int mg_levels;           // grid levels, level 0 the scene's grid, each next one halved
     int mg_tail;             // first level of at most kMgTailMaxN cells per side (solved in LDS)
@@ -359,6 +362,8 @@ void launch_apply_fixed(const Scene* s, hipStream_t st, float* T);
 void launch_bin_particles(Scene* s, hipStream_t st);
 void launch_p2g(const Launch& L);
 void launch_csr_build(Scene* s, hipStream_t st);
+void launch_mc_T(const Launch& L, float dt);
+void launch_mc_faces(const Launch& L, float dt);
 void launch_winding(const float* tri, long nt, const float* pts, long np, float threshold, int* flags, float* wn,
                     hipStream_t st);
 void launch_p2g_listed(const Launch& L);

@@ -786,11 +786,15 @@ int batch_substep(Batch* b, double dt) {
     if (listed) launch_p2g_listed(family(b, FAM_CELL));
     launch_heat_prep(family(b, FAM_CELL), dt32);
     launch_advect_T(family(b, FAM_CELL), dt32);
+    bool mc = false;
+    for (Scene* s : b->scenes) mc |= s->dev.maccormack != 0;
+    if (mc) launch_mc_T(family(b, FAM_CELL), dt32);
     CHECK_LAUNCH();
     const bool mixed = !b->scenes.empty() && b->scenes[0]->dev.mixed != 0;  // one precision per batch
     RC(mixed ? run_ir(b, 0, dt32) : run_pcg(b, 0, dt32));
     launch_g2p(family(b, FAM_G2P));
     launch_air_faces(family(b, FAM_FACE3), dt32);
+    if (mc) launch_mc_faces(family(b, FAM_FACE3), dt32);
     launch_divergence(family(b, FAM_CELL), dt);
     launch_rhs_mean(family(b, FAM_MEAN));
     CHECK_LAUNCH();
@@ -990,6 +994,10 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
         set_err("HFM_FLAG_FUSED_CG is not available with HFM_PREC_MIXED");
         return HFM_ERR_ARG;
     }
+    if (prm->precision == HFM_PREC_MIXED && (prm->flags & HFM_FLAG_MACCORMACK)) {
+        set_err("HFM_FLAG_MACCORMACK is available with HFM_PREC_FP32 only");
+        return HFM_ERR_ARG;
+    }
     const int ndev = hfm_device_count();
     if (device < 0 || device >= ndev) {
         set_err("device %d not available (%d HIP devices)", device, ndev);
@@ -1089,6 +1097,11 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
         A(d.mg_w, coarse);
         A(d.mg_s, coarse);
         A(d.mg_e, coarse);
+    }
+    if (prm->flags & HFM_FLAG_MACCORMACK) {   // the forward semi-Lagrangian fields of the MacCormack step
+        d.maccormack = 1;
+        A(d.mc_T, N);
+        for (int a = 0; a < 3; ++a) A(d.mc_u[a], F);
     }
     if (prm->precision == HFM_PREC_MIXED) {
         d.mixed = 1;

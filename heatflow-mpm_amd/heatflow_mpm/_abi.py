@@ -27,6 +27,7 @@ PREC_MIXED = 1
 FLAG_TIME_KERNELS = 1
 FLAG_FUSED_CG = 2
 FLAG_MG_PCG = 4
+FLAG_MACCORMACK = 8
 KT_PCG_DIR, KT_PCG_UPD, KT_PCG_FUSED = 0, 1, 2
 
 

@@ -205,16 +205,21 @@ class HeatflowMPM:
     far fewer iterations, fields equal to the Jacobi solve's to the solve tolerance, not in bits;
     `layout` picks the pressure PCG layout: None (chosen by grid size), "auto" / "auto-batch"
     (the autotuned table of heatflow_mpm.autotune for a scene stepped alone / in a batch), or
-    dict(dir_planes=0|-1|4|8|16, cg="warp"|"fused"); with cg="mg" only its pass-A tiling applies.
+    dict(dir_planes=0|-1|4|8|16, cg="warp"|"fused"); with cg="mg" only its pass-A tiling applies;
+    `advection="maccormack"` advects T and u, v, w by the limited MacCormack method (the
+    reference's prepared but unused limiter, fields.py:51-65; HFM_FLAG_MACCORMACK, fp32 only)
+    instead of the reference's semi-Lagrangian step.
     """
 
     CG_FORMS = ("warp", "fused", "mg")
+    ADVECTION = ("semi-lagrangian", "maccormack")
     PRECISIONS = {"fp32": _abi.PREC_FP32, "mixed": _abi.PREC_MIXED}
 
     def __init__(self, particles: np.ndarray, mat: Material = Material(), heater: Heater = Heater(),
                  grid: GridConfig = GridConfig(), spacing: float = 2.5e-3, t_init: float = 20.0,
                  kappa_air: float = KAPPA_AIR, device: str = "cuda:0", body=None, cg: str = "warp",
-                 inputs: dict | None = None, precision: str = "fp32", layout=None):
+                 inputs: dict | None = None, precision: str = "fp32", layout=None,
+                 advection: str = "semi-lagrangian"):
         if cg not in self.CG_FORMS:
             raise ValueError(f"cg must be one of {self.CG_FORMS}")
         dir_planes = 0
@@ -233,6 +238,11 @@ class HeatflowMPM:
         self.layout = dict(dir_planes=dir_planes, cg=cg)
         if precision not in self.PRECISIONS:
             raise ValueError(f"precision must be one of {tuple(self.PRECISIONS)}")
+        if advection not in self.ADVECTION:
+            raise ValueError(f"advection must be one of {self.ADVECTION}")
+        if precision == "mixed" and advection != "semi-lagrangian":
+            raise ValueError("advection='maccormack' is available with precision='fp32' only")
+        self.advection = advection
         if precision == "mixed" and cg != "warp":
             raise ValueError(f"the {cg} PCG form is not available with precision='mixed'")
         self.precision = precision
@@ -246,6 +256,8 @@ class HeatflowMPM:
             inp["params"]["flags"] |= _abi.FLAG_FUSED_CG
         elif cg == "mg":
             inp["params"]["flags"] |= _abi.FLAG_MG_PCG
+        if advection == "maccormack":
+            inp["params"]["flags"] |= _abi.FLAG_MACCORMACK
         inp["params"]["precision"] = self.PRECISIONS[precision]
         inp["params"]["dir_planes"] = dir_planes
         self._scene = _Scene(inp["params"], parse_device(device))

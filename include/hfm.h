@@ -94,6 +94,12 @@ typedef struct hfm_params {
  * agree with the Jacobi solve to the solve's tolerance, not bit for bit.  fp32 only, not with
  * HFM_FLAG_FUSED_CG. */
 #define HFM_FLAG_MG_PCG 4
+/* Limited MacCormack advection of T and of u, v, w (Selle et al., J. Sci. Comput. 35, 2008): the
+ * semi-Lagrangian value, corrected by half the error of tracing it back to the start, clamped to
+ * the range of the eight values around the departure point.  No counterpart in the reference's
+ * substep: it prepares the limiter (value_range, fields.py:51-65) but advects semi-Lagrangian
+ * only.  fp32 only (not with HFM_PREC_MIXED). */
+#define HFM_FLAG_MACCORMACK 8
 
 /* Per-substep statistics of the last hfm_substep. */
 typedef struct hfm_stats {

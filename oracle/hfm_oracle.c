@@ -119,6 +119,50 @@ static void scatter(real* a, const real pos[3], const real off[3], real h, int n
     }
 }
 
+static void backtrace(const real* u, const real* v, const real* w, const real x[3], real h, int n, real dt,
+                      real out[3]);
+
+/* fields.py:51-65: min and max of the eight lattice values around pos (the MacCormack limiter) */
+static void value_range(const real* a, const real pos[3], const real off[3], real h, int nx, int ny, int nz,
+                        real* lo, real* hi) {
+    int b[3];
+    real f[3];
+    base_index(pos, off, h, b, f);
+    real l = R(1.0e30), u = R(-1.0e30);
+    for (int dx = 0; dx < 2; ++dx) {
+        int ix = clampi(b[0] + dx, 0, nx - 1);
+        for (int dy = 0; dy < 2; ++dy) {
+            int iy = clampi(b[1] + dy, 0, ny - 1);
+            for (int dz = 0; dz < 2; ++dz) {
+                int iz = clampi(b[2] + dz, 0, nz - 1);
+                real v = a[lidx(ny, nz, ix, iy, iz)];
+                l = v < l ? v : l;
+                u = v > u ? v : u;
+            }
+        }
+    }
+    *lo = l;
+    *hi = u;
+}
+
+/* Limited MacCormack value at a point x of a lattice (HFM_FLAG_MACCORMACK; the reference prepares
+ * value_range, fields.py:51-65, but advects semi-Lagrangian only): with fwd the semi-Lagrangian
+ * value at x (field `fwd_field`), start the start field's value at x, d the departure point
+ * (RK2 with dt) and e the point traced forward (RK2 with -dt):
+ *   clamp(fwd + (start - fwd_field(e)) / 2, value_range(start_field, d)). */
+static real maccormack(const real* u, const real* v, const real* w, const real* start_field, const real* fwd_field,
+                       real fwd, real start, const real x[3], const real off[3], real h, int n, real dt, int nx,
+                       int ny, int nz) {
+    real d[3], e[3];
+    backtrace(u, v, w, x, h, n, dt, d);
+    backtrace(u, v, w, x, h, n, -dt, e);
+    real back = sample(fwd_field, e, off, h, nx, ny, nz);
+    real val = fwd + R(0.5) * (start - back);
+    real lo, hi;
+    value_range(start_field, d, off, h, nx, ny, nz, &lo, &hi);
+    return val < lo ? lo : (val > hi ? hi : val);
+}
+
 /* fields.py:68-72 */
 static void sample_velocity(const real* u, const real* v, const real* w, const real pos[3], real h, int n,
                             real vel[3]) {
@@ -815,6 +859,24 @@ static void st_advect_T(orc_scene* s, double dtd) {
             }
     if (s->bt_bf16)
         for (int a = 0; a < 3; ++a) free(bu[a]);
+    if (s->prm.flags & HFM_FLAG_MACCORMACK) {   /* T2 holds the semi-Lagrangian field */
+        real* mc = zalloc(s->N);
+        #pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j)
+                for (int k = 0; k < n; ++k) {
+                    long t = ((long)i * n + j) * n + k;
+                    if (s->alpha_s[t] > R(0.5)) {
+                        mc[t] = s->T2[t];
+                        continue;
+                    }
+                    real x[3] = {((real)i + R(0.5)) * h, ((real)j + R(0.5)) * h, ((real)k + R(0.5)) * h};
+                    mc[t] = maccormack(s->u[0], s->u[1], s->u[2], s->Tw, s->T2, s->T2[t], s->Tw[t], x, OFF_CELL, h, n,
+                                       dt, n, n, n);
+                }
+        memcpy(s->T2, mc, s->N * sizeof(real));
+        free(mc);
+    }
     store_round_array(s->T2, s->N, s->round_t);
 }
 
@@ -938,6 +1000,31 @@ static void st_advect_faces(orc_scene* s, double dtd) {
     }
     if (s->bt_bf16)
         for (int a = 0; a < 3; ++a) free(bu[a]);
+    if (s->prm.flags & HFM_FLAG_MACCORMACK) {   /* scr holds the semi-Lagrangian u, v, w; u the old ones */
+        real* mc[3];
+        for (int axis = 0; axis < 3; ++axis) {
+            int d[3];
+            face_dims(axis, n, d);
+            const real* off = face_offset(axis);
+            mc[axis] = zalloc(s->F[axis]);
+            #pragma omp parallel for schedule(static)
+            for (int i = 0; i < d[0]; ++i)
+                for (int j = 0; j < d[1]; ++j)
+                    for (int k = 0; k < d[2]; ++k) {
+                        long f = lidx(d[1], d[2], i, j, k);
+                        int idx = axis == 0 ? i : (axis == 1 ? j : k);
+                        if (idx == 0 || idx == n) continue;   /* wall faces stay 0 */
+                        real x[3];
+                        face_position(axis, i, j, k, h, x);
+                        mc[axis][f] = maccormack(s->u[0], s->u[1], s->u[2], s->u[axis], s->scr[axis], s->scr[axis][f],
+                                                 s->u[axis][f], x, off, h, n, dt, d[0], d[1], d[2]);
+                    }
+        }
+        for (int a = 0; a < 3; ++a) {
+            memcpy(s->scr[a], mc[a], s->F[a] * sizeof(real));
+            free(mc[a]);
+        }
+    }
     for (int a = 0; a < 3; ++a) {
         real* tmp = s->u[a];
         s->u[a] = s->scr[a];

@@ -105,7 +105,7 @@ class OracleSim:
 
     def __init__(self, particles, mat: Material = Material(), heater: Heater = Heater(),
                  grid: GridConfig = GridConfig(), spacing: float = 2.5e-3, t_init: float = 20.0,
-                 kappa_air: float = KAPPA_AIR, precision: str = "f32", velocities=None):
+                 kappa_air: float = KAPPA_AIR, precision: str = "f32", velocities=None, flags: int = 0):
         self.L = lib(precision)
         particles = np.ascontiguousarray(particles, np.float32).reshape(-1, 3)
         self.n_particles = len(particles)
@@ -115,6 +115,7 @@ class OracleSim:
         self.h = grid.h
         self.cfg = grid
         self.params = _abi.Params(**inp["params"])
+        self.params.flags |= int(flags)   # e.g. _abi.FLAG_MACCORMACK
         self.s = self.L.orc_create(C.byref(self.params))
         self.L.orc_set_fixed(self.s, np.ascontiguousarray(inp["fixed"], np.int32),
                              np.ascontiguousarray(inp["tfix"], np.float32))

@@ -241,3 +241,17 @@ def test_multigrid_argument_checks():
         assert b"HFM_FLAG_MG_PCG" in lib.hfm_last_error(), n
     with pytest.raises(ValueError, match="mg"):
         HeatflowMPM(np.zeros((0, 3), np.float32), grid=GridConfig(L=0.32, h=0.01), cg="mg", precision="mixed")
+
+
+def test_maccormack_argument_checks():
+    lib = _abi.load()
+    h = C.c_void_p()
+    p = _abi.Params(n=16, cg_max_iter=10, cg_check_every=2, heat_max_iter=10, heat_check_every=2, h=0.01,
+                    flags=_abi.FLAG_MACCORMACK, precision=_abi.PREC_MIXED)
+    assert lib.hfm_scene_create(C.byref(p), 0, C.byref(h)) == -1
+    assert b"HFM_FLAG_MACCORMACK" in lib.hfm_last_error()
+    with pytest.raises(ValueError, match="maccormack"):
+        HeatflowMPM(np.zeros((0, 3), np.float32), grid=GridConfig(L=0.16, h=0.01), advection="maccormack",
+                    precision="mixed")
+    with pytest.raises(ValueError, match="advection"):
+        HeatflowMPM(np.zeros((0, 3), np.float32), grid=GridConfig(L=0.16, h=0.01), advection="upwind")
