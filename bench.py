@@ -386,6 +386,8 @@ def main() -> int:
     ap.add_argument("--cg", choices=HeatflowMPM.CG_FORMS, default=None,
                     help="pressure PCG form: warp's two-pass Jacobi iteration (default), one fused launch per "
                          "iteration, or the multigrid-preconditioned iteration (mg; the served form's default)")
+    ap.add_argument("--advection", choices=HeatflowMPM.ADVECTION, default="semi-lagrangian",
+                    help="coupled: the reference's semi-Lagrangian step or limited MacCormack")
     ap.add_argument("--plain-cg", choices=HeatflowMPM.CG_FORMS, default="warp",
                     help="served: pressure PCG form of the plain baseline (the reference's Jacobi PCG)")
     a = ap.parse_args()
@@ -413,7 +415,8 @@ def main() -> int:
 
     prec = a.precision or "fp32"
     kw, dt, _ = coupled_scene(a.n)
-    sim = HeatflowMPM(**kw, device="cuda:0", cg=a.cg, precision=prec, layout=None if a.layout == "rule" else "auto")
+    sim = HeatflowMPM(**kw, device="cuda:0", cg=a.cg, precision=prec, layout=None if a.layout == "rule" else "auto",
+                      advection=a.advection)
     n, N, P = sim.grid.n, sim.grid.n ** 3, sim.n
     if a.warmup:
         sim.run(dt, a.warmup)
@@ -458,7 +461,8 @@ def main() -> int:
 
     cpu = None if a.no_cpu else cpu_baseline(kw, dt, state)
     end = {name: sim._scene.get_field(name) for name in ("T", "U", "V", "W")}
-    mg = None if a.no_mg or a.cg != "warp" or prec != "fp32" else bench_mg_form(a, kw, dt, end)
+    mg = (None if a.no_mg or a.cg != "warp" or prec != "fp32" or a.advection != "semi-lagrangian"
+          else bench_mg_form(a, kw, dt, end))
 
     out = dict(metric=METRIC, value=value, unit="cell-substeps/s", n_gpus=1, steps=a.steps, warmup=a.warmup,
                ms_per_step=wall * 1e3 / a.steps, higher_is_better=True, scaling="weak", vs_baseline=None,
@@ -466,7 +470,7 @@ def main() -> int:
                data="synthetic",
                config=dict(workload=f"coupled scene n={n} (h=5 mm), static ellipsoid P={P}, heater 400 W cap 120 C fan "
                                     f"2 m/s, PCG tol 1e-6, dt 2 ms", n=n, particles=P, dt=dt, cg_tol=1e-6,
-                           pressure_cg=sim.layout["cg"], precision=prec, layout=a.layout,
+                           pressure_cg=sim.layout["cg"], precision=prec, layout=a.layout, advection=a.advection,
                            dir_planes=sim.layout["dir_planes"],
                            pressure_iters_per_substep=round(I_p, 2), heat_iters_per_substep=round(I_h, 2),
                            parallelism="1 GPU"),
