@@ -474,7 +474,8 @@ def main() -> int:
                            dir_planes=sim.layout["dir_planes"],
                            pressure_iters_per_substep=round(I_p, 2), heat_iters_per_substep=round(I_h, 2),
                            parallelism="1 GPU"),
-               hbm_tb_s=round(hbm_tbs, 3), roofline=roofline, cpu_baseline=cpu, mg_form=mg)
+               hbm_tb_s=None if hbm_tbs is None else round(hbm_tbs, 3), roofline=roofline, cpu_baseline=cpu,
+               mg_form=mg)
     print(json.dumps(out), flush=True)
     sim.close()
     return 0

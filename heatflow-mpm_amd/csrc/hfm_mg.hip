@@ -205,12 +205,24 @@ __global__ __launch_bounds__(kBlock) void k_mg_coef(Src src, int l) {
 constexpr int kCells = 1;
 constexpr int kUp0Cells = 1;
 
-// s - A^ (w s) at fine cell (i, j, k) of level l (level 0: s = h^2 r, w s = omega D r)
-template <class A>
+// s - A^ (w s) at fine cell (i, j, k) of level l.  Level 0 of the fp32 PCG: s = h^2 r,
+// w s = omega D r; of the FP16 inner solve of a refinement step (IR), whose operator is already
+// A^: s = r (FP16), w s = omega (D / h^2) r.
+template <bool IR, class A>
 __device__ __forceinline__ float smoothed_residual(const A& a, int l, int i, int j, int k, int n) {
     const Stencil s = stencil(i, j, k, n);
     float c[6], e[6], ec, sc;
-    if (l == 0) {
+    if (l == 0 && IR) {
+        coefs0(a, s, c);
+        const _Float16* r = a.ir_r;
+        const float* D = a.pdiag;
+        const float rhh = a.rhh;
+#pragma unroll
+        for (int d = 0; d < 6; ++d) e[d] = kMgOmega * (float)r[s.q[d]] * (D[s.q[d]] * rhh);
+        const float rc = (float)r[s.t];
+        ec = kMgOmega * rc * (D[s.t] * rhh);
+        sc = rc;
+    } else if (l == 0) {
         coefs0(a, s, c);
         const float* r = a.cg_r;
         const float* D = a.pdiag;
@@ -232,7 +244,7 @@ __device__ __forceinline__ float smoothed_residual(const A& a, int l, int i, int
     return sc - apply6(ec, e, c);
 }
 
-template <class Src>
+template <bool IR, class Src>
 __global__ __launch_bounds__(kBlock) void k_mg_restrict(Src src, int l) {
     HFM_ENTRY(src);
     if (!a.cg_mg || l >= a.mg_tail) return;
@@ -252,7 +264,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(Src src, int l) {
         const unsigned T = base + u * kPerPass;
         int I, J, K;
         cell3(T < Mc ? T : Mc - 1, m, I, J, K);
-        tv[u] = smoothed_residual(a, l, 2 * I + di, 2 * J + dj, 2 * K + dk, n);
+        tv[u] = smoothed_residual<IR>(a, l, 2 * I + di, 2 * J + dj, 2 * K + dk, n);
     }
     float* out = a.mg_s + a.mg_off[l + 1];
 #pragma unroll
@@ -444,8 +456,10 @@ __global__ __launch_bounds__(kBlock) void k_mg_up(Src src, int l) {
 }
 
 // 1 -> 0: z = e + omega D (r - A e) with e = omega D r + e_1(parent); z and the partial sums of
-// r.z (row 1 of the PCG partials, in the PCG geometry, as pass B writes them).
-template <class Src>
+// r.z (row 1 of the PCG partials, in the PCG geometry, as pass B writes them).  IR: the FP16
+// inner solve of a refinement step (r and z FP16, D / h^2 for D, r.z of the rounded values, as
+// its pass B forms them).
+template <bool IR, class Src>
 __global__ __launch_bounds__(kBlock) void k_mg_up0(Src src) {
     HFM_ENTRY(src);
     if (!a.cg_mg) return;
@@ -453,9 +467,9 @@ __global__ __launch_bounds__(kBlock) void k_mg_up0(Src src) {
     __shared__ double lds[1][kWaves];
     const int n = a.n, m = a.mg_n[1];
     const unsigned N = (unsigned)a.N;
-    const float* r = a.cg_r;
     const float* D = a.pdiag;
     const float* P = a.mg_e + a.mg_off[1];
+    const float dsc = IR ? a.rhh : 1.0f;
     double acc[1] = {0.0};
     const int L_ = xcd_tile(lb, a.nb_upd);
     for (int u0 = 0; u0 < a.cpt; u0 += kUp0Cells) {
@@ -468,20 +482,36 @@ __global__ __launch_bounds__(kBlock) void k_mg_up0(Src src) {
             cell3(t, n, i, j, k);
             Stencil s = stencil(i, j, k, n);
             parents(s, i, j, k, n, m);
-            float c[6], e[6];
+            float c[6], e[6], rn[6];
             coefs0(a, s, c);
 #pragma unroll
-            for (int d = 0; d < 6; ++d) e[d] = kMgOmega * r[s.q[d]] * D[s.q[d]] + P[s.pq[d]];
-            const float rc = r[t], Dc = D[t];
-            const float ec = kMgOmega * rc * Dc + P[s.pc];
-            z[u] = ec + kMgOmega * Dc * a.rhh * (a.hh * rc - apply6(ec, e, c));
+            for (int d = 0; d < 6; ++d) rn[d] = IR ? (float)a.ir_r[s.q[d]] : a.cg_r[s.q[d]];
+            const float rc = IR ? (float)a.ir_r[t] : a.cg_r[t];
+            if (IR) {
+#pragma unroll
+                for (int d = 0; d < 6; ++d) e[d] = kMgOmega * rn[d] * (D[s.q[d]] * dsc) + P[s.pq[d]];
+                const float Dc = D[t] * dsc;
+                const float ec = kMgOmega * rc * Dc + P[s.pc];
+                const float zf = ec + kMgOmega * Dc * (rc - apply6(ec, e, c));
+                const _Float16 zh = (_Float16)zf;
+                z[u] = (float)zh;
+            } else {
+#pragma unroll
+                for (int d = 0; d < 6; ++d) e[d] = kMgOmega * rn[d] * D[s.q[d]] + P[s.pq[d]];
+                const float Dc = D[t];
+                const float ec = kMgOmega * rc * Dc + P[s.pc];
+                z[u] = ec + kMgOmega * Dc * a.rhh * (a.hh * rc - apply6(ec, e, c));
+            }
             rz[u] = rc * z[u];
         }
 #pragma unroll
         for (int u = 0; u < kUp0Cells; ++u) {
             const unsigned t = ((unsigned)L_ * a.cpt + u0 + u) * kBlock + threadIdx.x;
             if (u0 + u < a.cpt && t < N) {
-                a.cg_z[t] = z[u];
+                if (IR)
+                    a.ir_z[t] = (_Float16)z[u];
+                else
+                    a.cg_z[t] = z[u];
                 acc[0] += (double)rz[u];
             }
         }
@@ -498,13 +528,20 @@ void launch_mg_coefficients(const Batch* b, int levels) {
         HFM_LAUNCH(k_mg_coef, family(b, (Family)(FAM_MG0 + l + 1)), kBlock, nullptr, nullptr, l);
 }
 
-void launch_mg_vcycle(const Batch* b, int tail) {
-    for (int l = 0; l < tail; ++l)
-        HFM_LAUNCH(k_mg_restrict, family(b, (Family)(FAM_MG0 + l)), kBlock, nullptr, nullptr, l);
+void launch_mg_vcycle(const Batch* b, int tail, bool ir) {
+    for (int l = 0; l < tail; ++l) {
+        if (ir && l == 0)
+            HFM_LAUNCH_T(k_mg_restrict, true, family(b, FAM_MG0), kBlock, nullptr, nullptr, l);
+        else
+            HFM_LAUNCH_T(k_mg_restrict, false, family(b, (Family)(FAM_MG0 + l)), kBlock, nullptr, nullptr, l);
+    }
     HFM_LAUNCH(k_mg_tail, family(b, FAM_MGC), kTailBlock, nullptr, nullptr);
     for (int l = tail - 1; l >= 1; --l)
         HFM_LAUNCH(k_mg_up, family(b, (Family)(FAM_MG0 + l)), kBlock, nullptr, nullptr, l);
-    HFM_LAUNCH(k_mg_up0, family(b, FAM_PCG), kBlock, nullptr, nullptr);
+    if (ir)
+        HFM_LAUNCH_T(k_mg_up0, true, family(b, FAM_PCG), kBlock, nullptr, nullptr);
+    else
+        HFM_LAUNCH_T(k_mg_up0, false, family(b, FAM_PCG), kBlock, nullptr, nullptr);
 }
 
 }  // namespace hfm

@@ -388,7 +388,7 @@ void launch_bf16_copy(const Scene* s, hipStream_t st);
 constexpr int kMgMaxLevels = 8;
 constexpr int kMgTailMaxN = 12;   // levels of at most 12^3 cells are solved in one workgroup
 void launch_mg_coefficients(const Batch* b, int levels);
-void launch_mg_vcycle(const Batch* b, int tail);
+void launch_mg_vcycle(const Batch* b, int tail, bool ir = false);
 // fused pressure form: first-iteration scalars after launch_pcg_init, then one launch per iteration
 constexpr int kFusedRows = 5;
 void launch_pcg_init_fused(const Launch& L);

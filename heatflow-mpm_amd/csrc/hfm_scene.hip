@@ -492,7 +492,7 @@ int mg_depth(const Batch* b, bool tail) {
 // ir: the inner solves of a mixed-precision refinement step (FP16 kernels).
 int pcg_iterate(Batch* b, int op, float dt, bool ir, int maxeff, int pred) {
     const unsigned tag = ++b->solve_tag;
-    const int mg = op == 1 && !ir ? mg_depth(b, true) : 0;
+    const int mg = op == 1 ? mg_depth(b, true) : 0;
     // pass A launches: the heat operator uses the flat kernel with the PCG geometry; the
     // pressure operator one launch per tiling present in the batch, two-pass or fused
     Family dirs[8];
@@ -604,7 +604,7 @@ int pcg_iterate(Batch* b, int op, float dt, bool ir, int maxeff, int pred) {
                     hipEvent_t e0 = nullptr, e1 = nullptr;
                     if (timed) RC(timing_events(b, HFM_KT_PCG_UPD, &e0, &e1));
                     launch_pcg_upd(fl[FAM_PCG], op, k, e0, e1, ir);
-                    if (mg) launch_mg_vcycle(b, mg);
+                    if (mg) launch_mg_vcycle(b, mg, ir);
                 }
             }
             CHECK_LAUNCH();
@@ -702,8 +702,13 @@ int run_ir(Batch* b, int op, float dt) {
         pred = std::max(pred, op ? s->pres_pred : s->heat_pred);
     }
     const Launch fam = family(b, FAM_PCG);
+    // multigrid scenes: this solve's coarse levels, and z = M r after every scale step (the inner
+    // solves' init)
+    const int mg = op == 1 ? mg_depth(b, true) : 0;
+    if (mg) launch_mg_coefficients(b, mg_depth(b, false));
     launch_ir_residual(fam, op, dt, true);
     launch_ir_scale(fam, op, true);
+    if (mg) launch_mg_vcycle(b, mg, true);
     CHECK_LAUNCH();
     for (int step = 0;; ++step) {
         RC(pcg_iterate(b, op, dt, true, maxeff, pred));
@@ -724,6 +729,7 @@ int run_ir(Batch* b, int op, float dt) {
         }
         launch_ir_residual(fam, op, dt, false);
         launch_ir_scale(fam, op, false);
+        if (mg) launch_mg_vcycle(b, mg, true);
         CHECK_LAUNCH();
     }
     for (Scene* s : b->scenes) {
@@ -976,9 +982,8 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
         set_err("dir_planes %d: use 0, -1, 4, 8 or 16", prm->dir_planes);
         return HFM_ERR_ARG;
     }
-    if ((prm->flags & HFM_FLAG_MG_PCG) &&
-        (prm->precision != HFM_PREC_FP32 || (prm->flags & HFM_FLAG_FUSED_CG))) {
-        set_err("HFM_FLAG_MG_PCG is available with HFM_PREC_FP32 and the two-pass PCG only");
+    if ((prm->flags & HFM_FLAG_MG_PCG) && (prm->flags & HFM_FLAG_FUSED_CG)) {
+        set_err("HFM_FLAG_MG_PCG is available with the two-pass PCG only");
         return HFM_ERR_ARG;
     }
     if (prm->flags & HFM_FLAG_MG_PCG) {

@@ -201,8 +201,9 @@ class HeatflowMPM:
     `precision="mixed"` solves both systems by iterative refinement with FP16 inner vectors and
     traces advection through a BF16 copy of the velocity (HFM_PREC_MIXED); the state stays fp32;
     `cg="mg"` preconditions the pressure PCG with a geometric multigrid V-cycle instead of
-    Jacobi (HFM_FLAG_MG_PCG: fp32, n = m 2^k with m <= 8 and n >= 16): the same stopping rule in
-    far fewer iterations, fields equal to the Jacobi solve's to the solve tolerance, not in bits;
+    Jacobi (HFM_FLAG_MG_PCG: n = m 2^k with m <= 8 and n >= 16; with precision="mixed" it
+    preconditions the FP16 inner solves): the same stopping rule in far fewer iterations, fields
+    equal to the Jacobi solve's to the solve tolerance, not in bits;
     `layout` picks the pressure PCG layout: None (chosen by grid size), "auto" / "auto-batch"
     (the autotuned table of heatflow_mpm.autotune for a scene stepped alone / in a batch), or
     dict(dir_planes=0|-1|4|8|16, cg="warp"|"fused"); with cg="mg" only its pass-A tiling applies;
@@ -243,7 +244,7 @@ class HeatflowMPM:
         if precision == "mixed" and advection != "semi-lagrangian":
             raise ValueError("advection='maccormack' is available with precision='fp32' only")
         self.advection = advection
-        if precision == "mixed" and cg != "warp":
+        if precision == "mixed" and cg not in ("warp", "mg"):
             raise ValueError(f"the {cg} PCG form is not available with precision='mixed'")
         self.precision = precision
         self.mat, self.heater, self.dev, self.spacing, self.kappa_air = mat, heater, device, spacing, kappa_air

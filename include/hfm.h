@@ -89,10 +89,10 @@ typedef struct hfm_params {
 /* Pressure PCG preconditioned by a geometric multigrid V-cycle instead of Jacobi (no counterpart in
  * the reference): cell-centred levels halving n down to at most 8 cells per side (n = m 2^k,
  * m <= 8, n >= 16), face coefficients averaged from the fine faces, restriction by summing the 8
- * children, injection, one weighted-Jacobi sweep before and after the coarse correction, 30 on the
+ * children, injection, one weighted-Jacobi sweep before and after the coarse correction, 16 on the
  * coarsest level.  Same stopping rule and iteration as warp's CG, far fewer iterations; results
- * agree with the Jacobi solve to the solve's tolerance, not bit for bit.  fp32 only, not with
- * HFM_FLAG_FUSED_CG. */
+ * agree with the Jacobi solve to the solve's tolerance, not bit for bit.  With HFM_PREC_MIXED it
+ * preconditions the FP16 inner solves of the refinement.  Not with HFM_FLAG_FUSED_CG. */
 #define HFM_FLAG_MG_PCG 4
 /* Limited MacCormack advection of T and of u, v, w (Selle et al., J. Sci. Comput. 35, 2008): the
  * semi-Lagrangian value, corrected by half the error of tracing it back to the start, clamped to

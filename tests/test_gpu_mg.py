@@ -78,3 +78,48 @@ def test_mg_alone_equals_mg_in_a_batch(gpu):
         for sim, ref in zip(sims, alone):
             for f, v in ref.items():
                 assert np.array_equal(sim._scene.get_field(f), v), f
+
+
+def test_mg_mixed_plume_tracks_fp64(gpu):
+    """The multigrid V-cycle inside the FP16 inner solves of the refinement (precision="mixed")."""
+    kw, dt, _ = plume_scene()
+    K = 60
+    sim = HeatflowMPM(**kw, cg="mg", precision="mixed")
+    jac = HeatflowMPM(**kw, precision="mixed")
+    its = its_j = 0
+    for _ in range(K):
+        sim.substep(dt)
+        jac.substep(dt)
+        its += sim.stats()["p_iters_exact"]
+        its_j += jac.stats()["p_iters_exact"]
+    st = sim.stats()
+    f64 = O.OracleSim(**kw, precision="f64")
+    f64.run(dt, K)
+    eT = rel_l2(sim._scene.get_field("T"), f64.field("T"))
+    eu = rel_l2(_u(sim._scene.get_field), _u(f64.field))
+    dm, _ = sim._scene.diagnostics()
+    print(f"mixed + mg after {K} substeps: pressure its {its} (mixed Jacobi {its_j}), outer steps {st['p_outer']}, "
+          f"converged {st['p_converged']}, T {eT:.2e} u {eu:.2e} div {dm:.2e}", flush=True)
+    assert eT <= 1e-3 and eu <= 1e-3 and dm <= 1e-6
+    assert st["p_converged"] in (1, 2)
+    assert its * 4 <= its_j
+
+
+def test_mg_mixed_alone_equals_in_a_batch(gpu):
+    coupled48, dt, _ = coupled_scene(48)
+    kws = [dict(particles=NONE, heater=Heater(power=400.0, t_cap=120.0, fan=2.0), grid=GridConfig(L=0.32, h=0.01)),
+           coupled48]
+    alone = []
+    for kw in kws:
+        sim = HeatflowMPM(**kw, cg="mg", precision="mixed")
+        for _ in range(3):
+            sim.substep(dt)
+        alone.append({f: sim._scene.get_field(f) for f in ("T", "U", "V", "W", "P")})
+        sim.close()
+    sims = [HeatflowMPM(**kw, cg="mg", precision="mixed") for kw in kws]
+    with HeatflowBatch(sims) as batch:
+        for _ in range(3):
+            batch.substep(dt)
+        for sim, ref in zip(sims, alone):
+            for f, v in ref.items():
+                assert np.array_equal(sim._scene.get_field(f), v), f
