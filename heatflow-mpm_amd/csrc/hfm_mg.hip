@@ -458,7 +458,10 @@ __global__ __launch_bounds__(kBlock) void k_mg_up(Src src, int l) {
 // 1 -> 0: z = e + omega D (r - A e) with e = omega D r + e_1(parent); z and the partial sums of
 // r.z (row 1 of the PCG partials, in the PCG geometry, as pass B writes them).  IR: the FP16
 // inner solve of a refinement step (r and z FP16, D / h^2 for D, r.z of the rounded values, as
-// its pass B forms them).
+// its pass B forms them), with z stored as (M r) 4 / n^2: M r approximates the inverse of an
+// operator whose smallest nonzero eigenvalue is about (pi / n)^2, so it is up to ~(n / pi)^2
+// times r (r has RMS 1 in the inner solve) and would overflow FP16 from n ~ 128 on; CG is
+// invariant to a constant factor on the preconditioner (alpha grows by it, p shrinks by it).
 template <bool IR, class Src>
 __global__ __launch_bounds__(kBlock) void k_mg_up0(Src src) {
     HFM_ENTRY(src);
@@ -470,6 +473,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_up0(Src src) {
     const float* D = a.pdiag;
     const float* P = a.mg_e + a.mg_off[1];
     const float dsc = IR ? a.rhh : 1.0f;
+    const float zscale = 4.0f / ((float)n * (float)n);
     double acc[1] = {0.0};
     const int L_ = xcd_tile(lb, a.nb_upd);
     for (int u0 = 0; u0 < a.cpt; u0 += kUp0Cells) {
@@ -492,7 +496,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_up0(Src src) {
                 for (int d = 0; d < 6; ++d) e[d] = kMgOmega * rn[d] * (D[s.q[d]] * dsc) + P[s.pq[d]];
                 const float Dc = D[t] * dsc;
                 const float ec = kMgOmega * rc * Dc + P[s.pc];
-                const float zf = ec + kMgOmega * Dc * (rc - apply6(ec, e, c));
+                const float zf = (ec + kMgOmega * Dc * (rc - apply6(ec, e, c))) * zscale;
                 const _Float16 zh = (_Float16)zf;
                 z[u] = (float)zh;
             } else {
