@@ -361,6 +361,7 @@ void launch_apply_fixed(const Scene* s, hipStream_t st, float* T);
 // ---- particles (hfm_p2g.hip) ----
 void launch_bin_particles(Scene* s, hipStream_t st);
 void launch_p2g(const Launch& L);
+constexpr long kCsrMaxParticles = 1L << 27;   // scenes with more keep the general gather every substep
 void launch_csr_build(Scene* s, hipStream_t st);
 void launch_mc_T(const Launch& L, float dt);
 void launch_mc_faces(const Launch& L, float dt);

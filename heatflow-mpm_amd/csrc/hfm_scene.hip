@@ -780,7 +780,7 @@ int batch_substep(Batch* b, double dt) {
         if (s->bins_dirty) {
             launch_bin_particles(s, b->stream);
             s->csr_state = 0;
-        } else {
+        } else if (s->dev.P <= kCsrMaxParticles) {   // the list's 8 P terms are indexed by int
             s->csr_state = s->csr_state == 0 ? 1 : 2;
         }
         general |= s->csr_state < 2;
@@ -932,6 +932,8 @@ int hfm_mesh_inside(const float* tri, int64_t nt, const float* points, int64_t n
         set_err("device %d not available (%d HIP devices)", device, ndev);
         return HFM_ERR_ARG;
     }
+    int prev = 0;
+    HIP_OK(hipGetDevice(&prev));
     HIP_OK(hipSetDevice(device));
     float *d_tri = nullptr, *d_pts = nullptr, *d_wn = nullptr;
     int* d_flags = nullptr;
@@ -959,6 +961,7 @@ int hfm_mesh_inside(const float* tri, int64_t nt, const float* points, int64_t n
         fail(e, "hipMemcpy");
     for (void* q : {(void*)d_tri, (void*)d_pts, (void*)d_flags, (void*)d_wn})
         if (q) (void)hipFree(q);
+    (void)hipSetDevice(prev);   // the caller's current device is left as it was
     return rc;
 }
 
@@ -1310,8 +1313,8 @@ int hfm_set_particles(hfm_scene* h, int64_t P, const float* pos, const float* ve
         if ((rc = dalloc(o, &d.pos, 3 * P)) || (rc = dalloc(o, &d.vel, 3 * P)) || (rc = dalloc(o, &d.temp, P)) ||
             (rc = dalloc(o, &d.melt, P)) || (rc = dalloc(o, &d.yield, P)) || (rc = dalloc(o, &d.visc, P)) ||
             (rc = dalloc(o, &s->pkey, P)) || (rc = dalloc(o, &s->pslot, P)) || (rc = dalloc(o, &d.sorted, P)) ||
-            (rc = dalloc(o, &d.qpos, 3 * P)) || (rc = dalloc(o, &d.csr_p, 8 * P)) ||
-            (rc = dalloc(o, &d.csr_w, 8 * P))) {
+            (rc = dalloc(o, &d.qpos, 3 * P)) || (rc = dalloc(o, &d.csr_p, P <= kCsrMaxParticles ? 8 * P : 1)) ||
+            (rc = dalloc(o, &d.csr_w, P <= kCsrMaxParticles ? 8 * P : 1))) {
             d.P = 0;
             refresh_dev(s);
             return rc;
