@@ -780,7 +780,7 @@ int batch_substep(Batch* b, double dt) {
         if (s->bins_dirty) {
             launch_bin_particles(s, b->stream);
             s->csr_state = 0;
-        } else if (s->dev.P <= kCsrMaxParticles) {   // the list's 8 P terms are indexed by int
+        } else if (s->dev.P <= kCsrMaxParticles && 8L * s->dev.P <= s->csr_cap) {   // 8 P int-indexed terms
             s->csr_state = s->csr_state == 0 ? 1 : 2;
         }
         general |= s->csr_state < 2;
@@ -1315,11 +1315,13 @@ int hfm_set_particles(hfm_scene* h, int64_t P, const float* pos, const float* ve
             (rc = dalloc(o, &s->pkey, P)) || (rc = dalloc(o, &s->pslot, P)) || (rc = dalloc(o, &d.sorted, P)) ||
             (rc = dalloc(o, &d.qpos, 3 * P)) || (rc = dalloc(o, &d.csr_p, P <= kCsrMaxParticles ? 8 * P : 1)) ||
             (rc = dalloc(o, &d.csr_w, P <= kCsrMaxParticles ? 8 * P : 1))) {
+            s->csr_cap = 0;
             d.P = 0;
             refresh_dev(s);
             return rc;
         }
         s->P_cap = P;
+        s->csr_cap = P <= kCsrMaxParticles ? 8 * P : 1;
     }
     d.P = (int)P;
     refresh_dev(s);
