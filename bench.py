@@ -367,6 +367,9 @@ def main() -> int:
     ap.add_argument("--layout", choices=("rule", "auto"), default="auto",
                     help="pressure PCG layout: chosen by grid size, or the autotuned table (heatflow_mpm.autotune)")
     ap.add_argument("--fp64-n", type=int, default=0, help="mixed: grid size of the fp64-oracle error check (0: off)")
+    ap.add_argument("--cg-tol", type=float, default=None,
+                    help="coupled: pressure PCG tolerance (default 1e-6 of the configuration; the reference's own "
+                         "default is 1e-4)")
     ap.add_argument("--cg-max-iter", type=int, default=2000,
                     help="mixed and served: pressure PCG iteration cap (n=192 needs more than the default 400)")
     ap.add_argument("--rate", type=float, default=20.0, help="served: arrivals per second")
@@ -415,6 +418,9 @@ def main() -> int:
 
     prec = a.precision or "fp32"
     kw, dt, _ = coupled_scene(a.n)
+    if a.cg_tol is not None:
+        import dataclasses
+        kw["grid"] = dataclasses.replace(kw["grid"], cg_tol=a.cg_tol)
     sim = HeatflowMPM(**kw, device="cuda:0", cg=a.cg, precision=prec, layout=None if a.layout == "rule" else "auto",
                       advection=a.advection)
     n, N, P = sim.grid.n, sim.grid.n ** 3, sim.n
@@ -469,7 +475,8 @@ def main() -> int:
                dtype="f32" if prec == "fp32" else "f32 state, f16 PCG vectors, bf16 departure velocity",
                data="synthetic",
                config=dict(workload=f"coupled scene n={n} (h=5 mm), static ellipsoid P={P}, heater 400 W cap 120 C fan "
-                                    f"2 m/s, PCG tol 1e-6, dt 2 ms", n=n, particles=P, dt=dt, cg_tol=1e-6,
+                                    f"2 m/s, PCG tol {kw['grid'].cg_tol:g}, dt 2 ms", n=n, particles=P, dt=dt,
+                           cg_tol=kw["grid"].cg_tol,
                            pressure_cg=sim.layout["cg"], precision=prec, layout=a.layout, advection=a.advection,
                            dir_planes=sim.layout["dir_planes"],
                            pressure_iters_per_substep=round(I_p, 2), heat_iters_per_substep=round(I_h, 2),
