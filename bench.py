@@ -250,6 +250,47 @@ def bench_served(a) -> dict:
                 max_batch=max(e[3] for e in served.log))
 
 
+def bench_served_sweep(a) -> dict:
+    """The served stream at several arrival rates (the same seeded requests: the exponential gaps
+    scale with 1/rate, the scene draws do not change), served and plain at each rate."""
+    from heatflow_mpm.serve import GpuBackend
+    scg = a.cg or "mg"
+    prec = a.precision or ("mixed" if scg == "warp" else "fp32")
+    budget = int(a.hbm_gib * (1 << 30))
+    warm = [HeatflowMPM(**coupled_scene(32)[0], cg=scg) for _ in range(2)]
+    with HeatflowBatch(warm) as wb:
+        wb.run(2e-3, 2)
+    for w in warm:
+        w.close()
+    rows = []
+    for rate in a.rates:
+        reqs = poisson_requests(rate, a.requests, seed=a.seed, k_range=(a.kmin, a.kmax), cg_max_iter=a.cg_max_iter)
+        served = Server(GpuBackend(cg=scg, precision=prec, layout=None if a.layout == "rule" else "auto-batch"),
+                        hbm_budget=budget, max_active=a.slots)
+        t0 = time.perf_counter()
+        s_b = summarize(served.run(reqs), time.perf_counter() - t0)
+        plain = Server(GpuBackend(cg=a.plain_cg, layout=None if a.layout == "rule" else "auto"), hbm_budget=budget,
+                       max_active=1)
+        t0 = time.perf_counter()
+        s_p = summarize(plain.run(reqs), time.perf_counter() - t0)
+        rows.append(dict(rate_per_s=rate, served=s_b, plain=s_p,
+                         speedup=round(s_b["cell_substeps_per_s"] / s_p["cell_substeps_per_s"], 3),
+                         p99_ratio=round(s_p["p99_s"] / s_b["p99_s"], 3)))
+        print(f"rate {rate}/s: served {s_b['cell_substeps_per_s']:.3e} cell-substeps/s p50 {s_b['p50_s']:.2f} s "
+              f"p99 {s_b['p99_s']:.2f} s; plain {s_p['cell_substeps_per_s']:.3e} p50 {s_p['p50_s']:.2f} "
+              f"p99 {s_p['p99_s']:.2f}", file=sys.stderr, flush=True)
+    return dict(metric="served p50/p99 latency and aggregate cell-substeps/s against arrival rate",
+                value=min(r["speedup"] for r in rows), unit="x plain aggregate cell-substeps/s (lowest over rates)",
+                n_gpus=1, steps=None, warmup=0, ms_per_step=None, higher_is_better=True, scaling="weak",
+                vs_baseline=None, dtype="f32" if prec == "fp32" else "f32 state, f16 PCG vectors, bf16 departure velocity",
+                data="synthetic",
+                config=dict(workload=f"{a.requests} Poisson requests per rate, n in {{48,64,96,128,192}}, K uniform "
+                                     f"{a.kmin}-{a.kmax}, PCG tol 1e-6 (pressure cap {a.cg_max_iter}), dt 2 ms",
+                            rates=a.rates, seed=a.seed, pressure_cg=scg, plain_cg=a.plain_cg, precision=prec,
+                            hbm_budget_gib=a.hbm_gib, slots=a.slots, parallelism="1 GPU"),
+                sweep=rows)
+
+
 def bench_mixed(a) -> dict:
     """The coupled scene at n=192 (h = 2.5 mm) with the ellipsoid seeded to about 50k particles,
     K substeps from rest, once in fp32 and once in the mixed form (iterative refinement with FP16
@@ -373,6 +414,8 @@ def main() -> int:
     ap.add_argument("--cg-max-iter", type=int, default=2000,
                     help="mixed and served: pressure PCG iteration cap (n=192 needs more than the default 400)")
     ap.add_argument("--rate", type=float, default=20.0, help="served: arrivals per second")
+    ap.add_argument("--rates", type=float, nargs="+", default=None,
+                    help="served: sweep these arrival rates (one JSON line with a row per rate)")
     ap.add_argument("--requests", type=int, default=24, help="served: number of requests")
     ap.add_argument("--kmin", type=int, default=20, help="served: fewest substeps of a request")
     ap.add_argument("--kmax", type=int, default=100, help="served: most substeps of a request")
@@ -411,7 +454,7 @@ def main() -> int:
         print(json.dumps(bench_batched(a)), flush=True)
         return 0
     if a.workload == "served":
-        print(json.dumps(bench_served(a)), flush=True)
+        print(json.dumps(bench_served_sweep(a) if a.rates else bench_served(a)), flush=True)
         return 0
     if a.steps is None:
         a.steps = 500
