@@ -1,6 +1,7 @@
 """The reference's own acceptance tests (tests/test_solver.py:21-54) on the GPU path, at their
 full lengths and thresholds; the cold-solid test keeps its temperature/melt half with a static
-particle body (its height half needs Newton's MPM, outside this path)."""
+particle body (its height half needs Newton's MPM, outside this path).  The plume and fan tests also
+run with the multigrid-preconditioned pressure PCG and with limited MacCormack advection."""
 import math
 
 import numpy as np
@@ -32,8 +33,9 @@ def test_implicit_conduction_matches_erfc(gpu):
     assert err < 0.1
 
 
-def test_heater_plume_rises(gpu):
-    sim = HeatflowMPM(NONE, heater=Heater(power=400.0, t_cap=120.0))
+@pytest.mark.parametrize("form", [{}, {"cg": "mg"}, {"advection": "maccormack"}], ids=["plain", "mg", "maccormack"])
+def test_heater_plume_rises(gpu, form):
+    sim = HeatflowMPM(NONE, heater=Heater(power=400.0, t_cap=120.0), **form)
     run(sim, 3.0)
     T = sim.grid_T()
     hc = sim.heater_geom["centre"]
@@ -45,8 +47,9 @@ def test_heater_plume_rises(gpu):
     assert sim.grid.divergence_max() < 1e-3
 
 
-def test_fan_heats_the_material_position(gpu):
-    sim = HeatflowMPM(NONE, heater=Heater(power=400.0, t_cap=120.0, fan=2.0))
+@pytest.mark.parametrize("form", [{}, {"cg": "mg"}, {"advection": "maccormack"}], ids=["plain", "mg", "maccormack"])
+def test_fan_heats_the_material_position(gpu, form):
+    sim = HeatflowMPM(NONE, heater=Heater(power=400.0, t_cap=120.0, fan=2.0), **form)
     run(sim, 3.0)
     T = sim.grid_T()
     c = sim.grid.n // 2
