@@ -368,6 +368,34 @@ def bench_mixed(a) -> dict:
                 rel_l2_mixed_vs_fp32=vs_fp32, error_vs_fp64=vs_fp64)
 
 
+def pcg_kernel_fractions(n: int, warmup: int, steps: int) -> dict:
+    """The plain form's pressure-PCG passes on the coupled scene at a larger grid, event-timed like
+    the main roofline: per kernel the average in-dispatch time, the algorithmic GB/s and the
+    fraction of HBM peak (at 96^3 the passes are latency-bound; here they are not)."""
+    kw, dt, _ = coupled_scene(n, particles_target=50000)
+    import dataclasses
+    kw["grid"] = dataclasses.replace(kw["grid"], cg_max_iter=2000)
+    sim = HeatflowMPM(**kw, device="cuda:0", layout="auto")
+    sim.run(dt, warmup)
+    sim.synchronize()
+    sim._scene.set_kernel_timing(True)
+    sim._scene.kernel_time_reset()
+    st0 = sim.stats()
+    sim.run(dt, steps)
+    sim.synchronize()
+    iters = sim.stats()["p_iters_total"] - st0["p_iters_total"]
+    out = {}
+    for c in (_abi.KT_PCG_DIR, _abi.KT_PCG_UPD):
+        total_ms, launches = sim._scene.kernel_time(c)
+        if launches:
+            gbs = KERNEL_WORDS[c] * 4.0 * n ** 3 * iters / (total_ms * 1e-3) / 1e9
+            out[KERNEL_NAMES[c]] = dict(avg_launch_us=round(total_ms * 1e3 / launches, 2), achieved=round(gbs, 1),
+                                        unit="GB/s", frac=round(gbs / HBM_PEAK_GBS, 3))
+    sim._scene.set_kernel_timing(False)
+    sim.close()
+    return dict(n=n, substeps=steps, pressure_iters=iters, kernels=out)
+
+
 def bench_mg_form(a, kw, dt, end) -> dict:
     """The same scene and substeps with the multigrid-preconditioned pressure PCG (fp32): warmup,
     the timed substeps, then as many untimed ones again, so that its T and u stand beside the plain
@@ -427,6 +455,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=None, help="grid size (coupled: 96, mixed: 192)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-large", action="store_true",
+                    help="coupled: skip the event-timed PCG passes of the same scene at 192^3")
     ap.add_argument("--no-mg", action="store_true",
                     help="coupled: skip the multigrid-preconditioned form timed beside the plain one")
     ap.add_argument("--cg", choices=HeatflowMPM.CG_FORMS, default=None,
@@ -512,6 +542,8 @@ def main() -> int:
     end = {name: sim._scene.get_field(name) for name in ("T", "U", "V", "W")}
     mg = (None if a.no_mg or a.cg != "warp" or prec != "fp32" or a.advection != "semi-lagrangian"
           else bench_mg_form(a, kw, dt, end))
+    large = (None if a.no_large or a.cg != "warp" or prec != "fp32" or a.n != 96
+             else pcg_kernel_fractions(192, 5, 10))
 
     out = dict(metric=METRIC, value=value, unit="cell-substeps/s", n_gpus=1, steps=a.steps, warmup=a.warmup,
                ms_per_step=wall * 1e3 / a.steps, higher_is_better=True, scaling="weak", vs_baseline=None,
@@ -525,7 +557,7 @@ def main() -> int:
                            pressure_iters_per_substep=round(I_p, 2), heat_iters_per_substep=round(I_h, 2),
                            parallelism="1 GPU"),
                hbm_tb_s=None if hbm_tbs is None else round(hbm_tbs, 3), roofline=roofline, cpu_baseline=cpu,
-               mg_form=mg)
+               mg_form=mg, pcg_kernels_192=large)
     print(json.dumps(out), flush=True)
     sim.close()
     return 0
