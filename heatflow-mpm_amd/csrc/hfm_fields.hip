@@ -374,25 +374,46 @@ __global__ __launch_bounds__(kBlock) void k_rhs_mean(Src src) {
     const int* ops = prog + 2 + 2 * nleaves;
     const float* base = a.rhs + (long)c * kChunk;
     CgCtl* ctl = a.ctl + 1;
-    for (int l = threadIdx.x; l < nleaves; l += kBlock) leaves[l] = leaf_sum(base + lv[2 * l], lv[2 * l + 1]);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float stk[16];
-        int sp = 0;
-        for (int o = 0; o < nops; ++o) {
-            int op = ops[o];
-            if (op >= 0) {
-                stk[sp++] = leaves[op];
-            } else {
-                float r = stk[--sp];
-                float l = stk[--sp];
-                stk[sp++] = l + r;
+    if (!tail && a.N >= kChunk) {
+        // a full chunk: 64 leaves of 128 in order, and numpy's tree over them is balanced, so wave 0
+        // sums aligned pairs, then aligned quads, ... (fp32 addition commutes: the butterfly gives
+        // the bits of the tree's left + right at every node)
+        // the chunk is staged in LDS with coalesced loads, each 128-element leaf padded to 129
+        // floats so that the 64 lanes summing one leaf each read different banks
+        __shared__ float buf[64 * 129];
+        for (int e = threadIdx.x; e < kChunk; e += kBlock) buf[(e >> 7) * 129 + (e & 127)] = base[e];
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            float x = leaf_sum(buf + 129 * threadIdx.x, 128);
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) x += __shfl_xor(x, off);
+            if (threadIdx.x == 0) {
+                __hip_atomic_store(&a.chunk_sums[c], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                is_last = (take_ticket(&ctl->ticket2) == (unsigned)(a.nchunks - 1));
             }
         }
-        __hip_atomic_store(&a.chunk_sums[c], stk[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (take_ticket(&ctl->ticket2) == (unsigned)(a.nchunks - 1));
+        __syncthreads();
+    } else {
+        for (int l = threadIdx.x; l < nleaves; l += kBlock) leaves[l] = leaf_sum(base + lv[2 * l], lv[2 * l + 1]);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float stk[16];
+            int sp = 0;
+            for (int o = 0; o < nops; ++o) {
+                int op = ops[o];
+                if (op >= 0) {
+                    stk[sp++] = leaves[op];
+                } else {
+                    float r = stk[--sp];
+                    float l = stk[--sp];
+                    stk[sp++] = l + r;
+                }
+            }
+            __hip_atomic_store(&a.chunk_sums[c], stk[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            is_last = (take_ticket(&ctl->ticket2) == (unsigned)(a.nchunks - 1));
+        }
+        __syncthreads();
     }
-    __syncthreads();
     if (!is_last) return;
     // the chunk sums are loaded by all threads at once, then added in chunk order by one thread
     __shared__ float cs[kMaxChunksLds];

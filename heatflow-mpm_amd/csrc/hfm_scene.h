@@ -261,6 +261,7 @@ struct Scene {
     int bins_dirty;
     int csr_state;           // contribution list: 0 none, 1 built in this substep, 2 in use
     long csr_cap;            // terms the list arrays hold (8 P of the allocation, or 1)
+    int mg_coef_dirty;       // the solid fraction changed since the multigrid coefficients were formed
     unsigned version;       // bumped whenever the descriptor changes
     hfm_stats stats;
     int heat_pred, pres_pred;

@@ -487,6 +487,16 @@ int mg_depth(const Batch* b, bool tail) {
     return L;
 }
 
+// The multigrid coarse levels depend on the solid fraction only: formed again when any multigrid
+// scene of the batch has a new one (for every scene of the batch; unchanged ones get the same values).
+void launch_mg_coefficients_if_stale(Batch* b) {
+    bool stale = false;
+    for (Scene* s : b->scenes) stale |= s->dev.cg_mg && s->mg_coef_dirty;
+    if (!stale) return;
+    launch_mg_coefficients(b, mg_depth(b, false));
+    for (Scene* s : b->scenes) s->mg_coef_dirty = 0;
+}
+
 // Run PCG iterations from iteration 0 for every scene of the batch whose mirror is not done
 // (the init launches are enqueued), and return when the device reports all of them done.
 // ir: the inner solves of a mixed-precision refinement step (FP16 kernels).
@@ -653,8 +663,8 @@ int run_pcg(Batch* b, int op, float dt) {
     }
     launch_pcg_init(family(b, FAM_PCG), op, dt);
     if (any_fused) launch_pcg_init_fused(family(b, FAM_PCG));
-    if (op == 1 && mg_depth(b, false)) {   // multigrid: this solve's coarse levels, then z = M r
-        launch_mg_coefficients(b, mg_depth(b, false));
+    if (op == 1 && mg_depth(b, false)) {   // multigrid: the coarse levels (when the solid fraction changed), z = M r
+        launch_mg_coefficients_if_stale(b);
         launch_mg_vcycle(b, mg_depth(b, true));
     }
     CHECK_LAUNCH();
@@ -705,7 +715,7 @@ int run_ir(Batch* b, int op, float dt) {
     // multigrid scenes: this solve's coarse levels, and z = M r after every scale step (the inner
     // solves' init)
     const int mg = op == 1 ? mg_depth(b, true) : 0;
-    if (mg) launch_mg_coefficients(b, mg_depth(b, false));
+    if (mg) launch_mg_coefficients_if_stale(b);
     launch_ir_residual(fam, op, dt, true);
     launch_ir_scale(fam, op, true);
     if (mg) launch_mg_vcycle(b, mg, true);
@@ -780,6 +790,7 @@ int batch_substep(Batch* b, double dt) {
         if (s->bins_dirty) {
             launch_bin_particles(s, b->stream);
             s->csr_state = 0;
+            s->mg_coef_dirty = 1;   // this substep's gather forms a new solid fraction
         } else if (s->dev.P <= kCsrMaxParticles && 8L * s->dev.P <= s->csr_cap) {   // 8 P int-indexed terms
             s->csr_state = s->csr_state == 0 ? 1 : 2;
         }
@@ -878,6 +889,7 @@ int reset_particle_fields(Scene* s) {
     HIP_OK(hipMemsetAsync(d.alpha_p, 0, N * sizeof(float), s->stream));
     HIP_OK(hipMemsetAsync(d.tsum, 0, N * sizeof(float), s->stream));
     HIP_OK(hipMemcpyAsync(d.alpha_s, d.st_alpha, N * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
+    s->mg_coef_dirty = 1;
     for (int a = 0; a < 3; ++a) {
         HIP_OK(hipMemsetAsync(d.q[a], 0, F * sizeof(float), s->stream));
         HIP_OK(hipMemcpyAsync(d.kf[a], d.st_k[a], F * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
@@ -1029,6 +1041,7 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
     s->F = (long)(prm->n + 1) * prm->n * prm->n;
     s->heat_pred = 2;
     s->pres_pred = 8;
+    s->mg_coef_dirty = 1;
     {
         // minimum tiles per PCG workgroup (the workgroup cap may raise it); HFM_PCG_CPT overrides
         const char* e = getenv("HFM_PCG_CPT");
@@ -1246,6 +1259,7 @@ int hfm_set_static(hfm_scene* h, const float* alpha, const float* kx, const floa
         HIP_OK(hipMemcpy(s->dev.st_m[a], m[a], F * sizeof(float), hipMemcpyHostToDevice));
     }
     s->has_static = 1;
+    s->mg_coef_dirty = 1;
     if (s->dev.P == 0) {
         RC(reset_particle_fields(s));
         HIP_OK(hipStreamSynchronize(s->stream));
@@ -1424,6 +1438,7 @@ int hfm_set_field(hfm_scene* h, int32_t field, const float* in) {
     RC(check_finite(in, size, "field"));
     HIP_OK(hipStreamSynchronize(s->stream));
     HIP_OK(hipMemcpy(p, in, size * sizeof(float), hipMemcpyHostToDevice));
+    if (field == HFM_FIELD_ALPHA_S) s->mg_coef_dirty = 1;
     if (field == HFM_FIELD_U || field == HFM_FIELD_V || field == HFM_FIELD_W) {
         launch_bf16_copy(s, s->stream);   // mixed precision: the advection's copy of the velocity
         CHECK_LAUNCH();
