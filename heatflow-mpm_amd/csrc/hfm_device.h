@@ -55,8 +55,45 @@ __device__ __forceinline__ uint16_t to_bf16(float x) {
     return (uint16_t)(u >> 16);
 }
 
-// fields.py:25-36 — trilinear sample on a lattice of dims (nx, ny, nz), clamped indices, weights
-// from the unclamped fractions, accumulation order dx, dy, dz.
+// fields.py:5-10 on a quotient q = pos / h already formed: base_index1's operations, so a point's
+// quotient serves the lattices of every offset.
+__device__ __forceinline__ void split_index(float q, float off, int& i, float& f) {
+    const float g = q - off;
+    i = (int)floorf(g);
+    f = g - (float)i;
+}
+
+// fields.py:25-36 — trilinear sum on a lattice of dims (nx, ny, nz) from the base indices and
+// fractions of a point: clamped indices, weights from the unclamped fractions, accumulation order
+// dx, dy, dz.  Index arithmetic is 32-bit (a lattice holds at most (1024 + 1) * 1024^2 < 2^32
+// elements).
+template <class E = float>
+__device__ __forceinline__ float trilinear(const E* __restrict__ a, int bi, float fx, int bj, float fy, int bk,
+                                           float fz, int nx, int ny, int nz) {
+    float s = 0.0f;
+#pragma unroll
+    for (int dx = 0; dx < 2; ++dx) {
+        const unsigned ix = (unsigned)clampi(bi + dx, 0, nx - 1);
+        const float wx = dx ? fx : 1.0f - fx;
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy) {
+            const unsigned iy = (unsigned)clampi(bj + dy, 0, ny - 1);
+            const float wy = dy ? fy : 1.0f - fy;
+            const unsigned row = (ix * (unsigned)ny + iy) * (unsigned)nz;
+#pragma unroll
+            for (int dz = 0; dz < 2; ++dz) {
+                const unsigned iz = (unsigned)clampi(bk + dz, 0, nz - 1);
+                const float wz = dz ? fz : 1.0f - fz;
+                float w = wx * wy;
+                w = w * wz;
+                s += w * load_elem(a, (long)(row + iz));
+            }
+        }
+    }
+    return s;
+}
+
+// fields.py:25-36 — trilinear sample of a lattice with offsets (ox, oy, oz) at a point
 template <class E = float>
 __device__ __forceinline__ float sample(const E* __restrict__ a, float px, float py, float pz, float ox,
                                         float oy, float oz, float h, int nx, int ny, int nz) {
@@ -65,27 +102,7 @@ __device__ __forceinline__ float sample(const E* __restrict__ a, float px, float
     base_index1(px, ox, h, bi, fx);
     base_index1(py, oy, h, bj, fy);
     base_index1(pz, oz, h, bk, fz);
-    float s = 0.0f;
-#pragma unroll
-    for (int dx = 0; dx < 2; ++dx) {
-        int ix = clampi(bi + dx, 0, nx - 1);
-        float wx = dx ? fx : 1.0f - fx;
-#pragma unroll
-        for (int dy = 0; dy < 2; ++dy) {
-            int iy = clampi(bj + dy, 0, ny - 1);
-            float wy = dy ? fy : 1.0f - fy;
-            const E* row = a + ((long)ix * ny + iy) * nz;
-#pragma unroll
-            for (int dz = 0; dz < 2; ++dz) {
-                int iz = clampi(bk + dz, 0, nz - 1);
-                float wz = dz ? fz : 1.0f - fz;
-                float w = wx * wy;
-                w = w * wz;
-                s += w * load_elem(row, iz);
-            }
-        }
-    }
-    return s;
+    return trilinear<E>(a, bi, fx, bj, fy, bk, fz, nx, ny, nz);
 }
 
 // fields.py:51-65 — min and max of the eight lattice values around a point (the MacCormack
@@ -122,9 +139,19 @@ template <class E = float>
 __device__ __forceinline__ void sample_velocity(const E* __restrict__ u, const E* __restrict__ v,
                                                 const E* __restrict__ w, float px, float py, float pz, float h,
                                                 int n, float& vx, float& vy, float& vz) {
-    vx = sample<E>(u, px, py, pz, 0.0f, 0.5f, 0.5f, h, n + 1, n, n);
-    vy = sample<E>(v, px, py, pz, 0.5f, 0.0f, 0.5f, h, n, n + 1, n);
-    vz = sample<E>(w, px, py, pz, 0.5f, 0.5f, 0.0f, h, n, n, n + 1);
+    // the three face lattices take offsets 0 or 0.5 per axis: one quotient per axis, two splits
+    const float qx = px / h, qy = py / h, qz = pz / h;
+    int x0, x5, y0, y5, z0, z5;
+    float fx0, fx5, fy0, fy5, fz0, fz5;
+    split_index(qx, 0.0f, x0, fx0);
+    split_index(qx, 0.5f, x5, fx5);
+    split_index(qy, 0.0f, y0, fy0);
+    split_index(qy, 0.5f, y5, fy5);
+    split_index(qz, 0.0f, z0, fz0);
+    split_index(qz, 0.5f, z5, fz5);
+    vx = trilinear<E>(u, x0, fx0, y5, fy5, z5, fz5, n + 1, n, n);
+    vy = trilinear<E>(v, x5, fx5, y0, fy0, z5, fz5, n, n + 1, n);
+    vz = trilinear<E>(w, x5, fx5, y5, fy5, z0, fz0, n, n, n + 1);
 }
 
 // fields.py:111-116 — midpoint RK2 departure point (E = uint16_t: through BF16 copies of u, v, w)
