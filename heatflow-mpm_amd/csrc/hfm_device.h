@@ -63,10 +63,33 @@ __device__ __forceinline__ void split_index(float q, float off, int& i, float& f
     f = g - (float)i;
 }
 
+// The values at clamp(bk) and clamp(bk + 1) of a lattice row of nz >= 2 elements, from one load
+// of the adjacent pair at b = min(clamp(bk), nz - 2) (in the row for every bk): each index is b
+// or b + 1, so each value is the pair's low or high element.
+template <class E>
+__device__ __forceinline__ void row_pair(const E* __restrict__ row, int bk, int nz, float& v0, float& v1) {
+    const int i0 = clampi(bk, 0, nz - 1), i1 = clampi(bk + 1, 0, nz - 1);
+    const int b = i0 < nz - 2 ? i0 : nz - 2;
+    float lo, hi;
+    if constexpr (sizeof(E) == 4) {   // fp32
+        unsigned p[2];
+        __builtin_memcpy(p, row + b, 8);
+        lo = __uint_as_float(p[0]);
+        hi = __uint_as_float(p[1]);
+    } else {                          // BF16 patterns, the lower address in the low half
+        unsigned p;
+        __builtin_memcpy(&p, row + b, 4);
+        lo = __uint_as_float(p << 16);
+        hi = __uint_as_float(p & 0xffff0000u);
+    }
+    v0 = i0 == b ? lo : hi;
+    v1 = i1 == b ? lo : hi;
+}
+
 // fields.py:25-36 — trilinear sum on a lattice of dims (nx, ny, nz) from the base indices and
 // fractions of a point: clamped indices, weights from the unclamped fractions, accumulation order
 // dx, dy, dz.  Index arithmetic is 32-bit (a lattice holds at most (1024 + 1) * 1024^2 < 2^32
-// elements).
+// elements); the two dz values of a row come from one load (row_pair).
 template <class E = float>
 __device__ __forceinline__ float trilinear(const E* __restrict__ a, int bi, float fx, int bj, float fy, int bk,
                                            float fz, int nx, int ny, int nz) {
@@ -80,13 +103,14 @@ __device__ __forceinline__ float trilinear(const E* __restrict__ a, int bi, floa
             const unsigned iy = (unsigned)clampi(bj + dy, 0, ny - 1);
             const float wy = dy ? fy : 1.0f - fy;
             const unsigned row = (ix * (unsigned)ny + iy) * (unsigned)nz;
+            float v[2];
+            row_pair(a + row, bk, nz, v[0], v[1]);
 #pragma unroll
             for (int dz = 0; dz < 2; ++dz) {
-                const unsigned iz = (unsigned)clampi(bk + dz, 0, nz - 1);
                 const float wz = dz ? fz : 1.0f - fz;
                 float w = wx * wy;
                 w = w * wz;
-                s += w * load_elem(a, (long)(row + iz));
+                s += w * v[dz];
             }
         }
     }

@@ -96,7 +96,10 @@ __device__ __forceinline__ void heat_tail(const A& a, long t, int i, int j, int 
 
 // heat.py:31-41 (T advection) fused with the conductivity and heat rhs/diagonal.  With
 // HFM_FLAG_MACCORMACK this pass only stores the semi-Lagrangian value; k_mc_T finishes the cell.
-template <class Src>
+// MIXED (the batch's precision mode): departure points through the BF16 copies of u, v, w.  One
+// instantiation per mode keeps the fp32 kernel at the registers of its own path (9 waves per
+// SIMD instead of 6).
+template <bool MIXED, class Src>
 __global__ __launch_bounds__(kBlock) void k_advect_T(Src src, float dt) {
     HFM_ENTRY(src);
     const long t = (long)xcd_tile(lb, nblocks(a.N)) * kBlock + threadIdx.x;
@@ -111,7 +114,7 @@ __global__ __launch_bounds__(kBlock) void k_advect_T(Src src, float dt) {
         t2 = a.Tw[t];
     } else {
         float x = ((float)i + 0.5f) * h, y = ((float)j + 0.5f) * h, z = ((float)k + 0.5f) * h;
-        if (a.mixed)
+        if constexpr (MIXED)
             backtrace<uint16_t>(a.ub[0], a.ub[1], a.ub[2], x, y, z, h, n, dt);
         else
             backtrace<float>(a.u[0], a.u[1], a.u[2], x, y, z, h, n, dt);
@@ -216,8 +219,8 @@ __device__ __forceinline__ float face_tail(const A& a, int axis, long f, int i, 
 
 // grid.py:25-33 (face advection of u, v, w from the old fields) fused with buoyancy and
 // relaxation.  With HFM_FLAG_MACCORMACK this pass only stores the semi-Lagrangian value;
-// k_mc_faces finishes the face.
-template <class Src>
+// k_mc_faces finishes the face.  MIXED as for k_advect_T.
+template <bool MIXED, class Src>
 __global__ __launch_bounds__(kBlock) void k_air_faces(Src src, float dt) {
     HFM_ENTRY(src);
     const int axis = sub;
@@ -236,7 +239,7 @@ __global__ __launch_bounds__(kBlock) void k_air_faces(Src src, float dt) {
     float x, y, z, ox, oy, oz;
     int dx, dy, dz;
     face_geometry(axis, i, j, k, h, n, x, y, z, ox, oy, oz, dx, dy, dz);
-    if (a.mixed)
+    if constexpr (MIXED)
         backtrace<uint16_t>(a.ub[0], a.ub[1], a.ub[2], x, y, z, h, n, dt);
     else
         backtrace<float>(a.u[0], a.u[1], a.u[2], x, y, z, h, n, dt);
@@ -528,9 +531,19 @@ __global__ __launch_bounds__(kBlock) void k_apply_fixed(long N, const uint8_t* f
 }  // namespace
 
 void launch_heat_prep(const Launch& L, float dt) { HFM_LAUNCH(k_heat_prep, L, kBlock, nullptr, nullptr, dt); }
-void launch_advect_T(const Launch& L, float dt) { HFM_LAUNCH(k_advect_T, L, kBlock, nullptr, nullptr, dt); }
+void launch_advect_T(const Launch& L, float dt, bool mixed) {
+    if (mixed)
+        HFM_LAUNCH_T(k_advect_T, true, L, kBlock, nullptr, nullptr, dt);
+    else
+        HFM_LAUNCH_T(k_advect_T, false, L, kBlock, nullptr, nullptr, dt);
+}
 void launch_g2p(const Launch& L) { HFM_LAUNCH(k_g2p, L, kBlock, nullptr, nullptr); }
-void launch_air_faces(const Launch& L, float dt) { HFM_LAUNCH(k_air_faces, L, kBlock, nullptr, nullptr, dt); }
+void launch_air_faces(const Launch& L, float dt, bool mixed) {
+    if (mixed)
+        HFM_LAUNCH_T(k_air_faces, true, L, kBlock, nullptr, nullptr, dt);
+    else
+        HFM_LAUNCH_T(k_air_faces, false, L, kBlock, nullptr, nullptr, dt);
+}
 
 void launch_mc_T(const Launch& L, float dt) { HFM_LAUNCH(k_mc_T, L, kBlock, nullptr, nullptr, dt); }
 

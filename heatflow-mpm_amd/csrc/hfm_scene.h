@@ -346,9 +346,9 @@ inline void launch_scene_kernel(KB kbatch, KS ksolo, const Launch& L, int thread
 
 // ---- batched kernel launchers (hfm_fields.hip) ----
 void launch_heat_prep(const Launch& L, float dt);
-void launch_advect_T(const Launch& L, float dt);
+void launch_advect_T(const Launch& L, float dt, bool mixed);   // mixed: the batch's precision mode
 void launch_g2p(const Launch& L);
-void launch_air_faces(const Launch& L, float dt);
+void launch_air_faces(const Launch& L, float dt, bool mixed);   // mixed: the batch's precision mode
 void launch_divergence(const Launch& L, double dt);
 void launch_rhs_mean(const Launch& L);
 void launch_project(const Launch& L, double dt);

@@ -801,16 +801,16 @@ int batch_substep(Batch* b, double dt) {
     for (Scene* s : b->scenes)
         if (s->dev.P > 0 && s->csr_state == 1) launch_csr_build(s, b->stream);
     if (listed) launch_p2g_listed(family(b, FAM_CELL));
+    const bool mixed = !b->scenes.empty() && b->scenes[0]->dev.mixed != 0;  // one precision per batch
     launch_heat_prep(family(b, FAM_CELL), dt32);
-    launch_advect_T(family(b, FAM_CELL), dt32);
+    launch_advect_T(family(b, FAM_CELL), dt32, mixed);
     bool mc = false;
     for (Scene* s : b->scenes) mc |= s->dev.maccormack != 0;
     if (mc) launch_mc_T(family(b, FAM_CELL), dt32);
     CHECK_LAUNCH();
-    const bool mixed = !b->scenes.empty() && b->scenes[0]->dev.mixed != 0;  // one precision per batch
     RC(mixed ? run_ir(b, 0, dt32) : run_pcg(b, 0, dt32));
     launch_g2p(family(b, FAM_G2P));
-    launch_air_faces(family(b, FAM_FACE3), dt32);
+    launch_air_faces(family(b, FAM_FACE3), dt32, mixed);
     if (mc) launch_mc_faces(family(b, FAM_FACE3), dt32);
     launch_divergence(family(b, FAM_CELL), dt);
     launch_rhs_mean(family(b, FAM_MEAN));
