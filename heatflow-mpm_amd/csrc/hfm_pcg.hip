@@ -182,7 +182,10 @@ struct Operator<1> {
     __device__ static float inner_diag_scale(float rhh) { return rhh; }
 };
 
-// heat.py:55-105
+// heat.py:55-105.  Branch-free: a neighbour outside the grid is replaced by the cell itself, whose
+// term kf (x_c - x_c) is +0 (the combine ignores acc for a fixed cell, so the cell is free there);
+// acc starts at +0 and a sum that starts at +0 never becomes -0, so adding +0 leaves it as
+// skipping the term would, and every load of the stencil issues at once.
 template <>
 struct Operator<0> {
     template <class View, class Dir>
@@ -191,17 +194,21 @@ struct Operator<0> {
         const long nn = (long)n * n;
         const float kc = a.kap[t];
         const float xc = dir(t);
-        float acc = 0.0f;
-        const long nb[6] = {t - nn, t + nn, t - n, t + n, t - 1, t + 1};
-        const bool in[6] = {i > 0, i < n - 1, j > 0, j < n - 1, k > 0, k < n - 1};
+        const long nb[6] = {i > 0 ? t - nn : t, i < n - 1 ? t + nn : t, j > 0 ? t - n : t,
+                            j < n - 1 ? t + n : t,  k > 0 ? t - 1 : t,   k < n - 1 ? t + 1 : t};
+        float kn[6], xn[6];
+        bool fixed[6];
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
-            if (!in[s]) continue;
-            const float kf = face_kappa(kc, a.kap[nb[s]]);
-            if (a.flags[nb[s]] & FLAG_FIXED)
-                acc += kf * xc;
-            else
-                acc += kf * (xc - dir(nb[s]));
+            kn[s] = a.kap[nb[s]];
+            fixed[s] = (a.flags[nb[s]] & FLAG_FIXED) != 0;
+            xn[s] = dir(nb[s]);
+        }
+        float acc = 0.0f;
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            const float kf = face_kappa(kc, kn[s]);
+            acc += fixed[s] ? kf * xc : kf * (xc - xn[s]);
         }
         return acc;
     }
