@@ -151,7 +151,8 @@ __device__ __forceinline__ PcgViewT<typename VecOf<IR>::type> make_view(const Sc
 template <int OP>
 struct Operator;
 
-// grid.py:94-115 with alpha/beta applied by the caller
+// grid.py:94-115 with alpha/beta applied by the caller.  Branch-free as Operator<0>: a neighbour
+// outside the grid is the cell itself, whose term a_f (p_c - p_c) is +0 (a_f in [amin, 1]).
 template <>
 struct Operator<1> {
     template <class View, class Dir>
@@ -161,13 +162,18 @@ struct Operator<1> {
         const float* as = a.alpha_s;
         const float ac = as[t];
         const float pc = dir(t);
+        const long nb[6] = {i > 0 ? t - nn : t, i < n - 1 ? t + nn : t, j > 0 ? t - n : t,
+                            j < n - 1 ? t + n : t,  k > 0 ? t - 1 : t,   k < n - 1 ? t + 1 : t};
+        float an[6], pn[6];
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            an[s] = as[nb[s]];
+            pn[s] = dir(nb[s]);
+        }
         float acc = 0.0f;
-        if (i > 0) acc += face_fraction(as[t - nn], ac, a.amin) * (pc - dir(t - nn));
-        if (i < n - 1) acc += face_fraction(ac, as[t + nn], a.amin) * (pc - dir(t + nn));
-        if (j > 0) acc += face_fraction(as[t - n], ac, a.amin) * (pc - dir(t - n));
-        if (j < n - 1) acc += face_fraction(ac, as[t + n], a.amin) * (pc - dir(t + n));
-        if (k > 0) acc += face_fraction(as[t - 1], ac, a.amin) * (pc - dir(t - 1));
-        if (k < n - 1) acc += face_fraction(ac, as[t + 1], a.amin) * (pc - dir(t + 1));
+#pragma unroll
+        for (int s = 0; s < 6; ++s)   // the lower cell of a face first, as face_fraction takes them
+            acc += ((s & 1) ? face_fraction(ac, an[s], a.amin) : face_fraction(an[s], ac, a.amin)) * (pc - pn[s]);
         return acc;
     }
     // alpha * acc / h^2; the inner solve of a refinement applies A^ = h^2 A: alpha * acc
