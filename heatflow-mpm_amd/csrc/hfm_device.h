@@ -55,6 +55,17 @@ __device__ __forceinline__ uint16_t to_bf16(float x) {
     return (uint16_t)(u >> 16);
 }
 
+// (i, j, k) of flat index t of a lattice with ny x nz elements per i-slab, in 32-bit arithmetic
+// (every lattice holds at most (1024 + 1) * 1024^2 < 2^32 elements): two divisions instead of the
+// three 64-bit divisions of i = t / (ny nz), j = (t / nz) % ny, k = t % nz.
+__device__ __forceinline__ void split3(unsigned t, unsigned ny, unsigned nz, int& i, int& j, int& k) {
+    const unsigned row = t / nz;   // i ny + j
+    const unsigned a = row / ny;
+    i = (int)a;
+    j = (int)(row - a * ny);
+    k = (int)(t - row * nz);
+}
+
 // fields.py:5-10 on a quotient q = pos / h already formed: base_index1's operations, so a point's
 // quotient serves the lattices of every offset.
 __device__ __forceinline__ void split_index(float q, float off, int& i, float& f) {

@@ -21,19 +21,8 @@ namespace hfm {
 namespace {
 
 __device__ __forceinline__ void face_ijk(int axis, long f, int n, int& i, int& j, int& k) {
-    if (axis == 0) {
-        i = (int)(f / ((long)n * n));
-        j = (int)((f / n) % n);
-        k = (int)(f % n);
-    } else if (axis == 1) {
-        i = (int)(f / ((long)(n + 1) * n));
-        j = (int)((f / n) % (n + 1));
-        k = (int)(f % n);
-    } else {
-        i = (int)(f / ((long)n * (n + 1)));
-        j = (int)((f / (n + 1)) % n);
-        k = (int)(f % (n + 1));
-    }
+    const unsigned u = (unsigned)n, v = u + 1;
+    split3((unsigned)f, axis == 1 ? v : u, axis == 2 ? v : u, i, j, k);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -105,8 +94,8 @@ __global__ __launch_bounds__(kBlock) void k_advect_T(Src src, float dt) {
     const long t = (long)xcd_tile(lb, nblocks(a.N)) * kBlock + threadIdx.x;
     if (t >= a.N) return;
     const int n = a.n;
-    const long nn = (long)n * n;
-    const int i = (int)(t / nn), j = (int)((t / n) % n), k = (int)(t % n);
+    int i, j, k;
+    split3((unsigned)t, (unsigned)n, (unsigned)n, i, j, k);
     const float h = a.h;
     const float ac = a.alpha_s[t];
     float t2;
@@ -138,8 +127,8 @@ __global__ __launch_bounds__(kBlock) void k_mc_T(Src src, float dt) {
     const long t = (long)xcd_tile(lb, nblocks(a.N)) * kBlock + threadIdx.x;
     if (t >= a.N) return;
     const int n = a.n;
-    const long nn = (long)n * n;
-    const int i = (int)(t / nn), j = (int)((t / n) % n), k = (int)(t % n);
+    int i, j, k;
+    split3((unsigned)t, (unsigned)n, (unsigned)n, i, j, k);
     const float h = a.h;
     const float ac = a.alpha_s[t];
     float t2 = a.mc_T[t];
@@ -289,7 +278,8 @@ __device__ __forceinline__ void divergence_cell(const Scene_& a, const Vel& vel,
                                                 Out pdiag) {
     const int n = a.n;
     const long nn = (long)n * n;
-    const int i = (int)(t / nn), j = (int)((t / n) % n), k = (int)(t % n);
+    int i, j, k;
+    split3((unsigned)t, (unsigned)n, (unsigned)n, i, j, k);
     const auto as = a.alpha_s;
     const float amin = a.amin;
     const float ac = as[t];
@@ -482,7 +472,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_velocity(int n, long N, const f
     const long t = (long)blockIdx.x * kBlock + threadIdx.x;
     if (t >= N) return;
     const long nn = (long)n * n;
-    const int i = (int)(t / nn), j = (int)((t / n) % n), k = (int)(t % n);
+    int i, j, k;
+    split3((unsigned)t, (unsigned)n, (unsigned)n, i, j, k);
     const long fx0 = ((long)i * n + j) * n + k;
     const long fy0 = ((long)i * (n + 1) + j) * n + k;
     const long fz0 = ((long)i * n + j) * (n + 1) + k;

@@ -228,10 +228,7 @@ struct Operator<0> {
 };
 
 __device__ __forceinline__ void cell_ijk(long t, int n, int& i, int& j, int& k) {
-    const long nn = (long)n * n;
-    i = (int)(t / nn);
-    j = (int)((t / n) % n);
-    k = (int)(t % n);
+    split3((unsigned)t, (unsigned)n, (unsigned)n, i, j, k);
 }
 
 // Publish the result of a solve to pinned host memory.  Relaxed system-scope stores go straight
