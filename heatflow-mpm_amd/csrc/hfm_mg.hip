@@ -9,12 +9,14 @@
 // coefficients alone: level 0 takes the air fractions of the solid fraction (grid.py:46-59, the
 // pressure operator's own), a coarse face the average of the 4 fine faces it covers.
 //   down, level l -> l+1:  e = omega s / d (one weighted-Jacobi sweep from zero, formed where it is
-//                          read), t = s - A^ e, s_{l+1} = 0.5 * (sum of t over the 8 children)
+//                          read), t = s - A^ e, s_{l+1} = r * (sum of t over the 8 children)
 //   coarsest level:        16 weighted-Jacobi sweeps from zero
 //   up, level l+1 -> l:    e = omega s / d + e_{l+1}(parent) (injection), then one sweep
 //                          e += omega (s - A^ e) / d; at level 0 that is z, with r.z partials
-// Pre- and post-smoothing are the same symmetric sweep and injection is 2 x the transpose of the
-// restriction, so M is symmetric, as CG needs.  d = the sum of a cell's face coefficients; at
+// Pre- and post-smoothing are the same symmetric sweep and injection is 1 / r times the transpose
+// of the restriction, so M is symmetric, as CG needs, and positive definite for omega < 1 (the
+// eigenvalues of D^-1 A^ lie in [0, 2] for these diagonally dominant operators, so the sweep
+// contracts and the coarsest sweeps' polynomial (1 - (1 - omega lambda)^k) / lambda stays positive).  d = the sum of a cell's face coefficients; at
 // level 0, omega s / d = omega D r with the PCG's own Jacobi diagonal D; on the coarse levels
 // w = omega / d is stored once per solve.
 //
@@ -34,8 +36,11 @@ namespace hfm {
 
 namespace {
 
-constexpr float kMgOmega = 0.8f;
-constexpr float kMgRestrict = 0.5f;
+// omega and r measured on MI355X (coupled scene, 96^3 and 192^3): 0.8 / 0.5 took 12.8 / 20.8 PCG
+// iterations per substep, 0.9 / 0.5 10.9 / 17.6, 0.9 / 0.4 10.3 / 16.1; 0.93 and 0.95, 0.88, and
+// r = 0.35, 0.45 (at 192^3), 0.55 took more; 8 or 24 coarsest sweeps instead of 16 changed nothing.
+constexpr float kMgOmega = 0.9f;
+constexpr float kMgRestrict = 0.4f;
 constexpr int kMgCoarseSweeps = 16;
 constexpr int kTailBlock = 1024;
 constexpr int kTailFine = kMgTailMaxN * kMgTailMaxN * kMgTailMaxN;   // cells of the tail level
@@ -197,7 +202,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_coef(Src src, int l) {
 // ---------------------------------------------------------------------------------------------
 // Down-sweep l -> l + 1 (l below the tail level): one thread per fine cell (kCells passes of the
 // workgroup), 8 threads per coarse cell;
-// s_{l+1} = 0.5 * sum over the 8 children of (s - A^ (w s)).
+// s_{l+1} = r * sum over the 8 children of (s - A^ (w s)).
 // ---------------------------------------------------------------------------------------------
 // Cells per thread of the level passes and of the final up-sweep.  One is fastest on MI355X: at
 // 96^3 and 192^3, 2 or 4 cells per thread (their loads issued together) lowered the occupancy
