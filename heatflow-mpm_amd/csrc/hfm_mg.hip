@@ -18,7 +18,7 @@
 // eigenvalues of D^-1 A^ lie in [0, 2] for these diagonally dominant operators, so the sweep
 // contracts and the coarsest sweeps' polynomial (1 - (1 - omega lambda)^k) / lambda stays positive).  d = the sum of a cell's face coefficients; at
 // level 0, omega s / d = omega D r with the PCG's own Jacobi diagonal D; on the coarse levels
-// w = omega / d is stored once per solve.
+// w = omega_c / d is stored once per solve (omega_c: the coarse levels' own weight).
 //
 // Kernels.  The level-0 and large-level passes are memory-latency bound stencils: a neighbour
 // outside the grid is replaced by the cell itself, whose term c (e_c - e_c) is exactly 0, so
@@ -36,10 +36,13 @@ namespace hfm {
 
 namespace {
 
-// omega and r measured on MI355X (coupled scene, 96^3 and 192^3): 0.8 / 0.5 took 12.8 / 20.8 PCG
-// iterations per substep, 0.9 / 0.5 10.9 / 17.6, 0.9 / 0.4 10.3 / 16.1; 0.93 and 0.95, 0.88, and
-// r = 0.35, 0.45 (at 192^3), 0.55 took more; 8 or 24 coarsest sweeps instead of 16 changed nothing.
+// omega (level 0), omega on the coarse levels and r, measured on MI355X (coupled scene, PCG
+// iterations per substep at 96^3 / 192^3): 0.8, 0.8, 0.5: 12.8 / 20.8; 0.9, 0.9, 0.5: 10.9 / 17.6;
+// 0.9, 0.9, 0.4: 10.3 / 16.1; 0.9, 0.95, 0.4: 9.8 / 15.6; 0.9, 0.98, 0.4: 9.6 / 15.2.  A single omega
+// of 0.88, 0.93 or 0.95, r = 0.35, 0.45 or 0.55, coarse omega 0.85, and 8 or 12 coarsest sweeps took
+// more; 24 sweeps or a level-0 omega of 0.92 changed little.
 constexpr float kMgOmega = 0.9f;
+constexpr float kMgOmegaCoarse = 0.98f;
 constexpr float kMgRestrict = 0.4f;
 constexpr int kMgCoarseSweeps = 16;
 constexpr int kTailBlock = 1024;
@@ -196,7 +199,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_coef(Src src, int l) {
     a.mg_ay[o] = ay;
     a.mg_az[o] = az;
     const float d = ax + axp + ay + ayp + az + azp;
-    a.mg_w[o] = kMgOmega / (d > 1.0e-12f ? d : 1.0e-12f);
+    a.mg_w[o] = kMgOmegaCoarse / (d > 1.0e-12f ? d : 1.0e-12f);
 }
 
 // ---------------------------------------------------------------------------------------------
