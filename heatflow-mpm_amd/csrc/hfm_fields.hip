@@ -373,8 +373,17 @@ __global__ __launch_bounds__(kBlock) void k_rhs_mean(Src src) {
         // the bits of the tree's left + right at every node)
         // the chunk is staged in LDS with coalesced loads, each 128-element leaf padded to 129
         // floats so that the 64 lanes summing one leaf each read different banks
+        // (all 32 loads of a thread are issued before the first LDS store, so their latencies overlap)
         __shared__ float buf[64 * 129];
-        for (int e = threadIdx.x; e < kChunk; e += kBlock) buf[(e >> 7) * 129 + (e & 127)] = base[e];
+        constexpr int kPer = kChunk / kBlock;
+        float v[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) v[u] = base[threadIdx.x + u * kBlock];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int e = threadIdx.x + u * kBlock;
+            buf[(e >> 7) * 129 + (e & 127)] = v[u];
+        }
         __syncthreads();
         if (threadIdx.x < 64) {
             float x = leaf_sum(buf + 129 * threadIdx.x, 128);
