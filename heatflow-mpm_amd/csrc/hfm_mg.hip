@@ -40,7 +40,7 @@ namespace {
 // iterations per substep at 96^3 / 192^3): 0.8, 0.8, 0.5: 12.8 / 20.8; 0.9, 0.9, 0.5: 10.9 / 17.6;
 // 0.9, 0.9, 0.4: 10.3 / 16.1; 0.9, 0.95, 0.4: 9.8 / 15.6; 0.9, 0.98, 0.4: 9.6 / 15.2.  A single omega
 // of 0.88, 0.93 or 0.95, r = 0.35, 0.45 or 0.55, coarse omega 0.85, and 8 or 12 coarsest sweeps took
-// more; 24 sweeps or a level-0 omega of 0.92 changed little.
+// more; 24 sweeps, a level-0 omega of 0.92, or a separate r at level 0 (0.35-0.45) changed little.
 constexpr float kMgOmega = 0.9f;
 constexpr float kMgOmegaCoarse = 0.98f;
 constexpr float kMgRestrict = 0.4f;
