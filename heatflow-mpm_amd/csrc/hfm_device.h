@@ -44,10 +44,6 @@ __device__ __forceinline__ void base_index1(float pos, float off, float h, int& 
     f = g - (float)i;
 }
 
-// Lattice values are fp32, or BF16 bit patterns (the upper half of the fp32 pattern).
-__device__ __forceinline__ float load_elem(const float* a, long i) { return a[i]; }
-__device__ __forceinline__ float load_elem(const uint16_t* a, long i) { return __uint_as_float((unsigned)a[i] << 16); }
-
 // fp32 -> BF16 rounded to nearest even (finite values)
 __device__ __forceinline__ uint16_t to_bf16(float x) {
     unsigned u = __float_as_uint(x);
@@ -87,7 +83,8 @@ __device__ __forceinline__ void row_pair(const E* __restrict__ row, int bk, int 
         __builtin_memcpy(p, row + b, 8);
         lo = __uint_as_float(p[0]);
         hi = __uint_as_float(p[1]);
-    } else {                          // BF16 patterns, the lower address in the low half
+    } else {                          // BF16 patterns (the upper half of an fp32 pattern), the lower
+                                      // address in the low half
         unsigned p;
         __builtin_memcpy(&p, row + b, 4);
         lo = __uint_as_float(p << 16);
