@@ -18,9 +18,14 @@ def _u(get):
     return np.concatenate([np.asarray(get(f), np.float64).ravel() for f in ("U", "V", "W")])
 
 
-@pytest.mark.parametrize("scene", ["plume48", "coupled64"])
+# coupled80: levels 80, 40, 20, then the tail 10 -> 5; coupled112: levels 112, 56, 28, 14 and a
+# single-level tail of 7^3 (odd coarsest levels)
+@pytest.mark.parametrize("scene", ["plume48", "coupled64", "coupled80", "coupled112"])
 def test_mg_substeps_match_the_jacobi_oracle(gpu, scene):
-    kw, dt, _ = plume_scene() if scene == "plume48" else coupled_scene(64, particles_target=8000, seed=2)
+    if scene == "plume48":
+        kw, dt, _ = plume_scene()
+    else:
+        kw, dt, _ = coupled_scene(int(scene[7:]), particles_target=8000, seed=2)
     sim = HeatflowMPM(**kw, cg="mg")
     orc = O.OracleSim(**kw)
     for step in range(3):
