@@ -32,16 +32,33 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     return base + q;
 }
 
-__device__ __forceinline__ int clampi(int i, int lo, int hi) {
+// The index helpers below (clampi through trilinear) are also compiled for the host, so that a
+// host build can drive them with any input under the address and undefined-behaviour sanitizers.
+__host__ __device__ __forceinline__ int clampi(int i, int lo, int hi) {
     int a = i > lo ? i : lo;
     return a < hi ? a : hi;
 }
 
+// fields.py:5-10 — the lattice index i = int(floor(g)) and fraction f = g - i of g = pos / h - off.
+// The index is converted from floor(g) limited to [-2^30, 2^30] (NaN takes the lower limit) and
+// f = g - floor(g).  For |g| < 2^30 that is int(floor(g)) and g - float(i) bit for bit.  For any
+// other g it keeps the conversion defined and i + 1 free of signed overflow, so every clamped
+// lattice index stays in range, including for non-finite or huge departure points.  An unlimited
+// conversion of +inf (or of g >= 2^31) saturates to INT_MAX; the compiler forms
+// clampi(i + 1, 0, n - 1) as min(max(i, -1) + 1, n - 1), where that +1 wraps to INT_MIN and
+// survives the min: a gather 2^31 elements out of bounds.  A NaN g keeps a NaN fraction (NaN
+// weights), as in the unlimited form.
+__host__ __device__ __forceinline__ void split_floor(float g, int& i, float& f) {
+    constexpr float kLim = 1073741824.0f;   // 2^30
+    const float fl = floorf(g);
+    i = (int)fminf(fmaxf(fl, -kLim), kLim);
+    f = g - fl;
+}
+
 // fields.py:5-10 — g = pos/h - off; i = floor(g); f = g - i (true IEEE divide)
-__device__ __forceinline__ void base_index1(float pos, float off, float h, int& i, float& f) {
-    float g = pos / h - off;
-    i = (int)floorf(g);
-    f = g - (float)i;
+__host__ __device__ __forceinline__ void base_index1(float pos, float off, float h, int& i, float& f) {
+    const float g = pos / h - off;
+    split_floor(g, i, f);
 }
 
 // fp32 -> BF16 rounded to nearest even (finite values)
@@ -64,31 +81,30 @@ __device__ __forceinline__ void split3(unsigned t, unsigned ny, unsigned nz, int
 
 // fields.py:5-10 on a quotient q = pos / h already formed: base_index1's operations, so a point's
 // quotient serves the lattices of every offset.
-__device__ __forceinline__ void split_index(float q, float off, int& i, float& f) {
+__host__ __device__ __forceinline__ void split_index(float q, float off, int& i, float& f) {
     const float g = q - off;
-    i = (int)floorf(g);
-    f = g - (float)i;
+    split_floor(g, i, f);
 }
 
 // The values at clamp(bk) and clamp(bk + 1) of a lattice row of nz >= 2 elements, from one load
 // of the adjacent pair at b = min(clamp(bk), nz - 2) (in the row for every bk): each index is b
 // or b + 1, so each value is the pair's low or high element.
 template <class E>
-__device__ __forceinline__ void row_pair(const E* __restrict__ row, int bk, int nz, float& v0, float& v1) {
+__host__ __device__ __forceinline__ void row_pair(const E* __restrict__ row, int bk, int nz, float& v0, float& v1) {
     const int i0 = clampi(bk, 0, nz - 1), i1 = clampi(bk + 1, 0, nz - 1);
     const int b = i0 < nz - 2 ? i0 : nz - 2;
     float lo, hi;
     if constexpr (sizeof(E) == 4) {   // fp32
         unsigned p[2];
         __builtin_memcpy(p, row + b, 8);
-        lo = __uint_as_float(p[0]);
-        hi = __uint_as_float(p[1]);
+        lo = __builtin_bit_cast(float, p[0]);
+        hi = __builtin_bit_cast(float, p[1]);
     } else {                          // BF16 patterns (the upper half of an fp32 pattern), the lower
                                       // address in the low half
         unsigned p;
         __builtin_memcpy(&p, row + b, 4);
-        lo = __uint_as_float(p << 16);
-        hi = __uint_as_float(p & 0xffff0000u);
+        lo = __builtin_bit_cast(float, p << 16);
+        hi = __builtin_bit_cast(float, p & 0xffff0000u);
     }
     v0 = i0 == b ? lo : hi;
     v1 = i1 == b ? lo : hi;
@@ -99,7 +115,7 @@ __device__ __forceinline__ void row_pair(const E* __restrict__ row, int bk, int 
 // dx, dy, dz.  Index arithmetic is 32-bit (a lattice holds at most (1024 + 1) * 1024^2 < 2^32
 // elements); the two dz values of a row come from one load (row_pair).
 template <class E = float>
-__device__ __forceinline__ float trilinear(const E* __restrict__ a, int bi, float fx, int bj, float fy, int bk,
+__host__ __device__ __forceinline__ float trilinear(const E* __restrict__ a, int bi, float fx, int bj, float fy, int bk,
                                            float fz, int nx, int ny, int nz) {
     float s = 0.0f;
 #pragma unroll
@@ -127,7 +143,7 @@ __device__ __forceinline__ float trilinear(const E* __restrict__ a, int bi, floa
 
 // fields.py:25-36 — trilinear sample of a lattice with offsets (ox, oy, oz) at a point
 template <class E = float>
-__device__ __forceinline__ float sample(const E* __restrict__ a, float px, float py, float pz, float ox,
+__host__ __device__ __forceinline__ float sample(const E* __restrict__ a, float px, float py, float pz, float ox,
                                         float oy, float oz, float h, int nx, int ny, int nz) {
     int bi, bj, bk;
     float fx, fy, fz;
@@ -168,7 +184,7 @@ __device__ __forceinline__ void value_range(const float* __restrict__ a, float p
 
 // fields.py:68-72
 template <class E = float>
-__device__ __forceinline__ void sample_velocity(const E* __restrict__ u, const E* __restrict__ v,
+__host__ __device__ __forceinline__ void sample_velocity(const E* __restrict__ u, const E* __restrict__ v,
                                                 const E* __restrict__ w, float px, float py, float pz, float h,
                                                 int n, float& vx, float& vy, float& vz) {
     // the three face lattices take offsets 0 or 0.5 per axis: one quotient per axis, two splits
@@ -188,7 +204,7 @@ __device__ __forceinline__ void sample_velocity(const E* __restrict__ u, const E
 
 // fields.py:111-116 — midpoint RK2 departure point (E = uint16_t: through BF16 copies of u, v, w)
 template <class E = float>
-__device__ __forceinline__ void backtrace(const E* __restrict__ u, const E* __restrict__ v,
+__host__ __device__ __forceinline__ void backtrace(const E* __restrict__ u, const E* __restrict__ v,
                                           const E* __restrict__ w, float& x, float& y, float& z, float h, int n,
                                           float dt) {
     float vx, vy, vz;

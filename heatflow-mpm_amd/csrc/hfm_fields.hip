@@ -28,26 +28,37 @@ __device__ __forceinline__ void face_ijk(int axis, long f, int n, int& i, int& j
 // ---------------------------------------------------------------------------------------------
 // heat prep: T2 = T (previous temperature); Tw = blend(T); fixed; capped source
 // ---------------------------------------------------------------------------------------------
+// With a source, each workgroup also stores the fp64 sum of its cells' source rises (the new
+// minus the old Tw, exact in fp32) in src_part[lb]; the heat solve's first launch sums them.
 template <class Src>
 __global__ __launch_bounds__(kBlock) void k_heat_prep(Src src, float dt) {
     HFM_ENTRY(src);
     const long t = (long)xcd_tile(lb, nblocks(a.N)) * kBlock + threadIdx.x;
-    if (t >= a.N) return;
-    const float Tc = a.T[t];
-    a.T2[t] = Tc;
-    float tw = Tc;
-    if (a.P > 0) {
-        float al = a.alpha_p[t];
-        if (al > 0.05f) tw = a.tsum[t] / al;
+    double rise[1] = {0.0};
+    if (t < a.N) {
+        const float Tc = a.T[t];
+        a.T2[t] = Tc;
+        float tw = Tc;
+        if (a.P > 0) {
+            float al = a.alpha_p[t];
+            if (al > 0.05f) tw = a.tsum[t] / al;
+        }
+        const uint8_t fl = a.flags[t];
+        if (fl & FLAG_FIXED) tw = a.tfix[t];
+        if (a.rate_on && (fl & FLAG_SOURCE)) {
+            float up = tw + a.rate * dt;
+            float lo = tw > a.t_cap ? tw : a.t_cap;
+            const float tw0 = tw;
+            tw = up < lo ? up : lo;
+            rise[0] = (double)(tw - tw0);
+        }
+        a.Tw[t] = tw;
     }
-    const uint8_t fl = a.flags[t];
-    if (fl & FLAG_FIXED) tw = a.tfix[t];
-    if (a.rate_on && (fl & FLAG_SOURCE)) {
-        float up = tw + a.rate * dt;
-        float lo = tw > a.t_cap ? tw : a.t_cap;
-        tw = up < lo ? up : lo;
+    if (a.rate_on) {   // uniform over the workgroup: every thread reaches the reduction's barrier
+        __shared__ double lds[1][kWaves];
+        block_sum<1>(rise, lds);
+        if (threadIdx.x == 0) a.src_part[lb] = rise[0];
     }
-    a.Tw[t] = tw;
 }
 
 // ---------------------------------------------------------------------------------------------

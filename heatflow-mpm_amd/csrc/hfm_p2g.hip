@@ -23,7 +23,12 @@ namespace hfm {
 
 namespace {
 
-__device__ __forceinline__ int bin_of(float x, float h, int n) { return clampi((int)floorf(x / h), 0, n - 1); }
+__device__ __forceinline__ int bin_of(float x, float h, int n) {
+    int i;
+    float f;
+    split_floor(x / h, i, f);
+    return clampi(i, 0, n - 1);
+}
 
 __global__ __launch_bounds__(kBlock) void k_bin_count(long P, int n, float h, const float* pos, int* cnt, int* key,
                                                       int* slot) {
@@ -143,11 +148,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_pack(long P, float h, const floa
 // gather: sub-range 0 = cell lattice (alpha_s, alpha_p, tsum), 1..3 = face lattices (q, K, m)
 // ---------------------------------------------------------------------------------------------
 // fields.py:5-10 with q = pos / h already formed: g = q - off; i = floor(g); f = g - i
-__device__ __forceinline__ void base_index_q(float q, float off, int& i, float& f) {
-    const float g = q - off;
-    i = (int)floorf(g);
-    f = g - (float)i;
-}
+__device__ __forceinline__ void base_index_q(float q, float off, int& i, float& f) { split_index(q, off, i, f); }
 
 // Weight with which the particle at (qx,qy,qz) = pos / h reaches node (ni,nj,nk) of a lattice
 // with offsets (ox,oy,oz) and dims (dx,dy,dz): the sum over the corners whose clamped index is
@@ -328,7 +329,37 @@ __global__ __launch_bounds__(kBlock) void k_p2g_listed(Src src) {
     a.tsum[node] = acc1;
 }
 
+// Debug view of the binning (hfm_get_particle_bins): for the particle at bin-order slot s, its bin
+// and, from the stored pos / h exactly as the gather reads it, the base index and fraction on the
+// cell, x-, y- and z-face lattices (fields.py:5-10, offsets of fields.py:84-90), written at the
+// particle's own index.
+__global__ __launch_bounds__(kBlock) void k_particle_bins(long P, const int* sorted, const float* qpos, const int* key,
+                                                          int* bin, int* base, float* frac) {
+    const long s = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= P) return;
+    const long p = sorted[s];
+    bin[p] = key[p];
+    const float off[4][3] = {{0.5f, 0.5f, 0.5f}, {0.0f, 0.5f, 0.5f}, {0.5f, 0.0f, 0.5f}, {0.5f, 0.5f, 0.0f}};
+#pragma unroll
+    for (int l = 0; l < 4; ++l)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            int i;
+            float f;
+            split_index(qpos[3 * s + d], off[l][d], i, f);
+            base[12 * p + 3 * l + d] = i;
+            frac[12 * p + 3 * l + d] = f;
+        }
+}
+
 }  // namespace
+
+void launch_particle_bins(const Scene* s, hipStream_t st, int* bin, int* base, float* frac) {
+    const long P = s->dev.P;
+    if (P > 0)
+        hipLaunchKernelGGL(k_particle_bins, dim3(nblocks(P)), dim3(kBlock), 0, st, P, s->dev.sorted, s->dev.qpos,
+                           s->pkey, bin, base, frac);
+}
 
 void launch_bin_particles(Scene* s, hipStream_t st) {
     const long N = s->N, P = s->dev.P;

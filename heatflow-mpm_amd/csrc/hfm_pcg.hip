@@ -8,6 +8,14 @@
 // beta are zero, so x and r stop changing: the kernels here stop at that iteration (ctl.done)
 // and every later launch returns at once.
 //
+// Breakdown stop (no counterpart in warp, which runs to its cap whatever the scalars hold): the
+// fp32 solve also stops, with converged = -1, where an iteration would take a step that is not a
+// CG step: p.Ap not positive or alpha not finite (x and r then keep the last iterate), r.z of the
+// previous iteration not positive or beta not finite, or a non-finite ||r||^2.  A converging solve
+// never meets these conditions, so its arithmetic and bits are unchanged; a solve stalled below its
+// floating-point floor can (tiny p.Ap of either sign), and a step with such an alpha would fill x
+// with huge or non-finite values.
+//
 // One iteration is two fused passes over the cells:
 //   pass A (dir): p = z + beta p_old at the cell and its six neighbours, Ap = A p, sum p.Ap
 //   pass B (upd): x += alpha p, r -= alpha Ap, z = D r, sums r.r and r.z
@@ -309,7 +317,7 @@ __device__ __forceinline__ int reduce_bcast(const double (&acc)[NV], int nv, int
 // scalars written by earlier launches.
 struct PassAIn {
     double acc[3];
-    int done, maxeff, total;
+    int done, brk, maxeff, total;
     float rz_prev, atol_sq, gl;
     double atol, tol;
 };
@@ -322,6 +330,7 @@ __device__ __forceinline__ void pass_a_load(const View& a, PassAIn& p) {
     const bool first = a.k == 0 && !IR;
     load_rows<3>(a.part_upd, a.nb_upd, first ? 3 : 2, p.acc);
     p.done = c->done;
+    p.brk = c->brk;
     p.maxeff = c->maxeff;
     p.rz_prev = c->slot_rz[(a.k + 1) & 1];
     p.atol_sq = c->atol_sq;
@@ -361,10 +370,20 @@ __device__ bool finish_iteration(const View& a, const PassAIn& p, float& beta) {
         atol_sq = (float)(atol * atol);
     }
     const bool conv = rr <= atol_sq;
-    const bool stop = conv || k >= p.maxeff;
     // beta of iteration k (warp: rz_new / rz_old while rr > atol^2); the inner FP16 solve of a
-    // refinement also takes beta = 0 on a breakdown (r.z of the previous iteration not positive)
-    beta = (k == 0) ? 0.0f : ((rr > atol_sq && (!IR || p.rz_prev > 0.0f)) ? rz / p.rz_prev : 0.0f);
+    // refinement takes beta = 0 on a breakdown (r.z of the previous iteration not positive), the
+    // fp32 solve stops there (and on pass B's alpha breakdown, or a non-finite ||r||^2)
+    bool brk = false;
+    if (k == 0 || !(rr > atol_sq)) {
+        beta = 0.0f;
+    } else if (IR) {
+        beta = p.rz_prev > 0.0f ? rz / p.rz_prev : 0.0f;
+    } else {
+        beta = rz / p.rz_prev;
+        brk = !(p.rz_prev > 0.0f) || !(fabsf(beta) <= 3.4028235e38f);
+    }
+    if (!IR) brk = brk || p.brk || !(rr <= 3.4028235e38f);
+    const bool stop = conv || k >= p.maxeff || brk;
     if (a.lb == 0 && threadIdx.x == 0) {
         c->slot_rr[k & 1] = rr;
         c->slot_rz[k & 1] = rz;
@@ -374,7 +393,7 @@ __device__ bool finish_iteration(const View& a, const PassAIn& p, float& beta) {
         }
         if (stop) {
             c->done = 1;
-            c->converged = conv;
+            c->converged = conv ? 1 : (brk ? -1 : 0);
             c->it = k;
             if (IR) {
                 c->ir_total = p.total + k;
@@ -382,17 +401,33 @@ __device__ bool finish_iteration(const View& a, const PassAIn& p, float& beta) {
                 wait_stores();
                 __hip_atomic_store(&a.mirror->done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             } else {
-                mirror_final(a.mirror, k, rr, atol, conv);
+                mirror_final(a.mirror, k, rr, atol, conv ? 1 : (brk ? -1 : 0));
             }
         }
     }
     return stop;
 }
 
+// The heater energy balance of the substep: workgroup 0 of the heat solve's first launch sums the
+// heat prep's per-workgroup source rises in a fixed order (fp64) into ctl[0].src_dt and the host
+// mirror.
+template <class A>
+__device__ __forceinline__ void sum_source_rises(const A& a, int lb) {
+    if (lb != 0 || !a.rate_on) return;
+    double acc[1], tot[1];
+    load_rows<1>(a.src_part, nblocks(a.N), 1, acc);
+    reduce_bcast<1, kBlock>(acc, 1, 0, tot);
+    if (threadIdx.x == 0) {
+        a.ctl[0].src_dt = tot[0];
+        __hip_atomic_store(&a.mirror[0].src_dt, tot[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 // r = b - A x ; z = D r ; partial sums r.r, r.z, b.b
 template <int OP, class Src>
 __global__ __launch_bounds__(kBlock) void k_pcg_init(Src src, float dt) {
     HFM_ENTRY(src);
+    if (OP == 0) sum_source_rises(a, lb);
     __shared__ double lds[3][kWaves];
     const PcgView v = make_view<OP>(a, lb, 0, dt);
     double s[3] = {0.0, 0.0, 0.0};
@@ -422,6 +457,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(Src src, float dt) {
     store_partials<3>(s, v.part_upd, v.nb_upd, lb, lds);
     if (lb == 0 && threadIdx.x == 0) {  // a new solve: clear the previous one's stop
         v.ctl->done = 0;
+        v.ctl->brk = 0;
         v.ctl->converged = 0;
         v.ctl->it = 0;
     }
@@ -608,9 +644,22 @@ __global__ __launch_bounds__(kBlock) void k_pcg_upd(Src src, int kit) {
     double tot[1];
     if (reduce_bcast<1, kBlock>(acc, 1, done, tot)) return;
     const float pAp = (float)tot[0];
-    // the inner FP16 solve of a refinement takes alpha = 0 on a breakdown (p.Ap not positive, or not
-    // a number): x and r stay, the solve stops at its cap and the refinement step takes over
-    const float alpha = (rr > atol_sq && (!IR || pAp > 0.0f)) ? rz / pAp : 0.0f;
+    // alpha = 0 on a breakdown (p.Ap not positive or not a number; fp32 also alpha not finite): x and
+    // r stay.  The inner FP16 solve of a refinement then runs to its cap and the refinement step
+    // takes over; the fp32 solve is stopped by the next pass A (brk, written by workgroup 0 and read
+    // by later launches only).
+    float alpha = 0.0f;
+    if (rr > atol_sq) {
+        if (IR) {
+            alpha = pAp > 0.0f ? rz / pAp : 0.0f;
+        } else {
+            alpha = rz / pAp;
+            if (!(pAp > 0.0f) || !(fabsf(alpha) <= 3.4028235e38f)) {
+                alpha = 0.0f;
+                if (lb == 0 && threadIdx.x == 0) v.ctl->brk = 1;
+            }
+        }
+    }
     const float xa = IR ? step * alpha : alpha;
     double s[2] = {0.0, 0.0};
     auto update = [&](unsigned t, float x0, float d0, float r0, float a0, float D0) {
@@ -653,6 +702,7 @@ __global__ __launch_bounds__(kBlock) void k_ir_residual(Src src, float dt, int f
     HFM_ENTRY(src);
     CgCtl* c = a.ctl + OP;
     if (!first && c->ir_end) return;  // set by an earlier launch: this scene's solve has ended
+    if (OP == 0 && first) sum_source_rises(a, lb);
     __shared__ double lds[2][kWaves];
     const PcgView v = make_view<OP>(a, lb, 0, dt);
     double s[2] = {0.0, 0.0};
@@ -679,6 +729,7 @@ __global__ __launch_bounds__(kBlock) void k_ir_residual(Src src, float dt, int f
     store_partials<2>(s, a.partials + kIrPart, v.nb_upd, lb, lds);
     if (first && lb == 0 && threadIdx.x == 0) {
         c->done = 0;
+        c->brk = 0;
         c->converged = 0;
         c->it = 0;
         c->ir_total = 0;
@@ -875,10 +926,15 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_fused(Src src, int kit, unsigne
         atol_sq = (float)(atol * atol);
     }
     const bool conv = rr <= atol_sq;
-    const bool stop = conv || kit >= maxeff;
     const float beta = first ? 0.0f : (rr > atol_sq ? rz / rz_prev : 0.0f);
     const double pap = first ? zaz : zaz + 2.0 * (double)beta * zap + (double)beta * (double)beta * pap_prev;
     const float alpha = rr > atol_sq ? rz / (float)pap : 0.0f;
+    // breakdown stop, as the two-pass form: r.z of the previous iteration or p.Ap not positive,
+    // beta or alpha not finite, or a non-finite ||r||^2
+    const bool brk = !(rr <= 3.4028235e38f) ||
+                     (rr > atol_sq && ((!first && !(rz_prev > 0.0f)) || !(fabsf(beta) <= 3.4028235e38f) ||
+                                       !((float)pap > 0.0f) || !(fabsf(alpha) <= 3.4028235e38f)));
+    const bool stop = conv || kit >= maxeff || brk;
     if (lb == 0 && threadIdx.x == 0) {
         c->slot_rr[kit & 1] = rr;
         c->slot_rz[kit & 1] = rz;
@@ -889,9 +945,9 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_fused(Src src, int kit, unsigne
         }
         if (stop) {
             c->done = 1;
-            c->converged = conv;
+            c->converged = conv ? 1 : (brk ? -1 : 0);
             c->it = kit;
-            mirror_final(a.mirror + 1, kit, rr, atol, conv);
+            mirror_final(a.mirror + 1, kit, rr, atol, conv ? 1 : (brk ? -1 : 0));
         }
     }
     if (stop) return;

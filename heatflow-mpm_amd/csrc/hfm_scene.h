@@ -27,7 +27,9 @@ struct CgCtl {
     float atol_sq;     // fp32 threshold on ||r||^2
     int it;            // iterations done when the solve stopped
     int done;          // 1: converged or iteration cap reached; every later CG kernel skips the scene
-    int converged;
+    int converged;     // 1 converged, 0 cap reached, -1 stopped on a breakdown (see hfm_stats)
+    int brk;           // set by pass B when p.Ap is not positive or alpha not finite; the next pass A
+                       // stops the solve there
     int maxeff;        // iteration cap rounded up to the 2-iteration cycle
     int geom_dirty;    // ctl[0] only: particles were re-binned, the next gather rebuilds every lattice
     int csr_ready;     // ctl[0] only: the cell lattice's contribution list matches the binned particles
@@ -43,6 +45,7 @@ struct CgCtl {
     int ir_end;        // 1: the refinement has converged or reached the cap
     int ir_maxiter;    // iteration cap (inner iterations over the whole solve)
     double ir_rr;      // ||r||^2 (true residual) at the previous refinement step
+    double src_dt;     // ctl[0]: sum of the source step's temperature rises of this substep (fp64)
 };
 
 // Pinned host memory written by the device with system-scope stores (results of a solve),
@@ -57,6 +60,7 @@ struct CgMirror {
     double mean;
     int end;           // mixed precision: 1 when `done` ends the whole solve, 0 when one inner solve
     int pad;
+    double src_dt;     // [0]: CgCtl::src_dt of this substep (the heater energy balance)
 };
 
 // Pointer members of the per-scene descriptor.  The kernels read the descriptor from device
@@ -100,6 +104,7 @@ struct SceneView {
     ptr<double> partials;    // two-pass: [kMaxDirBlocks] pass A, then [3][nb_upd] pass B / init;
                              // fused: two parity buffers of [kFusedRows][kMaxDirBlocks]
     ptr<CgCtl> ctl;          // [0] heat, [1] pressure
+    ptr<double> src_part;    // [nblocks(N)] per-workgroup sums of the source step's rises (heat prep)
     ptr<CgMirror> mirror;    // device view of the pinned mirrors, [0] heat, [1] pressure
     int cpt, nb_upd, nb_dir, tiles_j, tiles_k, dtile;
     int cg_fused;            // pressure PCG in one launch per iteration (tiled scenes only)
@@ -362,6 +367,7 @@ void launch_apply_fixed(const Scene* s, hipStream_t st, float* T);
 
 // ---- particles (hfm_p2g.hip) ----
 void launch_bin_particles(Scene* s, hipStream_t st);
+void launch_particle_bins(const Scene* s, hipStream_t st, int* bin, int* base, float* frac);
 void launch_p2g(const Launch& L);
 constexpr long kCsrMaxParticles = 1L << 27;   // scenes with more keep the general gather every substep
 void launch_csr_build(Scene* s, hipStream_t st);

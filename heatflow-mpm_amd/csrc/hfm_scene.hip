@@ -638,7 +638,11 @@ int pcg_iterate(Batch* b, int op, float dt, bool ir, int maxeff, int pred) {
             }
             hipError_t e = hipStreamQuery(b->stream);
             if (e != hipSuccess && e != hipErrorNotReady) {
-                set_err("stream error during PCG: %s", hipGetErrorString(e));
+                set_err("HIP error seen while waiting for the %s PCG (substep %lld of the %s, iteration %d "
+                        "reached): %s.  The stream reports the first fault of any kernel enqueued before this "
+                        "poll, so the faulting kernel may be any earlier kernel of this substep, not the PCG",
+                        op ? "pressure" : "heat", (long long)b->scenes[0]->stats.substeps + 1,
+                        ns > 1 ? "batch" : "scene", prog, hipGetErrorString(e));
                 return HFM_ERR_HIP;
             }
             std::this_thread::yield();
@@ -669,10 +673,14 @@ int run_pcg(Batch* b, int op, float dt) {
     }
     CHECK_LAUNCH();
     RC(pcg_iterate(b, op, dt, false, maxeff, pred));
-    for (Scene* s : b->scenes) {
+    int bad = -1;
+    for (size_t si = 0; si < b->scenes.size(); ++si) {
+        Scene* s = b->scenes[si];
         CgMirror* m = &s->mirror[op];
         const int it = __atomic_load_n(&m->it, __ATOMIC_ACQUIRE);
         const int conv = __atomic_load_n(&m->converged, __ATOMIC_ACQUIRE);
+        // a breakdown stop keeps the last iterate; a non-finite residual means the state already is
+        if (conv < 0 && !(m->rr <= 1.0e300) && bad < 0) bad = (int)si;
         const int maxiter = op ? s->prm.cg_max_iter : s->prm.heat_max_iter;
         const int ce = op ? s->prm.cg_check_every : s->prm.heat_check_every;
         const int reported = it == 0 && conv ? 0 : warp_reported_count(it, conv, maxiter, ce);
@@ -687,6 +695,7 @@ int run_pcg(Batch* b, int op, float dt) {
             s->stats.rhs_mean = m->mean;
         } else {
             s->heat_pred = it > 2 ? it : 2;
+            s->stats.source_dT_sum = s->dev.rate_on ? m->src_dt : 0.0;
             s->stats.h_iters_total += it;
             s->stats.h_iters_exact = it;
             s->stats.h_iters = reported;
@@ -694,6 +703,11 @@ int run_pcg(Batch* b, int op, float dt) {
             s->stats.h_rr = m->rr;
             s->stats.h_atol = m->atol;
         }
+    }
+    if (bad >= 0) {
+        set_err("%s PCG of %s %d stopped on a non-finite residual: the scene's state is not finite",
+                op ? "pressure" : "heat", b->scenes.size() > 1 ? "batch member" : "scene", bad);
+        return HFM_ERR_NONFINITE;
     }
     return HFM_OK;
 }
@@ -764,6 +778,7 @@ int run_ir(Batch* b, int op, float dt) {
             s->stats.rhs_mean = m->mean;
         } else {
             s->heat_pred = inner > 2 ? inner : 2;
+            s->stats.source_dT_sum = s->dev.rate_on ? m->src_dt : 0.0;
             s->stats.h_iters_total += it;
             s->stats.h_iters_exact = it;
             s->stats.h_iters = it;
@@ -1104,6 +1119,7 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
     A(d.cg_d, 2 * N);
     A(d.cg_Ap, N);
     A(d.partials, std::max(kMaxDirBlocks + 5 * kMaxPcgBlocks, 2 * kFusedRows * kMaxDirBlocks));
+    A(d.src_part, nblocks(N));
     if (prm->flags & HFM_FLAG_MG_PCG) {   // multigrid levels: halve n down to at most 8 cells per side
         d.cg_mg = 1;
         int L = 1;
@@ -1472,6 +1488,39 @@ int hfm_get_particles(hfm_scene* h, float* temp, float* melt, float* yield_stres
     if (melt) HIP_OK(hipMemcpy(melt, s->dev.melt, bytes, hipMemcpyDeviceToHost));
     if (yield_stress) HIP_OK(hipMemcpy(yield_stress, s->dev.yield, bytes, hipMemcpyDeviceToHost));
     if (viscosity) HIP_OK(hipMemcpy(viscosity, s->dev.visc, bytes, hipMemcpyDeviceToHost));
+    return HFM_OK;
+}
+
+int hfm_get_particle_bins(hfm_scene* h, int32_t* bin, int32_t* base, float* frac) {
+    SCENE_OR_FAIL(h);
+    if (!bin || !base || !frac) {
+        set_err("null array");
+        return HFM_ERR_ARG;
+    }
+    const long P = s->dev.P;
+    if (P == 0) return HFM_OK;
+    if (s->bins_dirty) {   // bin now, exactly as the next substep would (its state kept in step)
+        launch_bin_particles(s, s->stream);
+        s->csr_state = 0;
+        s->mg_coef_dirty = 1;
+    }
+    int *d_bin = nullptr, *d_base = nullptr;
+    float* d_frac = nullptr;
+    hipError_t e = hipMalloc(&d_bin, P * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&d_base, 12 * P * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&d_frac, 12 * P * sizeof(float));
+    if (e == hipSuccess) {
+        launch_particle_bins(s, s->stream, d_bin, d_base, d_frac);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e == hipSuccess) e = hipMemcpy(bin, d_bin, P * sizeof(int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(base, d_base, 12 * P * sizeof(int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(frac, d_frac, 12 * P * sizeof(float), hipMemcpyDeviceToHost);
+    (void)hipFree(d_bin);
+    (void)hipFree(d_base);
+    (void)hipFree(d_frac);
+    HIP_OK(e);
     return HFM_OK;
 }
 

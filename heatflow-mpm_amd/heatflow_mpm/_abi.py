@@ -49,7 +49,8 @@ class Stats(C.Structure):
                 ("h_iters_exact", C.c_int32), ("p_converged", C.c_int32), ("h_converged", C.c_int32),
                 ("substeps", C.c_int64), ("p_rr", C.c_double), ("p_atol", C.c_double), ("h_rr", C.c_double),
                 ("h_atol", C.c_double), ("rhs_mean", C.c_double), ("p_iters_total", C.c_int64),
-                ("h_iters_total", C.c_int64), ("p_outer", C.c_int32), ("h_outer", C.c_int32)]
+                ("h_iters_total", C.c_int64), ("p_outer", C.c_int32), ("h_outer", C.c_int32),
+                ("source_dT_sum", C.c_double)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -75,6 +76,7 @@ SIGNATURES = {
     "hfm_field_size": (C.c_int64, [_VP, C.c_int32]),
     "hfm_get_cell_velocity": (C.c_int, [_VP, _F32P]),
     "hfm_get_particles": (C.c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "hfm_get_particle_bins": (C.c_int, [_VP, _I32P, _I32P, _F32P]),
     "hfm_diagnostics": (C.c_int, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "hfm_get_stats": (C.c_int, [_VP, C.POINTER(Stats)]),
     "hfm_kernel_time": (C.c_int, [_VP, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),

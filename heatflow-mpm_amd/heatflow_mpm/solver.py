@@ -164,6 +164,17 @@ class _Scene:
             _abi.check(self.lib.hfm_get_particles(self.handle, *[_abi.ptr(o) for o in outs]), "hfm_get_particles")
         return outs
 
+    def particle_bins(self):
+        """(bin (P,), base (P, 4, 3) int32, frac (P, 4, 3) float32) as the device forms them (hfm.h:
+        hfm_get_particle_bins); lattices in the order cells, x, y, z faces."""
+        P = max(self.n_particles, 0)
+        b = np.zeros(P, np.int32)
+        base = np.zeros((P, 4, 3), np.int32)
+        frac = np.zeros((P, 4, 3), np.float32)
+        if P:
+            _abi.check(self.lib.hfm_get_particle_bins(self.handle, b, base, frac), "hfm_get_particle_bins")
+        return b, base, frac
+
     def diagnostics(self):
         dm, sm = C.c_double(0.0), C.c_double(0.0)
         _abi.check(self.lib.hfm_diagnostics(self.handle, C.byref(dm), C.byref(sm)), "hfm_diagnostics")

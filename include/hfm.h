@@ -110,8 +110,12 @@ typedef struct hfm_stats {
     int32_t p_iters_exact;    /* iterations that changed the pressure */
     int32_t h_iters;          /* heat PCG count as warp returns it: HeatField.cg_iters */
     int32_t h_iters_exact;
-    int32_t p_converged;      /* 1 when ||r||^2 <= atol^2 at exit; HFM_PREC_MIXED also 2: stopped when a
-                                 refinement step no longer cut the true residual (its fp32 floor) */
+    int32_t p_converged;      /* 1 when ||r||^2 <= atol^2 at exit, 0 at the iteration cap; HFM_PREC_MIXED also
+                                 2: stopped when a refinement step no longer cut the true residual (its
+                                 fp32 floor).  -1: stopped on a breakdown (no counterpart in warp, which
+                                 runs on to its cap): p.Ap or r.z not positive, alpha or beta not finite,
+                                 or a non-finite ||r||^2.  x keeps its last iterate; if ||r||^2 is not
+                                 finite the substep returns HFM_ERR_NONFINITE */
     int32_t h_converged;
     int64_t substeps;         /* substeps run since creation */
     double p_rr;              /* final ||r||^2 of the pressure solve */
@@ -123,6 +127,12 @@ typedef struct hfm_stats {
     int64_t h_iters_total;    /* sum of h_iters_exact over all substeps */
     int32_t p_outer;          /* HFM_PREC_MIXED: refinement steps of the pressure solve (0 in fp32) */
     int32_t h_outer;          /* HFM_PREC_MIXED: refinement steps of the heat solve */
+    double source_dT_sum;     /* the heater energy balance of the last substep, no counterpart in the
+                                 reference: sum over source cells of the temperature rise the capped source
+                                 step gave (heat.py:154-158), fp32 rises summed in fp64 on the device in a
+                                 fixed order.  Times rho c_p h^3 it is the energy put into the air, equal to
+                                 P_W dt (heater.py:52) while no source cell is at its cap; 0 without a
+                                 source */
 } hfm_stats;
 
 /* Field identifiers for hfm_get_field / hfm_set_field. */
@@ -193,6 +203,16 @@ int hfm_get_cell_velocity(hfm_scene* s, float* out);
 /* Per-particle temperature, melt fraction, yield stress and viscosity (heat.py:161-174). */
 int hfm_get_particles(hfm_scene* s, float* temp, float* melt, float* yield_stress,
                       float* viscosity);
+
+/* Debug view of the particle binning, no counterpart in the reference (whose scatter forms the
+ * indices inside its atomic loop, fields.py:39-48): for every particle p (in the order given to
+ * hfm_set_particles), bin[p] = the cell (i n + j) n + k it is sorted into (floor(pos / h), clamped to
+ * the box), and base[12 p + 3 l + d] / frac[12 p + 3 l + d] the index int(floor(g)) and fraction
+ * g - floor(g) of g = pos_d / h - off_d on lattice l (0 cells, 1 x faces, 2 y faces, 3 z faces; the
+ * offsets of fields.py:84-90), computed on the device from the stored pos / h exactly as the particle
+ * gather reads it (fields.py:5-10).  Bins the particles first if they changed since the last
+ * substep. */
+int hfm_get_particle_bins(hfm_scene* s, int32_t* bin, int32_t* base, float* frac);
 
 /* AirGrid.divergence_max (grid.py:257-260) and AirGrid.speed_max (grid.py:262-263). */
 int hfm_diagnostics(hfm_scene* s, double* divergence_max, double* speed_max);

@@ -693,11 +693,24 @@ static void pcg(const opctx* op, const real* diag, const real* b, real* x, long 
     memcpy(p, z, (size_t)N * sizeof(real));
     int maxeff = maxiter + (maxiter & 1);
     int it = 0, converged = 0;
+    /* Breakdown stop of the build (not in warp, which runs on to maxiter): an iteration whose p.Ap
+     * is not positive or whose alpha is not finite takes alpha = 0 (x and r keep the last iterate),
+     * and the solve ends with converged = -1 there, as it does where r.z of the previous iteration is
+     * not positive, beta is not finite or ||r||^2 is not finite.  A converging solve never meets
+     * these conditions. */
     while (it < maxeff) {
         real rz_old = rz_new;
         op_apply(op, p, Ap, Ap, R(1.0), R(0.0));
         real pAp = dotr(p, Ap, N);
-        real alpha = (rr > atol_sq) ? rz_old / pAp : R(0.0);
+        int brk = 0;
+        real alpha = R(0.0);
+        if (rr > atol_sq) {
+            alpha = rz_old / pAp;
+            if (!(pAp > R(0.0)) || !isfinite(alpha)) {
+                alpha = R(0.0);
+                brk = 1;
+            }
+        }
         #pragma omp parallel for schedule(static)
         for (long t = 0; t < N; ++t) {
             x[t] = x[t] + alpha * p[t];
@@ -707,13 +720,22 @@ static void pcg(const opctx* op, const real* diag, const real* b, real* x, long 
         #pragma omp parallel for schedule(static)
         for (long t = 0; t < N; ++t) z[t] = diag[t] * r[t];
         rz_new = dotr(r, z, N);
-        real beta = (rr > atol_sq) ? rz_new / rz_old : R(0.0);
+        real beta = R(0.0);
+        if (rr > atol_sq) {
+            beta = rz_new / rz_old;
+            if (!(rz_old > R(0.0)) || !isfinite(beta)) brk = 1;
+        }
+        if (!isfinite(rr)) brk = 1;
         #pragma omp parallel for schedule(static)
         for (long t = 0; t < N; ++t) p[t] = z[t] + beta * p[t];
         ++it;
         /* past this point alpha = beta = 0: x and r are frozen, so stopping is exact */
         if (rr <= atol_sq) {
             converged = 1;
+            break;
+        }
+        if (brk) {
+            converged = -1;
             break;
         }
     }
@@ -816,18 +838,23 @@ static void st_blend(orc_scene* s) {
 }
 
 /* heat.py:154-158, launched only when rate > 0 (solver.py:76) */
+/* Also the build's heater energy balance (hfm_stats.source_dT_sum): the rises T_new - T_old of the
+ * source cells summed in fp64, serially in cell order. */
 static void st_source(orc_scene* s, double dt) {
+    s->st.source_dT_sum = 0.0;
     if (!(s->rate > 0)) return;
     real rate = (real)s->rate, tcap = (real)s->t_cap, d = (real)dt;
-    #pragma omp parallel for schedule(static)
+    double rise = 0.0;
     for (long t = 0; t < s->N; ++t) {
         if (s->src[t] == 1) {
             real T = s->Tw[t];
             real a = T + rate * d;
             real b = T > tcap ? T : tcap;
             s->Tw[t] = a < b ? a : b;
+            rise += (double)(s->Tw[t] - T);
         }
     }
+    s->st.source_dT_sum = rise;
 }
 
 static real* bf16_copy(const real* a, long n) {
