@@ -92,9 +92,8 @@ typedef struct hfm_params {
  * children, injection, one weighted-Jacobi sweep before and after the coarse correction, 16 on the
  * coarsest level.  Same stopping rule and iteration as warp's CG, far fewer iterations; results
  * agree with the Jacobi solve to the solve's tolerance, not bit for bit.  With HFM_PREC_MIXED it
- * preconditions the FP16 inner solves of the refinement.  Not with HFM_FLAG_FUSED_CG; needs
- * cg_tol >= 1e-7 and cg_max_iter <= 2000 (a smaller tolerance is below what fp32 reaches and only
- * runs a stalled solve to its cap). */
+ * preconditions the FP16 inner solves of the refinement.  Not with HFM_FLAG_FUSED_CG.  A tolerance
+ * below what fp32 reaches (about 1e-7 here) runs each solve to its cap or to a breakdown stop. */
 #define HFM_FLAG_MG_PCG 4
 /* Limited MacCormack advection of T and of u, v, w (Selle et al., J. Sci. Comput. 35, 2008): the
  * semi-Lagrangian value, corrected by half the error of tracing it back to the start, clamped to
