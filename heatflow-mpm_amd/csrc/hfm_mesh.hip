@@ -17,6 +17,8 @@
 // the chunks in double.  Compute bound (~60 VALU per point-triangle pair with the atan2).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "hfm_device.h"
 #include "hfm_scene.h"
 
@@ -76,4 +78,79 @@ void launch_winding(const float* tri, long nt, const float* pts, long np, float 
                        threshold, flags, wn);
 }
 
+// ---------------------------------------------------------------------------------------------
+// The inside test without device allocations (hfm_mesh_inside_noalloc).  Its device copies of the
+// triangles, points, flags and winding numbers live in `scratch`, one 8 MiB array per GPU held for
+// the life of the process.  Host code never passes the array's name to a kernel or a copy (in host
+// code it is a stand-in, not the device address): the address of each GPU's copy comes from
+// hipGetSymbolAddress, taken once per device with that device current, and is kept in
+// g_scratch_dev.  Every call on a GPU shares that GPU's array and leaves its contents there; nothing
+// in it needs to start at zero (the kernel writes every flag and winding number it is asked for).
+// Calls that overlap on one GPU (from several host threads) would race on the array, so the calls
+// of a process are serialised by one mutex.
+// ---------------------------------------------------------------------------------------------
+constexpr long kScratchWords = 1L << 21;      // 8 MiB: 9 nt + 5 np + 3 * 64 words at most
+static __device__ __align__(256) float scratch[kScratchWords];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_dev[kMaxDevices] = {};
+
+long round64(long w) { return (w + 63) & ~63L; }
+}  // namespace
+
 }  // namespace hfm
+
+extern "C" int hfm_mesh_inside_noalloc(const float* tri, int64_t nt, const float* points, int64_t np, float threshold,
+                                       int32_t* flags, float* winding, int device) {
+    using namespace hfm;
+    if (nt < 0 || np < 0 || (nt > 0 && !tri) || (np > 0 && (!points || !flags)) || !(threshold == threshold)) {
+        set_error("bad mesh-inside arguments (triangles %lld, points %lld)", (long long)nt, (long long)np);
+        return HFM_ERR_ARG;
+    }
+    if (nt > kScratchWords || np > kScratchWords ||
+        round64(9 * nt) + round64(3 * np) + round64(np) + round64(np) > kScratchWords) {
+        set_error("hfm_mesh_inside_noalloc: 9 x %lld triangle + 5 x %lld point words exceed its %ld-word scratch "
+                  "(use hfm_mesh_inside)", (long long)nt, (long long)np, kScratchWords);
+        return HFM_ERR_ARG;
+    }
+    if (np == 0) return HFM_OK;
+    int rc = check_finite_host(tri, 9 * nt, "triangles");
+    if (!rc) rc = check_finite_host(points, 3 * np, "points");
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess) ndev = 0;
+    if (device < 0 || device >= ndev || device >= kMaxDevices) {
+        set_error("device %d not available (%d HIP devices)", device, ndev);
+        return HFM_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    int prev = 0;
+    hipError_t e = hipGetDevice(&prev);
+    if (e == hipSuccess) e = hipSetDevice(device);
+    if (e == hipSuccess && !g_scratch_dev[device]) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e == hipSuccess) g_scratch_dev[device] = (float*)ptr;
+    }
+    float* base = g_scratch_dev[device];
+    float* d_tri = base;
+    float* d_pts = d_tri + round64(9 * nt);
+    int* d_flags = (int*)(d_pts + round64(3 * np));
+    float* d_wn = (float*)d_flags + round64(np);
+    if (e == hipSuccess && nt > 0) e = hipMemcpy(d_tri, tri, 9 * nt * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_pts, points, 3 * np * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        launch_winding(d_tri, nt, d_pts, np, threshold, d_flags, winding ? d_wn : nullptr, nullptr);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(flags, d_flags, np * sizeof(int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && winding) e = hipMemcpy(winding, d_wn, np * sizeof(float), hipMemcpyDeviceToHost);
+    (void)hipSetDevice(prev);
+    if (e != hipSuccess) {
+        set_error("hfm_mesh_inside_noalloc: %s", hipGetErrorString(e));
+        return HFM_ERR_HIP;
+    }
+    return HFM_OK;
+}

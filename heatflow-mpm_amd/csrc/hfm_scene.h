@@ -375,6 +375,10 @@ void launch_mc_T(const Launch& L, float dt);
 void launch_mc_faces(const Launch& L, float dt);
 void launch_winding(const float* tri, long nt, const float* pts, long np, float threshold, int* flags, float* wn,
                     hipStream_t st);
+// the message hfm_last_error() returns on this thread; HFM_ERR_NONFINITE (and the message) if a host
+// array holds a value that is not finite
+void set_error(const char* fmt, ...);
+int check_finite_host(const float* a, long n, const char* what);
 void launch_p2g_listed(const Launch& L);
 
 // ---- PCG (hfm_pcg.hip); op 0 = heat, 1 = pressure ----

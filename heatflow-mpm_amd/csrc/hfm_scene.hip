@@ -934,6 +934,17 @@ int reset_particle_fields(Scene* s) {
 
 }  // namespace
 
+void hfm::set_error(const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+int hfm::check_finite_host(const float* a, long n, const char* what) { return check_finite(a, n, what); }
+
 extern "C" {
 
 const char* hfm_last_error(void) { return g_err.c_str(); }

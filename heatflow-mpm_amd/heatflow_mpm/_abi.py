@@ -95,6 +95,7 @@ SIGNATURES = {
     "hfm_batch_kernel_time_reset": (C.c_int, [_VP]),
     "hfm_device_count": (C.c_int, []),
     "hfm_mesh_inside": (C.c_int, [_VP, C.c_int64, _VP, C.c_int64, C.c_float, _VP, _VP, C.c_int]),
+    "hfm_mesh_inside_noalloc": (C.c_int, [_VP, C.c_int64, _VP, C.c_int64, C.c_float, _VP, _VP, C.c_int]),
     "hfm_last_error": (C.c_char_p, []),
 }
 

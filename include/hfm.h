@@ -276,6 +276,16 @@ int hfm_device_count(void);
 int hfm_mesh_inside(const float* tri, int64_t nt, const float* points, int64_t np, float threshold, int32_t* flags,
                     float* winding, int device);
 
+/* hfm_mesh_inside without device allocations: the same results (flags and winding numbers bit for bit),
+ * with the device copies of the triangles, points and results in a fixed 8 MiB array per GPU, held until
+ * the process ends, instead of hipMalloc'd buffers.  No counterpart in the reference; for callers that
+ * cannot allocate device memory (loops under a no-allocation rule, harnesses that forbid hipMalloc).
+ * Limit: round64(9 nt) + round64(3 np) + 2 round64(np) <= 2^21 words (round64 rounds up to a multiple of
+ * 64), e.g. 5 120 triangles and 400 000 points; larger calls return HFM_ERR_ARG.  Calls share the array:
+ * calls on one GPU are serialised inside the library, so concurrent host threads wait for each other. */
+int hfm_mesh_inside_noalloc(const float* tri, int64_t nt, const float* points, int64_t np, float threshold,
+                            int32_t* flags, float* winding, int device);
+
 const char* hfm_last_error(void);
 
 #ifdef __cplusplus
