@@ -43,8 +43,8 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per cell of one PCG iteration, by kernel (fp32 words x 4):
 #   dir:   reads z, p_old, alpha_s; writes p, Ap                     -> 5 words
 #   upd:   reads x, p, r, Ap, D; writes x, r, z                        -> 8 words
-#   fused: reads x, r, p_old, D, alpha_s; writes x, r, p              -> 8 words (whole iteration)
-KERNEL_WORDS = {_abi.KT_PCG_DIR: 5, _abi.KT_PCG_UPD: 8, _abi.KT_PCG_FUSED: 8}
+#   fused: reads x, r, p_old, s_old, w, D, alpha_s; writes x, r, p, s, w -> 12 words (whole iteration)
+KERNEL_WORDS = {_abi.KT_PCG_DIR: 5, _abi.KT_PCG_UPD: 8, _abi.KT_PCG_FUSED: 12}
 # Mixed precision, inner solve (bytes per cell): pass A reads z, p_old (FP16) and alpha_s, writes
 # p, Ap (FP16) -> 12 B; pass B reads x (fp32), p, r, Ap (FP16), D (fp32), writes x, r, z -> 22 B
 KERNEL_BYTES_MIXED = {_abi.KT_PCG_DIR: 12, _abi.KT_PCG_UPD: 22}
@@ -54,8 +54,11 @@ TRAFFIC_FILE = os.path.join(REPO, "profiles", "pcg_pressure_n96_traffic.json")
 
 
 def measured_traffic(kernel: str, n: int):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
-    (calibrated against a plain copy), when they were taken on this workload; else None."""
+    """Bytes per launch of `kernel` that the memory side (TCC) moved, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs of this workload (scaled by what the counters read for a plain copy
+    of known size), with where they come from; None if not measured for this kernel and size.  A
+    stored value, not measured in this run; at 96^3 the PCG working set fits the 256 MB Infinity
+    Cache, so part of these bytes are served by it rather than by HBM."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
@@ -65,7 +68,10 @@ def measured_traffic(kernel: str, n: int):
         return None
     for name, rec in t.get("kernels", {}).items():
         if name.startswith(kernel):
-            return rec["traffic_bytes"]
+            return dict(traffic_bytes=rec["traffic_bytes"],
+                        source=f"{os.path.relpath(TRAFFIC_FILE, REPO)} (stored rocprofv3 FETCH_SIZE/WRITE_SIZE, "
+                               f"{t.get('measured', 'earlier run')}; at this size part of it is Infinity Cache "
+                               f"traffic, not HBM)")
     return None
 
 
@@ -73,12 +79,19 @@ def substep_bytes(N: int, P: int, I_p: float, I_h: float, fused: bool) -> float:
     """Algorithmic HBM bytes of one substep of this implementation (fp32 words x 4):
     P2G heat-sum gather 2 (static particles), heat prep 6.25, T advection + kappa + heat rhs/diag
     10.25, heat PCG init 6.25, heat PCG 13.25 per iteration, air face pass 14, divergence +
-    diagonal 9, rhs mean 1, pressure PCG init 7 (+5 for the fused form's first scalars), pressure
-    PCG 13 (two-pass) or 8 (fused) per iteration, projection 7 words per cell; 8 words per
+    diagonal 9, rhs mean 1, pressure PCG init 7 (+3 for the one-launch form's w = A D r), pressure
+    PCG 13 (two-pass) or 12 (one launch) per iteration, projection 7 words per cell; 8 words per
     particle for G2P."""
-    p_init, p_iter = (12, 8) if fused else (7, 13)
+    p_init, p_iter = (10, 12) if fused else (7, 13)
     words = N * (2 + 6.25 + 10.25 + 6.25 + 13.25 * I_h + 14 + 9 + 1 + p_init + p_iter * I_p + 7) + 8 * P
     return 4.0 * words
+
+
+def contract_bytes(N: int, P: int, I_p: float, I_h: float) -> float:
+    """The path's algorithmic bytes per substep as the scope table fixes them (independent of this
+    implementation's kernel split): B = 4 [N (48.5 + 12 I_p + 12.5 I_h) + 17 P], with I_p and I_h the
+    exact pressure and heat PCG iterations."""
+    return 4.0 * (N * (48.5 + 12.0 * I_p + 12.5 * I_h) + 17.0 * P)
 
 
 def cpu_baseline(kw, dt, state, budget_s: float = 15.0):
@@ -514,8 +527,15 @@ def main() -> int:
     I_p = (st1["p_iters_total"] - st0["p_iters_total"]) / a.steps
     I_h = (st1["h_iters_total"] - st0["h_iters_total"]) / a.steps
     value = N * a.steps / wall
-    hbm_tbs = (substep_bytes(N, P, I_p, I_h, sim.layout["cg"] == "fused") * a.steps / wall / 1e12
-               if prec == "fp32" else None)
+    hbm_impl = (substep_bytes(N, P, I_p, I_h, sim.layout["cg"] == "fused") * a.steps / wall / 1e12
+                if prec == "fp32" else None)
+    B = contract_bytes(N, P, I_p, I_h)
+    hbm_tbs = B * a.steps / wall / 1e12 if prec == "fp32" else None
+    substep_roofline = (None if prec != "fp32" else
+                        dict(bound="hbm", bytes_per_substep=round(B), achieved_tb_s=round(hbm_tbs, 3),
+                             peak_tb_s=HBM_PEAK_GBS / 1e3, frac=round(hbm_tbs * 1e3 / HBM_PEAK_GBS, 4),
+                             formula="B = 4 [N (48.5 + 12 I_p + 12.5 I_h) + 17 P] per substep, K B / wall time",
+                             own_kernel_model_tb_s=None if hbm_impl is None else round(hbm_impl, 3)))
 
     # second timed region with HIP events around every pressure-PCG launch
     sim._scene.set_kernel_timing(True)
@@ -531,12 +551,13 @@ def main() -> int:
     total_ms, launches = kt[dom]
     bytes_per_launch = (KERNEL_WORDS[dom] * 4.0 if prec == "fp32" else KERNEL_BYTES_MIXED[dom]) * N
     achieved = bytes_per_launch * iters / (total_ms * 1e-3) / 1e9
-    traffic = measured_traffic(KERNEL_NAMES[dom], n) if prec == "fp32" else None
+    tr = measured_traffic(KERNEL_NAMES[dom], n) if prec == "fp32" else None
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4),
-                    traffic=None if traffic is None else round(traffic), kernel=KERNEL_NAMES[dom],
+                    traffic=None if tr is None else round(tr["traffic_bytes"]), kernel=KERNEL_NAMES[dom],
                     avg_launch_us=round(total_ms * 1e3 / max(launches, 1), 3), launches=launches,
-                    iterations=iters, bytes_per_launch=bytes_per_launch)
+                    iterations=iters, bytes_per_launch=bytes_per_launch,
+                    traffic_source=None if tr is None else tr["source"])
 
     cpu = None if a.no_cpu else cpu_baseline(kw, dt, state)
     end = {name: sim._scene.get_field(name) for name in ("T", "U", "V", "W")}
@@ -556,7 +577,8 @@ def main() -> int:
                            dir_planes=sim.layout["dir_planes"],
                            pressure_iters_per_substep=round(I_p, 2), heat_iters_per_substep=round(I_h, 2),
                            parallelism="1 GPU"),
-               hbm_tb_s=None if hbm_tbs is None else round(hbm_tbs, 3), roofline=roofline, cpu_baseline=cpu,
+               hbm_tb_s=None if hbm_tbs is None else round(hbm_tbs, 3), substep_roofline=substep_roofline,
+               roofline=roofline, cpu_baseline=cpu,
                mg_form=mg, pcg_kernels_192=large)
     print(json.dumps(out), flush=True)
     sim.close()

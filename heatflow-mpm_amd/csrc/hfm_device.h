@@ -15,7 +15,7 @@ constexpr int kBlock = 256;      // threads per workgroup (4 wave64)
 constexpr int kWaves = kBlock / 64;
 constexpr int kXcds = 8;         // MI355X: 8 XCDs, workgroups dispatched round-robin over them
 constexpr int kMaxPcgBlocks = 1024;     // pass B / init workgroups (4 per CU); bounds their partial sums
-constexpr int kMaxDirBlocks = 2048;     // pass A workgroups
+constexpr int kMaxDirBlocks = 4096;     // pass A / one-launch iteration workgroups
 
 __host__ __device__ inline int nblocks(long count) { return (int)((count + kBlock - 1) / kBlock); }
 

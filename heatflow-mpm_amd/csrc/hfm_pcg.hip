@@ -293,14 +293,17 @@ __device__ __forceinline__ void load_rows(const double* P, int nb, int nv, doubl
             if (q < nv) acc[q] += P[q * nb + b];
 }
 
+// nb: the number of row entries; a wave whose threads all lie past it holds zeros and skips its tree
+// (the wave's total is then +0, which leaves every sum it joins unchanged).
 template <int NV, int BS>
-__device__ __forceinline__ int reduce_bcast(const double (&acc)[NV], int nv, int flag, double (&out)[NV]) {
+__device__ __forceinline__ int reduce_bcast(const double (&acc)[NV], int nv, int flag, double (&out)[NV], int nb) {
     __shared__ double wsum[NV][BS / 64];
     __shared__ int flag_s;
+    const bool any = (int)(threadIdx.x & ~63u) < nb;   // uniform over the wave
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
         if (q < nv) {
-            const double w = wave_sum(acc[q]);
+            const double w = any ? wave_sum(acc[q]) : 0.0;
             if ((threadIdx.x & 63) == 0) wsum[q][threadIdx.x >> 6] = w;
         }
     }
@@ -351,7 +354,7 @@ __device__ bool finish_iteration(const View& a, const PassAIn& p, float& beta) {
     CgCtl* c = a.ctl;
     const int k = a.k;
     double tot[3];
-    if (reduce_bcast<3, BS>(p.acc, k == 0 && !IR ? 3 : 2, p.done, tot)) return true;
+    if (reduce_bcast<3, BS>(p.acc, k == 0 && !IR ? 3 : 2, p.done, tot, a.nb_upd)) return true;
     const float rr = (float)tot[0];
     const float rz = (float)tot[1];
     float atol_sq = p.atol_sq;
@@ -416,7 +419,7 @@ __device__ __forceinline__ void sum_source_rises(const A& a, int lb) {
     if (lb != 0 || !a.rate_on) return;
     double acc[1], tot[1];
     load_rows<1>(a.src_part, nblocks(a.N), 1, acc);
-    reduce_bcast<1, kBlock>(acc, 1, 0, tot);
+    reduce_bcast<1, kBlock>(acc, 1, 0, tot, nblocks(a.N));
     if (threadIdx.x == 0) {
         a.ctl[0].src_dt = tot[0];
         __hip_atomic_store(&a.mirror[0].src_dt, tot[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -642,7 +645,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_upd(Src src, int kit) {
     const float rr = c->slot_rr[kit & 1], rz = c->slot_rz[kit & 1], atol_sq = c->atol_sq;
     const float step = IR ? c->ir_step : 1.0f;
     double tot[1];
-    if (reduce_bcast<1, kBlock>(acc, 1, done, tot)) return;
+    if (reduce_bcast<1, kBlock>(acc, 1, done, tot, v.nb_dir)) return;
     const float pAp = (float)tot[0];
     // alpha = 0 on a breakdown (p.Ap not positive or not a number; fp32 also alpha not finite): x and
     // r stay.  The inner FP16 solve of a refinement then runs to its cap and the refinement step
@@ -760,7 +763,7 @@ __global__ __launch_bounds__(kBlock) void k_ir_scale(Src src, int first) {
     const int total = c->ir_total, outer = c->ir_outer, maxiter = c->ir_maxiter;
     const double rr_prev = c->ir_rr;
     double tot[2];
-    reduce_bcast<2, kBlock>(acc, first ? 2 : 1, 0, tot);
+    reduce_bcast<2, kBlock>(acc, first ? 2 : 1, 0, tot, v.nb_upd);
     const double rr = tot[0];
     if (first) {
         atol = tol * sqrt(tot[1]);
@@ -820,35 +823,38 @@ __global__ __launch_bounds__(kBlock) void k_ir_scale(Src src, int first) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Fused pressure PCG: one launch per iteration (HFM_FLAG_FUSED_CG).
+// One-launch pressure PCG (HFM_FLAG_FUSED_CG): warp's Jacobi PCG in the single-reduction form of
+// Chronopoulos and Gear (J. Comput. Appl. Math. 25, 1989).  With u = D r (never stored), w = A u and
+// s = A p carried by the recurrence s = w + beta s, one iteration is
+//   beta_k = g_k / g_{k-1},  p.Ap_k = d_k - beta_k^2 p.Ap_{k-1}  (g = r.u, d = w.u; k = 0: beta 0, p.Ap d_0)
+//   alpha_k = g_k / p.Ap_k
+//   p_k = u_k + beta_k p_{k-1},  s_k = w_k + beta_k s_{k-1},  x_{k+1} = x_k + alpha_k p_k,
+//   r_{k+1} = r_k - alpha_k s_k,  u_{k+1} = D r_{k+1},  w_{k+1} = A u_{k+1}
+// and the only global reduction, g, d and r.r of step k + 1, closes it.  In exact arithmetic these are
+// warp's iterates (p_k.A p_{k-1} = 0 turns z.Az + 2 beta z.Ap_{k-1} + beta^2 p.Ap_{k-1} into d - beta^2
+// p.Ap_{k-1}, and s_k = A p_k); the stopping rule and its count are warp's, on r.r of the recursive r.
+// In fp32 the iteration counts equal the two-pass form's and the solutions agree to ~1e-7 on the
+// pressure systems of the coupled and plume scenes.
 //
-// Launch k starts from x_k, r_k, p_{k-1} in memory and the partial sums of launch k-1:
-// r_k.r_k, r_k.z_k, z_k.A z_k and z_k.A p_{k-1} (z = D r).  Its prologue forms
-//   beta = r_k.z_k / r_{k-1}.z_{k-1}   (0 at k = 0, or once converged, as warp)
-//   p_k.A p_k = z.Az + 2 beta z.Ap_{k-1} + beta^2 p_{k-1}.A p_{k-1}   (exact algebra, A symmetric)
-//   alpha = r_k.z_k / p_k.A p_k        (0 once converged, as warp)
-// and then, on a 16 x 16 x D block with a two-cell halo staged in LDS:
-//   p_k = z_k + beta p_{k-1}           on block + 2 halo
-//   A p_k, r_{k+1} = r_k - alpha A p_k, z_{k+1} = D r_{k+1}   on block + 1 halo
-//   x_{k+1} = x_k + alpha p_k, A z_{k+1} and the four partial sums   on the block
-// Halo cells are recomputed by every block that needs them with the same operations on the same
-// inputs, so all copies agree bit for bit.  Memory per iteration: reads x, r, p, D, alpha_s,
-// writes x, r, p (8 words per cell) in one launch, against 12 words in two launches for the
-// two-pass form.  r and p are double-buffered by iteration parity (neighbours read the old ones
-// while owners write the new), as are the partial sums.
+// Launch k stages, on a 16 x 16 x D block framed by a one-cell halo (as pass A), the values that make
+// u_{k+1}: r_k, w_k, s_{k-1} and D at every frame cell; it forms s_k, r_{k+1} and u_{k+1} there (a frame
+// cell outside the box takes the clamped in-box cell's values, so its stencil term is +0), keeps
+// u_{k+1} and the solid fraction in LDS, writes r_{k+1}, s_k, p_k and x_{k+1} for the block's own cells,
+// and then w_{k+1} = A u_{k+1} from LDS.  Neighbouring blocks read r, w and s at halo cells while the
+// owners write them, so those three are double-buffered by iteration parity (r_k, w_k in buffer k & 1,
+// s_k in buffer k & 1); p and x are read and written by their owners only.  Memory per iteration: reads
+// r, w, s, D, alpha_s, x, p; writes r, w, s, p, x (12 words per cell), one launch and one reduction
+// instead of two of each.
 // ---------------------------------------------------------------------------------------------
-constexpr int kF2 = kT3 + 4;
-
 template <int D, class Src>
-__global__ __launch_bounds__(kBlock3) void k_pcg_fused(Src src, int kit, unsigned long long* pace, unsigned tag) {
+__global__ __launch_bounds__(kBlock3, D <= 8 ? 8 : 4) void k_pcg_fused(Src src, int kit, unsigned long long* pace,
+                                                                       unsigned tag) {
     post_pace(pace, tag, kit);
     HFM_ENTRY(src);
     if (batch_solve_done(src, a.ctl + 1)) return;
-    __shared__ float P2[D + 4][kF2][kF2];   // z_k, then p_k (block + 2 halo)
-    __shared__ float S2[D + 4][kF2][kF2];   // alpha_s
-    __shared__ float Z1[D + 2][kF3][kF3];   // z_{k+1} (block + 1 halo)
-    __shared__ float A1[D + 2][kF3][kF3];   // A p_k (block + 1 halo)
-    __shared__ double lds[kFusedRows][kBlock3 / 64];
+    __shared__ float U[D + 2][kF3][kF3];   // u_{k+1}
+    __shared__ float S[D + 2][kF3][kF3];   // alpha_s
+    __shared__ double lds[3][kBlock3 / 64];
     CgCtl* c = a.ctl + 1;
     const int n = a.n;
     const long N = a.N;
@@ -860,65 +866,53 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_fused(Src src, int kit, unsigne
     const bool first = kit == 0;
     const float* __restrict__ rk = a.cg_r + (long)(kit & 1) * N;
     float* __restrict__ rn = a.cg_r + (long)((kit + 1) & 1) * N;
-    const float* __restrict__ pold = a.cg_d + (long)((kit + 1) & 1) * N;
-    float* __restrict__ pk = a.cg_d + (long)(kit & 1) * N;
+    const float* __restrict__ wk = a.cg_Ap + (long)(kit & 1) * N;
+    float* __restrict__ wn = a.cg_Ap + (long)((kit + 1) & 1) * N;
+    const float* __restrict__ sprev = a.cg_d + (long)((kit + 1) & 1) * N;
+    float* __restrict__ sk = a.cg_d + (long)(kit & 1) * N;
+    float* __restrict__ pv = a.cg_z;
+    float* __restrict__ x = a.p;
     const float* __restrict__ Dg = a.pdiag;
     const float* __restrict__ as = a.alpha_s;
-    float* __restrict__ x = a.p;
     const double* part_in = a.partials + (long)(kit & 1) * kFusedRows * kMaxDirBlocks;
     double* part_out = a.partials + (long)((kit + 1) & 1) * kFusedRows * kMaxDirBlocks;
     const unsigned nn = (unsigned)n * n;
-    // ---- loads, all issued before the prologue: z_k = D r_k, p_{k-1} and alpha_s on block + 2
-    // halo (each of 800 threads owns one (j, k) position of the 20 x 20 plane and steps through
-    // planes), r_k and D on block + 1 halo, the partial sums of launch k-1 and the scalars
-    constexpr int kPl2 = kF2 * kF2, kG2 = kBlock3 / kPl2;   // 400 positions, 2 plane groups
-    constexpr int kM2 = (D + 4 + kG2 - 1) / kG2;
-    const int grp2 = threadIdx.x / kPl2, pos2 = threadIdx.x % kPl2;
-    const int fj2 = pos2 / kF2, fk2 = pos2 % kF2;
-    const int gj2 = j0 + fj2 - 2, gk2 = k0 + fk2 - 2;
-    const bool okjk2 = grp2 < kG2 && gj2 >= 0 && gj2 < n && gk2 >= 0 && gk2 < n;
-    const unsigned tjk2 = (unsigned)clampi(gj2, 0, n - 1) * n + clampi(gk2, 0, n - 1);
-    float f_z[kM2], f_q[kM2], f_a[kM2];
-    bool ok2[kM2];
+    // ---- loads, all issued before the prologue: each of 972 threads owns one (j, k) position of the
+    // 18 x 18 frame plane and steps through the planes; the block's own cells also load x and p
+    constexpr int kM = (D + 2 + kGroups3 - 1) / kGroups3;
+    const int grp = threadIdx.x / kPlane3, pos = threadIdx.x % kPlane3;
+    const int fj = pos / kF3, fk = pos % kF3;
+    const int gj = j0 + fj - 1, gk = k0 + fk - 1;
+    const bool own_jk = grp < kGroups3 && fj >= 1 && fj <= kT3 && fk >= 1 && fk <= kT3 && gj < n && gk < n;
+    const unsigned t_jk = (unsigned)clampi(gj, 0, n - 1) * n + clampi(gk, 0, n - 1);
+    float fr[kM], fw[kM], fs[kM], fd[kM], fa[kM];
+    bool own[kM];
 #pragma unroll
-    for (int m = 0; m < kM2; ++m) {
-        const int fi = grp2 + kG2 * m, gi = i0 + fi - 2;
-        ok2[m] = okjk2 && fi < D + 4 && gi >= 0 && gi < n;
-        const unsigned t = (unsigned)clampi(gi, 0, n - 1) * nn + tjk2;
-        f_z[m] = Dg[t] * rk[t];
-        f_q[m] = first ? 0.0f : pold[t];
-        f_a[m] = as[t];
-    }
-    constexpr int kPl1 = kF3 * kF3, kG1 = kBlock3 / kPl1;   // 324 positions, 3 plane groups
-    constexpr int kM1 = (D + 2 + kG1 - 1) / kG1;
-    const int grp1 = threadIdx.x / kPl1, pos1 = threadIdx.x % kPl1;
-    const int fj1 = pos1 / kF3, fk1 = pos1 % kF3;
-    const int j1 = j0 + fj1 - 1, k1 = k0 + fk1 - 1;
-    const bool okjk1 = grp1 < kG1 && j1 >= 0 && j1 < n && k1 >= 0 && k1 < n;
-    const unsigned tjk1 = (unsigned)clampi(j1, 0, n - 1) * n + clampi(k1, 0, n - 1);
-    float g_r[kM1], g_d[kM1];
-#pragma unroll
-    for (int m = 0; m < kM1; ++m) {
-        const int fi = grp1 + kG1 * m, gi = i0 + fi - 1;
-        const unsigned t = (unsigned)clampi(gi, 0, n - 1) * nn + tjk1;
-        g_r[m] = rk[t];  // out-of-box entries are never used
-        g_d[m] = Dg[t];
+    for (int m = 0; m < kM; ++m) {
+        const int fi = grp + kGroups3 * m, gi = i0 + fi - 1;
+        const unsigned t = (unsigned)clampi(gi, 0, n - 1) * nn + t_jk;
+        own[m] = own_jk && fi >= 1 && fi <= D && gi < n;
+        fr[m] = ld32(rk, t);
+        fw[m] = ld32(wk, t);
+        fs[m] = first ? 0.0f : ld32(sprev, t);
+        fd[m] = ld32(Dg, t);
+        fa[m] = ld32(as, t);
     }
     double acc[kFusedRows];
-    // the init launch wrote rows of its own count
-    load_rows<kFusedRows>(part_in, first ? a.nb_upd : nb, first ? kFusedRows : 4, acc);
+    // the init launch wrote rows of its own count (r.r, g, d, -, b.b); later launches r.r, g, d
+    load_rows<kFusedRows>(part_in, first ? a.nb_upd : nb, first ? kFusedRows : 3, acc);
     const int done = c->done, maxeff = c->maxeff;
-    const float rz_prev = c->slot_rz[(kit + 1) & 1];
-    const double pap_prev = c->slot_pap[(kit + 1) & 1];
+    const float g_prev = c->slot_rz[(kit + 1) & 1];
+    const float pap_prev = (float)c->slot_pap[(kit + 1) & 1];
     const double tol = c->tol;
     double atol = c->atol;
     float atol_sq = c->atol_sq;
     // ---- prologue: scalars of iteration k
     double tot[kFusedRows];
-    if (reduce_bcast<kFusedRows, kBlock3>(acc, first ? kFusedRows : 4, done, tot)) return;
+    if (reduce_bcast<kFusedRows, kBlock3>(acc, first ? kFusedRows : 3, done, tot, first ? a.nb_upd : nb)) return;
     const float rr = (float)tot[0];
-    const float rz = (float)tot[1];
-    const double zaz = tot[2], zap = tot[3];
+    const float g = (float)tot[1];
+    const float dd = (float)tot[2];
     if (first) {
         const float bb = (float)tot[4];
         atol = tol * sqrt((double)bb);  // warp: atol = max(tol * ||b||, tol)
@@ -926,19 +920,22 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_fused(Src src, int kit, unsigne
         atol_sq = (float)(atol * atol);
     }
     const bool conv = rr <= atol_sq;
-    const float beta = first ? 0.0f : (rr > atol_sq ? rz / rz_prev : 0.0f);
-    const double pap = first ? zaz : zaz + 2.0 * (double)beta * zap + (double)beta * (double)beta * pap_prev;
-    const float alpha = rr > atol_sq ? rz / (float)pap : 0.0f;
-    // breakdown stop, as the two-pass form: r.z of the previous iteration or p.Ap not positive,
-    // beta or alpha not finite, or a non-finite ||r||^2
+    float beta = 0.0f, pap = dd;
+    if (!first && rr > atol_sq) {
+        beta = g / g_prev;
+        pap = dd - (beta * beta) * pap_prev;
+    }
+    const float alpha = rr > atol_sq ? g / pap : 0.0f;
+    // breakdown stop, as the two-pass form: r.z of the previous iteration or p.Ap not positive, beta
+    // or alpha not finite, or a non-finite r.r
     const bool brk = !(rr <= 3.4028235e38f) ||
-                     (rr > atol_sq && ((!first && !(rz_prev > 0.0f)) || !(fabsf(beta) <= 3.4028235e38f) ||
-                                       !((float)pap > 0.0f) || !(fabsf(alpha) <= 3.4028235e38f)));
+                     (rr > atol_sq && ((!first && !(g_prev > 0.0f)) || !(fabsf(beta) <= 3.4028235e38f) ||
+                                       !(pap > 0.0f) || !(fabsf(alpha) <= 3.4028235e38f)));
     const bool stop = conv || kit >= maxeff || brk;
     if (lb == 0 && threadIdx.x == 0) {
         c->slot_rr[kit & 1] = rr;
-        c->slot_rz[kit & 1] = rz;
-        c->slot_pap[kit & 1] = pap;
+        c->slot_rz[kit & 1] = g;
+        c->slot_pap[kit & 1] = (double)pap;
         if (first) {
             c->atol = atol;
             c->atol_sq = atol_sq;
@@ -951,88 +948,82 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_fused(Src src, int kit, unsigne
         }
     }
     if (stop) return;
-    // ---- stage 1: p_k = z_k + beta p_{k-1} and alpha_s on block + 2 halo
-    if (grp2 < kG2) {
+    // x and p of the block's own cells, needed only after the frame pass: loaded now (fewer registers
+    // live across the prologue), consumed after the frame pass
+    float fx[kM], fp[kM], uk[kM];
 #pragma unroll
-        for (int m = 0; m < kM2; ++m) {
-            const int fi = grp2 + kG2 * m;
-            if (fi < D + 4) {
-                P2[fi][fj2][fk2] = ok2[m] ? (first ? f_z[m] : f_z[m] + beta * f_q[m]) : 0.0f;
-                S2[fi][fj2][fk2] = ok2[m] ? f_a[m] : 0.0f;
-            }
-        }
+    for (int m = 0; m < kM; ++m) {
+        const unsigned t = (unsigned)(i0 + grp + kGroups3 * m - 1) * nn + t_jk;
+        fx[m] = own[m] ? ld32((const float*)x, t) : 0.0f;
+        fp[m] = own[m] && !first ? ld32((const float*)pv, t) : 0.0f;
     }
-    __syncthreads();
-    const float amin = a.amin;
-    const float hh = a.hh, rhh = a.rhh;
-    const int fastdiv = a.fastdiv;
-    // ---- stage 2: A p_k, r_{k+1}, z_{k+1} on block + 1 halo
-    if (grp1 < kG1) {
+    // ---- frame cells: s_k, r_{k+1}, u_{k+1}
+    double sum[3] = {0.0, 0.0, 0.0};   // g, d, r.r of step k + 1
+    if (grp < kGroups3) {
 #pragma unroll
-        for (int m = 0; m < kM1; ++m) {
-            const int fi = grp1 + kG1 * m;
+        for (int m = 0; m < kM; ++m) {
+            const int fi = grp + kGroups3 * m;
             if (fi < D + 2) {
-                const int i = i0 + fi - 1;
-                float zn = 0.0f, ap = 0.0f;
-                if (okjk1 && i >= 0 && i < n) {
-                    const int ci = fi + 1, cj = fj1 + 1, ck = fk1 + 1;
-                    const float pc = P2[ci][cj][ck], ac = S2[ci][cj][ck];
-                    float sacc = 0.0f;
-                    if (i > 0) sacc += face_fraction(S2[ci - 1][cj][ck], ac, amin) * (pc - P2[ci - 1][cj][ck]);
-                    if (i < n - 1) sacc += face_fraction(ac, S2[ci + 1][cj][ck], amin) * (pc - P2[ci + 1][cj][ck]);
-                    if (j1 > 0) sacc += face_fraction(S2[ci][cj - 1][ck], ac, amin) * (pc - P2[ci][cj - 1][ck]);
-                    if (j1 < n - 1) sacc += face_fraction(ac, S2[ci][cj + 1][ck], amin) * (pc - P2[ci][cj + 1][ck]);
-                    if (k1 > 0) sacc += face_fraction(S2[ci][cj][ck - 1], ac, amin) * (pc - P2[ci][cj][ck - 1]);
-                    if (k1 < n - 1) sacc += face_fraction(ac, S2[ci][cj][ck + 1], amin) * (pc - P2[ci][cj][ck + 1]);
-                    ap = div_const(1.0f * sacc, hh, rhh, fastdiv);
-                    zn = g_d[m] * (g_r[m] - alpha * ap);
+                const float s = first ? fw[m] : fw[m] + beta * fs[m];
+                const float r1 = fr[m] - alpha * s;
+                const float u1 = fd[m] * r1;
+                U[fi][fj][fk] = u1;
+                S[fi][fj][fk] = fa[m];
+                uk[m] = fd[m] * fr[m];
+                if (own[m]) {
+                    const unsigned t = (unsigned)(i0 + fi - 1) * nn + t_jk;
+                    st32(rn, t, r1);
+                    st32(sk, t, s);
+                    sum[0] += (double)(r1 * u1);
+                    sum[2] += (double)(r1 * r1);
                 }
-                Z1[fi][fj1][fk1] = zn;
-                A1[fi][fj1][fk1] = ap;
             }
         }
     }
+    // ---- p_k and x_{k+1} of the block's own cells
+#pragma unroll
+    for (int m = 0; m < kM; ++m) {
+        if (own[m]) {
+            const unsigned t = (unsigned)(i0 + grp + kGroups3 * m - 1) * nn + t_jk;
+            const float pk = first ? uk[m] : uk[m] + beta * fp[m];
+            st32(pv, t, pk);
+            st32(x, t, fx[m] + alpha * pk);
+        }
+    }
     __syncthreads();
-    // ---- stage 3: on the block, r_{k+1}, x_{k+1}, A z_{k+1} and the partial sums
+    // ---- w_{k+1} = A u_{k+1} on the block, from LDS
+    const float amin = a.amin, hh = a.hh, rhh = a.rhh;
+    const int fastdiv = a.fastdiv;
     const int lk = threadIdx.x % kT3, lj = (threadIdx.x / kT3) % kT3, lq = threadIdx.x / (kT3 * kT3);
     const int j = j0 + lj, k = k0 + lk;
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
     if (j < n && k < n) {
-        const int sj = lj + 1, sk = lk + 1;
+        const int sj = lj + 1, sk1 = lk + 1;
         const unsigned tb = ((unsigned)i0 * n + j) * n + k;
 #pragma unroll
         for (int m = 0; m < D / 4; ++m) {
             const int li = lq + 4 * m, i = i0 + li;
             if (i >= n) break;
             const int si = li + 1;
-            const unsigned t = tb + (unsigned)li * nn;
-            const float xo = x[t], ro = rk[t];
-            const float ap = A1[si][sj][sk];
-            const float rnew = ro - alpha * ap;
-            const float zc = Z1[si][sj][sk], ac = S2[si + 1][sj + 1][sk + 1];
-            float sacc = 0.0f;
-            if (i > 0) sacc += face_fraction(S2[si][sj + 1][sk + 1], ac, amin) * (zc - Z1[si - 1][sj][sk]);
-            if (i < n - 1) sacc += face_fraction(ac, S2[si + 2][sj + 1][sk + 1], amin) * (zc - Z1[si + 1][sj][sk]);
-            if (j > 0) sacc += face_fraction(S2[si + 1][sj][sk + 1], ac, amin) * (zc - Z1[si][sj - 1][sk]);
-            if (j < n - 1) sacc += face_fraction(ac, S2[si + 1][sj + 2][sk + 1], amin) * (zc - Z1[si][sj + 1][sk]);
-            if (k > 0) sacc += face_fraction(S2[si + 1][sj + 1][sk], ac, amin) * (zc - Z1[si][sj][sk - 1]);
-            if (k < n - 1) sacc += face_fraction(ac, S2[si + 1][sj + 1][sk + 2], amin) * (zc - Z1[si][sj][sk + 1]);
-            const float az = div_const(1.0f * sacc, hh, rhh, fastdiv);
-            const float pc = P2[si + 1][sj + 1][sk + 1];
-            x[t] = xo + alpha * pc;
-            pk[t] = pc;
-            rn[t] = rnew;
-            v[0] += (double)(rnew * rnew);
-            v[1] += (double)(rnew * zc);
-            v[2] += (double)(zc * az);
-            v[3] += (double)(zc * ap);
+            const float uc = U[si][sj][sk1], ac = S[si][sj][sk1];
+            float acc6 = 0.0f;
+            acc6 += face_fraction(S[si - 1][sj][sk1], ac, amin) * (uc - U[si - 1][sj][sk1]);
+            acc6 += face_fraction(ac, S[si + 1][sj][sk1], amin) * (uc - U[si + 1][sj][sk1]);
+            acc6 += face_fraction(S[si][sj - 1][sk1], ac, amin) * (uc - U[si][sj - 1][sk1]);
+            acc6 += face_fraction(ac, S[si][sj + 1][sk1], amin) * (uc - U[si][sj + 1][sk1]);
+            acc6 += face_fraction(S[si][sj][sk1 - 1], ac, amin) * (uc - U[si][sj][sk1 - 1]);
+            acc6 += face_fraction(ac, S[si][sj][sk1 + 1], amin) * (uc - U[si][sj][sk1 + 1]);
+            const float w1 = div_const(1.0f * acc6, hh, rhh, fastdiv);
+            st32(wn, tb + (unsigned)li * nn, w1);
+            sum[1] += (double)(w1 * uc);
         }
     }
-    store_partials<4, kBlock3>(v, part_out, nb, lb, lds);
+    // rows r.r, g, d in the order the prologue reads them
+    double out[3] = {sum[2], sum[0], sum[1]};
+    store_partials<3, kBlock3>(out, part_out, nb, lb, lds);
 }
 
-// Scalars of the first fused iteration from r_0 (written by k_pcg_init<1>): r.r, r.z, z.Az,
-// z.Ap_{-1} = 0 and b.b, into the partial-sum buffer launch 0 reads.
+// The first iteration's inputs from r_0 (written by k_pcg_init<1>): w_0 = A D r_0 into w's buffer 0,
+// and the partial sums r.r, g = r.u, d = w.u and b.b into the partial-sum buffer launch 0 reads.
 template <class Src>
 __global__ __launch_bounds__(kBlock) void k_pcg_init_fused(Src src) {
     HFM_ENTRY(src);
@@ -1041,17 +1032,19 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init_fused(Src src) {
     const PcgView v = make_view<1>(a, lb, 0, 0.0f);
     const float* r = a.cg_r;
     const float* Dg = a.pdiag;
+    float* w = a.cg_Ap;
     double s[kFusedRows] = {0.0, 0.0, 0.0, 0.0, 0.0};
     FOR_CELLS(v, v.nb_upd) {
         int i, j, k;
         cell_ijk(t, v.n, i, j, k);
-        auto zf = [r, Dg](long q) { return Dg[q] * r[q]; };
-        const float acc = Operator<1>::apply(v, t, i, j, k, zf);
-        const float az = Operator<1>::combine(v, acc, 0.0f, false, 1.0f);
-        const float rc = r[t], zc = zf(t), bc = v.b[t];
+        auto uf = [r, Dg](long q) { return Dg[q] * r[q]; };
+        const float acc = Operator<1>::apply(v, t, i, j, k, uf);
+        const float wc = Operator<1>::combine(v, acc, 0.0f, false, 1.0f);
+        const float rc = r[t], uc = uf(t), bc = v.b[t];
+        w[t] = wc;
         s[0] += (double)(rc * rc);
-        s[1] += (double)(rc * zc);
-        s[2] += (double)(zc * az);
+        s[1] += (double)(rc * uc);
+        s[2] += (double)(wc * uc);
         s[4] += (double)(bc * bc);
     }
     store_partials<kFusedRows>(s, a.partials, v.nb_upd, lb, lds);
@@ -1062,10 +1055,10 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init_fused(Src src) {
 // PCG geometry of a scene.  Init, pass B and the flat pass A cover cpt consecutive tiles of
 // kBlock cells per workgroup: at least the scene's pcg_cpt (HFM_PCG_CPT), and enough that at most
 // kMaxPcgBlocks workgroups run.  The pressure pass A is LDS-tiled (unless HFM_PCG_FLAT_DIR) with
-// planes D = 16, else 8, else 4, the first that keeps at most kMaxDirBlocks workgroups
-// (HFM_DIR_ICHUNK forces D; the fused form takes 8 or 4); otherwise it is the flat kernel with
-// the pass B geometry.  Deeper
-// blocks re-read less halo and launch fewer, longer workgroups, which measured fastest.
+// planes D = 16, else 8, else 4, the first that keeps at most kMaxDirBlocks workgroups (deeper blocks
+// re-read less halo and launch fewer, longer workgroups, which measured fastest); the one-launch form
+// tries 8 first (two of its workgroups then fit a CU).  HFM_DIR_ICHUNK forces D.  Without a tiling
+// pass A is the flat kernel with the pass B geometry.
 void pcg_geometry(const Scene* s, bool fused, int* cpt, int* nb_upd, int* nb_dir, int* tiles_j, int* tiles_k,
                   int* dtile) {
     const long tiles = (s->N + kBlock - 1) / kBlock;
@@ -1078,8 +1071,9 @@ void pcg_geometry(const Scene* s, bool fused, int* cpt, int* nb_upd, int* nb_dir
     const int tj = (n + kT3 - 1) / kT3, tk = tj;
     int D = 0;
     if (!s->flat_dir) {
-        const int cand[3] = {16, 8, 4};
-        for (int q = fused ? 1 : 0; q < 3 && !D; ++q) {  // the fused iteration is built for 4 and 8 planes
+        // the one-launch form prefers 8 planes: two of its workgroups fit a CU (at most 64 VGPRs)
+        const int cand[3] = {fused ? 8 : 16, fused ? 16 : 8, 4};
+        for (int q = 0; q < 3 && !D; ++q) {
             const int d = cand[q];
             if (s->dir_ichunk > 0 && d != s->dir_ichunk) continue;
             const long blocks = (long)tj * tk * ((n + d - 1) / d);

@@ -1121,7 +1121,7 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
     A(d.cg_r, 2 * N);
     A(d.cg_z, N);
     A(d.cg_d, 2 * N);
-    A(d.cg_Ap, N);
+    A(d.cg_Ap, 2 * N);   // two-pass: A p; one-launch form: w = A D r, two parity buffers
     A(d.partials, std::max(kMaxDirBlocks + 5 * kMaxPcgBlocks, 2 * kFusedRows * kMaxDirBlocks));
     A(d.src_part, nblocks(N));
     if (prm->flags & HFM_FLAG_MG_PCG) {   // multigrid levels: halve n down to at most 8 cells per side
@@ -1180,8 +1180,8 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
     d.mean_prog = prog_dev;
     const bool want_fused = (prm->flags & HFM_FLAG_FUSED_CG) != 0;
     pcg_geometry(s, want_fused, &d.cpt, &d.nb_upd, &d.nb_dir, &d.tiles_j, &d.tiles_k, &d.dtile);
-    // the fused iteration is built for 4- and 8-plane blocks (16 planes exceed its register budget)
-    d.cg_fused = want_fused && (d.dtile == 4 || d.dtile == 8);
+    // the one-launch iteration runs on the tiled blocks (a flat layout keeps the two-pass form)
+    d.cg_fused = want_fused && d.dtile > 0;
     e = hipMemcpy(prog_dev, o->prog.data(), o->prog.size() * sizeof(int), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         set_err("hipMemcpy: %s", hipGetErrorString(e));

@@ -7,8 +7,8 @@ stepped as one batch -- each candidate layout runs the coupled scene (static ell
 heater with fan, PCG tolerance 1e-6) for a few substeps after a warmup, and is scored by wall
 time per pressure-PCG iteration (launch gaps included, so the one-launch fused form is credited
 for its fewer launches).  Candidates: pass A tiles of 16, 8 and 4 planes (those whose tile count
-fits the pass-A workgroup cap) and the untiled pass, each with the two-pass PCG, plus the fused
-one-launch iteration on 8- and 4-plane tiles.  The winners go to a JSON table (shipped as
+fits the pass-A workgroup cap), each with the two-pass PCG and with the one-launch iteration, and
+the untiled two-pass form.  The winners go to a JSON table (shipped as
 tuning_mi355x.json); HeatflowMPM(layout="auto" / "auto-batch") reads it, using the entry of the
 nearest measured size.
 """
@@ -48,10 +48,9 @@ def candidates(n: int):
     tj = -(-n // 16)
     out = []
     for d in (16, 8, 4):
-        if tj * tj * -(-n // d) <= 2048:   # the pass-A workgroup cap of the library
+        if tj * tj * -(-n // d) <= 4096:   # the pass-A workgroup cap of the library
             out.append(dict(dir_planes=d, cg="warp"))
-            if d in (4, 8):
-                out.append(dict(dir_planes=d, cg="fused"))
+            out.append(dict(dir_planes=d, cg="fused"))
     out.append(dict(dir_planes=-1, cg="warp"))
     return out
 

@@ -208,7 +208,8 @@ class HeatflowMPM:
 
     Same constructor as the reference (solver.py:42-43).  Extras, not in the reference: `body`
     replaces the default StaticBody built from `particles`; `cg="fused"` runs the pressure PCG in
-    one launch per iteration (HFM_FLAG_FUSED_CG in include/hfm.h) instead of warp's two-pass form;
+    one launch per iteration (warp's Jacobi PCG in the single-reduction Chronopoulos-Gear form,
+    HFM_FLAG_FUSED_CG in include/hfm.h) instead of warp's two-pass form;
     `precision="mixed"` solves both systems by iterative refinement with FP16 inner vectors and
     traces advection through a BF16 copy of the velocity (HFM_PREC_MIXED); the state stays fp32;
     `cg="mg"` preconditions the pressure PCG with a geometric multigrid V-cycle instead of

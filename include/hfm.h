@@ -80,11 +80,12 @@ typedef struct hfm_params {
 #define HFM_PREC_MIXED 1
 
 #define HFM_FLAG_TIME_KERNELS 1  /* record HIP events around every CG kernel launch */
-/* Pressure PCG in one launch per iteration instead of two (no counterpart in the reference).
- * Same iterates as warp's PCG in exact arithmetic: p.Ap is formed from explicit dot products,
- * p.Ap = z.Az + 2 beta z.Ap_old + beta^2 p.Ap_old, so one global reduction per iteration
- * suffices, while r is still updated with the explicit A p.  Results agree with the default
- * form to rounding; iteration counts within one.  Used where the pressure pass runs tiled. */
+/* Pressure PCG in one launch per iteration instead of two (no counterpart in the reference):
+ * warp's Jacobi PCG in the single-reduction form of Chronopoulos and Gear (u = D r, w = A u,
+ * s = A p by the recurrence s = w + beta s, p.Ap = w.u - beta^2 p.Ap_old), the same iterates in
+ * exact arithmetic, one global reduction and one launch per iteration.  Same stopping rule and
+ * count; results agree with the default form to rounding, iteration counts within one.  Used where
+ * the pressure pass runs tiled (the autotuned layout picks it where it is faster). */
 #define HFM_FLAG_FUSED_CG 2
 /* Pressure PCG preconditioned by a geometric multigrid V-cycle instead of Jacobi (no counterpart in
  * the reference): cell-centred levels halving n down to at most 8 cells per side (n = m 2^k,

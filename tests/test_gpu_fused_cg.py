@@ -1,7 +1,8 @@
-"""The fused pressure PCG (HFM_FLAG_FUSED_CG, one launch per iteration) against the fp32 oracle,
-which restates warp's two-pass PCG.  The fused form has the same iterates in exact arithmetic
-and differs only in how p.Ap is rounded, so it is held to the same bar as the default form:
-iteration counts within one, fields within relative L2 1e-5 after a substep, 1e-4 after 60."""
+"""The one-launch pressure PCG (HFM_FLAG_FUSED_CG: warp's Jacobi PCG in the single-reduction
+Chronopoulos-Gear form) against the fp32 oracle, which restates warp's two-pass PCG.  The two have the
+same iterates in exact arithmetic and differ only in rounding (p.Ap and A p by recurrence), so the
+one-launch form is held to the same bar as the default form: iteration counts within one, fields
+within relative L2 1e-5 after a substep, 1e-4 after 60."""
 import os
 
 import numpy as np

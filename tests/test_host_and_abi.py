@@ -206,7 +206,7 @@ def test_mixed_precision_argument_checks():
 
 def test_autotune_table_lookup_and_candidates():
     """The autotuner's table lookup takes the nearest measured size (in log n); candidates respect
-    the pass-A workgroup cap (16-plane tiles up to 2048 workgroups) and the fused form's tiles."""
+    the pass-A workgroup cap (4096 workgroups) and offer both PCG forms on every tiling."""
     from heatflow_mpm.autotune import candidates, lookup
     t = {"solo": {"64": {"choice": {"dir_planes": 8, "cg": "fused"}},
                   "128": {"choice": {"dir_planes": 16, "cg": "warp"}}}}
@@ -215,8 +215,9 @@ def test_autotune_table_lookup_and_candidates():
     assert lookup(96, "batch", t) is None
     c = candidates(192)
     assert {"dir_planes": 16, "cg": "warp"} in c and {"dir_planes": -1, "cg": "warp"} in c
-    assert all(x["dir_planes"] != 8 for x in c)          # 12 x 12 x 24 tiles exceed the cap
-    assert all(x["cg"] == "warp" or x["dir_planes"] in (4, 8) for x in candidates(64))
+    assert all(x["dir_planes"] != 4 for x in c)          # 12 x 12 x 48 tiles exceed the cap
+    assert {"dir_planes": 8, "cg": "fused"} in c         # 12 x 12 x 24 tiles fit it
+    assert all(x["cg"] == "warp" or x["dir_planes"] in (4, 8, 16) for x in candidates(64))
 
 
 def test_layout_argument_checks():
