@@ -282,12 +282,32 @@ __device__ __forceinline__ void store_partials(double (&v)[NV], double* __restri
 // then waves in order) into every thread's out[0..nv), with one barrier that also broadcasts
 // thread 0's `flag`; it returns the flag in every thread.  (Letting wave 0 alone load and reduce
 // the rows measured slower: the other waves then idle at the barrier.)
+// The first entry of each row is taken as the thread's value (no add), so nothing waits for these
+// loads before the reduction uses them: loads issued after them (a kernel's data) stay in flight
+// meanwhile.  Further entries (rows longer than the workgroup) are added in order, four per round
+// with their loads issued together.
 template <int NV>
 __device__ __forceinline__ void load_rows(const double* P, int nb, int nv, double (&acc)[NV]) {
+    const int b0 = threadIdx.x, bd = blockDim.x;
 #pragma unroll
-    for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+    for (int q = 0; q < NV; ++q) acc[q] = (q < nv && b0 < nb) ? P[q * nb + b0] : 0.0;
+    int b = b0 + bd;
 #pragma unroll 1
-    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+    for (; b + 3 * bd < nb; b += 4 * bd) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            if (q < nv) {
+                const double v0 = P[q * nb + b], v1 = P[q * nb + b + bd], v2 = P[q * nb + b + 2 * bd],
+                             v3 = P[q * nb + b + 3 * bd];
+                acc[q] = acc[q] + v0;
+                acc[q] = acc[q] + v1;
+                acc[q] = acc[q] + v2;
+                acc[q] = acc[q] + v3;
+            }
+        }
+    }
+#pragma unroll 1
+    for (; b < nb; b += bd)
 #pragma unroll
         for (int q = 0; q < NV; ++q)
             if (q < nv) acc[q] += P[q * nb + b];
@@ -546,6 +566,10 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_dir_tile3(Src src, int kit, uns
     const int fj = pos / kF3, fk = pos % kF3;
     const int gj = j0 + fj - 1, gk = k0 + fk - 1;
     const unsigned t_jk = (unsigned)clampi(gj, 0, n - 1) * n + clampi(gk, 0, n - 1);
+    // the prologue's operands first (vector loads complete in issue order, so the prologue then waits
+    // for these only), then the frame, in flight during the prologue
+    PassAIn pre;
+    pass_a_load<kBlock3, IR>(v, pre);
     float fz[kM], fd[kM], fa[kM];
 #pragma unroll
     for (int m = 0; m < kM; ++m) {
@@ -555,8 +579,6 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_dir_tile3(Src src, int kit, uns
         fd[m] = first ? 0.0f : (float)ld32(ds, t);
         fa[m] = ld32(as, t);
     }
-    PassAIn pre;
-    pass_a_load<kBlock3, IR>(v, pre);
     float beta;
     if (finish_iteration<kBlock3, IR>(v, pre, beta)) return;
     if (grp < kGroups3) {
@@ -623,8 +645,14 @@ __global__ __launch_bounds__(kBlock) void k_pcg_upd(Src src, int kit) {
     // a multigrid-preconditioned pressure solve replaces z and the r.z sums in the V-cycle that
     // follows: this pass neither loads D nor stores z (6 words per cell instead of 8)
     const bool mgz = OP == 1 && a.cg_mg;
-    // the operands of the first kPre tiles, the partial sums of pass A and the scalars are all
-    // loaded before the p.Ap reduction, so their latencies overlap
+    // the partial sums of pass A and the scalars first (vector loads complete in issue order, so the
+    // p.Ap reduction then waits for these only), then the operands of the first kPre tiles, in flight
+    // during the reduction
+    double acc[1];
+    load_rows<1>(v.part_dir, v.nb_dir, 1, acc);
+    const int done = c->done;
+    const float rr = c->slot_rr[kit & 1], rz = c->slot_rz[kit & 1], atol_sq = c->atol_sq;
+    const float step = IR ? c->ir_step : 1.0f;
     constexpr int kPre = 4;
     float px[kPre], pd[kPre], pr[kPre], pa[kPre], pD[kPre];
     const unsigned N32 = (unsigned)v.N;
@@ -639,11 +667,6 @@ __global__ __launch_bounds__(kBlock) void k_pcg_upd(Src src, int kit) {
             pD[u] = mgz ? 0.0f : ld32(v.D, t);
         }
     }
-    double acc[1];
-    load_rows<1>(v.part_dir, v.nb_dir, 1, acc);
-    const int done = c->done;
-    const float rr = c->slot_rr[kit & 1], rz = c->slot_rz[kit & 1], atol_sq = c->atol_sq;
-    const float step = IR ? c->ir_step : 1.0f;
     double tot[1];
     if (reduce_bcast<1, kBlock>(acc, 1, done, tot, v.nb_dir)) return;
     const float pAp = (float)tot[0];
@@ -748,11 +771,16 @@ __global__ __launch_bounds__(kBlock) void k_ir_residual(Src src, float dt, int f
 // there too), at the iteration cap, or on a non-finite residual; otherwise write r^ = r / sigma
 // and z^ = (D / c) r^ in FP16 with their partial sums, which the inner solve's first pass A
 // reads, and set the inner solve's state.
+// step: the refinement step (0 the first), which the host counts; ||r||^2 of each step is kept in
+// slot step & 1, so this launch reads the previous step's slot and workgroup 0 writes its own (every
+// workgroup must reach the same decision, so no workgroup may read a value that workgroup 0 of the
+// same launch writes).
 template <int OP, class Src>
-__global__ __launch_bounds__(kBlock) void k_ir_scale(Src src, int first) {
+__global__ __launch_bounds__(kBlock) void k_ir_scale(Src src, int step) {
     HFM_ENTRY(src);
     CgCtl* c = a.ctl + OP;
     if (c->ir_end) return;
+    const bool first = step == 0;
     __shared__ double lds[2][kWaves];
     const auto v = make_view<OP, true>(a, lb, 0, 0.0f);
     const double* part = a.partials + kIrPart;
@@ -760,8 +788,8 @@ __global__ __launch_bounds__(kBlock) void k_ir_scale(Src src, int first) {
     load_rows<2>(part, v.nb_upd, first ? 2 : 1, acc);
     const double tol = c->tol;
     double atol = c->atol;
-    const int total = c->ir_total, outer = c->ir_outer, maxiter = c->ir_maxiter;
-    const double rr_prev = c->ir_rr;
+    const int total = c->ir_total, outer = step, maxiter = c->ir_maxiter;
+    const double rr_prev = c->ir_rr2[(step + 1) & 1];
     double tot[2];
     reduce_bcast<2, kBlock>(acc, first ? 2 : 1, 0, tot, v.nb_upd);
     const double rr = tot[0];
@@ -778,7 +806,7 @@ __global__ __launch_bounds__(kBlock) void k_ir_scale(Src src, int first) {
             c->done = 1;
             c->converged = conv;
             c->it = total;
-            c->atol = atol;
+            if (first) c->atol = atol;
             CgMirror* m = v.mirror;
             __hip_atomic_store(&m->rr, rr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             __hip_atomic_store(&m->atol, atol, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -813,8 +841,8 @@ __global__ __launch_bounds__(kBlock) void k_ir_scale(Src src, int first) {
         c->ir_outer = outer + 1;
         c->ir_gl = (float)(atol * atol / rr * (double)v.N);   // (atol / sigma)^2
         c->ir_step = (float)(sigma * cop);
-        c->ir_rr = rr;
-        c->atol = atol;
+        c->ir_rr2[step & 1] = rr;
+        if (first) c->atol = atol;
         c->done = 0;
         c->converged = 0;
         c->it = 0;
@@ -877,8 +905,19 @@ __global__ __launch_bounds__(kBlock3, D <= 8 ? 8 : 4) void k_pcg_fused(Src src, 
     const double* part_in = a.partials + (long)(kit & 1) * kFusedRows * kMaxDirBlocks;
     double* part_out = a.partials + (long)((kit + 1) & 1) * kFusedRows * kMaxDirBlocks;
     const unsigned nn = (unsigned)n * n;
-    // ---- loads, all issued before the prologue: each of 972 threads owns one (j, k) position of the
-    // 18 x 18 frame plane and steps through the planes; the block's own cells also load x and p
+    // ---- the prologue's operands first (vector loads complete in issue order, so the prologue then
+    // waits for these only): the partial sums of launch k - 1 (the init launch wrote rows of its own
+    // count: r.r, g, d, -, b.b; later launches r.r, g, d) and the control scalars
+    double acc[kFusedRows];
+    load_rows<kFusedRows>(part_in, first ? a.nb_upd : nb, first ? kFusedRows : 3, acc);
+    const int done = c->done, maxeff = c->maxeff;
+    const float g_prev = c->slot_rz[(kit + 1) & 1];
+    const float pap_prev = (float)c->slot_pap[(kit + 1) & 1];
+    const double tol = c->tol;
+    double atol = c->atol;
+    float atol_sq = c->atol_sq;
+    // ---- then the frame, in flight during the prologue: each of 972 threads owns one (j, k) position
+    // of the 18 x 18 frame plane and steps through the planes
     constexpr int kM = (D + 2 + kGroups3 - 1) / kGroups3;
     const int grp = threadIdx.x / kPlane3, pos = threadIdx.x % kPlane3;
     const int fj = pos / kF3, fk = pos % kF3;
@@ -898,15 +937,6 @@ __global__ __launch_bounds__(kBlock3, D <= 8 ? 8 : 4) void k_pcg_fused(Src src, 
         fd[m] = ld32(Dg, t);
         fa[m] = ld32(as, t);
     }
-    double acc[kFusedRows];
-    // the init launch wrote rows of its own count (r.r, g, d, -, b.b); later launches r.r, g, d
-    load_rows<kFusedRows>(part_in, first ? a.nb_upd : nb, first ? kFusedRows : 3, acc);
-    const int done = c->done, maxeff = c->maxeff;
-    const float g_prev = c->slot_rz[(kit + 1) & 1];
-    const float pap_prev = (float)c->slot_pap[(kit + 1) & 1];
-    const double tol = c->tol;
-    double atol = c->atol;
-    float atol_sq = c->atol_sq;
     // ---- prologue: scalars of iteration k
     double tot[kFusedRows];
     if (reduce_bcast<kFusedRows, kBlock3>(acc, first ? kFusedRows : 3, done, tot, first ? a.nb_upd : nb)) return;
@@ -1151,11 +1181,11 @@ void launch_ir_residual(const Launch& L, int op, float dt, bool first) {
         HFM_LAUNCH_T(k_ir_residual, 0, L, kBlock, nullptr, nullptr, dt, (int)first);
 }
 
-void launch_ir_scale(const Launch& L, int op, bool first) {
+void launch_ir_scale(const Launch& L, int op, int step) {
     if (op == 1)
-        HFM_LAUNCH_T(k_ir_scale, 1, L, kBlock, nullptr, nullptr, (int)first);
+        HFM_LAUNCH_T(k_ir_scale, 1, L, kBlock, nullptr, nullptr, step);
     else
-        HFM_LAUNCH_T(k_ir_scale, 0, L, kBlock, nullptr, nullptr, (int)first);
+        HFM_LAUNCH_T(k_ir_scale, 0, L, kBlock, nullptr, nullptr, step);
 }
 
 }  // namespace hfm

@@ -44,7 +44,7 @@ struct CgCtl {
     int ir_outer;      // refinement steps of this solve so far
     int ir_end;        // 1: the refinement has converged or reached the cap
     int ir_maxiter;    // iteration cap (inner iterations over the whole solve)
-    double ir_rr;      // ||r||^2 (true residual) at the previous refinement step
+    double ir_rr2[2];  // ||r||^2 (true residual) of refinement step s, in slot s & 1
     double src_dt;     // ctl[0]: sum of the source step's temperature rises of this substep (fp64)
 };
 
@@ -394,7 +394,7 @@ void launch_pcg_upd(const Launch& L, int op, int k, hipEvent_t ev0 = nullptr, hi
 // mixed precision: true residual of a refinement step (first: also b.b and the rhs mean removal),
 // then its scaling into the inner solve's FP16 vectors (or the end of the solve)
 void launch_ir_residual(const Launch& L, int op, float dt, bool first);
-void launch_ir_scale(const Launch& L, int op, bool first);
+void launch_ir_scale(const Launch& L, int op, int step);   // step: refinement step, 0 the first
 void launch_bf16_copy(const Scene* s, hipStream_t st);
 
 // ---- multigrid preconditioner of the pressure PCG (hfm_mg.hip) ----

@@ -731,7 +731,7 @@ int run_ir(Batch* b, int op, float dt) {
     const int mg = op == 1 ? mg_depth(b, true) : 0;
     if (mg) launch_mg_coefficients_if_stale(b);
     launch_ir_residual(fam, op, dt, true);
-    launch_ir_scale(fam, op, true);
+    launch_ir_scale(fam, op, 0);
     if (mg) launch_mg_vcycle(b, mg, true);
     CHECK_LAUNCH();
     for (int step = 0;; ++step) {
@@ -752,17 +752,20 @@ int run_ir(Batch* b, int op, float dt) {
             return HFM_ERR_STATE;
         }
         launch_ir_residual(fam, op, dt, false);
-        launch_ir_scale(fam, op, false);
+        launch_ir_scale(fam, op, step + 1);
         if (mg) launch_mg_vcycle(b, mg, true);
         CHECK_LAUNCH();
     }
-    for (Scene* s : b->scenes) {
+    for (size_t si = 0; si < b->scenes.size(); ++si) {
+        Scene* s = b->scenes[si];
         CgMirror* m = &s->mirror[op];
         const int it = __atomic_load_n(&m->it, __ATOMIC_ACQUIRE);
         const int conv = __atomic_load_n(&m->converged, __ATOMIC_ACQUIRE);
         const int outer = __atomic_load_n(&m->outer, __ATOMIC_ACQUIRE);
         if (conv < 0) {
-            set_err("%s solve: non-finite residual", op ? "pressure" : "heat");
+            set_err("%s solve of %s %zu (n = %d) in substep %lld: non-finite residual after %d inner iterations "
+                    "in %d refinement steps", op ? "pressure" : "heat", b->scenes.size() > 1 ? "batch member" : "scene",
+                    si, s->n, (long long)s->stats.substeps + 1, it, outer);
             return HFM_ERR_NONFINITE;
         }
         const int inner = outer > 0 ? (it + outer - 1) / outer : 2;

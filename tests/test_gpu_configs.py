@@ -143,3 +143,27 @@ def test_c5_served_mixed_stream_equals_alone(gpu):
             assert eT <= 1e-3 and eu <= 1e-3 and dm < 1e-3
         print(msg, flush=True)
         alone.close()
+
+
+def test_c4_mixed_form_batch_equals_alone(gpu):
+    """The 64 batched scenes in the mixed form, one batch, against n = 48, 64 and 128 scenes stepped
+    alone.  A batch this large runs each launch in several rounds of workgroups, so a workgroup of a
+    scene can start after workgroup 0 of the same scene finished: every value a launch decides with
+    must come from earlier launches (a refinement step once read its predecessor's residual norm
+    from a slot that its own workgroup 0 rewrote, and the late workgroups then took the solve for
+    stalled)."""
+    scenes, dt, _ = batched_scenes(64)
+    pick = [50, 9, 3, 0]
+    assert [scenes[i][1]["grid"].n for i in pick] == [48, 48, 64, 128]
+    alone = {i: HeatflowMPM(**scenes[i][1], precision="mixed") for i in pick}
+    sims = [HeatflowMPM(**kw, precision="mixed") for _, kw in scenes]
+    with HeatflowBatch(sims) as batch:
+        for step in range(3):
+            batch.substep(dt)
+            for i, a in alone.items():
+                a.substep(dt)
+                for f in ("T", "U", "V", "W", "P"):
+                    assert np.array_equal(a._scene.get_field(f), sims[i]._scene.get_field(f)), (step, i, f)
+                assert a.stats()["p_outer"] == sims[i].stats()["p_outer"], (step, i)
+    for s in list(alone.values()) + sims:
+        s.close()
