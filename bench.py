@@ -244,6 +244,23 @@ def bench_served(a) -> dict:
         err_T, err_u = max(err_T, eT), max(err_u, eu)
         per.append(dict(rid=rb.rid, n=rb.n, substeps=rb.substeps, T=eT, u=eu, p_converged=rb.stats.get("p_converged")))
     worst = sorted(per, key=lambda d: -d["u"])[:4]
+    # error of every served result against the fp64 CPU oracle, for the requests whose oracle run fits (n <= fp64-n)
+    vs_fp64 = []
+    if a.fp64_n:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle as O
+        for rb, q in zip(res, reqs):
+            if rb.n > a.fp64_n:
+                continue
+            orc = O.OracleSim(**q.kwargs, precision="f64")
+            orc.run(q.dt, q.substeps)
+            uo = np.concatenate([orc.field(f).ravel() for f in ("U", "V", "W")])
+            ub = np.concatenate([rb.fields[f].ravel() for f in ("U", "V", "W")])
+            d64, _ = orc.diagnostics()
+            vs_fp64.append(dict(rid=rb.rid, n=rb.n, substeps=rb.substeps, T=rel_l2(rb.fields["T"], orc.field("T")),
+                                u=rel_l2(ub, uo), divergence_max=rb.stats.get("divergence_max"),
+                                divergence_max_fp64=d64))
+            del orc
     return dict(metric="served p50/p99 latency and aggregate cell-substeps/s, continuous batching",
                 value=s_b["cell_substeps_per_s"], unit="cell-substeps/s", n_gpus=1, steps=None, warmup=0,
                 ms_per_step=None, higher_is_better=True, scaling="weak", vs_baseline=None,
@@ -258,6 +275,7 @@ def bench_served(a) -> dict:
                             layout=a.layout,
                             parallelism="1 GPU"),
                 max_rel_l2_vs_plain=dict(T=err_T, u=err_u), worst_requests_vs_plain=worst,
+                vs_fp64_oracle=vs_fp64 or None,
                 served=s_b, plain=s_p, speedup_vs_plain=round(s_b["cell_substeps_per_s"] / s_p["cell_substeps_per_s"], 3),
                 p99_ratio_plain_over_served=round(s_p["p99_s"] / s_b["p99_s"], 3),
                 max_batch=max(e[3] for e in served.log))
@@ -448,7 +466,9 @@ def main() -> int:
                          "batched server (mixed default; the plain form is fp32)")
     ap.add_argument("--layout", choices=("rule", "auto"), default="auto",
                     help="pressure PCG layout: chosen by grid size, or the autotuned table (heatflow_mpm.autotune)")
-    ap.add_argument("--fp64-n", type=int, default=0, help="mixed: grid size of the fp64-oracle error check (0: off)")
+    ap.add_argument("--fp64-n", type=int, default=0,
+                    help="mixed: grid size of the fp64-oracle error check; served: check every result of at most this "
+                         "grid size against the fp64 oracle (0: off)")
     ap.add_argument("--cg-tol", type=float, default=None,
                     help="coupled: pressure PCG tolerance (default 1e-6 of the configuration; the reference's own "
                          "default is 1e-4)")

@@ -104,12 +104,14 @@ class GpuBackend:
 
     def collect(self, sim, keep: bool):
         out = {}
+        st = sim.stats()
         if keep:
             out["T"] = sim.grid_T()
             for f in ("U", "V", "W"):
                 out[f] = sim._scene.get_field(f)
             out["temperatures"] = sim.temperatures().copy()
-        return out, sim.stats()
+            st["divergence_max"] = sim.grid.divergence_max()
+        return out, st
 
     def close_batch(self, batch) -> None:
         batch.close()
