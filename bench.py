@@ -204,10 +204,11 @@ def rel_l2(a, b) -> float:
 def bench_served(a) -> dict:
     """A seeded Poisson stream of rollout requests (n from {48, 64, 96, 128, 192}, K uniform in
     [kmin, kmax], heaters / fans / particle blobs as the batched scenes), served twice on the same
-    arrival times: with continuous batching (all admitted scenes advance in one batched substep;
-    admission in arrival order by device footprint), the multigrid-preconditioned pressure PCG and
-    fp32 state by default, and in the plain form (fp32, the reference's Jacobi PCG, one request at
-    a time, in arrival order).  Latency = completion - arrival.  Each served result is compared with
+    arrival times: with continuous batching (the admitted scenes of a size class advance in one batched
+    substep, the size classes (--lanes) side by side on their own streams; admission in arrival order by
+    device footprint), the multigrid-preconditioned pressure PCG and fp32 state by default, and in the
+    plain form (fp32, the reference's Jacobi PCG, one request at a time, in arrival order).
+    Latency = completion - arrival.  Each served result is compared with
     the plain result of the same request (relative L2 of T and u): both solve to the same relative
     residual, so they agree to the solve tolerance, not in bits."""
     reqs = poisson_requests(a.rate, a.requests, seed=a.seed, k_range=(a.kmin, a.kmax), cg_max_iter=a.cg_max_iter)
@@ -222,7 +223,7 @@ def bench_served(a) -> dict:
     budget = int(a.hbm_gib * (1 << 30))
     prec = a.precision or ("mixed" if scg == "warp" else "fp32")
     served = Server(GpuBackend(cg=scg, precision=prec, layout=None if a.layout == "rule" else "auto-batch"),
-                    hbm_budget=budget, max_active=a.slots, keep_results=True)
+                    hbm_budget=budget, max_active=a.slots, keep_results=True, lanes=_lanes(a))
     t0 = time.perf_counter()
     res = served.run(reqs)
     span = time.perf_counter() - t0
@@ -271,7 +272,8 @@ def bench_served(a) -> dict:
                                      f"{a.kmin}-{a.kmax}, heater 100-800 W, fan 0-3 m/s, 2k-50k particles, "
                                      f"PCG tol 1e-6 (pressure cap {a.cg_max_iter}), dt 2 ms", rate_per_s=a.rate,
                             requests=len(reqs), seed=a.seed, cg_max_iter=a.cg_max_iter,
-                            hbm_budget_gib=a.hbm_gib, slots=a.slots, pressure_cg=scg, plain_cg=a.plain_cg, precision=prec,
+                            hbm_budget_gib=a.hbm_gib, slots=a.slots, lanes=a.lanes, pressure_cg=scg,
+                            plain_cg=a.plain_cg, precision=prec,
                             layout=a.layout,
                             parallelism="1 GPU"),
                 max_rel_l2_vs_plain=dict(T=err_T, u=err_u), worst_requests_vs_plain=worst,
@@ -279,6 +281,13 @@ def bench_served(a) -> dict:
                 served=s_b, plain=s_p, speedup_vs_plain=round(s_b["cell_substeps_per_s"] / s_p["cell_substeps_per_s"], 3),
                 p99_ratio_plain_over_served=round(s_p["p99_s"] / s_b["p99_s"], 3),
                 max_batch=max(e[3] for e in served.log))
+
+
+def _lanes(a):
+    """Size-class lanes of the served form (MAXN:SLOTS each); "none" serves one batch of all requests."""
+    if not a.lanes or a.lanes == ["none"]:
+        return None
+    return [tuple(int(x) for x in lane.split(":")) for lane in a.lanes]
 
 
 def bench_served_sweep(a) -> dict:
@@ -297,7 +306,7 @@ def bench_served_sweep(a) -> dict:
     for rate in a.rates:
         reqs = poisson_requests(rate, a.requests, seed=a.seed, k_range=(a.kmin, a.kmax), cg_max_iter=a.cg_max_iter)
         served = Server(GpuBackend(cg=scg, precision=prec, layout=None if a.layout == "rule" else "auto-batch"),
-                        hbm_budget=budget, max_active=a.slots)
+                        hbm_budget=budget, max_active=a.slots, lanes=_lanes(a))
         t0 = time.perf_counter()
         s_b = summarize(served.run(reqs), time.perf_counter() - t0)
         plain = Server(GpuBackend(cg=a.plain_cg, layout=None if a.layout == "rule" else "auto"), hbm_budget=budget,
@@ -318,7 +327,7 @@ def bench_served_sweep(a) -> dict:
                 config=dict(workload=f"{a.requests} Poisson requests per rate, n in {{48,64,96,128,192}}, K uniform "
                                      f"{a.kmin}-{a.kmax}, PCG tol 1e-6 (pressure cap {a.cg_max_iter}), dt 2 ms",
                             rates=a.rates, seed=a.seed, pressure_cg=scg, plain_cg=a.plain_cg, precision=prec,
-                            hbm_budget_gib=a.hbm_gib, slots=a.slots, parallelism="1 GPU"),
+                            hbm_budget_gib=a.hbm_gib, slots=a.slots, lanes=a.lanes, parallelism="1 GPU"),
                 sweep=rows)
 
 
@@ -482,6 +491,10 @@ def main() -> int:
     ap.add_argument("--kmax", type=int, default=100, help="served: most substeps of a request")
     ap.add_argument("--hbm-gib", type=float, default=64.0, help="served: device memory budget of the scenes")
     ap.add_argument("--slots", type=int, default=64, help="served: most scenes in one batch")
+    ap.add_argument("--lanes", nargs="+", default=["64:64", "128:8", "1024:1"], metavar="MAXN:SLOTS",
+                    help="served: size-class lanes served side by side, each its own continuous batch on its own "
+                         "stream (default: n <= 64 in batches of up to 64, n <= 128 of up to 8, larger ones one at a "
+                         "time); 'none' for one batch of every request")
     ap.add_argument("--scenes", type=int, default=64, help="batched: number of scenes")
     ap.add_argument("--seed", type=int, default=0, help="batched: scene draw seed")
     ap.add_argument("--steps", type=int, default=None)

@@ -15,6 +15,7 @@ whether the request ran alone or in the mix.
 from __future__ import annotations
 
 import collections
+import threading
 import time
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
@@ -133,14 +134,24 @@ class Server:
     """Continuous-batching request loop (see the module docstring).
 
     hbm_budget: device bytes the admitted scenes may hold together; max_active: most scenes in
-    one batch (1 = the plain form: one request at a time, in arrival order)."""
+    one batch (1 = the plain form: one request at a time, in arrival order).
+
+    lanes: optional size classes served side by side, as [(largest n, most scenes in its batch), ...].
+    A request goes to the first lane whose largest n it does not exceed; each lane runs its own
+    continuous batch on its own stream from its own host thread, so a lane of small scenes keeps
+    stepping while a large scene's solve runs on another (the GPU runs the lanes' kernels
+    concurrently), and small requests do not wait on the slowest scene of one shared batch.  All
+    lanes admit against the one device budget.  Results are the same bits as without lanes."""
 
     def __init__(self, backend=None, hbm_budget: int = 64 << 30, max_active: int = 64, keep_results: bool = False,
-                 prepare_workers: int = 2, clock=time.perf_counter, sleep=time.sleep):
+                 prepare_workers: int = 2, clock=time.perf_counter, sleep=time.sleep, lanes=None):
         self.backend = backend if backend is not None else GpuBackend()
         self.budget, self.max_active, self.keep = int(hbm_budget), int(max_active), keep_results
         self.workers, self.clock, self.sleep = prepare_workers, clock, sleep
-        self.log = []   # (time, event, rid, active scenes, bytes in use)
+        self.lanes = sorted((int(m), int(k)) for m, k in lanes) if lanes else None
+        self.log = []   # (time, event, rid, active scenes of the lane, bytes in use)
+        self._lock = threading.Lock()
+        self._used = 0
 
     def run(self, requests) -> list:
         reqs = sorted(requests, key=lambda r: (r.arrival, r.rid))
@@ -153,13 +164,35 @@ class Server:
             if footprint_estimate(r.kwargs["grid"].n, len(r.kwargs["particles"]), prec, cg) > self.budget:
                 raise ValueError(f"request {r.rid} does not fit the device budget on its own")
         dt = dts.pop() if dts else 2e-3
+        self._used = 0
+        pool = ThreadPoolExecutor(max_workers=max(1, self.workers))
+        t0 = self.clock()
+        try:
+            if not self.lanes:
+                results = self._serve(reqs, self.max_active, dt, t0, pool, prec, cg)
+            else:
+                parts = [[] for _ in self.lanes]
+                for r in reqs:
+                    n = r.kwargs["grid"].n
+                    lane = next((i for i, (m, _) in enumerate(self.lanes) if n <= m), None)
+                    if lane is None:
+                        raise ValueError(f"request {r.rid} (n = {n}) is larger than every lane")
+                    parts[lane].append(r)
+                with ThreadPoolExecutor(max_workers=len(self.lanes)) as lanes:
+                    futs = [lanes.submit(self._serve, part, k, dt, t0, pool, prec, cg)
+                            for part, (_, k) in zip(parts, self.lanes) if part]
+                    results = [x for f in futs for x in f.result()]
+        finally:
+            pool.shutdown(wait=True)
+        return sorted(results, key=lambda r: r.rid)
+
+    def _serve(self, reqs, max_active: int, dt: float, t0: float, pool, prec: str, cg: str) -> list:
+        """One continuous batch over `reqs` (in arrival order), admitting against the shared budget."""
         pending = collections.deque(reqs)
         waiting = collections.deque()
         prepared = {}
         active, results = [], []
-        batch, dirty, used = None, False, 0
-        pool = ThreadPoolExecutor(max_workers=max(1, self.workers))
-        t0 = self.clock()
+        batch, dirty = None, False
         try:
             while pending or waiting or active:
                 now = self.clock() - t0
@@ -168,23 +201,26 @@ class Server:
                     waiting.append(r)
                     prepared[r.rid] = pool.submit(self.backend.prepare, r)
                 # admission: arrival order, while the head of the queue fits
-                while waiting and len(active) < self.max_active:
+                while waiting and len(active) < max_active:
                     r = waiting[0]
                     est = footprint_estimate(r.kwargs["grid"].n, len(r.kwargs["particles"]), prec, cg)
-                    if used + est > self.budget:
-                        break
+                    with self._lock:
+                        if self._used + est > self.budget:
+                            break
+                        self._used += est
                     waiting.popleft()
                     sim = self.backend.create(r, prepared.pop(r.rid).result())
                     nbytes = self.backend.bytes(sim)
-                    used += nbytes
+                    with self._lock:
+                        self._used += nbytes - est
+                        used = self._used
                     active.append(_Slot(r, sim, r.substeps, self.clock() - t0, nbytes))
                     dirty = True
                     self.log.append((self.clock() - t0, "admit", r.rid, len(active), used))
                 if not active:
-                    if pending:
-                        wait = pending[0].arrival - (self.clock() - t0)
-                        if wait > 0:
-                            self.sleep(min(wait, 0.01))
+                    wait = (pending[0].arrival if pending else now + 0.01) - (self.clock() - t0)
+                    if wait > 0:
+                        self.sleep(min(wait, 0.01))
                     continue
                 if dirty:
                     if batch is None:
@@ -205,7 +241,9 @@ class Server:
                         results.append(Result(s.req.rid, self.backend.n_of(s.sim), s.req.substeps, s.req.arrival,
                                               s.admitted, fin, s.nbytes, fields, stats))
                         self.backend.close_sim(s.sim)
-                        used -= s.nbytes
+                        with self._lock:
+                            self._used -= s.nbytes
+                            used = self._used
                         self.log.append((fin, "finish", s.req.rid, len(active) - len(done), used))
                     active = [s for s in active if s.remaining > 0]
         finally:
@@ -213,8 +251,7 @@ class Server:
                 self.backend.close_batch(batch)
             for s in active:
                 self.backend.close_sim(s.sim)
-            pool.shutdown(wait=True)
-        return sorted(results, key=lambda r: r.rid)
+        return results
 
 
 def poisson_requests(rate: float, count: int, seed: int = 0, sizes=(48, 64, 96, 128, 192), k_range=(50, 1000),

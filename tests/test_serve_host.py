@@ -138,3 +138,24 @@ def test_multigrid_footprint_covers_the_coarse_levels():
             m //= 2
             coarse += m ** 3
         assert footprint_estimate(n, 1000, cg="mg") - footprint_estimate(n, 1000) >= 24 * coarse
+
+
+def test_lanes_keep_size_classes_apart_and_serve_every_request():
+    """Size-class lanes: every request runs its substeps, a lane's batches hold its size class only,
+    and the shared budget is never exceeded."""
+    clock = Clock()
+    be = FakeBackend(clock)
+    reqs = _reqs(16, seed=3)
+    budget = 6 * footprint_estimate(32, 100)
+    srv = Server(be, hbm_budget=budget, keep_results=True, clock=clock, sleep=clock.sleep, lanes=[(24, 8), (32, 1)])
+    res = srv.run(reqs)
+    assert [r.rid for r in res] == [q.rid for q in reqs]
+    for r, q in zip(res, reqs):
+        assert r.fields["steps"] == q.substeps
+    by_rid = {q.rid: q.kwargs["grid"].n for q in reqs}
+    for b in be.batches:
+        sizes = {by_rid[rid] for rid in b}
+        assert sizes <= {16, 24} or (sizes == {32} and len(b) == 1), b
+    assert max(e[4] for e in srv.log) <= budget
+    with pytest.raises(ValueError, match="larger than every lane"):
+        Server(be, lanes=[(16, 4)], clock=clock, sleep=clock.sleep).run(reqs)
