@@ -44,12 +44,14 @@ HBM_PEAK_GBS = 8000.0
 #   dir:   reads z, p_old, alpha_s; writes p, Ap                     -> 5 words
 #   upd:   reads x, p, r, Ap, D; writes x, r, z                        -> 8 words
 #   fused: reads x, r, p_old, s_old, w, D, alpha_s; writes x, r, p, s, w -> 12 words (whole iteration)
-KERNEL_WORDS = {_abi.KT_PCG_DIR: 5, _abi.KT_PCG_UPD: 8, _abi.KT_PCG_FUSED: 12}
+#   persistent: the fused iteration's algorithmic bytes, per iteration of its one launch per solve (it
+#   keeps the vectors on chip and moves only tile faces and partial sums through memory)
+KERNEL_WORDS = {_abi.KT_PCG_DIR: 5, _abi.KT_PCG_UPD: 8, _abi.KT_PCG_FUSED: 12, _abi.KT_PCG_PERSIST: 12}
 # Mixed precision, inner solve (bytes per cell): pass A reads z, p_old (FP16) and alpha_s, writes
 # p, Ap (FP16) -> 12 B; pass B reads x (fp32), p, r, Ap (FP16), D (fp32), writes x, r, z -> 22 B
 KERNEL_BYTES_MIXED = {_abi.KT_PCG_DIR: 12, _abi.KT_PCG_UPD: 22}
 KERNEL_NAMES = {_abi.KT_PCG_DIR: "k_pcg_dir_tile3", _abi.KT_PCG_UPD: "k_pcg_upd<1>",
-                _abi.KT_PCG_FUSED: "k_pcg_fused"}
+                _abi.KT_PCG_FUSED: "k_pcg_fused", _abi.KT_PCG_PERSIST: "k_pcg_persist"}
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "pcg_pressure_n96_traffic.json")
 
 
@@ -560,7 +562,7 @@ def main() -> int:
     I_p = (st1["p_iters_total"] - st0["p_iters_total"]) / a.steps
     I_h = (st1["h_iters_total"] - st0["h_iters_total"]) / a.steps
     value = N * a.steps / wall
-    hbm_impl = (substep_bytes(N, P, I_p, I_h, sim.layout["cg"] == "fused") * a.steps / wall / 1e12
+    hbm_impl = (substep_bytes(N, P, I_p, I_h, sim.layout["cg"] in ("fused", "persistent")) * a.steps / wall / 1e12
                 if prec == "fp32" else None)
     B = contract_bytes(N, P, I_p, I_h)
     hbm_tbs = B * a.steps / wall / 1e12 if prec == "fp32" else None

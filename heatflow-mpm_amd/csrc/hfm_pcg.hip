@@ -285,20 +285,22 @@ __device__ __forceinline__ void store_partials(double (&v)[NV], double* __restri
 // The first entry of each row is taken as the thread's value (no add), so nothing waits for these
 // loads before the reduction uses them: loads issued after them (a kernel's data) stay in flight
 // meanwhile.  Further entries (rows longer than the workgroup) are added in order, four per round
-// with their loads issued together.
-template <int NV>
+// with their loads issued together.  AGENT: the rows are read with agent-scope loads (rows that
+// other workgroups of the same launch wrote); the sums are the same.
+template <int NV, bool AGENT = false>
 __device__ __forceinline__ void load_rows(const double* P, int nb, int nv, double (&acc)[NV]) {
+    auto rd = [P](int i) -> double { return AGENT ? load_agent(P + i) : P[i]; };
     const int b0 = threadIdx.x, bd = blockDim.x;
 #pragma unroll
-    for (int q = 0; q < NV; ++q) acc[q] = (q < nv && b0 < nb) ? P[q * nb + b0] : 0.0;
+    for (int q = 0; q < NV; ++q) acc[q] = (q < nv && b0 < nb) ? rd(q * nb + b0) : 0.0;
     int b = b0 + bd;
 #pragma unroll 1
     for (; b + 3 * bd < nb; b += 4 * bd) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
             if (q < nv) {
-                const double v0 = P[q * nb + b], v1 = P[q * nb + b + bd], v2 = P[q * nb + b + 2 * bd],
-                             v3 = P[q * nb + b + 3 * bd];
+                const double v0 = rd(q * nb + b), v1 = rd(q * nb + b + bd), v2 = rd(q * nb + b + 2 * bd),
+                             v3 = rd(q * nb + b + 3 * bd);
                 acc[q] = acc[q] + v0;
                 acc[q] = acc[q] + v1;
                 acc[q] = acc[q] + v2;
@@ -310,7 +312,7 @@ __device__ __forceinline__ void load_rows(const double* P, int nb, int nv, doubl
     for (; b < nb; b += bd)
 #pragma unroll
         for (int q = 0; q < NV; ++q)
-            if (q < nv) acc[q] += P[q * nb + b];
+            if (q < nv) acc[q] += rd(q * nb + b);
 }
 
 // nb: the number of row entries; a wave whose threads all lie past it holds zeros and skips its tree
@@ -864,16 +866,132 @@ __global__ __launch_bounds__(kBlock) void k_ir_scale(Src src, int step) {
 // In fp32 the iteration counts equal the two-pass form's and the solutions agree to ~1e-7 on the
 // pressure systems of the coupled and plume scenes.
 //
-// Launch k stages, on a 16 x 16 x D block framed by a one-cell halo (as pass A), the values that make
-// u_{k+1}: r_k, w_k, s_{k-1} and D at every frame cell; it forms s_k, r_{k+1} and u_{k+1} there (a frame
-// cell outside the box takes the clamped in-box cell's values, so its stencil term is +0), keeps
-// u_{k+1} and the solid fraction in LDS, writes r_{k+1}, s_k, p_k and x_{k+1} for the block's own cells,
-// and then w_{k+1} = A u_{k+1} from LDS.  Neighbouring blocks read r, w and s at halo cells while the
-// owners write them, so those three are double-buffered by iteration parity (r_k, w_k in buffer k & 1,
-// s_k in buffer k & 1); p and x are read and written by their owners only.  Memory per iteration: reads
-// r, w, s, D, alpha_s, x, p; writes r, w, s, p, x (12 words per cell), one launch and one reduction
-// instead of two of each.
+// Launch k works on a 16 x 16 x D block framed by a one-cell halo (as pass A).  Every frame cell gets
+// s_k, r_{k+1} and u_{k+1} from r_k, w_k, s_{k-1} and D (a frame cell outside the box takes the clamped
+// in-box cell's values, so its stencil term is +0); u_{k+1} and the solid fraction go to LDS, and then
+// w_{k+1} = A u_{k+1} is formed from LDS.  The halo cells (the frame minus the block: the two end planes
+// and the 68-cell rings of the inner planes) are handled in a compact numbering, at most two per
+// thread; the block's own cells in the stencil's layout, D / 4 per thread, which also writes their
+// r_{k+1}, s_k, p_k, x_{k+1} and then w_{k+1}, and sums r.u and r.r (then w.u) over its cells in that
+// order.  Neighbouring blocks read r, w and s at halo cells while the owners write them, so those three
+// are double-buffered by iteration parity (r_k, w_k in buffer k & 1, s_k in buffer k & 1); p and x are
+// read and written by their owners only.  Memory per iteration: reads r, w, s, D, alpha_s, x, p; writes
+// r, w, s, p, x (12 words per cell), one launch and one reduction instead of two of each.
 // ---------------------------------------------------------------------------------------------
+template <int D>
+constexpr int kHalo = 2 * kPlane3 + D * (kPlane3 - kT3 * kT3);
+
+// Frame coordinates of halo cell h: the end planes fi = 0 and D + 1 (18 x 18 each), then for each inner
+// plane its ring: the rows fj = 0 and 17, then the columns fk = 0 and 17 of rows 1 .. 16.
+template <int D>
+__device__ __forceinline__ void halo_cell(int h, int& fi, int& fj, int& fk) {
+    constexpr int kRing = kPlane3 - kT3 * kT3;   // 68
+    if (h < 2 * kPlane3) {
+        const bool lo = h < kPlane3;
+        const int q = lo ? h : h - kPlane3;
+        fi = lo ? 0 : D + 1;
+        fj = q / kF3;
+        fk = q % kF3;
+    } else {
+        const int q = h - 2 * kPlane3;
+        fi = 1 + q / kRing;
+        const int r = q % kRing;
+        if (r < 2 * kF3) {
+            fj = r < kF3 ? 0 : kF3 - 1;
+            fk = r < kF3 ? r : r - kF3;
+        } else {
+            const int col = r - 2 * kF3;
+            fj = 1 + (col >> 1);
+            fk = (col & 1) ? kF3 - 1 : 0;
+        }
+    }
+}
+
+// Memory index of frame cell (fi, fj, fk) of the block at (i0, j0, k0), clamped into the box.
+__device__ __forceinline__ unsigned frame_index(int i0, int j0, int k0, int fi, int fj, int fk, int n) {
+    const unsigned ci = (unsigned)clampi(i0 + fi - 1, 0, n - 1), cj = (unsigned)clampi(j0 + fj - 1, 0, n - 1),
+                   ck = (unsigned)clampi(k0 + fk - 1, 0, n - 1);
+    return (ci * (unsigned)n + cj) * (unsigned)n + ck;
+}
+
+// Scalars of iteration k from the reduced rows r.r, g, d (and b.b, row 4, at k = 0), with the breakdown
+// stop of the two-pass form: r.z of the previous iteration or p.Ap not positive, beta or alpha not
+// finite, or a non-finite r.r.
+struct CgScalars {
+    float rr, g, alpha, beta, pap;
+    bool conv, brk, stop;
+};
+__device__ __forceinline__ CgScalars cg_scalars(const double* tot, bool first, float g_prev, float pap_prev,
+                                                double tol, double& atol, float& atol_sq, int kit, int maxeff) {
+    CgScalars q;
+    q.rr = (float)tot[0];
+    q.g = (float)tot[1];
+    const float dd = (float)tot[2];
+    if (first) {
+        const float bb = (float)tot[4];
+        atol = tol * sqrt((double)bb);   // warp: atol = max(tol * ||b||, tol)
+        if (atol < tol) atol = tol;
+        atol_sq = (float)(atol * atol);
+    }
+    q.conv = q.rr <= atol_sq;
+    q.beta = 0.0f;
+    q.pap = dd;
+    if (!first && q.rr > atol_sq) {
+        q.beta = q.g / g_prev;
+        q.pap = dd - (q.beta * q.beta) * pap_prev;
+    }
+    q.alpha = q.rr > atol_sq ? q.g / q.pap : 0.0f;
+    q.brk = !(q.rr <= 3.4028235e38f) ||
+            (q.rr > atol_sq && ((!first && !(g_prev > 0.0f)) || !(fabsf(q.beta) <= 3.4028235e38f) ||
+                                !(q.pap > 0.0f) || !(fabsf(q.alpha) <= 3.4028235e38f)));
+    q.stop = q.conv || kit >= maxeff || q.brk;
+    return q;
+}
+
+// Workgroup 0's record of iteration k's scalars, and of the solve's end.
+__device__ __forceinline__ void cg_record(CgCtl* c, CgMirror* mirror, const CgScalars& q, bool first, int kit,
+                                          double atol, float atol_sq) {
+    c->slot_rr[kit & 1] = q.rr;
+    c->slot_rz[kit & 1] = q.g;
+    c->slot_pap[kit & 1] = (double)q.pap;
+    if (first) {
+        c->atol = atol;
+        c->atol_sq = atol_sq;
+    }
+    if (q.stop) {
+        const int cv = q.conv ? 1 : (q.brk ? -1 : 0);
+        c->done = 1;
+        c->converged = cv;
+        c->it = kit;
+        mirror_final(mirror, kit, q.rr, atol, cv);
+    }
+}
+
+// s_k, r_{k+1} and (returned) u_{k+1} of a frame cell
+__device__ __forceinline__ float cg_frame(float r, float w, float s_old, float d, bool first, float alpha, float beta,
+                                          float& s, float& r1) {
+    s = first ? w : w + beta * s_old;
+    r1 = r - alpha * s;
+    return d * r1;
+}
+
+// w = A u at block cell (li, lj, lk), from u (returned in uc) and alpha_s on the frame in LDS
+template <int D>
+__device__ __forceinline__ float cg_stencil(const float (*U)[kF3][kF3], const float (*S)[kF3][kF3], int li, int lj,
+                                            int lk, float amin, float hh, float rhh, int fastdiv, float& uc) {
+    const int si = li + 1, sj = lj + 1, sk = lk + 1;
+    uc = U[si][sj][sk];
+    const float ac = S[si][sj][sk];
+    float acc6 = 0.0f;
+    acc6 += face_fraction(S[si - 1][sj][sk], ac, amin) * (uc - U[si - 1][sj][sk]);
+    acc6 += face_fraction(ac, S[si + 1][sj][sk], amin) * (uc - U[si + 1][sj][sk]);
+    acc6 += face_fraction(S[si][sj - 1][sk], ac, amin) * (uc - U[si][sj - 1][sk]);
+    acc6 += face_fraction(ac, S[si][sj + 1][sk], amin) * (uc - U[si][sj + 1][sk]);
+    acc6 += face_fraction(S[si][sj][sk - 1], ac, amin) * (uc - U[si][sj][sk - 1]);
+    acc6 += face_fraction(ac, S[si][sj][sk + 1], amin) * (uc - U[si][sj][sk + 1]);
+    return div_const(1.0f * acc6, hh, rhh, fastdiv);
+}
+
 template <int D, class Src>
 __global__ __launch_bounds__(kBlock3, D <= 8 ? 8 : 4) void k_pcg_fused(Src src, int kit, unsigned long long* pace,
                                                                        unsigned tag) {
@@ -904,7 +1022,6 @@ __global__ __launch_bounds__(kBlock3, D <= 8 ? 8 : 4) void k_pcg_fused(Src src, 
     const float* __restrict__ as = a.alpha_s;
     const double* part_in = a.partials + (long)(kit & 1) * kFusedRows * kMaxDirBlocks;
     double* part_out = a.partials + (long)((kit + 1) & 1) * kFusedRows * kMaxDirBlocks;
-    const unsigned nn = (unsigned)n * n;
     // ---- the prologue's operands first (vector loads complete in issue order, so the prologue then
     // waits for these only): the partial sums of launch k - 1 (the init launch wrote rows of its own
     // count: r.r, g, d, -, b.b; later launches r.r, g, d) and the control scalars
@@ -916,139 +1033,101 @@ __global__ __launch_bounds__(kBlock3, D <= 8 ? 8 : 4) void k_pcg_fused(Src src, 
     const double tol = c->tol;
     double atol = c->atol;
     float atol_sq = c->atol_sq;
-    // ---- then the frame, in flight during the prologue: each of 972 threads owns one (j, k) position
-    // of the 18 x 18 frame plane and steps through the planes
-    constexpr int kM = (D + 2 + kGroups3 - 1) / kGroups3;
-    const int grp = threadIdx.x / kPlane3, pos = threadIdx.x % kPlane3;
-    const int fj = pos / kF3, fk = pos % kF3;
-    const int gj = j0 + fj - 1, gk = k0 + fk - 1;
-    const bool own_jk = grp < kGroups3 && fj >= 1 && fj <= kT3 && fk >= 1 && fk <= kT3 && gj < n && gk < n;
-    const unsigned t_jk = (unsigned)clampi(gj, 0, n - 1) * n + clampi(gk, 0, n - 1);
-    float fr[kM], fw[kM], fs[kM], fd[kM], fa[kM];
-    bool own[kM];
+    // ---- then the frame, in flight during the prologue: halo cells, then the block's own cells
+    constexpr int kH = kHalo<D>, kHM = (kH + kBlock3 - 1) / kBlock3, kO = D / 4;
+    // (threads past the last halo cell load the last one's values and do not use them: no branch, so
+    // every load issues at once)
+    float hr[kHM], hw[kHM], hs[kHM], hd[kHM], ha[kHM];
 #pragma unroll
-    for (int m = 0; m < kM; ++m) {
-        const int fi = grp + kGroups3 * m, gi = i0 + fi - 1;
-        const unsigned t = (unsigned)clampi(gi, 0, n - 1) * nn + t_jk;
-        own[m] = own_jk && fi >= 1 && fi <= D && gi < n;
-        fr[m] = ld32(rk, t);
-        fw[m] = ld32(wk, t);
-        fs[m] = first ? 0.0f : ld32(sprev, t);
-        fd[m] = ld32(Dg, t);
-        fa[m] = ld32(as, t);
+    for (int q = 0; q < kHM; ++q) {
+        const int h = (int)threadIdx.x + q * kBlock3;
+        int fi, fj, fk;
+        halo_cell<D>(h < kH ? h : kH - 1, fi, fj, fk);
+        const unsigned t = frame_index(i0, j0, k0, fi, fj, fk, n);
+        hr[q] = ld32(rk, t);
+        hw[q] = ld32(wk, t);
+        hs[q] = first ? 0.0f : ld32(sprev, t);
+        hd[q] = ld32(Dg, t);
+        ha[q] = ld32(as, t);
+    }
+    const int lk = threadIdx.x % kT3, lj = (threadIdx.x / kT3) % kT3, lq = threadIdx.x / (kT3 * kT3);
+    const int j = j0 + lj, k = k0 + lk;
+    float orr[kO], ow[kO], os[kO], od[kO], oa[kO];
+#pragma unroll
+    for (int m = 0; m < kO; ++m) {
+        const unsigned t = frame_index(i0, j0, k0, lq + 4 * m + 1, lj + 1, lk + 1, n);
+        orr[m] = ld32(rk, t);
+        ow[m] = ld32(wk, t);
+        os[m] = first ? 0.0f : ld32(sprev, t);
+        od[m] = ld32(Dg, t);
+        oa[m] = ld32(as, t);
     }
     // ---- prologue: scalars of iteration k
     double tot[kFusedRows];
     if (reduce_bcast<kFusedRows, kBlock3>(acc, first ? kFusedRows : 3, done, tot, first ? a.nb_upd : nb)) return;
-    const float rr = (float)tot[0];
-    const float g = (float)tot[1];
-    const float dd = (float)tot[2];
-    if (first) {
-        const float bb = (float)tot[4];
-        atol = tol * sqrt((double)bb);  // warp: atol = max(tol * ||b||, tol)
-        if (atol < tol) atol = tol;
-        atol_sq = (float)(atol * atol);
+    const CgScalars q = cg_scalars(tot, first, g_prev, pap_prev, tol, atol, atol_sq, kit, maxeff);
+    if (lb == 0 && threadIdx.x == 0) cg_record(c, a.mirror + 1, q, first, kit, atol, atol_sq);
+    if (q.stop) return;
+    // x and p of the own cells, needed only after the frame pass: loaded now (fewer registers live
+    // across the prologue)
+    float ox[kO], op[kO];
+#pragma unroll
+    for (int m = 0; m < kO; ++m) {
+        const int i = i0 + lq + 4 * m;
+        const bool in_box = i < n && j < n && k < n;
+        const unsigned t = frame_index(i0, j0, k0, lq + 4 * m + 1, lj + 1, lk + 1, n);
+        ox[m] = in_box ? ld32((const float*)x, t) : 0.0f;
+        op[m] = in_box && !first ? ld32((const float*)pv, t) : 0.0f;
     }
-    const bool conv = rr <= atol_sq;
-    float beta = 0.0f, pap = dd;
-    if (!first && rr > atol_sq) {
-        beta = g / g_prev;
-        pap = dd - (beta * beta) * pap_prev;
-    }
-    const float alpha = rr > atol_sq ? g / pap : 0.0f;
-    // breakdown stop, as the two-pass form: r.z of the previous iteration or p.Ap not positive, beta
-    // or alpha not finite, or a non-finite r.r
-    const bool brk = !(rr <= 3.4028235e38f) ||
-                     (rr > atol_sq && ((!first && !(g_prev > 0.0f)) || !(fabsf(beta) <= 3.4028235e38f) ||
-                                       !(pap > 0.0f) || !(fabsf(alpha) <= 3.4028235e38f)));
-    const bool stop = conv || kit >= maxeff || brk;
-    if (lb == 0 && threadIdx.x == 0) {
-        c->slot_rr[kit & 1] = rr;
-        c->slot_rz[kit & 1] = g;
-        c->slot_pap[kit & 1] = (double)pap;
-        if (first) {
-            c->atol = atol;
-            c->atol_sq = atol_sq;
-        }
-        if (stop) {
-            c->done = 1;
-            c->converged = conv ? 1 : (brk ? -1 : 0);
-            c->it = kit;
-            mirror_final(a.mirror + 1, kit, rr, atol, conv ? 1 : (brk ? -1 : 0));
+    // ---- halo cells: u_{k+1}
+#pragma unroll
+    for (int qh = 0; qh < kHM; ++qh) {
+        const int h = (int)threadIdx.x + qh * kBlock3;
+        if (h < kH) {
+            int fi, fj, fk;
+            halo_cell<D>(h, fi, fj, fk);
+            float s, r1;
+            U[fi][fj][fk] = cg_frame(hr[qh], hw[qh], hs[qh], hd[qh], first, q.alpha, q.beta, s, r1);
+            S[fi][fj][fk] = ha[qh];
         }
     }
-    if (stop) return;
-    // x and p of the block's own cells, needed only after the frame pass: loaded now (fewer registers
-    // live across the prologue), consumed after the frame pass
-    float fx[kM], fp[kM], uk[kM];
+    // ---- own cells: s_k, r_{k+1}, u_{k+1}, p_k, x_{k+1}
+    double sum_g = 0.0, sum_rr = 0.0, sum_d = 0.0;   // r.u, r.r, w.u of step k + 1
 #pragma unroll
-    for (int m = 0; m < kM; ++m) {
-        const unsigned t = (unsigned)(i0 + grp + kGroups3 * m - 1) * nn + t_jk;
-        fx[m] = own[m] ? ld32((const float*)x, t) : 0.0f;
-        fp[m] = own[m] && !first ? ld32((const float*)pv, t) : 0.0f;
-    }
-    // ---- frame cells: s_k, r_{k+1}, u_{k+1}
-    double sum[3] = {0.0, 0.0, 0.0};   // g, d, r.r of step k + 1
-    if (grp < kGroups3) {
-#pragma unroll
-        for (int m = 0; m < kM; ++m) {
-            const int fi = grp + kGroups3 * m;
-            if (fi < D + 2) {
-                const float s = first ? fw[m] : fw[m] + beta * fs[m];
-                const float r1 = fr[m] - alpha * s;
-                const float u1 = fd[m] * r1;
-                U[fi][fj][fk] = u1;
-                S[fi][fj][fk] = fa[m];
-                uk[m] = fd[m] * fr[m];
-                if (own[m]) {
-                    const unsigned t = (unsigned)(i0 + fi - 1) * nn + t_jk;
-                    st32(rn, t, r1);
-                    st32(sk, t, s);
-                    sum[0] += (double)(r1 * u1);
-                    sum[2] += (double)(r1 * r1);
-                }
-            }
-        }
-    }
-    // ---- p_k and x_{k+1} of the block's own cells
-#pragma unroll
-    for (int m = 0; m < kM; ++m) {
-        if (own[m]) {
-            const unsigned t = (unsigned)(i0 + grp + kGroups3 * m - 1) * nn + t_jk;
-            const float pk = first ? uk[m] : uk[m] + beta * fp[m];
+    for (int m = 0; m < kO; ++m) {
+        const int li = lq + 4 * m, i = i0 + li;
+        float s, r1;
+        const float u1 = cg_frame(orr[m], ow[m], os[m], od[m], first, q.alpha, q.beta, s, r1);
+        U[li + 1][lj + 1][lk + 1] = u1;
+        S[li + 1][lj + 1][lk + 1] = oa[m];
+        if (i < n && j < n && k < n) {
+            const unsigned t = ((unsigned)i * n + j) * n + k;
+            const float uk = od[m] * orr[m];
+            const float pk = first ? uk : uk + q.beta * op[m];
+            st32(rn, t, r1);
+            st32(sk, t, s);
             st32(pv, t, pk);
-            st32(x, t, fx[m] + alpha * pk);
+            st32(x, t, ox[m] + q.alpha * pk);
+            sum_g += (double)(r1 * u1);
+            sum_rr += (double)(r1 * r1);
         }
     }
     __syncthreads();
     // ---- w_{k+1} = A u_{k+1} on the block, from LDS
     const float amin = a.amin, hh = a.hh, rhh = a.rhh;
     const int fastdiv = a.fastdiv;
-    const int lk = threadIdx.x % kT3, lj = (threadIdx.x / kT3) % kT3, lq = threadIdx.x / (kT3 * kT3);
-    const int j = j0 + lj, k = k0 + lk;
-    if (j < n && k < n) {
-        const int sj = lj + 1, sk1 = lk + 1;
-        const unsigned tb = ((unsigned)i0 * n + j) * n + k;
 #pragma unroll
-        for (int m = 0; m < D / 4; ++m) {
-            const int li = lq + 4 * m, i = i0 + li;
-            if (i >= n) break;
-            const int si = li + 1;
-            const float uc = U[si][sj][sk1], ac = S[si][sj][sk1];
-            float acc6 = 0.0f;
-            acc6 += face_fraction(S[si - 1][sj][sk1], ac, amin) * (uc - U[si - 1][sj][sk1]);
-            acc6 += face_fraction(ac, S[si + 1][sj][sk1], amin) * (uc - U[si + 1][sj][sk1]);
-            acc6 += face_fraction(S[si][sj - 1][sk1], ac, amin) * (uc - U[si][sj - 1][sk1]);
-            acc6 += face_fraction(ac, S[si][sj + 1][sk1], amin) * (uc - U[si][sj + 1][sk1]);
-            acc6 += face_fraction(S[si][sj][sk1 - 1], ac, amin) * (uc - U[si][sj][sk1 - 1]);
-            acc6 += face_fraction(ac, S[si][sj][sk1 + 1], amin) * (uc - U[si][sj][sk1 + 1]);
-            const float w1 = div_const(1.0f * acc6, hh, rhh, fastdiv);
-            st32(wn, tb + (unsigned)li * nn, w1);
-            sum[1] += (double)(w1 * uc);
+    for (int m = 0; m < kO; ++m) {
+        const int li = lq + 4 * m, i = i0 + li;
+        if (i < n && j < n && k < n) {
+            float uc;
+            const float w1 = cg_stencil<D>(U, S, li, lj, lk, amin, hh, rhh, fastdiv, uc);
+            st32(wn, ((unsigned)i * n + j) * n + k, w1);
+            sum_d += (double)(w1 * uc);
         }
     }
     // rows r.r, g, d in the order the prologue reads them
-    double out[3] = {sum[2], sum[0], sum[1]};
+    double out[3] = {sum_rr, sum_g, sum_d};
     store_partials<3, kBlock3>(out, part_out, nb, lb, lds);
 }
 
@@ -1078,6 +1157,253 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init_fused(Src src) {
         s[4] += (double)(bc * bc);
     }
     store_partials<kFusedRows>(s, a.partials, v.nb_upd, lb, lds);
+    // the persistent form's partial-sum slots start empty
+    if (a.cg_persist) {
+        double* slots = a.partials + kPersistSlots;
+        for (int q = lb * kBlock + (int)threadIdx.x; q < 9 * a.nb_dir; q += v.nb_upd * kBlock)
+            slots[q] = __builtin_bit_cast(double, kEmptySum);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Persistent pressure PCG (HFM_FLAG_PERSISTENT_CG): every iteration of k_pcg_fused in one
+// cooperative launch of one workgroup per tile, all of them resident for the whole solve.  A
+// workgroup keeps its own cells' r, w, s, p, x and D in registers (the stencil's layout, D / 4 cells per
+// thread), D of its halo cells in registers and the frame's solid fraction in LDS, so memory carries
+// only what a neighbour needs: the r, s and w of the cells on the tile's faces, and the three partial
+// sums.  The launch boundary that separates two iterations of the one-launch form becomes the partial
+// sums themselves: iteration k stores its faces, waits for those stores to complete, then stores its
+// three sums in slot k % 3; iteration k + 1 starts when no sum of that slot reads as empty (a signalling
+// NaN pattern, which no arithmetic produces).  A workgroup runs at most one iteration ahead of any
+// other (it cannot pass an iteration before every workgroup has posted its sums), so the faces keep the
+// one-launch form's two parity buffers, and a workgroup empties its entries of slot (k + 1) % 3 as
+// iteration k starts: every reader of what they held (the sums of iteration k - 2) has passed
+// iteration k - 1, and the next sums they take are those of iteration k + 1, posted after the emptying
+// (stores of one thread complete in order of its waits).  Every cross-workgroup access is a relaxed
+// agent-scope atomic, coherent across the XCDs without L2 write-back or invalidation.
+//
+// Each workgroup computes what its launch of k_pcg_fused computes, from the same values, in the same
+// order: the same tiles and layouts, the same per-workgroup partial sums and the same reduction of
+// them, so the iterates, the scalars and the iteration count are the one-launch form's bit for bit.
+// Every workgroup reaches the stop decision from the same sums at the same iteration; the final x and
+// p go to memory where the one-launch form leaves them.  A wait is bounded: a workgroup whose sums do
+// not arrive within `spin_limit` polls reports the solve as stopped with converged = -2 (the host turns
+// that into an error) and ends, and so, in turn, does every workgroup waiting on it.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ float load_agent_f(const float* p, unsigned i) {
+    return __hip_atomic_load(const_cast<float*>(p) + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_agent_f(float* p, unsigned i, float v) {
+    __hip_atomic_store(p + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool empty_sum(double v) {
+    return __builtin_bit_cast(unsigned long long, v) == kEmptySum;
+}
+
+// Rows r.r, g, d of slot P (row stride nb): thread b < nb polls its entries until none reads as empty and
+// takes them as its share of the reduction (load_rows' first entries); false in every thread if one did
+// not arrive within `limit` polls.
+__device__ __forceinline__ bool wait_sums(const double* P, int nb, int limit, double (&acc)[kFusedRows]) {
+    int late = 0;
+#pragma unroll
+    for (int q = 0; q < kFusedRows; ++q) acc[q] = 0.0;
+    const int b = (int)threadIdx.x;
+    if (b < nb) {
+        for (int spins = 0;; ++spins) {
+            const double v0 = load_agent(P + b), v1 = load_agent(P + nb + b), v2 = load_agent(P + 2 * nb + b);
+            if (!empty_sum(v0) && !empty_sum(v1) && !empty_sum(v2)) {
+                acc[0] = v0;
+                acc[1] = v1;
+                acc[2] = v2;
+                break;
+            }
+            if (spins >= limit) {
+                late = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    return __syncthreads_or(late) == 0;
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock3) void k_pcg_persist(SoloSrc src, int spin_limit) {
+    HFM_ENTRY(src);
+    __shared__ float U[D + 2][kF3][kF3];   // u_{k+1} on the frame
+    __shared__ float S[D + 2][kF3][kF3];   // alpha_s on the frame
+    __shared__ double lds[3][kBlock3 / 64];
+    CgCtl* c = a.ctl + 1;
+    const int n = a.n;
+    const long N = a.N;
+    const int nb = a.nb_dir;
+    const int L = xcd_tile(lb, nb);
+    const int tjk = a.tiles_j * a.tiles_k;
+    const int ti = L / tjk, rem = L % tjk;
+    const int i0 = ti * D, j0 = (rem / a.tiles_k) * kT3, k0 = (rem % a.tiles_k) * kT3;
+    float* __restrict__ pv = a.cg_z;
+    float* __restrict__ x = a.p;
+    double* slots = a.partials + kPersistSlots;   // [3][3 rows][nb]
+    // the layouts of k_pcg_fused: halo cells in the compact numbering, own cells in the stencil's layout
+    constexpr int kH = kHalo<D>, kHM = (kH + kBlock3 - 1) / kBlock3, kO = D / 4;
+    const int lk = threadIdx.x % kT3, lj = (threadIdx.x / kT3) % kT3, lq = threadIdx.x / (kT3 * kT3);
+    const int j = j0 + lj, k = k0 + lk;
+    auto in_box = [&](int m) { return i0 + lq + 4 * m < n && j < n && k < n; };
+    // an own cell is published (its r, s, w go to memory every iteration) when it lies on a face of the
+    // block or of the box: the cells that some frame reads
+    const bool edge_jk = lj == 0 || lj == kT3 - 1 || j == n - 1 || lk == 0 || lk == kT3 - 1 || k == n - 1;
+    auto pub = [&](int m) {
+        const int li = lq + 4 * m;
+        return edge_jk || li == 0 || li == D - 1 || i0 + li == n - 1;
+    };
+    auto own_index = [&](int m) { return frame_index(i0, j0, k0, lq + 4 * m + 1, lj + 1, lk + 1, n); };
+    // constants of the solve: D at the halo and own cells, alpha_s on the frame
+    float hd[kHM], od[kO];
+#pragma unroll
+    for (int q = 0; q < kHM; ++q) {
+        const int h = (int)threadIdx.x + q * kBlock3;
+        hd[q] = 0.0f;
+        if (h < kH) {
+            int fi, fj, fk;
+            halo_cell<D>(h, fi, fj, fk);
+            const unsigned t = frame_index(i0, j0, k0, fi, fj, fk, n);
+            hd[q] = ld32((const float*)a.pdiag, t);
+            S[fi][fj][fk] = ld32((const float*)a.alpha_s, t);
+        }
+    }
+    // own cells: r_k, w_k, s_{k-1}, p_{k-1}, x_k (registers for the whole solve)
+    float orr[kO], ow[kO], os[kO], op[kO], ox[kO];
+#pragma unroll
+    for (int m = 0; m < kO; ++m) {
+        const unsigned t = own_index(m);
+        od[m] = ld32((const float*)a.pdiag, t);
+        S[lq + 4 * m + 1][lj + 1][lk + 1] = ld32((const float*)a.alpha_s, t);
+        ox[m] = in_box(m) ? ld32((const float*)x, t) : 0.0f;   // x_0
+        orr[m] = ow[m] = os[m] = op[m] = 0.0f;
+    }
+    const float amin = a.amin, hh = a.hh, rhh = a.rhh;
+    const int fastdiv = a.fastdiv;
+    const int maxeff = c->maxeff;
+    const double tol = c->tol;
+    double atol = c->atol;
+    float atol_sq = c->atol_sq;
+    float g_prev = 0.0f, pap_prev = 0.0f;
+    for (int kit = 0;; ++kit) {
+        const bool first = kit == 0;
+        const float* __restrict__ rk = a.cg_r + (long)(kit & 1) * N;
+        float* __restrict__ rn = a.cg_r + (long)((kit + 1) & 1) * N;
+        const float* __restrict__ wk = a.cg_Ap + (long)(kit & 1) * N;
+        float* __restrict__ wn = a.cg_Ap + (long)((kit + 1) & 1) * N;
+        const float* __restrict__ sprev = a.cg_d + (long)((kit + 1) & 1) * N;
+        float* __restrict__ sk = a.cg_d + (long)(kit & 1) * N;
+        // ---- every workgroup's partial sums of iteration k - 1 (the init launch's at k = 0)
+        double acc[kFusedRows];
+        if (first) {
+            load_rows<kFusedRows>(a.partials, a.nb_upd, kFusedRows, acc);
+        } else if (!wait_sums(slots + ((kit - 1) % 3) * 3 * nb, nb, spin_limit, acc)) {
+            if (threadIdx.x == 0) {
+                c->done = 1;
+                mirror_final(a.mirror + 1, kit, __builtin_nanf(""), atol, -2);
+            }
+            return;
+        }
+        if (kit >= 2 && threadIdx.x == 0)   // empty this workgroup's sums of iteration k - 2
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+                store_agent(slots + ((kit + 1) % 3) * 3 * nb + r * nb + lb, __builtin_bit_cast(double, kEmptySum));
+        // ---- r_k, w_k, s_{k-1} of the halo cells, and of the own cells at k = 0 (and of positions of the
+        // block outside the box: the clamped cell's), from memory
+        // (threads past the last halo cell load the last one's values and do not use them)
+        float hr[kHM], hw[kHM], hs[kHM];
+#pragma unroll
+        for (int q = 0; q < kHM; ++q) {
+            const int h = (int)threadIdx.x + q * kBlock3;
+            int fi, fj, fk;
+            halo_cell<D>(h < kH ? h : kH - 1, fi, fj, fk);
+            const unsigned t = frame_index(i0, j0, k0, fi, fj, fk, n);
+            hr[q] = load_agent_f(rk, t);
+            hw[q] = load_agent_f(wk, t);
+            hs[q] = first ? 0.0f : load_agent_f(sprev, t);
+        }
+#pragma unroll
+        for (int m = 0; m < kO; ++m) {
+            if (first || !in_box(m)) {
+                const unsigned t = own_index(m);
+                orr[m] = load_agent_f(rk, t);
+                ow[m] = load_agent_f(wk, t);
+                os[m] = first ? 0.0f : load_agent_f(sprev, t);
+            }
+        }
+        // ---- scalars of iteration k
+        double tot[kFusedRows];
+        reduce_bcast<kFusedRows, kBlock3>(acc, first ? kFusedRows : 3, 0, tot, first ? a.nb_upd : nb);
+        const CgScalars q = cg_scalars(tot, first, g_prev, pap_prev, tol, atol, atol_sq, kit, maxeff);
+        if (lb == 0 && threadIdx.x == 0) cg_record(c, a.mirror + 1, q, first, kit, atol, atol_sq);
+        if (q.stop) {
+            // x_k and p_{k-1} where the one-launch form leaves them
+#pragma unroll
+            for (int m = 0; m < kO; ++m)
+                if (in_box(m)) {
+                    st32(x, own_index(m), ox[m]);
+                    if (!first) st32(pv, own_index(m), op[m]);
+                }
+            return;
+        }
+        // ---- halo cells: u_{k+1}
+#pragma unroll
+        for (int qh = 0; qh < kHM; ++qh) {
+            const int h = (int)threadIdx.x + qh * kBlock3;
+            if (h < kH) {
+                int fi, fj, fk;
+                halo_cell<D>(h, fi, fj, fk);
+                float s, r1;
+                U[fi][fj][fk] = cg_frame(hr[qh], hw[qh], hs[qh], hd[qh], first, q.alpha, q.beta, s, r1);
+            }
+        }
+        // ---- own cells: s_k, r_{k+1}, u_{k+1}, p_k, x_{k+1}
+        double sum_g = 0.0, sum_rr = 0.0, sum_d = 0.0;   // r.u, r.r, w.u of step k + 1
+#pragma unroll
+        for (int m = 0; m < kO; ++m) {
+            float s, r1;
+            const float u1 = cg_frame(orr[m], ow[m], os[m], od[m], first, q.alpha, q.beta, s, r1);
+            U[lq + 4 * m + 1][lj + 1][lk + 1] = u1;
+            if (in_box(m)) {
+                const float uk = od[m] * orr[m];
+                const float pk = first ? uk : uk + q.beta * op[m];
+                op[m] = pk;
+                ox[m] = ox[m] + q.alpha * pk;
+                orr[m] = r1;
+                os[m] = s;
+                if (pub(m)) {
+                    store_agent_f(rn, own_index(m), r1);
+                    store_agent_f(sk, own_index(m), s);
+                }
+                sum_g += (double)(r1 * u1);
+                sum_rr += (double)(r1 * r1);
+            }
+        }
+        __syncthreads();
+        // ---- w_{k+1} = A u_{k+1} on the block, from LDS
+#pragma unroll
+        for (int m = 0; m < kO; ++m) {
+            if (in_box(m)) {
+                float uc;
+                const float w1 = cg_stencil<D>(U, S, lq + 4 * m, lj, lk, amin, hh, rhh, fastdiv, uc);
+                ow[m] = w1;
+                if (pub(m)) store_agent_f(wn, own_index(m), w1);
+                sum_d += (double)(w1 * uc);
+            }
+        }
+        // ---- partial sums (rows r.r, g, d) in slot k % 3, once every store of this iteration is done
+        wait_stores();
+        double out[3] = {sum_rr, sum_g, sum_d};
+        block_sum<3, kBlock3>(out, lds);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int r = 0; r < 3; ++r) store_agent(slots + (kit % 3) * 3 * nb + r * nb + lb, out[r]);
+        g_prev = q.g;
+        pap_prev = q.pap;
+    }
 }
 
 }  // namespace
@@ -1087,10 +1413,11 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init_fused(Src src) {
 // kMaxPcgBlocks workgroups run.  The pressure pass A is LDS-tiled (unless HFM_PCG_FLAT_DIR) with
 // planes D = 16, else 8, else 4, the first that keeps at most kMaxDirBlocks workgroups (deeper blocks
 // re-read less halo and launch fewer, longer workgroups, which measured fastest); the one-launch form
-// tries 8 first (two of its workgroups then fit a CU).  HFM_DIR_ICHUNK forces D.  Without a tiling
-// pass A is the flat kernel with the pass B geometry.
-void pcg_geometry(const Scene* s, bool fused, int* cpt, int* nb_upd, int* nb_dir, int* tiles_j, int* tiles_k,
-                  int* dtile) {
+// tries 8 first (two of its workgroups then fit a CU), its persistent form 16 (fewest workgroups to
+// keep resident).  HFM_DIR_ICHUNK forces D.  Without a tiling pass A is the flat kernel with the pass B
+// geometry.
+void pcg_geometry(const Scene* s, bool fused, bool persist, int* cpt, int* nb_upd, int* nb_dir, int* tiles_j,
+                  int* tiles_k, int* dtile) {
     const long tiles = (s->N + kBlock - 1) / kBlock;
     const long need = (tiles + kMaxPcgBlocks - 1) / kMaxPcgBlocks;
     const int c0 = s->pcg_cpt > 0 ? s->pcg_cpt : 1;
@@ -1102,7 +1429,7 @@ void pcg_geometry(const Scene* s, bool fused, int* cpt, int* nb_upd, int* nb_dir
     int D = 0;
     if (!s->flat_dir) {
         // the one-launch form prefers 8 planes: two of its workgroups fit a CU (at most 64 VGPRs)
-        const int cand[3] = {fused ? 8 : 16, fused ? 16 : 8, 4};
+        const int cand[3] = {fused && !persist ? 8 : 16, fused && !persist ? 16 : 8, 4};
         for (int q = 0; q < 3 && !D; ++q) {
             const int d = cand[q];
             if (s->dir_ichunk > 0 && d != s->dir_ichunk) continue;
@@ -1161,6 +1488,37 @@ void launch_pcg_fused(const Launch& L, Family fam, int k, unsigned long long* pa
         case FAM_FUSED16: HFM_LAUNCH_T(k_pcg_fused, 16, L, kBlock3, ev0, ev1, k, pace, tag); break;
         default: break;
     }
+}
+
+static const void* persist_kernel(int D) {
+    switch (D) {
+        case 4: return (const void*)k_pcg_persist<4>;
+        case 8: return (const void*)k_pcg_persist<8>;
+        case 16: return (const void*)k_pcg_persist<16>;
+        default: return nullptr;
+    }
+}
+
+int pcg_persist_capacity(int D, int device) {
+    const void* fn = persist_kernel(D);
+    if (!fn) return 0;
+    int per_cu = 0, cus = 0, coop = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock3, 0) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device) != hipSuccess || !coop) return 0;
+    return per_cu * cus;
+}
+
+hipError_t launch_pcg_persist(const Launch& L, int D, hipEvent_t ev0, hipEvent_t ev1) {
+    const void* fn = persist_kernel(D);
+    if (!fn || !L.solo) return hipErrorInvalidValue;
+    SoloSrc src{*reinterpret_cast<const SceneG*>(L.solo)};
+    int limit = kPersistSpinLimit;
+    void* args[] = {(void*)&src, (void*)&limit};
+    if (ev0) (void)hipEventRecord(ev0, L.stream);
+    const hipError_t e = hipLaunchCooperativeKernel(fn, L.sgrid, dim3(kBlock3), args, 0, L.stream);
+    if (ev1) (void)hipEventRecord(ev1, L.stream);
+    return e;
 }
 
 void launch_pcg_upd(const Launch& L, int op, int k, hipEvent_t ev0, hipEvent_t ev1, bool ir) {

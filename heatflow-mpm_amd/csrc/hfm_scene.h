@@ -108,6 +108,7 @@ struct SceneView {
     ptr<CgMirror> mirror;    // device view of the pinned mirrors, [0] heat, [1] pressure
     int cpt, nb_upd, nb_dir, tiles_j, tiles_k, dtile;
     int cg_fused;            // pressure PCG in one launch per iteration (tiled scenes only)
+    int cg_persist;          // ... and all its iterations in one cooperative launch (HFM_FLAG_PERSISTENT_CG)
     ptr<float> chunk_sums;
     ptr<const int> mean_prog;
     int mean_tail;           // offset of the tail chunk's program in mean_prog, -1 if none
@@ -382,8 +383,8 @@ int check_finite_host(const float* a, long n, const char* what);
 void launch_p2g_listed(const Launch& L);
 
 // ---- PCG (hfm_pcg.hip); op 0 = heat, 1 = pressure ----
-void pcg_geometry(const Scene* s, bool fused, int* cpt, int* nb_upd, int* nb_dir, int* tiles_j, int* tiles_k,
-                  int* dtile);
+void pcg_geometry(const Scene* s, bool fused, bool persist, int* cpt, int* nb_upd, int* nb_dir, int* tiles_j,
+                  int* tiles_k, int* dtile);
 void launch_pcg_init(const Launch& L, int op, float dt);
 // pace: when non-null, workgroup 0 posts (tag << 32) | k to it as the launch starts
 // ir: the inner iteration of a mixed-precision refinement (FP16 vectors, scaled operator)
@@ -407,5 +408,17 @@ constexpr int kFusedRows = 5;
 void launch_pcg_init_fused(const Launch& L);
 void launch_pcg_fused(const Launch& L, Family fam, int k, unsigned long long* pace, unsigned tag,
                       hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+// persistent form: every iteration of a solo scene's pressure solve in one cooperative launch of
+// nb_dir workgroups (D planes per tile), after launch_pcg_init_fused; the capacity is the most such
+// workgroups the device keeps resident at once (0: no cooperative launch, or no kernel for D)
+constexpr int kPersistSpinLimit = 1 << 22;   // polls of one partial-sum wait before the solve gives up
+// at most one workgroup per thread of a workgroup (each thread polls one workgroup's sums)
+constexpr int kPersistMaxBlocks = 1024;
+// the persistent form's partial-sum slots, [3][3 rows][nb_dir] doubles of the partial-sum buffer (the
+// one-launch form's second parity buffer), and the pattern of an empty entry (a signalling NaN)
+constexpr long kPersistSlots = (long)kFusedRows * kMaxDirBlocks;
+constexpr unsigned long long kEmptySum = 0x7FF4A5A5A5A5A5A5ull;
+int pcg_persist_capacity(int D, int device);
+hipError_t launch_pcg_persist(const Launch& L, int D, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 
 }  // namespace hfm

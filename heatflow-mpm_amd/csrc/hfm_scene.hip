@@ -140,6 +140,12 @@ int warp_reported_count(int exact, int converged, int maxiter, int check_every) 
     return cur;
 }
 
+// Serialises the persistent pressure solves of one device.
+std::mutex& persist_mutex(int device) {
+    static std::mutex mu[64];
+    return mu[device & 63];
+}
+
 // Whether div_const's corrected product x * RN(1/b) + residual equals the IEEE quotient x / b for
 // every significand (x in [1, 2)); with all intermediate values normal that covers every x with
 // 2^-100 <= |x| <= 2^100 (the quotients scale exactly with x).  Results are cached per b.
@@ -588,6 +594,24 @@ int pcg_iterate(Batch* b, int op, float dt, bool ir, int maxeff, int pred) {
     const auto t0 = std::chrono::steady_clock::now();
     long spins = 0;
     const bool timed = op == 1 && b->timing;
+    // The persistent form runs the whole solve in one cooperative launch; the loop below then only
+    // waits for it.  One such launch at a time per device: two of them, each partly resident, would
+    // wait on each other until their spin limits.
+    std::unique_lock<std::mutex> persist_lock;
+    if (op == 1 && !ir && ns == 1 && ndir == 1 && b->scenes[0]->dev.cg_persist && fl[dirs[0]].solo &&
+        fl[dirs[0]].blocks > 0) {
+        persist_lock = std::unique_lock<std::mutex>(persist_mutex(b->device));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (timed) RC(timing_events(b, HFM_KT_PCG_PERSIST, &e0, &e1));
+        const hipError_t e = launch_pcg_persist(fl[dirs[0]], b->scenes[0]->dev.dtile, e0, e1);
+        if (e != hipSuccess) {
+            set_err("persistent pressure PCG: cooperative launch of %d workgroups failed: %s", fl[dirs[0]].blocks,
+                    hipGetErrorString(e));
+            return HFM_ERR_HIP;
+        }
+        launched = maxeff;
+        closed = true;
+    }
     for (;;) {
         bool all_done = true;
         for (Scene* s : b->scenes)
@@ -679,6 +703,11 @@ int run_pcg(Batch* b, int op, float dt) {
         CgMirror* m = &s->mirror[op];
         const int it = __atomic_load_n(&m->it, __ATOMIC_ACQUIRE);
         const int conv = __atomic_load_n(&m->converged, __ATOMIC_ACQUIRE);
+        if (conv == -2) {
+            set_err("persistent pressure PCG: a workgroup waited longer than %d polls for the others at "
+                    "iteration %d", kPersistSpinLimit, it);
+            return HFM_ERR_HIP;
+        }
         // a breakdown stop keeps the last iterate; a non-finite residual means the state already is
         if (conv < 0 && !(m->rr <= 1.0e300) && bad < 0) bad = (int)si;
         const int maxiter = op ? s->prm.cg_max_iter : s->prm.heat_max_iter;
@@ -1182,9 +1211,13 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
 #undef A
     d.mean_prog = prog_dev;
     const bool want_fused = (prm->flags & HFM_FLAG_FUSED_CG) != 0;
-    pcg_geometry(s, want_fused, &d.cpt, &d.nb_upd, &d.nb_dir, &d.tiles_j, &d.tiles_k, &d.dtile);
-    // the one-launch iteration runs on the tiled blocks (a flat layout keeps the two-pass form)
+    const bool want_persist = want_fused && (prm->flags & HFM_FLAG_PERSISTENT_CG) != 0;
+    pcg_geometry(s, want_fused, want_persist, &d.cpt, &d.nb_upd, &d.nb_dir, &d.tiles_j, &d.tiles_k, &d.dtile);
+    // the one-launch iteration runs on the tiled blocks (a flat layout keeps the two-pass form); its
+    // persistent form where every tile's workgroup stays resident at once
     d.cg_fused = want_fused && d.dtile > 0;
+    d.cg_persist = d.cg_fused && want_persist && d.nb_dir <= kPersistMaxBlocks &&
+                   d.nb_dir <= pcg_persist_capacity(d.dtile, device);
     e = hipMemcpy(prog_dev, o->prog.data(), o->prog.size() * sizeof(int), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         set_err("hipMemcpy: %s", hipGetErrorString(e));

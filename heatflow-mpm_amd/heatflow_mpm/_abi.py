@@ -28,7 +28,8 @@ FLAG_TIME_KERNELS = 1
 FLAG_FUSED_CG = 2
 FLAG_MG_PCG = 4
 FLAG_MACCORMACK = 8
-KT_PCG_DIR, KT_PCG_UPD, KT_PCG_FUSED = 0, 1, 2
+FLAG_PERSISTENT_CG = 16
+KT_PCG_DIR, KT_PCG_UPD, KT_PCG_FUSED, KT_PCG_PERSIST = 0, 1, 2, 3
 
 
 class Params(C.Structure):

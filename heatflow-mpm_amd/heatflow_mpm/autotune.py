@@ -7,8 +7,9 @@ stepped as one batch -- each candidate layout runs the coupled scene (static ell
 heater with fan, PCG tolerance 1e-6) for a few substeps after a warmup, and is scored by wall
 time per pressure-PCG iteration (launch gaps included, so the one-launch fused form is credited
 for its fewer launches).  Candidates: pass A tiles of 16, 8 and 4 planes (those whose tile count
-fits the pass-A workgroup cap), each with the two-pass PCG and with the one-launch iteration, and
-the untiled two-pass form.  The winners go to a JSON table (shipped as
+fits the pass-A workgroup cap), each with the two-pass PCG and with the one-launch iteration, the
+persistent form of the one-launch iteration where the tiles fit one per CU (alone only: a batch
+does not run it), and the untiled two-pass form.  The winners go to a JSON table (shipped as
 tuning_mi355x.json); HeatflowMPM(layout="auto" / "auto-batch") reads it, using the entry of the
 nearest measured size.
 """
@@ -44,13 +45,16 @@ def lookup(n: int, context: str = "solo", table: dict | None = None) -> dict | N
     return dict(t[best]["choice"])
 
 
-def candidates(n: int):
+def candidates(n: int, context: str = "solo", cus: int = 256):
     tj = -(-n // 16)
     out = []
     for d in (16, 8, 4):
-        if tj * tj * -(-n // d) <= 4096:   # the pass-A workgroup cap of the library
+        tiles = tj * tj * -(-n // d)
+        if tiles <= 4096:   # the pass-A workgroup cap of the library
             out.append(dict(dir_planes=d, cg="warp"))
             out.append(dict(dir_planes=d, cg="fused"))
+            if context == "solo" and tiles <= cus:   # one resident workgroup per tile
+                out.append(dict(dir_planes=d, cg="persistent"))
     out.append(dict(dir_planes=-1, cg="warp"))
     return out
 
@@ -92,7 +96,7 @@ def tune(sizes=(32, 48, 64, 96, 128, 192), batch_scenes: int = 8, log=print) -> 
     for ctx, scenes in (("solo", 1), ("batch", batch_scenes)):
         for n in sizes:
             scores = []
-            for c in candidates(n):
+            for c in candidates(n, ctx):
                 ms = probe(n, c, scenes=scenes)
                 scores.append((ms, c))
                 if log:

@@ -209,7 +209,10 @@ class HeatflowMPM:
     Same constructor as the reference (solver.py:42-43).  Extras, not in the reference: `body`
     replaces the default StaticBody built from `particles`; `cg="fused"` runs the pressure PCG in
     one launch per iteration (warp's Jacobi PCG in the single-reduction Chronopoulos-Gear form,
-    HFM_FLAG_FUSED_CG in include/hfm.h) instead of warp's two-pass form;
+    HFM_FLAG_FUSED_CG in include/hfm.h) instead of warp's two-pass form; `cg="persistent"` runs
+    that same iteration, with the same bits, as one cooperative launch per solve that keeps the
+    vectors on chip (HFM_FLAG_PERSISTENT_CG: scenes stepped on their own whose 16^3 tiles the GPU
+    holds resident at once, n <= 96 on MI355X; otherwise it is cg="fused");
     `precision="mixed"` solves both systems by iterative refinement with FP16 inner vectors and
     traces advection through a BF16 copy of the velocity (HFM_PREC_MIXED); the state stays fp32;
     `cg="mg"` preconditions the pressure PCG with a geometric multigrid V-cycle instead of
@@ -218,13 +221,14 @@ class HeatflowMPM:
     equal to the Jacobi solve's to the solve tolerance, not in bits;
     `layout` picks the pressure PCG layout: None (chosen by grid size), "auto" / "auto-batch"
     (the autotuned table of heatflow_mpm.autotune for a scene stepped alone / in a batch), or
-    dict(dir_planes=0|-1|4|8|16, cg="warp"|"fused"); with cg="mg" only its pass-A tiling applies;
+    dict(dir_planes=0|-1|4|8|16, cg="warp"|"fused"|"persistent"); with cg="mg" only its pass-A
+    tiling applies;
     `advection="maccormack"` advects T and u, v, w by the limited MacCormack method (the
     reference's prepared but unused limiter, fields.py:51-65; HFM_FLAG_MACCORMACK, fp32 only)
     instead of the reference's semi-Lagrangian step.
     """
 
-    CG_FORMS = ("warp", "fused", "mg")
+    CG_FORMS = ("warp", "fused", "persistent", "mg")
     ADVECTION = ("semi-lagrangian", "maccormack")
     PRECISIONS = {"fp32": _abi.PREC_FP32, "mixed": _abi.PREC_MIXED}
 
@@ -246,7 +250,7 @@ class HeatflowMPM:
                 choice = dict(layout)
             if choice:
                 dir_planes = int(choice.get("dir_planes", 0))
-                if choice.get("cg") in ("warp", "fused") and precision == "fp32" and cg != "mg":
+                if choice.get("cg") in ("warp", "fused", "persistent") and precision == "fp32" and cg != "mg":
                     cg = choice["cg"]
         self.layout = dict(dir_planes=dir_planes, cg=cg)
         if precision not in self.PRECISIONS:
@@ -267,6 +271,8 @@ class HeatflowMPM:
         inp = dict(inp, params=dict(inp["params"]))
         if cg == "fused":
             inp["params"]["flags"] |= _abi.FLAG_FUSED_CG
+        elif cg == "persistent":
+            inp["params"]["flags"] |= _abi.FLAG_FUSED_CG | _abi.FLAG_PERSISTENT_CG
         elif cg == "mg":
             inp["params"]["flags"] |= _abi.FLAG_MG_PCG
         if advection == "maccormack":

@@ -102,6 +102,16 @@ typedef struct hfm_params {
  * substep: it prepares the limiter (value_range, fields.py:51-65) but advects semi-Lagrangian
  * only.  fp32 only (not with HFM_PREC_MIXED). */
 #define HFM_FLAG_MACCORMACK 8
+/* With HFM_FLAG_FUSED_CG: run every iteration of a pressure solve in one cooperative launch of one
+ * workgroup per tile, all resident for the whole solve (no counterpart in the reference).  Each
+ * workgroup keeps its tile's vectors on chip and exchanges only the tile's faces and its partial
+ * sums with the others through memory; the results, scalars and iteration counts are those of
+ * HFM_FLAG_FUSED_CG bit for bit.  Used when the scene is stepped on its own (alone or as a batch of
+ * one) and the device keeps all of its tiles resident at once (n <= 96 on MI355X: 216 tiles of
+ * 16 x 16 x 16 cells, one per CU); otherwise the scene runs HFM_FLAG_FUSED_CG as if this flag were
+ * absent.  Persistent solves of different scenes on one device run one at a time.  A workgroup that
+ * waits longer than a fixed number of polls for the others ends the solve with HFM_ERR_HIP. */
+#define HFM_FLAG_PERSISTENT_CG 16
 
 /* Per-substep statistics of the last hfm_substep. */
 typedef struct hfm_stats {
@@ -225,7 +235,8 @@ int hfm_get_stats(hfm_scene* s, hfm_stats* out);
 #define HFM_KT_PCG_DIR 0      /* pressure PCG pass A: direction update + SpMV + p.Ap */
 #define HFM_KT_PCG_UPD 1      /* pressure PCG pass B: x, r update + r.r, r.z */
 #define HFM_KT_PCG_FUSED 2    /* pressure PCG, HFM_FLAG_FUSED_CG: one whole iteration */
-#define HFM_KT_COUNT 3
+#define HFM_KT_PCG_PERSIST 3  /* pressure PCG, HFM_FLAG_PERSISTENT_CG: one whole solve */
+#define HFM_KT_COUNT 4
 int hfm_kernel_time(hfm_scene* s, int32_t kernel_class, double* total_ms, int64_t* launches);
 int hfm_kernel_time_reset(hfm_scene* s);
 

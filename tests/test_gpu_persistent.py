@@ -1,0 +1,145 @@
+"""The persistent pressure PCG (HFM_FLAG_PERSISTENT_CG): every iteration of the one-launch form in one
+cooperative launch per solve, each workgroup keeping its tile's vectors on chip.  It restates the
+one-launch form on the same tiles with the same partial sums and reduction order, so it is held to
+equality with it: the same bits in every field and the same iteration counts, convergence flags and
+residuals, substep after substep.  The one-launch form is held to the fp32 oracle elsewhere; one
+substep of the persistent form is checked against the oracle here as well.  Every case asserts that the
+persistent kernel really ran (its kernel-timing class counts one launch per pressure solve), or, where
+the scene cannot run it, that it did not."""
+import threading
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import rel_l2
+from heatflow_mpm import HeatflowBatch, HeatflowMPM, coupled_scene, plume_scene
+from heatflow_mpm import _abi
+
+pytestmark = pytest.mark.gpu
+FIELDS = ["T", "U", "V", "W", "P", "RHS"]
+STATS = ["p_iters", "p_iters_exact", "p_converged", "p_rr", "h_iters_exact", "source_dT_sum"]
+
+
+def _sim(kw, cg, planes):
+    sim = HeatflowMPM(**kw, cg=cg, layout=dict(dir_planes=planes))
+    sim._scene.set_kernel_timing(True)
+    return sim
+
+
+def _persist_launches(sim):
+    return sim._scene.kernel_time(_abi.KT_PCG_PERSIST)[1]
+
+
+def _assert_same(a, b, where):
+    sa, sb = a.stats(), b.stats()
+    for k in STATS:
+        assert sa[k] == sb[k], (where, k, sa[k], sb[k])
+    for f in FIELDS:
+        x, y = a._scene.get_field(f), b._scene.get_field(f)
+        assert np.array_equal(x, y), (where, f, float(np.abs(x - y).max()))
+
+
+def _lockstep(kw, dt, substeps, planes=16, expect_persistent=True):
+    ref, per = _sim(kw, "fused", planes), _sim(kw, "persistent", planes)
+    try:
+        for k in range(substeps):
+            ref.substep(dt)
+            per.substep(dt)
+            _assert_same(ref, per, f"substep {k + 1}")
+        st = per.stats()
+        print("n", kw["grid"].n, "planes", planes, "p_iters_exact", st["p_iters_exact"], "persistent launches",
+              _persist_launches(per), flush=True)
+        assert _persist_launches(per) == (substeps if expect_persistent else 0)
+        assert _persist_launches(ref) == 0
+    finally:
+        ref.close()
+        per.close()
+
+
+def test_persistent_coupled_96_equals_one_launch_form(gpu):
+    """216 tiles of 16^3, one per CU; the first substep from rest runs to the 400-iteration cap."""
+    kw, dt, _ = coupled_scene(96)
+    _lockstep(kw, dt, 4)
+
+
+def test_persistent_plume_many_substeps_equal_one_launch_form(gpu):
+    kw, dt, _ = plume_scene()
+    _lockstep(kw, dt, 40)
+
+
+def test_persistent_ragged_tiles_and_shallow_blocks(gpu):
+    """n = 40: tiles cut by the box on every axis, with 16-, 8- and 4-plane blocks (27, 45, 90 tiles)."""
+    kw, dt, _ = coupled_scene(40)
+    for planes in (16, 8, 4):
+        _lockstep(kw, dt, 6, planes=planes)
+
+
+def test_persistent_one_substep_matches_oracle(gpu):
+    kw, dt, _ = plume_scene()
+    sim, orc = _sim(kw, "persistent", 16), O.OracleSim(**kw)
+    sim.substep(dt)
+    orc.substep(dt)
+    st, so = sim.stats(), orc.stats()
+    print("persistent", st["p_iters_exact"], "oracle", so["p_iters_exact"], flush=True)
+    assert _persist_launches(sim) == 1
+    assert abs(st["p_iters_exact"] - so["p_iters_exact"]) <= 1
+    for f in FIELDS:
+        err = rel_l2(sim._scene.get_field(f), orc.field(f))
+        print(f, "rel_l2", err, flush=True)
+        assert err <= 1e-5, f
+    sim.close()
+
+
+def test_persistent_falls_back_where_it_cannot_run(gpu):
+    """A batch of two runs the one-launch form (same bits as each scene stepped alone in the persistent
+    form); at n = 128 the 512 tiles do not fit one per CU and the scene runs the one-launch form."""
+    kw, dt, _ = coupled_scene(48)
+    alone = [_sim(kw, "persistent", 16) for _ in range(2)]
+    pair = [_sim(kw, "persistent", 16) for _ in range(2)]
+    try:
+        with HeatflowBatch(pair) as batch:
+            for k in range(5):
+                batch.substep(dt)
+                for s in alone:
+                    s.substep(dt)
+                for a, b in zip(alone, pair):
+                    _assert_same(a, b, f"batch substep {k + 1}")
+        assert all(_persist_launches(s) == 5 for s in alone)
+        assert all(_persist_launches(s) == 0 for s in pair)
+    finally:
+        for s in alone + pair:
+            s.close()
+    kw, dt, _ = coupled_scene(128)
+    _lockstep(kw, dt, 2, expect_persistent=False)
+
+
+def test_persistent_solves_from_two_threads(gpu):
+    """Two scenes stepped from two host threads at once (the library runs their persistent solves one
+    at a time on the device): each ends with the bits of the same scene stepped alone."""
+    kw, dt, _ = plume_scene()
+    ref = _sim(kw, "persistent", 16)
+    for _ in range(12):
+        ref.substep(dt)
+    sims = [_sim(kw, "persistent", 16) for _ in range(2)]
+    errors = []
+
+    def run(s):
+        try:
+            for _ in range(12):
+                s.substep(dt)
+            s.synchronize()
+        except Exception as e:   # reported below, in the test's thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=run, args=(s,)) for s in sims]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    for s in sims:
+        _assert_same(ref, s, "threaded")
+        assert _persist_launches(s) == 12
+    for s in sims + [ref]:
+        s.close()

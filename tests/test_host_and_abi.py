@@ -206,7 +206,8 @@ def test_mixed_precision_argument_checks():
 
 def test_autotune_table_lookup_and_candidates():
     """The autotuner's table lookup takes the nearest measured size (in log n); candidates respect
-    the pass-A workgroup cap (4096 workgroups) and offer both PCG forms on every tiling."""
+    the pass-A workgroup cap (4096 workgroups) and offer both PCG forms on every tiling; the
+    persistent form only for a scene alone whose tiles fit one per CU."""
     from heatflow_mpm.autotune import candidates, lookup
     t = {"solo": {"64": {"choice": {"dir_planes": 8, "cg": "fused"}},
                   "128": {"choice": {"dir_planes": 16, "cg": "warp"}}}}
@@ -218,6 +219,10 @@ def test_autotune_table_lookup_and_candidates():
     assert all(x["dir_planes"] != 4 for x in c)          # 12 x 12 x 48 tiles exceed the cap
     assert {"dir_planes": 8, "cg": "fused"} in c         # 12 x 12 x 24 tiles fit it
     assert all(x["cg"] == "warp" or x["dir_planes"] in (4, 8, 16) for x in candidates(64))
+    assert all(x["cg"] != "persistent" for x in c)        # 576 tiles of 16 planes > 256 CUs
+    assert {"dir_planes": 16, "cg": "persistent"} in candidates(96)        # 6 x 6 x 6 tiles
+    assert {"dir_planes": 8, "cg": "persistent"} not in candidates(96)     # 432 tiles
+    assert all(x["cg"] != "persistent" for x in candidates(96, "batch"))
 
 
 def test_layout_argument_checks():
