@@ -584,15 +584,26 @@ def main() -> int:
     sim._scene.set_kernel_timing(False)
     dom = max(kt, key=lambda c: kt[c][0])
     total_ms, launches = kt[dom]
-    bytes_per_launch = (KERNEL_WORDS[dom] * 4.0 if prec == "fp32" else KERNEL_BYTES_MIXED[dom]) * N
-    achieved = bytes_per_launch * iters / (total_ms * 1e-3) / 1e9
+    bytes_per_iteration = (KERNEL_WORDS[dom] * 4.0 if prec == "fp32" else KERNEL_BYTES_MIXED[dom]) * N
+    # one launch per iteration, except the persistent form: one launch per solve
+    bytes_per_launch = bytes_per_iteration * iters / max(launches, 1)
+    achieved = bytes_per_iteration * iters / (total_ms * 1e-3) / 1e9
     tr = measured_traffic(KERNEL_NAMES[dom], n) if prec == "fp32" else None
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4),
-                    traffic=None if tr is None else round(tr["traffic_bytes"]), kernel=KERNEL_NAMES[dom],
+                    traffic=None if tr is None else round(tr["traffic_bytes"] * (iters / max(launches, 1)
+                                                                                if dom == _abi.KT_PCG_PERSIST
+                                                                                else 1.0)),
+                    kernel=KERNEL_NAMES[dom],
                     avg_launch_us=round(total_ms * 1e3 / max(launches, 1), 3), launches=launches,
-                    iterations=iters, bytes_per_launch=bytes_per_launch,
+                    iterations=iters, bytes_per_launch=round(bytes_per_launch),
+                    bytes_per_iteration=bytes_per_iteration,
                     traffic_source=None if tr is None else tr["source"])
+    if dom == _abi.KT_PCG_PERSIST:
+        roofline["note"] = ("achieved counts the one-launch iteration's algorithmic bytes (12 words per cell per "
+                            "iteration); the persistent kernel keeps the vectors on chip and moves only tile faces "
+                            "and partial sums through memory (traffic), so it is bound by instruction issue and the "
+                            "grid-wide wait, not by HBM")
 
     cpu = None if a.no_cpu else cpu_baseline(kw, dt, state)
     end = {name: sim._scene.get_field(name) for name in ("T", "U", "V", "W")}
@@ -611,7 +622,7 @@ def main() -> int:
                            pressure_cg=sim.layout["cg"], precision=prec, layout=a.layout, advection=a.advection,
                            dir_planes=sim.layout["dir_planes"],
                            pressure_iters_per_substep=round(I_p, 2), heat_iters_per_substep=round(I_h, 2),
-                           parallelism="1 GPU"),
+                           pressure_iters_since_start=int(st3["p_iters_total"]), parallelism="1 GPU"),
                hbm_tb_s=None if hbm_tbs is None else round(hbm_tbs, 3), substep_roofline=substep_roofline,
                roofline=roofline, cpu_baseline=cpu,
                mg_form=mg, pcg_kernels_192=large)

@@ -51,6 +51,8 @@
 // (eta = 1e-2), and adds its
 // update to x directly: x += sigma c alpha p.  The FP16 vectors hold values of order one: |r^|
 // <= sqrt(N), |z^| <= |r^| / sum a_f with sum a_f >= 3 amin.
+#include <cstdlib>
+
 #include <hip/hip_ext.h>
 
 #include "hfm_device.h"
@@ -1168,8 +1170,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init_fused(Src src) {
 // ---------------------------------------------------------------------------------------------
 // Persistent pressure PCG (HFM_FLAG_PERSISTENT_CG): every iteration of k_pcg_fused in one
 // cooperative launch of one workgroup per tile, all of them resident for the whole solve.  A
-// workgroup keeps its own cells' r, w, s, p, x and D in registers (the stencil's layout, D / 4 cells per
-// thread), D of its halo cells in registers and the frame's solid fraction in LDS, so memory carries
+// workgroup keeps its own cells' r, w, s and D in registers and their p and x in LDS (the stencil's
+// layout, D / 4 cells per thread), D of its halo cells in registers and the frame's solid fraction in LDS, so memory carries
 // only what a neighbour needs: the r, s and w of the cells on the tile's faces, and the three partial
 // sums.  The launch boundary that separates two iterations of the one-launch form becomes the partial
 // sums themselves: iteration k stores its faces, waits for those stores to complete, then stores its
@@ -1198,6 +1200,32 @@ __device__ __forceinline__ void store_agent_f(float* p, unsigned i, float v) {
 }
 __device__ __forceinline__ bool empty_sum(double v) {
     return __builtin_bit_cast(unsigned long long, v) == kEmptySum;
+}
+
+// The workgroup index of tile L (the inverse of xcd_tile).
+__device__ __forceinline__ int tile_workgroup(int L, int nb) {
+    const int per = nb / kXcds, rem = nb % kXcds, big = rem * (per + 1);
+    const int x = L < big ? L / (per + 1) : rem + (L - big) / per;
+    const int q = L < big ? L % (per + 1) : (L - big) % per;
+    return q * kXcds + x;
+}
+
+// Wait until the r.r entry of workgroup `nbr` in slot P reads as posted (threads with nbr < 0 do not
+// wait); false in every thread if it did not within `limit` polls.  A workgroup posts its sums only
+// after its faces of that iteration are in memory.
+__device__ __forceinline__ bool wait_neighbour(const double* P, int nbr, int limit) {
+    int late = 0;
+    if (nbr >= 0) {
+        for (int spins = 0;; ++spins) {
+            if (!empty_sum(load_agent(P + nbr))) break;
+            if (spins >= limit) {
+                late = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    return __syncthreads_or(late) == 0;
 }
 
 // Rows r.r, g, d of slot P (row stride nb): thread b < nb polls its entries until none reads as empty and
@@ -1232,6 +1260,8 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(SoloSrc src, int spin_l
     HFM_ENTRY(src);
     __shared__ float U[D + 2][kF3][kF3];   // u_{k+1} on the frame
     __shared__ float S[D + 2][kF3][kF3];   // alpha_s on the frame
+    __shared__ float Pl[D][kT3][kT3];      // p_{k-1} of the own cells
+    __shared__ float Xl[D][kT3][kT3];      // x_k of the own cells
     __shared__ double lds[3][kBlock3 / 64];
     CgCtl* c = a.ctl + 1;
     const int n = a.n;
@@ -1240,10 +1270,20 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(SoloSrc src, int spin_l
     const int L = xcd_tile(lb, nb);
     const int tjk = a.tiles_j * a.tiles_k;
     const int ti = L / tjk, rem = L % tjk;
-    const int i0 = ti * D, j0 = (rem / a.tiles_k) * kT3, k0 = (rem % a.tiles_k) * kT3;
+    const int tj = rem / a.tiles_k, tk = rem % a.tiles_k;
+    const int i0 = ti * D, j0 = tj * kT3, k0 = tk * kT3;
     float* __restrict__ pv = a.cg_z;
     float* __restrict__ x = a.p;
     double* slots = a.partials + kPersistSlots;   // [3][3 rows][nb]
+    // the workgroups of the tiles around this one (thread t < 27, t != 13: offset (t / 9, t / 3 % 3,
+    // t % 3) - 1), whose faces the frame reads; -1 for none
+    int nbr = -1;
+    if (threadIdx.x < 27 && threadIdx.x != 13) {
+        const int ni = ti + (int)threadIdx.x / 9 - 1, nj = tj + (int)threadIdx.x / 3 % 3 - 1,
+                  nk = tk + (int)threadIdx.x % 3 - 1;
+        if (ni >= 0 && ni < (n + D - 1) / D && nj >= 0 && nj < a.tiles_j && nk >= 0 && nk < a.tiles_k)
+            nbr = tile_workgroup((ni * a.tiles_j + nj) * a.tiles_k + nk, nb);
+    }
     // the layouts of k_pcg_fused: halo cells in the compact numbering, own cells in the stencil's layout
     constexpr int kH = kHalo<D>, kHM = (kH + kBlock3 - 1) / kBlock3, kO = D / 4;
     const int lk = threadIdx.x % kT3, lj = (threadIdx.x / kT3) % kT3, lq = threadIdx.x / (kT3 * kT3);
@@ -1271,15 +1311,15 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(SoloSrc src, int spin_l
             S[fi][fj][fk] = ld32((const float*)a.alpha_s, t);
         }
     }
-    // own cells: r_k, w_k, s_{k-1}, p_{k-1}, x_k (registers for the whole solve)
-    float orr[kO], ow[kO], os[kO], op[kO], ox[kO];
+    // own cells: r_k, w_k, s_{k-1} in registers, p_{k-1} and x_k in LDS, for the whole solve
+    float orr[kO], ow[kO], os[kO];
 #pragma unroll
     for (int m = 0; m < kO; ++m) {
         const unsigned t = own_index(m);
         od[m] = ld32((const float*)a.pdiag, t);
         S[lq + 4 * m + 1][lj + 1][lk + 1] = ld32((const float*)a.alpha_s, t);
-        ox[m] = in_box(m) ? ld32((const float*)x, t) : 0.0f;   // x_0
-        orr[m] = ow[m] = os[m] = op[m] = 0.0f;
+        Xl[lq + 4 * m][lj][lk] = in_box(m) ? ld32((const float*)x, t) : 0.0f;   // x_0
+        orr[m] = ow[m] = os[m] = 0.0f;
     }
     const float amin = a.amin, hh = a.hh, rhh = a.rhh;
     const int fastdiv = a.fastdiv;
@@ -1296,21 +1336,15 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(SoloSrc src, int spin_l
         float* __restrict__ wn = a.cg_Ap + (long)((kit + 1) & 1) * N;
         const float* __restrict__ sprev = a.cg_d + (long)((kit + 1) & 1) * N;
         float* __restrict__ sk = a.cg_d + (long)(kit & 1) * N;
-        // ---- every workgroup's partial sums of iteration k - 1 (the init launch's at k = 0)
+        // ---- the partial sums of iteration k - 1 (the init launch's at k = 0): first the neighbours' (their
+        // faces are then in memory, and the halo loads go out while the others arrive), then everyone's
         double acc[kFusedRows];
-        if (first) {
+        const double* P = slots + ((kit + 2) % 3) * 3 * nb;   // slot (k - 1) % 3
+        bool ok = true;
+        if (first)
             load_rows<kFusedRows>(a.partials, a.nb_upd, kFusedRows, acc);
-        } else if (!wait_sums(slots + ((kit - 1) % 3) * 3 * nb, nb, spin_limit, acc)) {
-            if (threadIdx.x == 0) {
-                c->done = 1;
-                mirror_final(a.mirror + 1, kit, __builtin_nanf(""), atol, -2);
-            }
-            return;
-        }
-        if (kit >= 2 && threadIdx.x == 0)   // empty this workgroup's sums of iteration k - 2
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-                store_agent(slots + ((kit + 1) % 3) * 3 * nb + r * nb + lb, __builtin_bit_cast(double, kEmptySum));
+        else
+            ok = wait_neighbour(P, nbr, spin_limit);
         // ---- r_k, w_k, s_{k-1} of the halo cells, and of the own cells at k = 0 (and of positions of the
         // block outside the box: the clamped cell's), from memory
         // (threads past the last halo cell load the last one's values and do not use them)
@@ -1334,6 +1368,18 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(SoloSrc src, int spin_l
                 os[m] = first ? 0.0f : load_agent_f(sprev, t);
             }
         }
+        if (!first && ok) ok = wait_sums(P, nb, spin_limit, acc);
+        if (!ok) {
+            if (threadIdx.x == 0) {
+                c->done = 1;
+                mirror_final(a.mirror + 1, kit, __builtin_nanf(""), atol, -2);
+            }
+            return;
+        }
+        if (kit >= 2 && threadIdx.x == 0)   // empty this workgroup's sums of iteration k - 2
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+                store_agent(slots + ((kit + 1) % 3) * 3 * nb + r * nb + lb, __builtin_bit_cast(double, kEmptySum));
         // ---- scalars of iteration k
         double tot[kFusedRows];
         reduce_bcast<kFusedRows, kBlock3>(acc, first ? kFusedRows : 3, 0, tot, first ? a.nb_upd : nb);
@@ -1344,8 +1390,8 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(SoloSrc src, int spin_l
 #pragma unroll
             for (int m = 0; m < kO; ++m)
                 if (in_box(m)) {
-                    st32(x, own_index(m), ox[m]);
-                    if (!first) st32(pv, own_index(m), op[m]);
+                    st32(x, own_index(m), Xl[lq + 4 * m][lj][lk]);
+                    if (!first) st32(pv, own_index(m), Pl[lq + 4 * m][lj][lk]);
                 }
             return;
         }
@@ -1369,9 +1415,11 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(SoloSrc src, int spin_l
             U[lq + 4 * m + 1][lj + 1][lk + 1] = u1;
             if (in_box(m)) {
                 const float uk = od[m] * orr[m];
-                const float pk = first ? uk : uk + q.beta * op[m];
-                op[m] = pk;
-                ox[m] = ox[m] + q.alpha * pk;
+                float& pl = Pl[lq + 4 * m][lj][lk];
+                float& xl = Xl[lq + 4 * m][lj][lk];
+                const float pk = first ? uk : uk + q.beta * pl;
+                pl = pk;
+                xl = xl + q.alpha * pk;
                 orr[m] = r1;
                 os[m] = s;
                 if (pub(m)) {
@@ -1509,14 +1557,23 @@ int pcg_persist_capacity(int D, int device) {
     return per_cu * cus;
 }
 
+// HFM_PCG_PERSIST_ORDINARY=1 launches the same kernel as an ordinary dispatch, for profilers that cannot
+// trace a cooperative launch (rocprofv3 7.2 ends without writing its results).  Co-residency is then not
+// guaranteed by the runtime, only likely on an otherwise idle device (one workgroup per CU); the waits
+// stay bounded either way.
 hipError_t launch_pcg_persist(const Launch& L, int D, hipEvent_t ev0, hipEvent_t ev1) {
+    static const bool ordinary = [] {
+        const char* e = getenv("HFM_PCG_PERSIST_ORDINARY");
+        return e && atoi(e) != 0;
+    }();
     const void* fn = persist_kernel(D);
     if (!fn || !L.solo) return hipErrorInvalidValue;
     SoloSrc src{*reinterpret_cast<const SceneG*>(L.solo)};
     int limit = kPersistSpinLimit;
     void* args[] = {(void*)&src, (void*)&limit};
     if (ev0) (void)hipEventRecord(ev0, L.stream);
-    const hipError_t e = hipLaunchCooperativeKernel(fn, L.sgrid, dim3(kBlock3), args, 0, L.stream);
+    const hipError_t e = ordinary ? hipLaunchKernel(fn, L.sgrid, dim3(kBlock3), args, 0, L.stream)
+                                  : hipLaunchCooperativeKernel(fn, L.sgrid, dim3(kBlock3), args, 0, L.stream);
     if (ev1) (void)hipEventRecord(ev1, L.stream);
     return e;
 }
