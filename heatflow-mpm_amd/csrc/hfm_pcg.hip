@@ -994,6 +994,24 @@ __device__ __forceinline__ float cg_stencil(const float (*U)[kF3][kF3], const fl
     return div_const(1.0f * acc6, hh, rhh, fastdiv);
 }
 
+// cg_stencil with the block's face fractions formed ahead (FX below plane li, FY below row lj, FZ below
+// column lk): the same six terms in the same order, so the same bits
+template <int D>
+__device__ __forceinline__ float cg_stencil_ff(const float (*U)[kF3][kF3], const float (*FX)[kT3][kT3],
+                                               const float (*FY)[kT3 + 1][kT3], const float (*FZ)[kT3][kT3 + 1],
+                                               int li, int lj, int lk, float hh, float rhh, int fastdiv, float& uc) {
+    const int si = li + 1, sj = lj + 1, sk = lk + 1;
+    uc = U[si][sj][sk];
+    float acc6 = 0.0f;
+    acc6 += FX[li][lj][lk] * (uc - U[si - 1][sj][sk]);
+    acc6 += FX[li + 1][lj][lk] * (uc - U[si + 1][sj][sk]);
+    acc6 += FY[li][lj][lk] * (uc - U[si][sj - 1][sk]);
+    acc6 += FY[li][lj + 1][lk] * (uc - U[si][sj + 1][sk]);
+    acc6 += FZ[li][lj][lk] * (uc - U[si][sj][sk - 1]);
+    acc6 += FZ[li][lj][lk + 1] * (uc - U[si][sj][sk + 1]);
+    return div_const(1.0f * acc6, hh, rhh, fastdiv);
+}
+
 template <int D, class Src>
 __global__ __launch_bounds__(kBlock3, D <= 8 ? 8 : 4) void k_pcg_fused(Src src, int kit, unsigned long long* pace,
                                                                        unsigned tag) {
@@ -1258,10 +1276,14 @@ __device__ __forceinline__ bool wait_sums(const double* P, int nb, int limit, do
 template <int D>
 __global__ __launch_bounds__(kBlock3) void k_pcg_persist(SoloSrc src, int spin_limit) {
     HFM_ENTRY(src);
-    __shared__ float U[D + 2][kF3][kF3];   // u_{k+1} on the frame
-    __shared__ float S[D + 2][kF3][kF3];   // alpha_s on the frame
+    constexpr int kH = kHalo<D>, kHM = (kH + kBlock3 - 1) / kBlock3, kO = D / 4;
+    __shared__ float U[D + 2][kF3][kF3];   // u_{k+1} on the frame (alpha_s while the face fractions are formed)
+    __shared__ float FX[D + 1][kT3][kT3];  // face fractions of the block's cells: x-face below plane li
+    __shared__ float FY[D][kT3 + 1][kT3];  // y-face below row lj
+    __shared__ float FZ[D][kT3][kT3 + 1];  // z-face below column lk
     __shared__ float Pl[D][kT3][kT3];      // p_{k-1} of the own cells
     __shared__ float Xl[D][kT3][kT3];      // x_k of the own cells
+    __shared__ uint2 Htab[kHM * kBlock3];  // halo cell h: memory index (clamped into the box), offset in U
     __shared__ double lds[3][kBlock3 / 64];
     CgCtl* c = a.ctl + 1;
     const int n = a.n;
@@ -1285,7 +1307,6 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(SoloSrc src, int spin_l
             nbr = tile_workgroup((ni * a.tiles_j + nj) * a.tiles_k + nk, nb);
     }
     // the layouts of k_pcg_fused: halo cells in the compact numbering, own cells in the stencil's layout
-    constexpr int kH = kHalo<D>, kHM = (kH + kBlock3 - 1) / kBlock3, kO = D / 4;
     const int lk = threadIdx.x % kT3, lj = (threadIdx.x / kT3) % kT3, lq = threadIdx.x / (kT3 * kT3);
     const int j = j0 + lj, k = k0 + lk;
     auto in_box = [&](int m) { return i0 + lq + 4 * m < n && j < n && k < n; };
@@ -1297,19 +1318,19 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(SoloSrc src, int spin_l
         return edge_jk || li == 0 || li == D - 1 || i0 + li == n - 1;
     };
     auto own_index = [&](int m) { return frame_index(i0, j0, k0, lq + 4 * m + 1, lj + 1, lk + 1, n); };
-    // constants of the solve: D at the halo and own cells, alpha_s on the frame
+    // constants of the solve: D at the halo and own cells, the halo cells' indices (each thread reads
+    // only the entries it writes) and, from alpha_s on the frame, the face fractions of the block's cells
+    float (*S)[kF3][kF3] = U;
     float hd[kHM], od[kO];
 #pragma unroll
     for (int q = 0; q < kHM; ++q) {
         const int h = (int)threadIdx.x + q * kBlock3;
-        hd[q] = 0.0f;
-        if (h < kH) {
-            int fi, fj, fk;
-            halo_cell<D>(h, fi, fj, fk);
-            const unsigned t = frame_index(i0, j0, k0, fi, fj, fk, n);
-            hd[q] = ld32((const float*)a.pdiag, t);
-            S[fi][fj][fk] = ld32((const float*)a.alpha_s, t);
-        }
+        int fi, fj, fk;
+        halo_cell<D>(h < kH ? h : kH - 1, fi, fj, fk);
+        const unsigned t = frame_index(i0, j0, k0, fi, fj, fk, n);
+        Htab[h] = make_uint2(t, (unsigned)((fi * kF3 + fj) * kF3 + fk));
+        hd[q] = ld32((const float*)a.pdiag, t);
+        if (h < kH) S[fi][fj][fk] = ld32((const float*)a.alpha_s, t);
     }
     // own cells: r_k, w_k, s_{k-1} in registers, p_{k-1} and x_k in LDS, for the whole solve
     float orr[kO], ow[kO], os[kO];
@@ -1323,6 +1344,20 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(SoloSrc src, int spin_l
     }
     const float amin = a.amin, hh = a.hh, rhh = a.rhh;
     const int fastdiv = a.fastdiv;
+    __syncthreads();
+    for (int e = (int)threadIdx.x; e < (D + 1) * kT3 * kT3; e += kBlock3) {
+        const int li = e / (kT3 * kT3), lj2 = e / kT3 % kT3, lk2 = e % kT3;
+        FX[li][lj2][lk2] = face_fraction(S[li][lj2 + 1][lk2 + 1], S[li + 1][lj2 + 1][lk2 + 1], amin);
+    }
+    for (int e = (int)threadIdx.x; e < D * (kT3 + 1) * kT3; e += kBlock3) {
+        const int li = e / ((kT3 + 1) * kT3), jj = e / kT3 % (kT3 + 1), lk2 = e % kT3;
+        FY[li][jj][lk2] = face_fraction(S[li + 1][jj][lk2 + 1], S[li + 1][jj + 1][lk2 + 1], amin);
+    }
+    for (int e = (int)threadIdx.x; e < D * kT3 * (kT3 + 1); e += kBlock3) {
+        const int li = e / (kT3 * (kT3 + 1)), lj2 = e / (kT3 + 1) % kT3, kk = e % (kT3 + 1);
+        FZ[li][lj2][kk] = face_fraction(S[li + 1][lj2 + 1][kk], S[li + 1][lj2 + 1][kk + 1], amin);
+    }
+    __syncthreads();   // U is u from here on
     const int maxeff = c->maxeff;
     const double tol = c->tol;
     double atol = c->atol;
@@ -1351,10 +1386,7 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(SoloSrc src, int spin_l
         float hr[kHM], hw[kHM], hs[kHM];
 #pragma unroll
         for (int q = 0; q < kHM; ++q) {
-            const int h = (int)threadIdx.x + q * kBlock3;
-            int fi, fj, fk;
-            halo_cell<D>(h < kH ? h : kH - 1, fi, fj, fk);
-            const unsigned t = frame_index(i0, j0, k0, fi, fj, fk, n);
+            const unsigned t = Htab[threadIdx.x + q * kBlock3].x;
             hr[q] = load_agent_f(rk, t);
             hw[q] = load_agent_f(wk, t);
             hs[q] = first ? 0.0f : load_agent_f(sprev, t);
@@ -1400,10 +1432,8 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(SoloSrc src, int spin_l
         for (int qh = 0; qh < kHM; ++qh) {
             const int h = (int)threadIdx.x + qh * kBlock3;
             if (h < kH) {
-                int fi, fj, fk;
-                halo_cell<D>(h, fi, fj, fk);
                 float s, r1;
-                U[fi][fj][fk] = cg_frame(hr[qh], hw[qh], hs[qh], hd[qh], first, q.alpha, q.beta, s, r1);
+                (&U[0][0][0])[Htab[h].y] = cg_frame(hr[qh], hw[qh], hs[qh], hd[qh], first, q.alpha, q.beta, s, r1);
             }
         }
         // ---- own cells: s_k, r_{k+1}, u_{k+1}, p_k, x_{k+1}
@@ -1436,7 +1466,7 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(SoloSrc src, int spin_l
         for (int m = 0; m < kO; ++m) {
             if (in_box(m)) {
                 float uc;
-                const float w1 = cg_stencil<D>(U, S, lq + 4 * m, lj, lk, amin, hh, rhh, fastdiv, uc);
+                const float w1 = cg_stencil_ff<D>(U, FX, FY, FZ, lq + 4 * m, lj, lk, hh, rhh, fastdiv, uc);
                 ow[m] = w1;
                 if (pub(m)) store_agent_f(wn, own_index(m), w1);
                 sum_d += (double)(w1 * uc);
