@@ -72,8 +72,9 @@ def measured_traffic(kernel: str, n: int):
         if name.startswith(kernel):
             return dict(traffic_bytes=rec["traffic_bytes"],
                         source=f"{os.path.relpath(TRAFFIC_FILE, REPO)} (stored rocprofv3 FETCH_SIZE/WRITE_SIZE, "
-                               f"{t.get('measured', 'earlier run')}; at this size part of it is Infinity Cache "
-                               f"traffic, not HBM)")
+                               f"{rec.get('measured', t.get('measured', 'earlier run'))}"
+                               f"{', per iteration' if rec.get('per') == 'iteration' else ''}; at this size part of "
+                               f"it is Infinity Cache traffic, not HBM)")
     return None
 
 
