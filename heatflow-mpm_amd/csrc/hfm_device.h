@@ -55,9 +55,46 @@ __host__ __device__ __forceinline__ void split_floor(float g, int& i, float& f) 
     f = g - fl;
 }
 
-// fields.py:5-10 — g = pos/h - off; i = floor(g); f = g - i (true IEEE divide)
-__host__ __device__ __forceinline__ void base_index1(float pos, float off, float h, int& i, float& f) {
-    const float g = pos / h - off;
+// x / b rounded correctly, for a scene constant b with rb = RN(1/b): q = RN(x rb), then one
+// residual correction q + RN(x - q b) rb (Markstein's theorem: with rb the correctly rounded
+// reciprocal and q within an ulp, the corrected quotient is the correctly rounded one).  The
+// host enables it (fast = 1) only after checking every significand of one binade of x against
+// IEEE division for this b; all intermediate values stay normal for 2^-100 <= |x| <= 2^100, so
+// that check covers the whole range, and anything outside it (or fast = 0), NaN and infinities
+// included, divides.
+__host__ __device__ __forceinline__ float div_const(float x, float b, float rb, int fast) {
+    const float ax = fabsf(x);
+    if (fast && ((ax >= 0x1p-100f && ax <= 0x1p100f) || x == 0.0f)) {
+        const float q = x * rb;
+        const float e = __builtin_fmaf(-q, b, x);
+        return __builtin_fmaf(e, rb, q);
+    }
+    return x / b;
+}
+
+// The cell size h as the index helpers divide by it: pos / h correctly rounded, through div_const's
+// corrected product where the host has checked it for this h (fast), else by division.  A plain
+// float converts to the division form.
+struct CellSize {
+    float h, rh;
+    int fast;
+    __host__ __device__ CellSize(float h_) : h(h_), rh(1.0f), fast(0) {}
+    __host__ __device__ CellSize(float h_, float rh_, int fast_) : h(h_), rh(rh_), fast(fast_) {}
+    __host__ __device__ float quotient(float pos) const { return div_const(pos, h, rh, fast); }
+};
+
+// a * b for a, b < 2^24 with a product below 2^32 (lattice row offsets): the full-rate 24-bit multiply
+__host__ __device__ __forceinline__ unsigned umul24(unsigned a, unsigned b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul24(a, b);
+#else
+    return a * b;
+#endif
+}
+
+// fields.py:5-10 — g = pos/h - off; i = floor(g); f = g - i (pos/h correctly rounded)
+__host__ __device__ __forceinline__ void base_index1(float pos, float off, CellSize hs, int& i, float& f) {
+    const float g = hs.quotient(pos) - off;
     split_floor(g, i, f);
 }
 
@@ -126,7 +163,7 @@ __host__ __device__ __forceinline__ float trilinear(const E* __restrict__ a, int
         for (int dy = 0; dy < 2; ++dy) {
             const unsigned iy = (unsigned)clampi(bj + dy, 0, ny - 1);
             const float wy = dy ? fy : 1.0f - fy;
-            const unsigned row = (ix * (unsigned)ny + iy) * (unsigned)nz;
+            const unsigned row = umul24(umul24(ix, (unsigned)ny) + iy, (unsigned)nz);
             float v[2];
             row_pair(a + row, bk, nz, v[0], v[1]);
 #pragma unroll
@@ -144,7 +181,7 @@ __host__ __device__ __forceinline__ float trilinear(const E* __restrict__ a, int
 // fields.py:25-36 — trilinear sample of a lattice with offsets (ox, oy, oz) at a point
 template <class E = float>
 __host__ __device__ __forceinline__ float sample(const E* __restrict__ a, float px, float py, float pz, float ox,
-                                        float oy, float oz, float h, int nx, int ny, int nz) {
+                                        float oy, float oz, CellSize h, int nx, int ny, int nz) {
     int bi, bj, bk;
     float fx, fy, fz;
     base_index1(px, ox, h, bi, fx);
@@ -156,7 +193,7 @@ __host__ __device__ __forceinline__ float sample(const E* __restrict__ a, float 
 // fields.py:51-65 — min and max of the eight lattice values around a point (the MacCormack
 // limiter), in the loop order of sample.
 __device__ __forceinline__ void value_range(const float* __restrict__ a, float px, float py, float pz, float ox,
-                                            float oy, float oz, float h, int nx, int ny, int nz, float& lo,
+                                            float oy, float oz, CellSize h, int nx, int ny, int nz, float& lo,
                                             float& hi) {
     int bi, bj, bk;
     float fx, fy, fz;
@@ -185,10 +222,10 @@ __device__ __forceinline__ void value_range(const float* __restrict__ a, float p
 // fields.py:68-72
 template <class E = float>
 __host__ __device__ __forceinline__ void sample_velocity(const E* __restrict__ u, const E* __restrict__ v,
-                                                const E* __restrict__ w, float px, float py, float pz, float h,
+                                                const E* __restrict__ w, float px, float py, float pz, CellSize h,
                                                 int n, float& vx, float& vy, float& vz) {
     // the three face lattices take offsets 0 or 0.5 per axis: one quotient per axis, two splits
-    const float qx = px / h, qy = py / h, qz = pz / h;
+    const float qx = h.quotient(px), qy = h.quotient(py), qz = h.quotient(pz);
     int x0, x5, y0, y5, z0, z5;
     float fx0, fx5, fy0, fy5, fz0, fz5;
     split_index(qx, 0.0f, x0, fx0);
@@ -205,7 +242,7 @@ __host__ __device__ __forceinline__ void sample_velocity(const E* __restrict__ u
 // fields.py:111-116 — midpoint RK2 departure point (E = uint16_t: through BF16 copies of u, v, w)
 template <class E = float>
 __host__ __device__ __forceinline__ void backtrace(const E* __restrict__ u, const E* __restrict__ v,
-                                          const E* __restrict__ w, float& x, float& y, float& z, float h, int n,
+                                          const E* __restrict__ w, float& x, float& y, float& z, CellSize h, int n,
                                           float dt) {
     float vx, vy, vz;
     sample_velocity<E>(u, v, w, x, y, z, h, n, vx, vy, vz);
@@ -225,21 +262,6 @@ __device__ __forceinline__ float face_fraction(float a_left, float a_right, floa
     return __builtin_amdgcn_fmed3f(__builtin_fmaf(-0.5f, a_left + a_right, 1.0f), amin, 1.0f);
 }
 
-// x / b rounded correctly, for a scene constant b with rb = RN(1/b): q = RN(x rb), then one
-// residual correction q + RN(x - q b) rb (Markstein's theorem: with rb the correctly rounded
-// reciprocal and q within an ulp, the corrected quotient is the correctly rounded one).  The
-// host enables it (fast = 1) only after checking every significand of one binade of x against
-// IEEE division for this b; all intermediate values stay normal for 2^-100 <= |x| <= 2^100, so
-// that check covers the whole range, and anything outside it (or fast = 0) divides.
-__device__ __forceinline__ float div_const(float x, float b, float rb, int fast) {
-    const float ax = fabsf(x);
-    if (fast && ((ax >= 0x1p-100f && ax <= 0x1p100f) || x == 0.0f)) {
-        const float q = x * rb;
-        const float e = __builtin_fmaf(-q, b, x);
-        return __builtin_fmaf(e, rb, q);
-    }
-    return x / b;
-}
 
 // heat.py:44-47
 __device__ __forceinline__ float conductivity(float alpha, float ka, float km) {

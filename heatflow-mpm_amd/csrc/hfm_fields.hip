@@ -102,6 +102,7 @@ __device__ __forceinline__ void heat_tail(const A& a, long t, int i, int j, int 
 template <bool MIXED, class Src>
 __global__ __launch_bounds__(kBlock) void k_advect_T(Src src, float dt) {
     HFM_ENTRY(src);
+    const CellSize hs(a.h, a.rh, a.fast_h);
     const long t = (long)xcd_tile(lb, nblocks(a.N)) * kBlock + threadIdx.x;
     if (t >= a.N) return;
     const int n = a.n;
@@ -115,10 +116,10 @@ __global__ __launch_bounds__(kBlock) void k_advect_T(Src src, float dt) {
     } else {
         float x = ((float)i + 0.5f) * h, y = ((float)j + 0.5f) * h, z = ((float)k + 0.5f) * h;
         if constexpr (MIXED)
-            backtrace<uint16_t>(a.ub[0], a.ub[1], a.ub[2], x, y, z, h, n, dt);
+            backtrace<uint16_t>(a.ub[0], a.ub[1], a.ub[2], x, y, z, hs, n, dt);
         else
-            backtrace<float>(a.u[0], a.u[1], a.u[2], x, y, z, h, n, dt);
-        t2 = sample(a.Tw, x, y, z, 0.5f, 0.5f, 0.5f, h, n, n, n);
+            backtrace<float>(a.u[0], a.u[1], a.u[2], x, y, z, hs, n, dt);
+        t2 = sample(a.Tw, x, y, z, 0.5f, 0.5f, 0.5f, hs, n, n, n);
     }
     if (a.maccormack) {
         a.mc_T[t] = t2;
@@ -134,6 +135,7 @@ __global__ __launch_bounds__(kBlock) void k_advect_T(Src src, float dt) {
 template <class Src>
 __global__ __launch_bounds__(kBlock) void k_mc_T(Src src, float dt) {
     HFM_ENTRY(src);
+    const CellSize hs(a.h, a.rh, a.fast_h);
     if (!a.maccormack) return;
     const long t = (long)xcd_tile(lb, nblocks(a.N)) * kBlock + threadIdx.x;
     if (t >= a.N) return;
@@ -146,12 +148,12 @@ __global__ __launch_bounds__(kBlock) void k_mc_T(Src src, float dt) {
     if (!(ac > 0.5f)) {
         const float xc = ((float)i + 0.5f) * h, yc = ((float)j + 0.5f) * h, zc = ((float)k + 0.5f) * h;
         float dx = xc, dy = yc, dz = zc, fx = xc, fy = yc, fz = zc;
-        backtrace<float>(a.u[0], a.u[1], a.u[2], dx, dy, dz, h, n, dt);
-        backtrace<float>(a.u[0], a.u[1], a.u[2], fx, fy, fz, h, n, -dt);
-        const float back = sample(a.mc_T, fx, fy, fz, 0.5f, 0.5f, 0.5f, h, n, n, n);
+        backtrace<float>(a.u[0], a.u[1], a.u[2], dx, dy, dz, hs, n, dt);
+        backtrace<float>(a.u[0], a.u[1], a.u[2], fx, fy, fz, hs, n, -dt);
+        const float back = sample(a.mc_T, fx, fy, fz, 0.5f, 0.5f, 0.5f, hs, n, n, n);
         float v = t2 + 0.5f * (a.Tw[t] - back);
         float lo, hi;
-        value_range(a.Tw, dx, dy, dz, 0.5f, 0.5f, 0.5f, h, n, n, n, lo, hi);
+        value_range(a.Tw, dx, dy, dz, 0.5f, 0.5f, 0.5f, hs, n, n, n, lo, hi);
         t2 = v < lo ? lo : (v > hi ? hi : v);
     }
     heat_tail(a, t, i, j, k, t2, ac, dt);
@@ -163,13 +165,14 @@ __global__ __launch_bounds__(kBlock) void k_mc_T(Src src, float dt) {
 template <class Src>
 __global__ __launch_bounds__(kBlock) void k_g2p(Src src) {
     HFM_ENTRY(src);
+    const CellSize hs(a.h, a.rh, a.fast_h);
     if (lb == 0 && threadIdx.x == 0) a.ctl[0].geom_dirty = 0;  // this substep's gather rebuilt the geometry
     const long p = (long)lb * kBlock + threadIdx.x;
     if (p >= a.P) return;
     const float x = a.pos[3 * p], y = a.pos[3 * p + 1], z = a.pos[3 * p + 2];
     const int n = a.n;
-    const float dT = sample(a.T, x, y, z, 0.5f, 0.5f, 0.5f, a.h, n, n, n) -
-                     sample(a.Tw, x, y, z, 0.5f, 0.5f, 0.5f, a.h, n, n, n);
+    const float dT = sample(a.T, x, y, z, 0.5f, 0.5f, 0.5f, hs, n, n, n) -
+                     sample(a.Tw, x, y, z, 0.5f, 0.5f, 0.5f, hs, n, n, n);
     float tp = a.temp[p] + dT;
     tp = tp < 0.0f ? 0.0f : (tp > a.t_max ? a.t_max : tp);
     a.temp[p] = tp;
@@ -183,20 +186,20 @@ __global__ __launch_bounds__(kBlock) void k_g2p(Src src) {
 // ---------------------------------------------------------------------------------------------
 // air face pass: advect (old u, v, w) -> buoyancy (new T) -> relax, into us; sub-range = axis
 // ---------------------------------------------------------------------------------------------
-// Face position, lattice offsets and dims of face f of an axis (fields.py:75-108).
+// Face position, lattice offsets and dims of face f of an axis (fields.py:75-108).  Written with selects
+// (no assignment through a branch on the axis, which the compiler kept as a per-thread array in scratch
+// memory); (float)i + 0 is (float)i exactly, so the positions are the reference's.
 __device__ __forceinline__ void face_geometry(int axis, int i, int j, int k, float h, int n, float& x, float& y,
                                               float& z, float& ox, float& oy, float& oz, int& dx, int& dy, int& dz) {
-    dx = dy = dz = n;
-    if (axis == 0) {
-        x = (float)i * h; y = ((float)j + 0.5f) * h; z = ((float)k + 0.5f) * h;
-        ox = 0.0f; oy = 0.5f; oz = 0.5f; dx = n + 1;
-    } else if (axis == 1) {
-        x = ((float)i + 0.5f) * h; y = (float)j * h; z = ((float)k + 0.5f) * h;
-        ox = 0.5f; oy = 0.0f; oz = 0.5f; dy = n + 1;
-    } else {
-        x = ((float)i + 0.5f) * h; y = ((float)j + 0.5f) * h; z = (float)k * h;
-        ox = 0.5f; oy = 0.5f; oz = 0.0f; dz = n + 1;
-    }
+    ox = axis == 0 ? 0.0f : 0.5f;
+    oy = axis == 1 ? 0.0f : 0.5f;
+    oz = axis == 2 ? 0.0f : 0.5f;
+    x = ((float)i + ox) * h;
+    y = ((float)j + oy) * h;
+    z = ((float)k + oz) * h;
+    dx = n + (axis == 0 ? 1 : 0);
+    dy = n + (axis == 1 ? 1 : 0);
+    dz = n + (axis == 2 ? 1 : 0);
 }
 
 // The rest of the face pass once the advected value of an interior face is known: buoyancy
@@ -223,6 +226,7 @@ __device__ __forceinline__ float face_tail(const A& a, int axis, long f, int i, 
 template <bool MIXED, class Src>
 __global__ __launch_bounds__(kBlock) void k_air_faces(Src src, float dt) {
     HFM_ENTRY(src);
+    const CellSize hs(a.h, a.rh, a.fast_h);
     const int axis = sub;
     const long f = (long)xcd_tile(lb, nblocks(a.F)) * kBlock + threadIdx.x;
     if (f >= a.F) return;
@@ -240,10 +244,10 @@ __global__ __launch_bounds__(kBlock) void k_air_faces(Src src, float dt) {
     int dx, dy, dz;
     face_geometry(axis, i, j, k, h, n, x, y, z, ox, oy, oz, dx, dy, dz);
     if constexpr (MIXED)
-        backtrace<uint16_t>(a.ub[0], a.ub[1], a.ub[2], x, y, z, h, n, dt);
+        backtrace<uint16_t>(a.ub[0], a.ub[1], a.ub[2], x, y, z, hs, n, dt);
     else
-        backtrace<float>(a.u[0], a.u[1], a.u[2], x, y, z, h, n, dt);
-    const float val = sample(a.u[axis], x, y, z, ox, oy, oz, h, dx, dy, dz);
+        backtrace<float>(a.u[0], a.u[1], a.u[2], x, y, z, hs, n, dt);
+    const float val = sample(a.u[axis], x, y, z, ox, oy, oz, hs, dx, dy, dz);
     out[f] = a.maccormack ? val : face_tail(a, axis, f, i, j, k, val, dt);
 }
 
@@ -252,6 +256,7 @@ __global__ __launch_bounds__(kBlock) void k_air_faces(Src src, float dt) {
 template <class Src>
 __global__ __launch_bounds__(kBlock) void k_mc_faces(Src src, float dt) {
     HFM_ENTRY(src);
+    const CellSize hs(a.h, a.rh, a.fast_h);
     if (!a.maccormack) return;
     const int axis = sub;
     const long f = (long)xcd_tile(lb, nblocks(a.F)) * kBlock + threadIdx.x;
@@ -270,13 +275,13 @@ __global__ __launch_bounds__(kBlock) void k_mc_faces(Src src, float dt) {
     int dx, dy, dz;
     face_geometry(axis, i, j, k, h, n, x, y, z, ox, oy, oz, dx, dy, dz);
     float px = x, py = y, pz = z, fx = x, fy = y, fz = z;
-    backtrace<float>(a.u[0], a.u[1], a.u[2], px, py, pz, h, n, dt);
-    backtrace<float>(a.u[0], a.u[1], a.u[2], fx, fy, fz, h, n, -dt);
+    backtrace<float>(a.u[0], a.u[1], a.u[2], px, py, pz, hs, n, dt);
+    backtrace<float>(a.u[0], a.u[1], a.u[2], fx, fy, fz, hs, n, -dt);
     const float fwd = a.mc_u[axis][f];
-    const float back = sample(a.mc_u[axis], fx, fy, fz, ox, oy, oz, h, dx, dy, dz);
+    const float back = sample(a.mc_u[axis], fx, fy, fz, ox, oy, oz, hs, dx, dy, dz);
     float v = fwd + 0.5f * (a.u[axis][f] - back);
     float lo, hi;
-    value_range(a.u[axis], px, py, pz, ox, oy, oz, h, dx, dy, dz, lo, hi);
+    value_range(a.u[axis], px, py, pz, ox, oy, oz, hs, dx, dy, dz, lo, hi);
     v = v < lo ? lo : (v > hi ? hi : v);
     out[f] = face_tail(a, axis, f, i, j, k, v, dt);
 }

@@ -93,6 +93,8 @@ struct SceneView {
     float h, hh;
     float rhh;               // RN(1 / hh)
     int fastdiv;             // x / hh through div_const's corrected product (checked on the host)
+    float rh;                // RN(1 / h)
+    int fast_h;              // pos / h through div_const's corrected product (checked on the host)
     float rho, t_amb, g, amin, ka, km, frac, k0, t_sol, t_liq, ys, mus, mul, t_max, rate, t_cap;
     int rate_on, buoyancy;
     double rho_d;

@@ -187,6 +187,8 @@ void refresh_dev(Scene* s) {
     d.hh = d.h * d.h;
     d.rhh = 1.0f / d.hh;
     d.fastdiv = const_division_exact(d.hh) ? 1 : 0;
+    d.rh = 1.0f / d.h;
+    d.fast_h = const_division_exact(d.h) ? 1 : 0;
     d.rho = (float)p.rho;
     d.rho_d = p.rho;
     d.t_amb = (float)p.t_amb;

@@ -30,17 +30,20 @@ int main() {
     const float big[] = {0.0f, 0.5f * h, h, 2.5f * h, (float)n * h, -h, 1e4f, -1e4f, 3e7f, -3e7f, 2.2e9f * h,
                          -2.2e9f * h, 1e30f, -1e30f, 3.4e38f, -3.4e38f, inf, -inf, nan};
     const int nb = (int)(sizeof(big) / sizeof(big[0]));
-    for (int a = 0; a < nb; ++a)
-        for (int b = 0; b < nb; b += 3)
-            for (int c = 0; c < nb; c += 5) {
-                float x = big[a], y = big[b], z = big[c];
-                float vx, vy, vz;
-                hfm::sample_velocity<float>(u, v, w, x, y, z, h, n, vx, vy, vz);
-                float bx = x, by = y, bz = z;
-                hfm::backtrace<float>(u, v, w, bx, by, bz, h, n, dt);
-                const float t = hfm::sample<float>(T, bx, by, bz, 0.5f, 0.5f, 0.5f, h, n, n, n);
-                std::printf("%g %g %g %g\n", (double)vx, (double)vy, (double)vz, (double)t);
-            }
+    // pos / h by division, then by the corrected product of the reciprocal (the kernels' fast form)
+    const hfm::CellSize forms[2] = {hfm::CellSize(h), hfm::CellSize(h, 1.0f / h, 1)};
+    for (const hfm::CellSize& hs : forms)
+        for (int a = 0; a < nb; ++a)
+            for (int b = 0; b < nb; b += 3)
+                for (int c = 0; c < nb; c += 5) {
+                    float x = big[a], y = big[b], z = big[c];
+                    float vx, vy, vz;
+                    hfm::sample_velocity<float>(u, v, w, x, y, z, hs, n, vx, vy, vz);
+                    float bx = x, by = y, bz = z;
+                    hfm::backtrace<float>(u, v, w, bx, by, bz, hs, n, dt);
+                    const float t = hfm::sample<float>(T, bx, by, bz, 0.5f, 0.5f, 0.5f, hs, n, n, n);
+                    std::printf("%g %g %g %g\n", (double)vx, (double)vy, (double)vz, (double)t);
+                }
     // velocities that are themselves huge or non-finite (a diverged pressure projection)
     const float vel[] = {1e30f, -1e30f, inf, -inf, nan};
     for (float s : vel) {
