@@ -1247,9 +1247,9 @@ __device__ __forceinline__ bool wait_neighbour(const double* P, int nbr, int lim
 }
 
 // Rows r.r, g, d of slot P (row stride nb): thread b < nb polls its entries until none reads as empty and
-// takes them as its share of the reduction (load_rows' first entries); false in every thread if one did
-// not arrive within `limit` polls.
-__device__ __forceinline__ bool wait_sums(const double* P, int nb, int limit, double (&acc)[kFusedRows]) {
+// takes them as its share of the reduction (load_rows' first entries); returns 1 if they did not arrive
+// within `limit` polls (no barrier: the caller's next one combines the flags).
+__device__ __forceinline__ int poll_sums(const double* P, int nb, int limit, double (&acc)[kFusedRows]) {
     int late = 0;
 #pragma unroll
     for (int q = 0; q < kFusedRows; ++q) acc[q] = 0.0;
@@ -1270,7 +1270,7 @@ __device__ __forceinline__ bool wait_sums(const double* P, int nb, int limit, do
             __builtin_amdgcn_s_sleep(1);
         }
     }
-    return __syncthreads_or(late) == 0;
+    return late;
 }
 
 template <int D>
@@ -1285,6 +1285,8 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(SoloSrc src, int spin_l
     __shared__ float Xl[D][kT3][kT3];      // x_k of the own cells
     __shared__ uint2 Htab[kHM * kBlock3];  // halo cell h: memory index (clamped into the box), offset in U
     __shared__ double lds[3][kBlock3 / 64];
+    __shared__ double wsum[kFusedRows][kBlock3 / 64];   // per-wave sums of the rows
+    __shared__ CgScalars qs;                            // iteration k's scalars, formed by wave 0
     CgCtl* c = a.ctl + 1;
     const int n = a.n;
     const long N = a.N;
@@ -1400,23 +1402,45 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(SoloSrc src, int spin_l
                 os[m] = first ? 0.0f : load_agent_f(sprev, t);
             }
         }
-        if (!first && ok) ok = wait_sums(P, nb, spin_limit, acc);
-        if (!ok) {
+        int late = ok ? 0 : 1;
+        if (!first && ok) late = poll_sums(P, nb, spin_limit, acc);
+        // ---- scalars of iteration k: the waves holding entries reduce them (reduce_bcast's tree), wave 0
+        // adds the waves and forms the scalars once, the workgroup reads them from LDS
+        {
+            const int nrows = first ? kFusedRows : 3, count = first ? a.nb_upd : nb;
+            const bool any = (int)(threadIdx.x & ~63u) < count;
+#pragma unroll
+            for (int r = 0; r < kFusedRows; ++r)
+                if (r < nrows) {
+                    const double w = any ? wave_sum(acc[r]) : 0.0;
+                    if ((threadIdx.x & 63) == 0) wsum[r][threadIdx.x >> 6] = w;
+                }
+        }
+        if (__syncthreads_or(late)) {
             if (threadIdx.x == 0) {
                 c->done = 1;
                 mirror_final(a.mirror + 1, kit, __builtin_nanf(""), atol, -2);
             }
             return;
         }
-        if (kit >= 2 && threadIdx.x == 0)   // empty this workgroup's sums of iteration k - 2
+        if (threadIdx.x < 64) {
+            double tot[kFusedRows];
 #pragma unroll
-            for (int r = 0; r < 3; ++r)
-                store_agent(slots + ((kit + 1) % 3) * 3 * nb + r * nb + lb, __builtin_bit_cast(double, kEmptySum));
-        // ---- scalars of iteration k
-        double tot[kFusedRows];
-        reduce_bcast<kFusedRows, kBlock3>(acc, first ? kFusedRows : 3, 0, tot, first ? a.nb_upd : nb);
-        const CgScalars q = cg_scalars(tot, first, g_prev, pap_prev, tol, atol, atol_sq, kit, maxeff);
-        if (lb == 0 && threadIdx.x == 0) cg_record(c, a.mirror + 1, q, first, kit, atol, atol_sq);
+            for (int r = 0; r < kFusedRows; ++r)
+                tot[r] = r < (first ? kFusedRows : 3) ? sum_wave_values<kBlock3 / 64>(wsum[r]) : 0.0;
+            const CgScalars s0 = cg_scalars(tot, first, g_prev, pap_prev, tol, atol, atol_sq, kit, maxeff);
+            if (threadIdx.x == 0) {
+                qs = s0;
+                if (lb == 0) cg_record(c, a.mirror + 1, s0, first, kit, atol, atol_sq);
+                if (kit >= 2)   // empty this workgroup's sums of iteration k - 2
+#pragma unroll
+                    for (int r = 0; r < 3; ++r)
+                        store_agent(slots + ((kit + 1) % 3) * 3 * nb + r * nb + lb,
+                                    __builtin_bit_cast(double, kEmptySum));
+            }
+        }
+        __syncthreads();
+        const CgScalars q = qs;
         if (q.stop) {
             // x_k and p_{k-1} where the one-launch form leaves them
 #pragma unroll
