@@ -1229,11 +1229,13 @@ __device__ __forceinline__ int tile_workgroup(int L, int nb) {
 }
 
 // Wait until the r.r entry of workgroup `nbr` in slot P reads as posted (threads with nbr < 0 do not
-// wait); false in every thread if it did not within `limit` polls.  A workgroup posts its sums only
-// after its faces of that iteration are in memory.
+// wait); false in every thread if it did not within `limit` polls (at once for a negative limit).  A
+// workgroup posts its sums only after its faces of that iteration are in memory.
 __device__ __forceinline__ bool wait_neighbour(const double* P, int nbr, int limit) {
     int late = 0;
-    if (nbr >= 0) {
+    if (nbr >= 0 && limit < 0) {
+        late = 1;
+    } else if (nbr >= 0) {
         for (int spins = 0;; ++spins) {
             if (!empty_sum(load_agent(P + nbr))) break;
             if (spins >= limit) {
@@ -1248,13 +1250,15 @@ __device__ __forceinline__ bool wait_neighbour(const double* P, int nbr, int lim
 
 // Rows r.r, g, d of slot P (row stride nb): thread b < nb polls its entries until none reads as empty and
 // takes them as its share of the reduction (load_rows' first entries); returns 1 if they did not arrive
-// within `limit` polls (no barrier: the caller's next one combines the flags).
+// within `limit` polls, at once for a negative limit (no barrier: the caller's next one combines the flags).
 __device__ __forceinline__ int poll_sums(const double* P, int nb, int limit, double (&acc)[kFusedRows]) {
     int late = 0;
 #pragma unroll
     for (int q = 0; q < kFusedRows; ++q) acc[q] = 0.0;
     const int b = (int)threadIdx.x;
-    if (b < nb) {
+    if (b < nb && limit < 0) {
+        late = 1;
+    } else if (b < nb) {
         for (int spins = 0;; ++spins) {
             const double v0 = load_agent(P + b), v1 = load_agent(P + nb + b), v2 = load_agent(P + 2 * nb + b);
             if (!empty_sum(v0) && !empty_sum(v1) && !empty_sum(v2)) {
@@ -1614,7 +1618,8 @@ int pcg_persist_capacity(int D, int device) {
 // HFM_PCG_PERSIST_ORDINARY=1 launches the same kernel as an ordinary dispatch, for profilers that cannot
 // trace a cooperative launch (rocprofv3 7.2 ends without writing its results).  Co-residency is then not
 // guaranteed by the runtime, only likely on an otherwise idle device (one workgroup per CU); the waits
-// stay bounded either way.
+// stay bounded either way.  HFM_PCG_PERSIST_SPINS (read at every launch) replaces the polls a wait may
+// take; a negative value gives up at the first wait, which forces the timeout path from the host.
 hipError_t launch_pcg_persist(const Launch& L, int D, hipEvent_t ev0, hipEvent_t ev1) {
     static const bool ordinary = [] {
         const char* e = getenv("HFM_PCG_PERSIST_ORDINARY");
@@ -1624,6 +1629,7 @@ hipError_t launch_pcg_persist(const Launch& L, int D, hipEvent_t ev0, hipEvent_t
     if (!fn || !L.solo) return hipErrorInvalidValue;
     SoloSrc src{*reinterpret_cast<const SceneG*>(L.solo)};
     int limit = kPersistSpinLimit;
+    if (const char* e = getenv("HFM_PCG_PERSIST_SPINS")) limit = atoi(e);
     void* args[] = {(void*)&src, (void*)&limit};
     if (ev0) (void)hipEventRecord(ev0, L.stream);
     const hipError_t e = ordinary ? hipLaunchKernel(fn, L.sgrid, dim3(kBlock3), args, 0, L.stream)

@@ -706,8 +706,8 @@ int run_pcg(Batch* b, int op, float dt) {
         const int it = __atomic_load_n(&m->it, __ATOMIC_ACQUIRE);
         const int conv = __atomic_load_n(&m->converged, __ATOMIC_ACQUIRE);
         if (conv == -2) {
-            set_err("persistent pressure PCG: a workgroup waited longer than %d polls for the others at "
-                    "iteration %d", kPersistSpinLimit, it);
+            set_err("persistent pressure PCG: a workgroup's wait for the others ran out of polls at iteration %d "
+                    "(every workgroup has ended; the pressure is not solved)", it);
             return HFM_ERR_HIP;
         }
         // a breakdown stop keeps the last iterate; a non-finite residual means the state already is

@@ -114,6 +114,21 @@ def test_persistent_falls_back_where_it_cannot_run(gpu):
     _lockstep(kw, dt, 2, expect_persistent=False)
 
 
+def test_persistent_wait_timeout_is_reported(gpu, monkeypatch):
+    """The bounded wait's failure path, forced from the host: with HFM_PCG_PERSIST_SPINS=-1 every workgroup
+    gives up at its first wait for the others, reports the solve as stopped and ends; the substep returns
+    the error.  Afterwards, with the limit back, scenes solve as before (same bits as the one-launch form)."""
+    kw, dt, _ = coupled_scene(48)   # from rest the fan drives a nonzero divergence: the solve iterates
+    sim = _sim(kw, "persistent", 16)
+    monkeypatch.setenv("HFM_PCG_PERSIST_SPINS", "-1")
+    with pytest.raises(RuntimeError, match="ran out of polls"):
+        sim.substep(dt)
+    assert _persist_launches(sim) == 1
+    sim.close()
+    monkeypatch.delenv("HFM_PCG_PERSIST_SPINS")
+    _lockstep(kw, dt, 2)
+
+
 def test_persistent_solves_from_two_threads(gpu):
     """Two scenes stepped from two host threads at once (the library runs their persistent solves one
     at a time on the device): each ends with the bits of the same scene stepped alone."""
