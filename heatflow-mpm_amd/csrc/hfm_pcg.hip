@@ -1277,8 +1277,8 @@ __device__ __forceinline__ int poll_sums(const double* P, int nb, int limit, dou
     return late;
 }
 
-template <int D>
-__global__ __launch_bounds__(kBlock3) void k_pcg_persist(SoloSrc src, int spin_limit) {
+template <int D, class Src>
+__global__ __launch_bounds__(kBlock3) void k_pcg_persist(Src src, int spin_limit) {
     HFM_ENTRY(src);
     constexpr int kH = kHalo<D>, kHM = (kH + kBlock3 - 1) / kBlock3, kO = D / 4;
     __shared__ float U[D + 2][kF3][kF3];   // u_{k+1} on the frame (alpha_s while the face fractions are formed)
@@ -1596,17 +1596,18 @@ void launch_pcg_fused(const Launch& L, Family fam, int k, unsigned long long* pa
     }
 }
 
+template <class Src>
 static const void* persist_kernel(int D) {
     switch (D) {
-        case 4: return (const void*)k_pcg_persist<4>;
-        case 8: return (const void*)k_pcg_persist<8>;
-        case 16: return (const void*)k_pcg_persist<16>;
+        case 4: return (const void*)k_pcg_persist<4, Src>;
+        case 8: return (const void*)k_pcg_persist<8, Src>;
+        case 16: return (const void*)k_pcg_persist<16, Src>;
         default: return nullptr;
     }
 }
 
-int pcg_persist_capacity(int D, int device) {
-    const void* fn = persist_kernel(D);
+int pcg_persist_capacity(int D, int device, bool batch) {
+    const void* fn = batch ? persist_kernel<BatchSrc>(D) : persist_kernel<SoloSrc>(D);
     if (!fn) return 0;
     int per_cu = 0, cus = 0, coop = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock3, 0) != hipSuccess) return 0;
@@ -1620,12 +1621,17 @@ int pcg_persist_capacity(int D, int device) {
 // guaranteed by the runtime, only likely on an otherwise idle device (one workgroup per CU); the waits
 // stay bounded either way.  HFM_PCG_PERSIST_SPINS (read at every launch) replaces the polls a wait may
 // take; a negative value gives up at the first wait, which forces the timeout path from the host.
-hipError_t launch_pcg_persist(const Launch& L, int D, hipEvent_t ev0, hipEvent_t ev1) {
+static bool persist_ordinary() {
     static const bool ordinary = [] {
         const char* e = getenv("HFM_PCG_PERSIST_ORDINARY");
         return e && atoi(e) != 0;
     }();
-    const void* fn = persist_kernel(D);
+    return ordinary;
+}
+
+hipError_t launch_pcg_persist(const Launch& L, int D, hipEvent_t ev0, hipEvent_t ev1) {
+    const bool ordinary = persist_ordinary();
+    const void* fn = persist_kernel<SoloSrc>(D);
     if (!fn || !L.solo) return hipErrorInvalidValue;
     SoloSrc src{*reinterpret_cast<const SceneG*>(L.solo)};
     int limit = kPersistSpinLimit;
@@ -1635,6 +1641,23 @@ hipError_t launch_pcg_persist(const Launch& L, int D, hipEvent_t ev0, hipEvent_t
     const hipError_t e = ordinary ? hipLaunchKernel(fn, L.sgrid, dim3(kBlock3), args, 0, L.stream)
                                   : hipLaunchCooperativeKernel(fn, L.sgrid, dim3(kBlock3), args, 0, L.stream);
     if (ev1) (void)hipEventRecord(ev1, L.stream);
+    return e;
+}
+
+hipError_t launch_pcg_persist_batch(const SceneDev* tab, const MapEntry* map, int blocks, int D, hipStream_t stream,
+                                    hipEvent_t ev0, hipEvent_t ev1) {
+    const void* fn = persist_kernel<BatchSrc>(D);
+    if (!fn || blocks <= 0) return hipErrorInvalidValue;
+    BatchSrc src;
+    src.tab = (const HFM_CONST SceneG*)(const void*)tab;
+    src.map = (const HFM_CONST MapEntry*)(const void*)map;
+    int limit = kPersistSpinLimit;
+    if (const char* e = getenv("HFM_PCG_PERSIST_SPINS")) limit = atoi(e);
+    void* args[] = {(void*)&src, (void*)&limit};
+    if (ev0) (void)hipEventRecord(ev0, stream);
+    const hipError_t e = persist_ordinary() ? hipLaunchKernel(fn, dim3(blocks), dim3(kBlock3), args, 0, stream)
+                                            : hipLaunchCooperativeKernel(fn, dim3(blocks), dim3(kBlock3), args, 0, stream);
+    if (ev1) (void)hipEventRecord(ev1, stream);
     return e;
 }
 

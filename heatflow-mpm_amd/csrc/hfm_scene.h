@@ -299,6 +299,10 @@ struct Batch {
     MapEntry* fstage[3] = {};
     size_t fstage_cap[3] = {};
     int fslot = 0;
+    MapEntry* pmap = nullptr;                // persistent groups' block maps (device, pinned staging)
+    size_t pmap_cap = 0;
+    MapEntry* pstage = nullptr;
+    size_t pstage_cap = 0;
     unsigned long long* pace = nullptr;      // pinned: (solve tag << 32) | iterations completed
     unsigned long long* pace_dev = nullptr;
     unsigned solve_tag = 0;
@@ -420,7 +424,11 @@ constexpr int kPersistMaxBlocks = 1024;
 // one-launch form's second parity buffer), and the pattern of an empty entry (a signalling NaN)
 constexpr long kPersistSlots = (long)kFusedRows * kMaxDirBlocks;
 constexpr unsigned long long kEmptySum = 0x7FF4A5A5A5A5A5A5ull;
-int pcg_persist_capacity(int D, int device);
+int pcg_persist_capacity(int D, int device, bool batch = false);
 hipError_t launch_pcg_persist(const Launch& L, int D, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+// the same for scenes of a batch: one cooperative launch of `blocks` workgroups over a block map of
+// whole scenes (each scene's workgroups wait only on each other)
+hipError_t launch_pcg_persist_batch(const SceneDev* tab, const MapEntry* map, int blocks, int D, hipStream_t stream,
+                                    hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 
 }  // namespace hfm

@@ -260,6 +260,12 @@ void batch_release(Batch* b) {
         if (b->fstage[r]) (void)hipHostFree(b->fstage[r]);
         b->fstage[r] = nullptr;
     }
+    if (b->pmap) (void)hipFree(b->pmap);
+    b->pmap = nullptr;
+    b->pmap_cap = 0;
+    if (b->pstage) (void)hipHostFree(b->pstage);
+    b->pstage = nullptr;
+    b->pstage_cap = 0;
     if (b->pace) (void)hipHostFree(b->pace);
     b->pace = nullptr;
     if (b->ev_pool) {
@@ -505,6 +511,84 @@ void launch_mg_coefficients_if_stale(Batch* b) {
     for (Scene* s : b->scenes) s->mg_coef_dirty = 0;
 }
 
+// The pressure solves of a batch's scenes marked in `persist` (each one's tiles fit the device at once) as
+// persistent launches: scenes of one tile depth packed first-fit, largest first, into cooperative
+// launches of at most the device's resident workgroups, each scene's block-map run whole (its padding to
+// the XCD stride included).  Every scene's workgroups wait only on each other, so the scenes of a group
+// solve independently and a group's launch ends with its slowest solve.  All groups' maps go to the device
+// in one copy ahead of the launches; the staging buffer is rewritten only by the next solve, after this
+// one has ended.
+int launch_persist_groups(Batch* b, const std::vector<char>& persist, bool timed) {
+    const size_t ns = b->scenes.size();
+    struct Group {
+        int D;
+        std::vector<size_t> scenes;
+        size_t len = 0;
+    };
+    std::vector<Group> groups;
+    for (int D : {16, 8, 4}) {
+        const Family f = D == 16 ? FAM_FUSED16 : (D == 8 ? FAM_FUSED8 : FAM_FUSED4);
+        const int cap = pcg_persist_capacity(D, b->device, true);
+        std::vector<size_t> idx;
+        for (size_t i = 0; i < ns; ++i)
+            if (persist[i] && b->scenes[i]->dev.dtile == D) idx.push_back(i);
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) {
+            const auto rx = b->ranges[f][x], ry = b->ranges[f][y];
+            return rx.second - rx.first > ry.second - ry.first;
+        });
+        const size_t first = groups.size();
+        for (size_t i : idx) {
+            const size_t len = b->ranges[f][i].second - b->ranges[f][i].first;
+            if (len > (size_t)cap) {
+                set_err("persistent pressure PCG: scene %zu needs %zu workgroups, the device holds %d", i, len, cap);
+                return HFM_ERR_STATE;
+            }
+            size_t g = first;
+            while (g < groups.size() && groups[g].len + len > (size_t)cap) ++g;
+            if (g == groups.size()) {
+                groups.push_back(Group());
+                groups.back().D = D;
+            }
+            groups[g].scenes.push_back(i);
+            groups[g].len += len;
+        }
+    }
+    if (groups.empty()) return HFM_OK;
+    size_t total = 0;
+    for (const Group& g : groups) total += g.len;
+    if (total > b->pstage_cap) {
+        if (b->pstage) HIP_OK(hipHostFree(b->pstage));
+        b->pstage = nullptr;
+        b->pstage_cap = 0;
+        HIP_OK(hipHostMalloc((void**)&b->pstage, total * sizeof(MapEntry), hipHostMallocDefault));
+        b->pstage_cap = total;
+    }
+    size_t off = 0;
+    for (const Group& g : groups) {
+        const Family f = g.D == 16 ? FAM_FUSED16 : (g.D == 8 ? FAM_FUSED8 : FAM_FUSED4);
+        for (size_t i : g.scenes) {
+            const auto rg = b->ranges[f][i];
+            memcpy(b->pstage + off, b->hmap[f].data() + rg.first, (rg.second - rg.first) * sizeof(MapEntry));
+            off += rg.second - rg.first;
+        }
+    }
+    RC(ensure_capacity(&b->pmap, &b->pmap_cap, total));
+    HIP_OK(hipMemcpyAsync(b->pmap, b->pstage, total * sizeof(MapEntry), hipMemcpyHostToDevice, b->stream));
+    off = 0;
+    for (const Group& g : groups) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (timed) RC(timing_events(b, HFM_KT_PCG_PERSIST, &e0, &e1));
+        const hipError_t e = launch_pcg_persist_batch(b->table, b->pmap + off, (int)g.len, g.D, b->stream, e0, e1);
+        if (e != hipSuccess) {
+            set_err("persistent pressure PCG: cooperative launch of %zu workgroups (%zu scenes) failed: %s", g.len,
+                    g.scenes.size(), hipGetErrorString(e));
+            return HFM_ERR_HIP;
+        }
+        off += g.len;
+    }
+    return HFM_OK;
+}
+
 // Run PCG iterations from iteration 0 for every scene of the batch whose mirror is not done
 // (the init launches are enqueued), and return when the device reports all of them done.
 // ir: the inner solves of a mixed-precision refinement step (FP16 kernels).
@@ -613,6 +697,32 @@ int pcg_iterate(Batch* b, int op, float dt, bool ir, int maxeff, int pred) {
         }
         launched = maxeff;
         closed = true;
+    } else if (op == 1 && !ir && ns > 1) {
+        // a batch: the scenes that can run the persistent form solve in cooperative groups first, the
+        // others (if any) in the loop below, from which the persistent ones are left out
+        std::vector<char> persist(ns, 0);
+        size_t np = 0;
+        for (size_t i = 0; i < ns; ++i)
+            if (active[i] && b->scenes[i]->dev.cg_persist) {
+                persist[i] = 1;
+                ++np;
+            }
+        if (np > 0) {
+            persist_lock = std::unique_lock<std::mutex>(persist_mutex(b->device));
+            RC(launch_persist_groups(b, persist, timed));
+            CHECK_LAUNCH();
+            for (size_t i = 0; i < ns; ++i)
+                if (persist[i]) {
+                    active[i] = 0;
+                    --n_active;
+                }
+            if (n_active == 0) {
+                launched = maxeff;
+                closed = true;
+            } else {
+                RC(filter_families(b, iter_fams, niter, active, fl));
+            }
+        }
     }
     for (;;) {
         bool all_done = true;

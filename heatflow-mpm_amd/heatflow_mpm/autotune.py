@@ -8,8 +8,8 @@ heater with fan, PCG tolerance 1e-6) for a few substeps after a warmup, and is s
 time per pressure-PCG iteration (launch gaps included, so the one-launch fused form is credited
 for its fewer launches).  Candidates: pass A tiles of 16, 8 and 4 planes (those whose tile count
 fits the pass-A workgroup cap), each with the two-pass PCG and with the one-launch iteration, the
-persistent form of the one-launch iteration where the tiles fit one per CU (alone only: a batch
-does not run it), and the untiled two-pass form.  The winners go to a JSON table (shipped as
+persistent form of the one-launch iteration where a scene's tiles fit one per CU (a batch packs such
+scenes into cooperative launches), and the untiled two-pass form.  The winners go to a JSON table (shipped as
 tuning_mi355x.json); HeatflowMPM(layout="auto" / "auto-batch") reads it, using the entry of the
 nearest measured size.
 """
@@ -53,7 +53,7 @@ def candidates(n: int, context: str = "solo", cus: int = 256):
         if tiles <= 4096:   # the pass-A workgroup cap of the library
             out.append(dict(dir_planes=d, cg="warp"))
             out.append(dict(dir_planes=d, cg="fused"))
-            if context == "solo" and tiles <= cus:   # one resident workgroup per tile
+            if tiles <= cus:   # one resident workgroup per tile (a batch packs scenes into such launches)
                 out.append(dict(dir_planes=d, cg="persistent"))
     out.append(dict(dir_planes=-1, cg="warp"))
     return out

@@ -92,8 +92,8 @@ def test_persistent_one_substep_matches_oracle(gpu):
 
 
 def test_persistent_falls_back_where_it_cannot_run(gpu):
-    """A batch of two runs the one-launch form (same bits as each scene stepped alone in the persistent
-    form); at n = 128 the 512 tiles do not fit one per CU and the scene runs the one-launch form."""
+    """A batch of two runs the persistent form in one cooperative launch (same bits as each scene stepped
+    alone); at n = 128 the 512 tiles do not fit one per CU and the scene runs the one-launch form."""
     kw, dt, _ = coupled_scene(48)
     alone = [_sim(kw, "persistent", 16) for _ in range(2)]
     pair = [_sim(kw, "persistent", 16) for _ in range(2)]
@@ -106,12 +106,67 @@ def test_persistent_falls_back_where_it_cannot_run(gpu):
                 for a, b in zip(alone, pair):
                     _assert_same(a, b, f"batch substep {k + 1}")
         assert all(_persist_launches(s) == 5 for s in alone)
-        assert all(_persist_launches(s) == 0 for s in pair)
     finally:
         for s in alone + pair:
             s.close()
     kw, dt, _ = coupled_scene(128)
     _lockstep(kw, dt, 2, expect_persistent=False)
+
+
+def _solo_fused_run(kw, dt, substeps, planes=16):
+    sim = _sim(kw, "fused", planes)
+    for _ in range(substeps):
+        sim.substep(dt)
+    return sim
+
+
+def test_persistent_batch_equals_scenes_alone(gpu):
+    """A batch of scenes with the persistent form: those whose tiles fit the device (n = 32, 40 with ragged
+    tiles, 48, 64) solve in one cooperative launch, each scene's workgroups waiting only on each other; the
+    n = 128 scene (512 tiles) runs the one-launch form in the same batch.  Every scene ends with the bits of
+    the one-launch form stepped alone."""
+    from heatflow_mpm.scenes import random_scene
+    rng = np.random.default_rng(7)
+    kws = [random_scene(rng, n, name=f"b{i}")[1] for i, n in enumerate((32, 40, 48, 64, 48, 128))]
+    dt, K = 2e-3, 4
+    sims = [_sim(kw, "persistent", 16) for kw in kws]
+    try:
+        with HeatflowBatch(sims) as batch:
+            batch.set_kernel_timing(True)
+            for _ in range(K):
+                batch.substep(dt)
+            groups = batch.kernel_time(_abi.KT_PCG_PERSIST)[1]
+        print("persistent group launches", groups, flush=True)
+        assert groups == K          # the five fitting scenes share one launch per substep
+        for i, (kw, s) in enumerate(zip(kws, sims)):
+            ref = _solo_fused_run(kw, dt, K)
+            _assert_same(ref, s, f"scene {i} (n = {kw['grid'].n})")
+            ref.close()
+    finally:
+        for s in sims:
+            s.close()
+
+
+def test_persistent_batch_of_large_scenes_takes_several_launches(gpu):
+    """Three 96^3 scenes (216 workgroups each) do not fit one launch: one group each, in turn."""
+    from heatflow_mpm.scenes import random_scene
+    rng = np.random.default_rng(11)
+    kws = [random_scene(rng, 96, name=f"l{i}")[1] for i in range(3)]
+    dt, K = 2e-3, 2
+    sims = [_sim(kw, "persistent", 16) for kw in kws]
+    try:
+        with HeatflowBatch(sims) as batch:
+            batch.set_kernel_timing(True)
+            for _ in range(K):
+                batch.substep(dt)
+            assert batch.kernel_time(_abi.KT_PCG_PERSIST)[1] == 3 * K
+        for i, (kw, s) in enumerate(zip(kws, sims)):
+            ref = _solo_fused_run(kw, dt, K)
+            _assert_same(ref, s, f"scene {i}")
+            ref.close()
+    finally:
+        for s in sims:
+            s.close()
 
 
 def test_persistent_wait_timeout_is_reported(gpu, monkeypatch):
