@@ -277,12 +277,15 @@ __device__ __forceinline__ float face_kappa(float ka, float kb) { return 2.0f * 
 // Deterministic grid reductions
 // ---------------------------------------------------------------------------------------------
 
-// One DPP step on a double: each lane receives the value of the lane CTRL selects (rows outside
-// ROW_MASK receive 0).
+// One DPP step on a double: each lane of the rows in ROW_MASK receives the value of the lane CTRL
+// selects; lanes of the other rows receive an unspecified value.  The trees below read only lanes
+// that every step wrote (the quad and mirror steps write all rows; after the row broadcasts only
+// lane 63 is read, whose chain passes through rows 1 and 3 alone), so no step needs its
+// destination cleared first (a DPP move with a defined old value costs a v_mov per half).
 template <int CTRL, int ROW_MASK = 0xf>
 __device__ __forceinline__ double dpp_double(double x) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, ROW_MASK, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, ROW_MASK, 0xf, false);
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, ROW_MASK, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, ROW_MASK, 0xf, false);
     return __hiloint2double(hi, lo);
 }
 
@@ -300,6 +303,30 @@ __device__ __forceinline__ double wave_sum(double x) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(x), 63);
     const int hi = __builtin_amdgcn_readlane(__double2hiint(x), 63);
     return __hiloint2double(hi, lo);
+}
+
+// wave_sum of NV values at once: the same tree for each value, the values' steps interleaved (no value
+// waits on its own previous step while another's can issue)
+template <int NV>
+__device__ __forceinline__ void wave_sums(double (&x)[NV]) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) x[q] += dpp_double<0xB1>(x[q]);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) x[q] += dpp_double<0x4E>(x[q]);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) x[q] += dpp_double<0x141>(x[q]);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) x[q] += dpp_double<0x140>(x[q]);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) x[q] += dpp_double<0x142, 0xA>(x[q]);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) x[q] += dpp_double<0x143, 0xC>(x[q]);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        const int lo = __builtin_amdgcn_readlane(__double2loint(x[q]), 63);
+        const int hi = __builtin_amdgcn_readlane(__double2hiint(x[q]), 63);
+        x[q] = __hiloint2double(hi, lo);
+    }
 }
 
 // Sum of the per-wave values w[0..NW) (NW <= 16, in LDS) in a fixed tree, returned to every lane:
@@ -324,11 +351,13 @@ __device__ __forceinline__ double sum_wave_values(const double* w) {
 template <int NV, int BS = kBlock>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double (*lds)[BS / 64]) {
     int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double s[NV];
 #pragma unroll
-    for (int q = 0; q < NV; ++q) {
-        double s = wave_sum(v[q]);
-        if (lane == 0) lds[q][wv] = s;
-    }
+    for (int q = 0; q < NV; ++q) s[q] = v[q];
+    wave_sums<NV>(s);
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+        if (lane == 0) lds[q][wv] = s[q];
     __syncthreads();
     if (threadIdx.x < 64) {
 #pragma unroll

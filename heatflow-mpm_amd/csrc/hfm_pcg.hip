@@ -1411,14 +1411,23 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(Src src, int spin_limit
         // ---- scalars of iteration k: the waves holding entries reduce them (reduce_bcast's tree), wave 0
         // adds the waves and forms the scalars once, the workgroup reads them from LDS
         {
-            const int nrows = first ? kFusedRows : 3, count = first ? a.nb_upd : nb;
-            const bool any = (int)(threadIdx.x & ~63u) < count;
+            const int count = first ? a.nb_upd : nb;
+            const bool any = (int)(threadIdx.x & ~63u) < count;   // uniform over the wave
+            if (first) {
+                double w[kFusedRows];
 #pragma unroll
-            for (int r = 0; r < kFusedRows; ++r)
-                if (r < nrows) {
-                    const double w = any ? wave_sum(acc[r]) : 0.0;
-                    if ((threadIdx.x & 63) == 0) wsum[r][threadIdx.x >> 6] = w;
-                }
+                for (int r = 0; r < kFusedRows; ++r) w[r] = acc[r];
+                if (any) wave_sums<kFusedRows>(w);
+#pragma unroll
+                for (int r = 0; r < kFusedRows; ++r)
+                    if ((threadIdx.x & 63) == 0) wsum[r][threadIdx.x >> 6] = any ? w[r] : 0.0;
+            } else {
+                double w[3] = {acc[0], acc[1], acc[2]};
+                if (any) wave_sums<3>(w);
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+                    if ((threadIdx.x & 63) == 0) wsum[r][threadIdx.x >> 6] = any ? w[r] : 0.0;
+            }
         }
         if (__syncthreads_or(late)) {
             if (threadIdx.x == 0) {
