@@ -1,15 +1,18 @@
 """The batched, mixed-precision and served configurations at their own scale, on the GPU.
 
 * C4, batched: the 64 uneven scenes (n in {32, 48, 64, 96, 128}, 100-800 W, fan 0-3 m/s, 2k-50k
-  particles) with the autotuned batch layouts, stepped in one batch; one scene of every size is
-  stepped alone with the same layout and must give the same bits.
-* C3, mixed precision: the coupled scene at n = 192 with about 50k particles, 20 substeps in the
-  mixed form (FP16 PCG vectors inside iterative refinement, BF16 departure velocity) and in fp32:
-  finite, projected (divergence_max small), mixed within 1e-3 of fp32.  The same scene at n = 48
-  against the fp64 oracle: the error of T, u and div(u) of both forms, per result.
-* C5, served: a seeded Poisson stream over n in {48, 64, 96, 128, 192} (K 50-120 substeps) in the
-  mixed form with continuous batching; every result equals the request run alone, bit for bit, and
-  the n = 48 results are held to 1e-3 against the fp64 oracle.
+  particles) with the autotuned batch layouts, stepped in one batch for the configuration's 200
+  substeps; one scene of every size is stepped alone with the same layout and must give the same bits.
+* C3, mixed precision: the coupled scene at n = 192 with about 50k particles, the configuration's 100
+  substeps in the mixed form (FP16 PCG vectors inside iterative refinement, BF16 departure velocity)
+  and in fp32: finite, projected (divergence_max small), mixed within 1e-3 of fp32.  The same scene at
+  n = 48 against the fp64 oracle after 100 substeps: the error of T, u and div(u) of both forms, per
+  result.
+* C5, served: seeded Poisson streams over n in {48, 64, 96, 128, 192} in the mixed form with
+  continuous batching, one with K 50-120 substeps (Jacobi-preconditioned refinement) and one over the
+  configuration's K range 50-1000 (multigrid-preconditioned refinement, the served form); every result
+  equals the request run alone, bit for bit, and the n = 48 results are held to 1e-3 against the fp64
+  oracle.
 """
 import time
 
@@ -37,7 +40,7 @@ def _u_orc(orc):
 
 def test_c4_batched_64_scenes_with_autotuned_layouts(gpu):
     scenes, dt, _ = batched_scenes(64)
-    steps = 3
+    steps = 200
     t0 = time.time()
     sims = [HeatflowMPM(**kw, layout=lookup(kw["grid"].n, "batch")) for _, kw in scenes]
     cells = sum(s.grid.n ** 3 for s in sims)
@@ -71,7 +74,7 @@ def test_c4_batched_64_scenes_with_autotuned_layouts(gpu):
 def test_c3_mixed_precision_n192_50k_particles(gpu):
     kw, dt, _ = coupled_scene(192, particles_target=50000)
     kw["grid"].cg_max_iter = 2000
-    steps = 20
+    steps = 100
     mixed = HeatflowMPM(**kw, precision="mixed")
     plain = HeatflowMPM(**kw)
     print("particles", mixed.n, flush=True)
@@ -98,7 +101,7 @@ def test_c3_mixed_precision_n192_50k_particles(gpu):
 
 def test_c3_same_scene_at_n48_against_fp64(gpu):
     kw, dt, _ = coupled_scene(48, particles_target=50000)
-    steps = 20
+    steps = 100
     f64 = O.OracleSim(**kw, precision="f64")
     sims = {"mixed": HeatflowMPM(**kw, precision="mixed"), "fp32": HeatflowMPM(**kw)}
     f64.run(dt, steps)
@@ -128,6 +131,42 @@ def test_c5_served_mixed_stream_equals_alone(gpu):
     assert len(res) == len(reqs)
     for r, q in zip(res, reqs):
         alone = HeatflowMPM(**q.kwargs, precision="mixed")
+        alone.run(q.dt, q.substeps)
+        assert np.array_equal(alone.grid_T(), r.fields["T"]), r.rid
+        for f in ("U", "V", "W"):
+            assert np.array_equal(alone._scene.get_field(f), r.fields[f]), (r.rid, f)
+        assert np.array_equal(alone.temperatures(), r.fields["temperatures"]), r.rid
+        msg = f"request {r.rid}: n={r.n} K={r.substeps} alone == served"
+        if r.n == 48:
+            orc = O.OracleSim(**q.kwargs, precision="f64")
+            orc.run(q.dt, q.substeps)
+            eT, eu = rel_l2(r.fields["T"], orc.field("T")), rel_l2(_u(r.fields), _u_orc(orc))
+            dm = alone.grid.divergence_max()
+            msg += f"; vs fp64 oracle T {eT:.2e}, u {eu:.2e}, divergence_max {dm:.2e}"
+            assert eT <= 1e-3 and eu <= 1e-3 and dm < 1e-3
+        print(msg, flush=True)
+        alone.close()
+
+
+def test_c5_served_stream_at_configuration_scale(gpu):
+    """The configuration's stream: K drawn from 50-1000 substeps over all five sizes (seed 3: 8.1e9
+    cell-substeps), served in the mixed form with the multigrid-preconditioned refinement and continuous
+    batching; each result equals the request run alone, and both n = 48 requests (275 and 158 substeps)
+    are within 1e-3 of the fp64 oracle."""
+    reqs = poisson_requests(50.0, 8, seed=3, k_range=(50, 1000), cg_max_iter=2000)
+    assert {q.kwargs["grid"].n for q in reqs} == {48, 64, 96, 128, 192}
+    srv = Server(GpuBackend(precision="mixed", cg="mg"), keep_results=True, max_active=16)
+    t0 = time.perf_counter()
+    res = srv.run(reqs)
+    span = time.perf_counter() - t0
+    s = summarize(res, span)
+    print(f"served (mixed, multigrid, continuous batching), K {min(q.substeps for q in reqs)}-"
+          f"{max(q.substeps for q in reqs)}: {s['cell_substeps']:.3g} cell-substeps, "
+          f"{s['cell_substeps_per_s']:.3g} cell-substeps/s, p50 {s['p50_s']:.2f} s, p99 {s['p99_s']:.2f} s",
+          flush=True)
+    assert len(res) == len(reqs)
+    for r, q in zip(res, reqs):
+        alone = HeatflowMPM(**q.kwargs, precision="mixed", cg="mg")
         alone.run(q.dt, q.substeps)
         assert np.array_equal(alone.grid_T(), r.fields["T"]), r.rid
         for f in ("U", "V", "W"):
