@@ -303,6 +303,12 @@ struct Batch {
     size_t pmap_cap = 0;
     MapEntry* pstage = nullptr;
     size_t pstage_cap = 0;
+    unsigned long long maps_epoch = 0;       // bumped whenever batch_prepare rebuilds the table and maps
+    // the multigrid V-cycle captured as a HIP graph, [0] fp32, [1] the mixed form's inner solve: valid for
+    // the tail depth and maps epoch it was captured with
+    hipGraphExec_t mg_exec[2] = {};
+    int mg_exec_tail[2] = {};
+    unsigned long long mg_exec_epoch[2] = {};
     unsigned long long* pace = nullptr;      // pinned: (solve tag << 32) | iterations completed
     unsigned long long* pace_dev = nullptr;
     unsigned solve_tag = 0;
@@ -408,7 +414,10 @@ void launch_bf16_copy(const Scene* s, hipStream_t st);
 constexpr int kMgMaxLevels = 8;
 constexpr int kMgTailMaxN = 12;   // levels of at most 12^3 cells are solved in one workgroup
 void launch_mg_coefficients(const Batch* b, int levels);
-void launch_mg_vcycle(const Batch* b, int tail, bool ir = false);
+// the V-cycle of every multigrid scene of the batch (z = M r and its r.z partial sums): its launches one
+// by one, or with HFM_MG_GRAPH=1 one replay of the batch's captured graph (captured on first use and
+// again after the batch's maps were rebuilt)
+int launch_mg_vcycle(Batch* b, int tail, bool ir = false);
 // fused pressure form: first-iteration scalars after launch_pcg_init, then one launch per iteration
 constexpr int kFusedRows = 5;
 void launch_pcg_init_fused(const Launch& L);

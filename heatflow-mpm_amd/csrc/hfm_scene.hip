@@ -268,6 +268,10 @@ void batch_release(Batch* b) {
     b->pstage_cap = 0;
     if (b->pace) (void)hipHostFree(b->pace);
     b->pace = nullptr;
+    for (int i = 0; i < 2; ++i) {
+        if (b->mg_exec[i]) (void)hipGraphExecDestroy(b->mg_exec[i]);
+        b->mg_exec[i] = nullptr;
+    }
     if (b->ev_pool) {
         for (int i = 0; i < b->ev_cap; ++i) (void)hipEventDestroy(b->ev_pool[i]);
         delete[] b->ev_pool;
@@ -370,6 +374,7 @@ int batch_prepare(Batch* b) {
     }
     b->versions.resize(ns);
     for (size_t i = 0; i < ns; ++i) b->versions[i] = b->scenes[i]->version;
+    b->maps_epoch += 1;   // graphs captured over the old table and maps are stale
     return HFM_OK;
 }
 
@@ -750,7 +755,7 @@ int pcg_iterate(Batch* b, int op, float dt, bool ir, int maxeff, int pred) {
                     hipEvent_t e0 = nullptr, e1 = nullptr;
                     if (timed) RC(timing_events(b, HFM_KT_PCG_UPD, &e0, &e1));
                     launch_pcg_upd(fl[FAM_PCG], op, k, e0, e1, ir);
-                    if (mg) launch_mg_vcycle(b, mg, ir);
+                    if (mg) RC(launch_mg_vcycle(b, mg, ir));
                 }
             }
             CHECK_LAUNCH();
@@ -805,7 +810,7 @@ int run_pcg(Batch* b, int op, float dt) {
     if (any_fused) launch_pcg_init_fused(family(b, FAM_PCG));
     if (op == 1 && mg_depth(b, false)) {   // multigrid: the coarse levels (when the solid fraction changed), z = M r
         launch_mg_coefficients_if_stale(b);
-        launch_mg_vcycle(b, mg_depth(b, true));
+        RC(launch_mg_vcycle(b, mg_depth(b, true)));
     }
     CHECK_LAUNCH();
     RC(pcg_iterate(b, op, dt, false, maxeff, pred));
@@ -873,7 +878,7 @@ int run_ir(Batch* b, int op, float dt) {
     if (mg) launch_mg_coefficients_if_stale(b);
     launch_ir_residual(fam, op, dt, true);
     launch_ir_scale(fam, op, 0);
-    if (mg) launch_mg_vcycle(b, mg, true);
+    if (mg) RC(launch_mg_vcycle(b, mg, true));
     CHECK_LAUNCH();
     for (int step = 0;; ++step) {
         RC(pcg_iterate(b, op, dt, true, maxeff, pred));
@@ -894,7 +899,7 @@ int run_ir(Batch* b, int op, float dt) {
         }
         launch_ir_residual(fam, op, dt, false);
         launch_ir_scale(fam, op, step + 1);
-        if (mg) launch_mg_vcycle(b, mg, true);
+        if (mg) RC(launch_mg_vcycle(b, mg, true));
         CHECK_LAUNCH();
     }
     for (size_t si = 0; si < b->scenes.size(); ++si) {
