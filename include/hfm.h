@@ -106,10 +106,11 @@ typedef struct hfm_params {
  * workgroup per tile, all resident for the whole solve (no counterpart in the reference).  Each
  * workgroup keeps its tile's vectors on chip and exchanges only the tile's faces and its partial
  * sums with the others through memory; the results, scalars and iteration counts are those of
- * HFM_FLAG_FUSED_CG bit for bit.  Used when the scene is stepped on its own (alone or as a batch of
- * one) and the device keeps all of its tiles resident at once (n <= 96 on MI355X: 216 tiles of
- * 16 x 16 x 16 cells, one per CU); otherwise the scene runs HFM_FLAG_FUSED_CG as if this flag were
- * absent.  Persistent solves of different scenes on one device run one at a time.  A workgroup that
+ * HFM_FLAG_FUSED_CG bit for bit.  Used when the device keeps all of the scene's tiles resident at
+ * once (n <= 96 on MI355X: 216 tiles of 16 x 16 x 16 cells, one per CU); in a batch such scenes are
+ * packed into cooperative launches of at most the device's resident workgroups, each scene's
+ * workgroups waiting only on each other.  A scene whose tiles do not fit runs HFM_FLAG_FUSED_CG as if
+ * this flag were absent.  Persistent launches on one device run one at a time.  A workgroup that
  * waits longer than a fixed number of polls for the others ends the solve with HFM_ERR_HIP. */
 #define HFM_FLAG_PERSISTENT_CG 16
 
