@@ -2,7 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cg warp|fused] [--precision fp32|mixed]
     python bench.py --workload batched [--scenes 64] [--steps 200]
-    python bench.py --workload served [--rate 20] [--requests 24] [--precision mixed]
+    python bench.py --workload served [--rate 20] [--requests 24] [--precision mixed|fp32] [--fp64-n 64]
     python bench.py --workload mixed [--n 192] [--steps 100] [--fp64-n 48]
 
 Workload: the default coupled scene at n=96 (0.48 m box, h = 5 mm, a static ellipsoid of ~26.8k
@@ -209,8 +209,9 @@ def bench_served(a) -> dict:
     [kmin, kmax], heaters / fans / particle blobs as the batched scenes), served twice on the same
     arrival times: with continuous batching (the admitted scenes of a size class advance in one batched
     substep, the size classes (--lanes) side by side on their own streams; admission in arrival order by
-    device footprint), the multigrid-preconditioned pressure PCG and fp32 state by default, and in the
-    plain form (fp32, the reference's Jacobi PCG, one request at a time, in arrival order).
+    device footprint), the multigrid-preconditioned pressure PCG and the mixed-precision form (FP16
+    refinement vectors, BF16 departure velocity, fp32 state) by default, and in the plain form (fp32,
+    the reference's Jacobi PCG, one request at a time, in arrival order).
     Latency = completion - arrival.  Each served result is compared with
     the plain result of the same request (relative L2 of T and u): both solve to the same relative
     residual, so they agree to the solve tolerance, not in bits."""
@@ -224,7 +225,7 @@ def bench_served(a) -> dict:
         w.close()
     from heatflow_mpm.serve import GpuBackend
     budget = int(a.hbm_gib * (1 << 30))
-    prec = a.precision or ("mixed" if scg == "warp" else "fp32")
+    prec = a.precision or "mixed"   # the served configuration names the mixed-precision form
     served = Server(GpuBackend(cg=scg, precision=prec, layout=None if a.layout == "rule" else "auto-batch"),
                     hbm_budget=budget, max_active=a.slots, keep_results=True, lanes=_lanes(a))
     t0 = time.perf_counter()
@@ -298,7 +299,7 @@ def bench_served_sweep(a) -> dict:
     scale with 1/rate, the scene draws do not change), served and plain at each rate."""
     from heatflow_mpm.serve import GpuBackend
     scg = a.cg or "mg"
-    prec = a.precision or ("mixed" if scg == "warp" else "fp32")
+    prec = a.precision or "mixed"   # the served configuration names the mixed-precision form
     budget = int(a.hbm_gib * (1 << 30))
     warm = [HeatflowMPM(**coupled_scene(32)[0], cg=scg) for _ in range(2)]
     with HeatflowBatch(warm) as wb:
