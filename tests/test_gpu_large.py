@@ -1,0 +1,79 @@
+"""The largest grid of the tests: the coupled scene at n = 512 (134 million cells, h = 0.94 mm, about
+24 GB of device state), where the kernels' 32-bit lattice and face indices run up to 1.3e8.  No oracle run
+fits this size, so the checks are the size-independent ones:
+
+* binning and lattice indices of every particle bit for bit against numpy's float32 op sequence
+  (fields.py:5-10);
+* the heater energy balance (heater.py:52, heat.py:154-158): the device's fp64 sum of the source step's
+  rises times rho c_p h^3 equals P dt while no source cell is capped;
+* finite fields, and with the multigrid-preconditioned pressure PCG (levels 512 down to 8) converged
+  solves that leave divergence_max below 1e-3.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from heatflow_mpm import HeatflowMPM, coupled_scene
+from heatflow_mpm.solver import KAPPA_AIR, scene_inputs
+
+pytestmark = pytest.mark.gpu
+
+N_LARGE = 512
+OFFS = np.array([[0.5, 0.5, 0.5], [0.0, 0.5, 0.5], [0.5, 0.0, 0.5], [0.5, 0.5, 0.0]], np.float32)
+
+
+def test_n512_bins_energy_and_projection(gpu):
+    kw, dt, _ = coupled_scene(N_LARGE)
+    t0 = time.time()
+    inputs = scene_inputs(len(kw["particles"]), kw["mat"], kw["heater"], kw["grid"], kw["spacing"], kw["t_init"],
+                          KAPPA_AIR)
+    print(f"n={N_LARGE}: {N_LARGE ** 3 / 1e6:.0f} M cells, {len(kw['particles'])} particles, host setup "
+          f"{time.time() - t0:.0f} s", flush=True)
+    pos = np.asarray(kw["particles"], np.float32)
+    h32 = np.float32(kw["grid"].h)
+
+    sim = HeatflowMPM(**kw, inputs=inputs)
+    try:
+        print("device footprint", sim._scene.device_bytes() / 2 ** 30, "GiB", flush=True)
+        b, base, frac = sim._scene.particle_bins()
+        q = pos / h32
+        for lat in range(4):
+            g = q - OFFS[lat]
+            fl = np.floor(g)
+            assert np.array_equal(base[:, lat, :], fl.astype(np.int32)), lat
+            assert np.array_equal(frac[:, lat, :].view(np.uint32), (g - fl).astype(np.float32).view(np.uint32)), lat
+        n = N_LARGE
+        cell = np.clip(np.floor(q).astype(np.int64), 0, n - 1)
+        assert np.array_equal(b, ((cell[:, 0] * n + cell[:, 1]) * n + cell[:, 2]).astype(np.int32))
+        t0 = time.time()
+        sim.substep(dt)
+        sim.synchronize()
+        st = sim.stats()
+        e = st["source_dT_sum"] * 1.2 * 1005.0 * float(kw["grid"].h) ** 3
+        print(f"Jacobi substep {time.time() - t0:.2f} s: pressure its {st['p_iters_exact']} (converged "
+              f"{st['p_converged']}), heat its {st['h_iters_exact']}; heater energy {e:.6f} J against P dt "
+              f"{kw['heater'].power * dt:.6f} J", flush=True)
+        assert e == pytest.approx(kw["heater"].power * dt, rel=1e-5)
+        assert np.isfinite(sim.grid_T()).all() and np.isfinite(sim.temperatures()).all()
+        for f in ("U", "V", "W", "P"):
+            assert np.isfinite(sim._scene.get_field(f)).all(), f
+    finally:
+        sim.close()
+
+    sim = HeatflowMPM(**kw, inputs=inputs, cg="mg")
+    try:
+        for step in range(2):
+            t0 = time.time()
+            sim.substep(dt)
+            sim.synchronize()
+            st = sim.stats()
+            dm = sim.grid.divergence_max()
+            print(f"multigrid substep {step + 1} {time.time() - t0:.2f} s: pressure its {st['p_iters_exact']} "
+                  f"(converged {st['p_converged']}), divergence_max {dm:.3e}, speed_max {sim.grid.speed_max():.3e}",
+                  flush=True)
+            assert st["p_converged"] == 1
+            assert dm < 1e-3
+        assert np.isfinite(sim.grid_T()).all()
+    finally:
+        sim.close()
