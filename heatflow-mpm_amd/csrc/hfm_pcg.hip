@@ -92,6 +92,7 @@ struct PcgViewT {
     int tiles_j, tiles_k;
     int k;                 // completed iterations before this launch
     int lb;                // workgroup index inside the scene
+    int floor_stop;        // the pressure solve of a multigrid scene: stop at the fp32 floor
 };
 using PcgView = PcgViewT<float>;
 
@@ -107,6 +108,13 @@ struct VecOf<true> {
 constexpr float kIrEta2 = 1.0e-4f;  // inner solve: reduce ||r||^2 by eta^2, eta = 1e-2
 constexpr int kIrInnerMax = 256;    // iterations of one inner solve at most (then a refinement step)
 constexpr int kIrPart = kMaxDirBlocks + 3 * kMaxPcgBlocks;  // residual partials of a refinement step
+// The floor stop of a multigrid-preconditioned fp32 solve (no counterpart in warp, whose Jacobi PCG
+// stalls at the floor of its fp32 residual): with the V-cycle as preconditioner the iteration diverges
+// once it runs below that floor (the recursively updated residual grows without bound), so the solve
+// stops when ||r||^2 has set no new minimum for kFloorStall iterations or has grown kFloorGrowth times
+// above its minimum.  A converging solve lowers ||r||^2 at (nearly) every iteration and never meets it.
+constexpr int kFloorStall = 8;
+constexpr float kFloorGrowth = 100.0f;
 
 template <int OP, bool IR = false>
 __device__ __forceinline__ PcgViewT<typename VecOf<IR>::type> make_view(const SceneG& a, int lb, int k, float dt) {
@@ -155,6 +163,7 @@ __device__ __forceinline__ PcgViewT<typename VecOf<IR>::type> make_view(const Sc
     v.tiles_k = a.tiles_k;
     v.k = k;
     v.lb = lb;
+    v.floor_stop = OP == 1 && !IR && a.cg_mg;
     return v;
 }
 
@@ -347,6 +356,8 @@ struct PassAIn {
     int done, brk, maxeff, total;
     float rz_prev, atol_sq, gl;
     double atol, tol;
+    float rr_min;   // floor stop: smallest ||r||^2 before iteration k, and its iteration
+    int k_min;
 };
 
 // IR: the inner solve of a refinement step, whose first launch takes r.r and r.z from the scale
@@ -365,6 +376,8 @@ __device__ __forceinline__ void pass_a_load(const View& a, PassAIn& p) {
     p.tol = c->tol;
     p.total = c->ir_total;
     p.gl = c->ir_gl;
+    p.rr_min = c->slot_rrmin[(a.k + 1) & 1];
+    p.k_min = c->slot_kmin[(a.k + 1) & 1];
 }
 
 // Prologue of every pass A: finish iteration k from pass B's partial sums (r.r, r.z; b.b too at
@@ -410,17 +423,32 @@ __device__ bool finish_iteration(const View& a, const PassAIn& p, float& beta) {
         brk = !(p.rz_prev > 0.0f) || !(fabsf(beta) <= 3.4028235e38f);
     }
     if (!IR) brk = brk || p.brk || !(rr <= 3.4028235e38f);
-    const bool stop = conv || k >= p.maxeff || brk;
+    // the floor stop of a multigrid scene (see kFloorStall); iteration 0 sets the first minimum
+    float rr_min = rr;
+    int k_min = k;
+    bool floor = false;
+    if (a.floor_stop && k > 0) {
+        const bool lower = rr < p.rr_min;
+        rr_min = lower ? rr : p.rr_min;
+        k_min = lower ? k : p.k_min;
+        floor = !conv && (k - k_min >= kFloorStall || rr > kFloorGrowth * p.rr_min);
+    }
+    const bool stop = conv || k >= p.maxeff || brk || floor;
     if (a.lb == 0 && threadIdx.x == 0) {
         c->slot_rr[k & 1] = rr;
         c->slot_rz[k & 1] = rz;
+        if (a.floor_stop) {
+            c->slot_rrmin[k & 1] = rr_min;
+            c->slot_kmin[k & 1] = k_min;
+        }
         if (k == 0) {
             if (!IR) c->atol = atol;
             c->atol_sq = atol_sq;
         }
         if (stop) {
+            const int cv = conv ? 1 : (brk ? -1 : (floor ? 2 : 0));
             c->done = 1;
-            c->converged = conv ? 1 : (brk ? -1 : 0);
+            c->converged = cv;
             c->it = k;
             if (IR) {
                 c->ir_total = p.total + k;
@@ -428,7 +456,7 @@ __device__ bool finish_iteration(const View& a, const PassAIn& p, float& beta) {
                 wait_stores();
                 __hip_atomic_store(&a.mirror->done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             } else {
-                mirror_final(a.mirror, k, rr, atol, conv ? 1 : (brk ? -1 : 0));
+                mirror_final(a.mirror, k, rr, atol, cv);
             }
         }
     }

@@ -46,6 +46,10 @@ struct CgCtl {
     int ir_maxiter;    // iteration cap (inner iterations over the whole solve)
     double ir_rr2[2];  // ||r||^2 (true residual) of refinement step s, in slot s & 1
     double src_dt;     // ctl[0]: sum of the source step's temperature rises of this substep (fp64)
+    // multigrid-preconditioned fp32 solves: smallest ||r||^2 so far and its iteration, after iteration k in
+    // slot k & 1 (the floor stop)
+    float slot_rrmin[2];
+    int slot_kmin[2];
 };
 
 // Pinned host memory written by the device with system-scope stores (results of a solve),

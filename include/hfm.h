@@ -121,9 +121,11 @@ typedef struct hfm_stats {
     int32_t p_iters_exact;    /* iterations that changed the pressure */
     int32_t h_iters;          /* heat PCG count as warp returns it: HeatField.cg_iters */
     int32_t h_iters_exact;
-    int32_t p_converged;      /* 1 when ||r||^2 <= atol^2 at exit, 0 at the iteration cap; HFM_PREC_MIXED also
-                                 2: stopped when a refinement step no longer cut the true residual (its
-                                 fp32 floor).  -1: stopped on a breakdown (no counterpart in warp, which
+    int32_t p_converged;      /* 1 when ||r||^2 <= atol^2 at exit, 0 at the iteration cap; 2: stopped at the
+                                 fp32 floor of the residual, above atol (HFM_PREC_MIXED: a refinement step no
+                                 longer cut the true residual; HFM_FLAG_MG_PCG in fp32: ||r||^2 set no new
+                                 minimum for 8 iterations or grew 100 times above its minimum, where the
+                                 multigrid-preconditioned iteration would diverge).  -1: stopped on a breakdown (no counterpart in warp, which
                                  runs on to its cap): p.Ap or r.z not positive, alpha or beta not finite,
                                  or a non-finite ||r||^2.  x keeps its last iterate; if ||r||^2 is not
                                  finite the substep returns HFM_ERR_NONFINITE */

@@ -1,14 +1,18 @@
-"""The largest grid of the tests: the coupled scene at n = 512 (134 million cells, h = 0.94 mm, about
-24 GB of device state), where the kernels' 32-bit lattice and face indices run up to 1.3e8.  No oracle run
-fits this size, so the checks are the size-independent ones:
+"""The largest grids: the coupled scene at n = 512 (134 million cells, h = 0.94 mm, about 23 GiB of device
+state), and with HFM_TEST_N1024=1 at n = 1024, the largest size the library accepts (1.07e9 cells, about
+190 GiB of device state and ~100 GB of host memory while the scene is set up, a few minutes), where the
+kernels' 32-bit face indices reach 1.07e9 and the cell arrays 2^30 elements.  No oracle run fits these
+sizes, so the checks are the size-independent ones:
 
 * binning and lattice indices of every particle bit for bit against numpy's float32 op sequence
   (fields.py:5-10);
 * the heater energy balance (heater.py:52, heat.py:154-158): the device's fp64 sum of the source step's
   rises times rho c_p h^3 equals P dt while no source cell is capped;
-* finite fields, and with the multigrid-preconditioned pressure PCG (levels 512 down to 8) converged
-  solves that leave divergence_max below 1e-3.
+* finite fields, and with the multigrid-preconditioned pressure PCG (levels n down to 8) solves that end
+  converged, or at the fp32 floor of the residual (converged = 2, which a 1024^3 solve with a small
+  right-hand side reaches above the relative tolerance 1e-6), and leave divergence_max below 1e-3.
 """
+import os
 import time
 
 import numpy as np
@@ -19,11 +23,17 @@ from heatflow_mpm.solver import KAPPA_AIR, scene_inputs
 
 pytestmark = pytest.mark.gpu
 
-N_LARGE = 512
 OFFS = np.array([[0.5, 0.5, 0.5], [0.0, 0.5, 0.5], [0.5, 0.0, 0.5], [0.5, 0.5, 0.0]], np.float32)
 
 
-def test_n512_bins_energy_and_projection(gpu):
+@pytest.mark.parametrize("n_large", [
+    512,
+    pytest.param(1024, marks=pytest.mark.skipif(os.environ.get("HFM_TEST_N1024") != "1",
+                                                 reason="n = 1024 needs ~190 GiB of device memory and minutes; "
+                                                        "set HFM_TEST_N1024=1")),
+])
+def test_large_grid_bins_energy_and_projection(gpu, n_large):
+    N_LARGE = n_large
     kw, dt, _ = coupled_scene(N_LARGE)
     t0 = time.time()
     inputs = scene_inputs(len(kw["particles"]), kw["mat"], kw["heater"], kw["grid"], kw["spacing"], kw["t_init"],
@@ -72,7 +82,7 @@ def test_n512_bins_energy_and_projection(gpu):
             print(f"multigrid substep {step + 1} {time.time() - t0:.2f} s: pressure its {st['p_iters_exact']} "
                   f"(converged {st['p_converged']}), divergence_max {dm:.3e}, speed_max {sim.grid.speed_max():.3e}",
                   flush=True)
-            assert st["p_converged"] == 1
+            assert st["p_converged"] in (1, 2)
             assert dm < 1e-3
         assert np.isfinite(sim.grid_T()).all()
     finally:
