@@ -345,6 +345,29 @@ __device__ __forceinline__ double sum_wave_values(const double* w) {
     return __hiloint2double(hi, lo);
 }
 
+// sum_wave_values of rows w[0..NV) at once: the same tree for each row, the rows' steps interleaved
+template <int NW, int NV, int STRIDE>
+__device__ __forceinline__ void sum_wave_values_n(const double (*w)[STRIDE], double (&out)[NV]) {
+    static_assert(NW <= 16 && NW <= STRIDE, "at most 16 waves per workgroup");
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) out[q] = lane < NW ? w[q][lane] : 0.0;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) out[q] += dpp_double<0xB1>(out[q]);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) out[q] += dpp_double<0x4E>(out[q]);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) out[q] += dpp_double<0x141>(out[q]);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) out[q] += dpp_double<0x140>(out[q]);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        const int lo = __builtin_amdgcn_readlane(__double2loint(out[q]), 0);
+        const int hi = __builtin_amdgcn_readlane(__double2hiint(out[q]), 0);
+        out[q] = __hiloint2double(hi, lo);
+    }
+}
+
 // Sum of one double per thread over a workgroup of BS threads; the result is valid in thread 0.
 // The tree is fixed (wave tree, then waves in order), so the value does not depend on timing.
 // lds must hold NV x (BS / 64) doubles and is not reused afterwards (the caller ends the kernel).
@@ -359,10 +382,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double (*lds)[BS / 64
     for (int q = 0; q < NV; ++q)
         if (lane == 0) lds[q][wv] = s[q];
     __syncthreads();
-    if (threadIdx.x < 64) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) v[q] = sum_wave_values<BS / 64>(lds[q]);
-    }
+    if (threadIdx.x < 64) sum_wave_values_n<BS / 64, NV>(lds, v);
 }
 
 // Element i of an array through a 32-bit byte offset (i * sizeof(T) < 2^32: arrays of at most

@@ -1466,9 +1466,14 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(Src src, int spin_limit
         }
         if (threadIdx.x < 64) {
             double tot[kFusedRows];
+            if (first) {
+                sum_wave_values_n<kBlock3 / 64, kFusedRows>(wsum, tot);
+            } else {
+                double t3[3];
+                sum_wave_values_n<kBlock3 / 64, 3>(wsum, t3);
 #pragma unroll
-            for (int r = 0; r < kFusedRows; ++r)
-                tot[r] = r < (first ? kFusedRows : 3) ? sum_wave_values<kBlock3 / 64>(wsum[r]) : 0.0;
+                for (int r = 0; r < kFusedRows; ++r) tot[r] = r < 3 ? t3[r] : 0.0;
+            }
             const CgScalars s0 = cg_scalars(tot, first, g_prev, pap_prev, tol, atol, atol_sq, kit, maxeff);
             if (threadIdx.x == 0) {
                 qs = s0;
@@ -1501,20 +1506,17 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(Src src, int spin_limit
                 (&U[0][0][0])[Htab[h].y] = cg_frame(hr[qh], hw[qh], hs[qh], hd[qh], first, q.alpha, q.beta, s, r1);
             }
         }
-        // ---- own cells: s_k, r_{k+1}, u_{k+1}, p_k, x_{k+1}
+        // ---- own cells: s_k, r_{k+1}, u_{k+1} (p_k and x_{k+1}, which no later step of the iteration
+        // reads, once the partial sums are posted)
         double sum_g = 0.0, sum_rr = 0.0, sum_d = 0.0;   // r.u, r.r, w.u of step k + 1
+        float rk_own[kO];
 #pragma unroll
         for (int m = 0; m < kO; ++m) {
             float s, r1;
             const float u1 = cg_frame(orr[m], ow[m], os[m], od[m], first, q.alpha, q.beta, s, r1);
             U[lq + 4 * m + 1][lj + 1][lk + 1] = u1;
+            rk_own[m] = orr[m];
             if (in_box(m)) {
-                const float uk = od[m] * orr[m];
-                float& pl = Pl[lq + 4 * m][lj][lk];
-                float& xl = Xl[lq + 4 * m][lj][lk];
-                const float pk = first ? uk : uk + q.beta * pl;
-                pl = pk;
-                xl = xl + q.alpha * pk;
                 orr[m] = r1;
                 os[m] = s;
                 if (pub(m)) {
@@ -1544,6 +1546,18 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(Src src, int spin_limit
         if (threadIdx.x == 0)
 #pragma unroll
             for (int r = 0; r < 3; ++r) store_agent(slots + (kit % 3) * 3 * nb + r * nb + lb, out[r]);
+        // ---- p_k and x_{k+1} of the own cells, while the other workgroups' sums are on their way
+#pragma unroll
+        for (int m = 0; m < kO; ++m) {
+            if (in_box(m)) {
+                const float uk = od[m] * rk_own[m];
+                float& pl = Pl[lq + 4 * m][lj][lk];
+                float& xl = Xl[lq + 4 * m][lj][lk];
+                const float pk = first ? uk : uk + q.beta * pl;
+                pl = pk;
+                xl = xl + q.alpha * pk;
+            }
+        }
         g_prev = q.g;
         pap_prev = q.pap;
     }
