@@ -1040,6 +1040,41 @@ __device__ __forceinline__ float cg_stencil_ff(const float (*U)[kF3][kF3], const
     return div_const(1.0f * acc6, hh, rhh, fastdiv);
 }
 
+// The persistent form's frame of u in LDS, laid out so that the stencil's reads of a wave (4 rows of 16
+// k-consecutive cells) hit 64 distinct banks: rows of the 16 inner k positions, [D + 2][18][16], then the
+// two k-halo positions of each row, [D + 2][18][2] (the same 18 x 18 x (D + 2) words as the plain frame).
+template <int D>
+__device__ __forceinline__ unsigned urow_offset(int fi, int fj, int fk) {
+    return fk >= 1 && fk <= kT3 ? (unsigned)((fi * kF3 + fj) * kT3 + fk - 1)
+                                : (unsigned)((D + 2) * kF3 * kT3 + (fi * kF3 + fj) * 2 + (fk == 0 ? 0 : 1));
+}
+
+// cg_stencil_ff on the row layout: u at k - 1 and k + 1 comes from the neighbouring lanes of the 16-lane
+// row (DPP row_shr:1 / row_shl:1), the row's first and last lanes take the k-halo pair; the same six
+// terms in the same order, so the same bits.  Every lane of the wave must run it (the moves read the
+// neighbours' registers).
+template <int D>
+__device__ __forceinline__ float cg_stencil_rows(const float (*UI)[kF3][kT3], const float2 (*UK)[kF3],
+                                                 const float (*FX)[kT3][kT3], const float (*FY)[kT3 + 1][kT3],
+                                                 const float (*FZ)[kT3][kT3 + 1], int li, int lj, int lk, float hh,
+                                                 float rhh, int fastdiv, float& uc) {
+    const int si = li + 1, sj = lj + 1;
+    uc = UI[si][sj][lk];
+    const float2 kh = UK[si][sj];
+    const float ukm = __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(kh.x), __float_as_int(uc), 0x111, 0xf, 0xf, false));  // row_shr:1
+    const float ukp = __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(kh.y), __float_as_int(uc), 0x101, 0xf, 0xf, false));  // row_shl:1
+    float acc6 = 0.0f;
+    acc6 += FX[li][lj][lk] * (uc - UI[si - 1][sj][lk]);
+    acc6 += FX[li + 1][lj][lk] * (uc - UI[si + 1][sj][lk]);
+    acc6 += FY[li][lj][lk] * (uc - UI[si][sj - 1][lk]);
+    acc6 += FY[li][lj + 1][lk] * (uc - UI[si][sj + 1][lk]);
+    acc6 += FZ[li][lj][lk] * (uc - ukm);
+    acc6 += FZ[li][lj][lk + 1] * (uc - ukp);
+    return div_const(1.0f * acc6, hh, rhh, fastdiv);
+}
+
 template <int D, class Src>
 __global__ __launch_bounds__(kBlock3, D <= 8 ? 8 : 4) void k_pcg_fused(Src src, int kit, unsigned long long* pace,
                                                                        unsigned tag) {
@@ -1309,7 +1344,9 @@ template <int D, class Src>
 __global__ __launch_bounds__(kBlock3) void k_pcg_persist(Src src, int spin_limit) {
     HFM_ENTRY(src);
     constexpr int kH = kHalo<D>, kHM = (kH + kBlock3 - 1) / kBlock3, kO = D / 4;
-    __shared__ float U[D + 2][kF3][kF3];   // u_{k+1} on the frame (alpha_s while the face fractions are formed)
+    // u_{k+1} on the frame in the row layout (urow_offset; alpha_s in the plain layout while the face
+    // fractions are formed)
+    __shared__ __attribute__((aligned(16))) float U[D + 2][kF3][kF3];
     __shared__ float FX[D + 1][kT3][kT3];  // face fractions of the block's cells: x-face below plane li
     __shared__ float FY[D][kT3 + 1][kT3];  // y-face below row lj
     __shared__ float FZ[D][kT3][kT3 + 1];  // z-face below column lk
@@ -1362,7 +1399,7 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(Src src, int spin_limit
         int fi, fj, fk;
         halo_cell<D>(h < kH ? h : kH - 1, fi, fj, fk);
         const unsigned t = frame_index(i0, j0, k0, fi, fj, fk, n);
-        Htab[h] = make_uint2(t, (unsigned)((fi * kF3 + fj) * kF3 + fk));
+        Htab[h] = make_uint2(t, urow_offset<D>(fi, fj, fk));
         hd[q] = ld32((const float*)a.pdiag, t);
         if (h < kH) S[fi][fj][fk] = ld32((const float*)a.alpha_s, t);
     }
@@ -1391,7 +1428,9 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(Src src, int spin_limit
         const int li = e / (kT3 * (kT3 + 1)), lj2 = e / (kT3 + 1) % kT3, kk = e % (kT3 + 1);
         FZ[li][lj2][kk] = face_fraction(S[li + 1][lj2 + 1][kk], S[li + 1][lj2 + 1][kk + 1], amin);
     }
-    __syncthreads();   // U is u from here on
+    __syncthreads();   // U is u from here on, in the row layout
+    float (*UI)[kF3][kT3] = reinterpret_cast<float (*)[kF3][kT3]>(&U[0][0][0]);
+    const float2 (*UK)[kF3] = reinterpret_cast<const float2 (*)[kF3]>(&U[0][0][0] + (D + 2) * kF3 * kT3);
     const int maxeff = c->maxeff;
     const double tol = c->tol;
     double atol = c->atol;
@@ -1514,7 +1553,7 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(Src src, int spin_limit
         for (int m = 0; m < kO; ++m) {
             float s, r1;
             const float u1 = cg_frame(orr[m], ow[m], os[m], od[m], first, q.alpha, q.beta, s, r1);
-            U[lq + 4 * m + 1][lj + 1][lk + 1] = u1;
+            UI[lq + 4 * m + 1][lj + 1][lk] = u1;
             rk_own[m] = orr[m];
             if (in_box(m)) {
                 orr[m] = r1;
@@ -1531,9 +1570,9 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(Src src, int spin_limit
         // ---- w_{k+1} = A u_{k+1} on the block, from LDS
 #pragma unroll
         for (int m = 0; m < kO; ++m) {
+            float uc;   // every lane forms its cell's w (the k neighbours come from the row's lanes)
+            const float w1 = cg_stencil_rows<D>(UI, UK, FX, FY, FZ, lq + 4 * m, lj, lk, hh, rhh, fastdiv, uc);
             if (in_box(m)) {
-                float uc;
-                const float w1 = cg_stencil_ff<D>(U, FX, FY, FZ, lq + 4 * m, lj, lk, hh, rhh, fastdiv, uc);
                 ow[m] = w1;
                 if (pub(m)) store_agent_f(wn, own_index(m), w1);
                 sum_d += (double)(w1 * uc);
