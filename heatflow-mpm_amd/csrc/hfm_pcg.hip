@@ -1291,10 +1291,20 @@ __device__ __forceinline__ int tile_workgroup(int L, int nb) {
     return q * kXcds + x;
 }
 
+// A workgroup's late flag (LDS): a thread whose bounded wait ran out sets it, every thread reads it after
+// the barrier that follows.  It is never cleared: the workgroup ends once it reads the flag set, so one
+// plain store and one barrier do what a workgroup OR (three barriers) would.
+__device__ __forceinline__ bool late_any(int late, int* late_s) {
+    if (late) *late_s = 1;
+    __syncthreads();
+    return *(volatile int*)late_s != 0;
+}
+
 // Wait until the r.r entry of workgroup `nbr` in slot P reads as posted (threads with nbr < 0 do not
-// wait); false in every thread if it did not within `limit` polls (at once for a negative limit).  A
-// workgroup posts its sums only after its faces of that iteration are in memory.
-__device__ __forceinline__ bool wait_neighbour(const double* P, int nbr, int limit) {
+// wait); false in every thread if it did not within `limit` polls (at once for a negative limit), or if
+// the late flag was already set.  A workgroup posts its sums only after its faces of that iteration
+// are in memory.
+__device__ __forceinline__ bool wait_neighbour(const double* P, int nbr, int limit, int* late_s) {
     int late = 0;
     if (nbr >= 0 && limit < 0) {
         late = 1;
@@ -1308,7 +1318,7 @@ __device__ __forceinline__ bool wait_neighbour(const double* P, int nbr, int lim
             __builtin_amdgcn_s_sleep(1);
         }
     }
-    return __syncthreads_or(late) == 0;
+    return !late_any(late, late_s);
 }
 
 // Rows r.r, g, d of slot P (row stride nb): thread b < nb polls its entries until none reads as empty and
@@ -1355,6 +1365,7 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(Src src, int spin_limit
     __shared__ uint2 Htab[kHM * kBlock3];  // halo cell h: memory index (clamped into the box), offset in U
     __shared__ double lds[3][kBlock3 / 64];
     __shared__ double wsum[kFusedRows][kBlock3 / 64];   // per-wave sums of the rows
+    __shared__ int late_s;                              // the late flag (late_any)
     __shared__ CgScalars qs;                            // iteration k's scalars, formed by wave 0
     CgCtl* c = a.ctl + 1;
     const int n = a.n;
@@ -1413,6 +1424,7 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(Src src, int spin_limit
         Xl[lq + 4 * m][lj][lk] = in_box(m) ? ld32((const float*)x, t) : 0.0f;   // x_0
         orr[m] = ow[m] = os[m] = 0.0f;
     }
+    if (threadIdx.x == 0) late_s = 0;
     const float amin = a.amin, hh = a.hh, rhh = a.rhh;
     const int fastdiv = a.fastdiv;
     __syncthreads();
@@ -1452,7 +1464,7 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(Src src, int spin_limit
         if (first)
             load_rows<kFusedRows>(a.partials, a.nb_upd, kFusedRows, acc);
         else
-            ok = wait_neighbour(P, nbr, spin_limit);
+            ok = wait_neighbour(P, nbr, spin_limit, &late_s);
         // ---- r_k, w_k, s_{k-1} of the halo cells, and of the own cells at k = 0 (and of positions of the
         // block outside the box: the clamped cell's), from memory
         // (threads past the last halo cell load the last one's values and do not use them)
@@ -1496,7 +1508,7 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(Src src, int spin_limit
                     if ((threadIdx.x & 63) == 0) wsum[r][threadIdx.x >> 6] = any ? w[r] : 0.0;
             }
         }
-        if (__syncthreads_or(late)) {
+        if (late_any(late, &late_s)) {
             if (threadIdx.x == 0) {
                 c->done = 1;
                 mirror_final(a.mirror + 1, kit, __builtin_nanf(""), atol, -2);
