@@ -1240,11 +1240,13 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init_fused(Src src) {
         s[4] += (double)(bc * bc);
     }
     store_partials<kFusedRows>(s, a.partials, v.nb_upd, lb, lds);
-    // the persistent form's partial-sum slots start empty
+    // the persistent form's partial-sum slots start empty (every copy)
     if (a.cg_persist) {
         double* slots = a.partials + kPersistSlots;
         for (int q = lb * kBlock + (int)threadIdx.x; q < 9 * a.nb_dir; q += v.nb_upd * kBlock)
-            slots[q] = __builtin_bit_cast(double, kEmptySum);
+#pragma unroll
+            for (int c = 0; c < kPersistReplicas; ++c)
+                slots[c * kPersistReplicaStride + q] = __builtin_bit_cast(double, kEmptySum);
     }
 }
 
@@ -1378,7 +1380,8 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(Src src, int spin_limit
     const int i0 = ti * D, j0 = tj * kT3, k0 = tk * kT3;
     float* __restrict__ pv = a.cg_z;
     float* __restrict__ x = a.p;
-    double* slots = a.partials + kPersistSlots;   // [3][3 rows][nb]
+    double* slots = a.partials + kPersistSlots;   // kPersistReplicas copies of [3][3 rows][nb]
+    const double* rslots = slots + (lb % kXcds) * kPersistReplicaStride;   // the copy this workgroup polls
     // the workgroups of the tiles around this one (thread t < 27, t != 13: offset (t / 9, t / 3 % 3,
     // t % 3) - 1), whose faces the frame reads; -1 for none
     int nbr = -1;
@@ -1459,7 +1462,7 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(Src src, int spin_limit
         // ---- the partial sums of iteration k - 1 (the init launch's at k = 0): first the neighbours' (their
         // faces are then in memory, and the halo loads go out while the others arrive), then everyone's
         double acc[kFusedRows];
-        const double* P = slots + ((kit + 2) % 3) * 3 * nb;   // slot (k - 1) % 3
+        const double* P = rslots + ((kit + 2) % 3) * 3 * nb;   // slot (k - 1) % 3
         bool ok = true;
         if (first)
             load_rows<kFusedRows>(a.partials, a.nb_upd, kFusedRows, acc);
@@ -1529,12 +1532,12 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(Src src, int spin_limit
             if (threadIdx.x == 0) {
                 qs = s0;
                 if (lb == 0) cg_record(c, a.mirror + 1, s0, first, kit, atol, atol_sq);
-                if (kit >= 2)   // empty this workgroup's sums of iteration k - 2
-#pragma unroll
-                    for (int r = 0; r < 3; ++r)
-                        store_agent(slots + ((kit + 1) % 3) * 3 * nb + r * nb + lb,
-                                    __builtin_bit_cast(double, kEmptySum));
             }
+            if (kit >= 2 && threadIdx.x < kPersistReplicas)   // empty this workgroup's sums of iteration
+#pragma unroll                                                 // k - 2, thread c in copy c
+                for (int r = 0; r < 3; ++r)
+                    store_agent(slots + threadIdx.x * kPersistReplicaStride + ((kit + 1) % 3) * 3 * nb + r * nb + lb,
+                                __builtin_bit_cast(double, kEmptySum));
         }
         __syncthreads();
         const CgScalars q = qs;
@@ -1594,9 +1597,10 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(Src src, int spin_limit
         wait_stores();
         double out[3] = {sum_rr, sum_g, sum_d};
         block_sum<3, kBlock3>(out, lds);
-        if (threadIdx.x == 0)
+        if (threadIdx.x < kPersistReplicas)   // thread c posts to copy c
 #pragma unroll
-            for (int r = 0; r < 3; ++r) store_agent(slots + (kit % 3) * 3 * nb + r * nb + lb, out[r]);
+            for (int r = 0; r < 3; ++r)
+                store_agent(slots + threadIdx.x * kPersistReplicaStride + (kit % 3) * 3 * nb + r * nb + lb, out[r]);
         // ---- p_k and x_{k+1} of the own cells, while the other workgroups' sums are on their way
 #pragma unroll
         for (int m = 0; m < kO; ++m) {

@@ -437,6 +437,12 @@ constexpr int kPersistMaxBlocks = 1024;
 // one-launch form's second parity buffer), and the pattern of an empty entry (a signalling NaN)
 constexpr long kPersistSlots = (long)kFusedRows * kMaxDirBlocks;
 constexpr unsigned long long kEmptySum = 0x7FF4A5A5A5A5A5A5ull;
+// Every workgroup posts its sums to kPersistReplicas copies of the slots, kPersistReplicaStride doubles
+// apart (an odd number of cache lines, so the copies fall on different pages), and polls the copy of
+// its XCD: the polls of the whole device spread over eight sets of pages instead of one.
+constexpr int kPersistReplicas = 8;
+constexpr long kPersistReplicaStride = 9L * kPersistMaxBlocks + 16 * 37;
+constexpr long kPartialsDoubles = kPersistSlots + kPersistReplicas * kPersistReplicaStride;
 int pcg_persist_capacity(int D, int device, bool batch = false);
 hipError_t launch_pcg_persist(const Launch& L, int D, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 // the same for scenes of a batch: one cooperative launch of `blocks` workgroups over a block map of

@@ -1271,7 +1271,7 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
     A(d.cg_z, N);
     A(d.cg_d, 2 * N);
     A(d.cg_Ap, 2 * N);   // two-pass: A p; one-launch form: w = A D r, two parity buffers
-    A(d.partials, std::max(kMaxDirBlocks + 5 * kMaxPcgBlocks, 2 * kFusedRows * kMaxDirBlocks));
+    A(d.partials, std::max((long)kMaxDirBlocks + 5 * kMaxPcgBlocks, kPartialsDoubles));
     A(d.src_part, nblocks(N));
     if (prm->flags & HFM_FLAG_MG_PCG) {   // multigrid levels: halve n down to at most 8 cells per side
         d.cg_mg = 1;
