@@ -1049,15 +1049,15 @@ __device__ __forceinline__ unsigned urow_offset(int fi, int fj, int fk) {
                                 : (unsigned)((D + 2) * kF3 * kT3 + (fi * kF3 + fj) * 2 + (fk == 0 ? 0 : 1));
 }
 
-// cg_stencil_ff on the row layout: u at k - 1 and k + 1 comes from the neighbouring lanes of the 16-lane
-// row (DPP row_shr:1 / row_shl:1), the row's first and last lanes take the k-halo pair; the same six
-// terms in the same order, so the same bits.  Every lane of the wave must run it (the moves read the
-// neighbours' registers).
+// The six-term sum of cg_stencil_ff on the row layout (w before its division by h^2): u at k - 1 and
+// k + 1 comes from the neighbouring lanes of the 16-lane row (DPP row_shr:1 / row_shl:1), the row's
+// first and last lanes take the k-halo pair; the same six terms in the same order, so the same bits.
+// Every lane of the wave must run it (the moves read the neighbours' registers).
 template <int D>
-__device__ __forceinline__ float cg_stencil_rows(const float (*UI)[kF3][kT3], const float2 (*UK)[kF3],
-                                                 const float (*FX)[kT3][kT3], const float (*FY)[kT3 + 1][kT3],
-                                                 const float (*FZ)[kT3][kT3 + 1], int li, int lj, int lk, float hh,
-                                                 float rhh, int fastdiv, float& uc) {
+__device__ __forceinline__ float cg_stencil_rows_sum(const float (*UI)[kF3][kT3], const float2 (*UK)[kF3],
+                                                     const float (*FX)[kT3][kT3], const float (*FY)[kT3 + 1][kT3],
+                                                     const float (*FZ)[kT3][kT3 + 1], int li, int lj, int lk,
+                                                     float& uc) {
     const int si = li + 1, sj = lj + 1;
     uc = UI[si][sj][lk];
     const float2 kh = UK[si][sj];
@@ -1072,7 +1072,29 @@ __device__ __forceinline__ float cg_stencil_rows(const float (*UI)[kF3][kT3], co
     acc6 += FY[li][lj + 1][lk] * (uc - UI[si][sj + 1][lk]);
     acc6 += FZ[li][lj][lk] * (uc - ukm);
     acc6 += FZ[li][lj][lk + 1] * (uc - ukp);
-    return div_const(1.0f * acc6, hh, rhh, fastdiv);
+    return acc6;
+}
+
+// cg_stencil_rows_sum for a cell whose six face fractions are exactly 1 (no solid in or beside it): the
+// same terms without reading the fractions (1 x d is d exactly), so the same bits.
+template <int D>
+__device__ __forceinline__ float cg_stencil_rows_open(const float (*UI)[kF3][kT3], const float2 (*UK)[kF3], int li,
+                                                      int lj, int lk, float& uc) {
+    const int si = li + 1, sj = lj + 1;
+    uc = UI[si][sj][lk];
+    const float2 kh = UK[si][sj];
+    const float ukm = __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(kh.x), __float_as_int(uc), 0x111, 0xf, 0xf, false));  // row_shr:1
+    const float ukp = __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(kh.y), __float_as_int(uc), 0x101, 0xf, 0xf, false));  // row_shl:1
+    float acc6 = 0.0f;
+    acc6 += uc - UI[si - 1][sj][lk];
+    acc6 += uc - UI[si + 1][sj][lk];
+    acc6 += uc - UI[si][sj - 1][lk];
+    acc6 += uc - UI[si][sj + 1][lk];
+    acc6 += uc - ukm;
+    acc6 += uc - ukp;
+    return acc6;
 }
 
 template <int D, class Src>
@@ -1444,6 +1466,16 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(Src src, int spin_limit
         FZ[li][lj2][kk] = face_fraction(S[li + 1][lj2 + 1][kk], S[li + 1][lj2 + 1][kk + 1], amin);
     }
     __syncthreads();   // U is u from here on, in the row layout
+    // bit m: cell m of every lane of this wave has six open faces (fraction exactly 1), so the wave forms
+    // its stencil without reading the fractions (uniform over the wave)
+    unsigned open_cells = 0;
+#pragma unroll
+    for (int m = 0; m < kO; ++m) {
+        const int li = lq + 4 * m;
+        const bool open = FX[li][lj][lk] == 1.0f && FX[li + 1][lj][lk] == 1.0f && FY[li][lj][lk] == 1.0f &&
+                          FY[li][lj + 1][lk] == 1.0f && FZ[li][lj][lk] == 1.0f && FZ[li][lj][lk + 1] == 1.0f;
+        if (__all(open)) open_cells |= 1u << m;
+    }
     float (*UI)[kF3][kT3] = reinterpret_cast<float (*)[kF3][kT3]>(&U[0][0][0]);
     const float2 (*UK)[kF3] = reinterpret_cast<const float2 (*)[kF3]>(&U[0][0][0] + (D + 2) * kF3 * kT3);
     const int maxeff = c->maxeff;
@@ -1586,7 +1618,10 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_persist(Src src, int spin_limit
 #pragma unroll
         for (int m = 0; m < kO; ++m) {
             float uc;   // every lane forms its cell's w (the k neighbours come from the row's lanes)
-            const float w1 = cg_stencil_rows<D>(UI, UK, FX, FY, FZ, lq + 4 * m, lj, lk, hh, rhh, fastdiv, uc);
+            const float acc6 = (open_cells >> m) & 1u
+                                   ? cg_stencil_rows_open<D>(UI, UK, lq + 4 * m, lj, lk, uc)
+                                   : cg_stencil_rows_sum<D>(UI, UK, FX, FY, FZ, lq + 4 * m, lj, lk, uc);
+            const float w1 = div_const(1.0f * acc6, hh, rhh, fastdiv);
             if (in_box(m)) {
                 ow[m] = w1;
                 if (pub(m)) store_agent_f(wn, own_index(m), w1);
