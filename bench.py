@@ -210,7 +210,7 @@ def bench_served(a) -> dict:
     arrival times: with continuous batching (the admitted scenes of a size class advance in one batched
     substep, the size classes (--lanes) side by side on their own streams; admission in arrival order by
     device footprint), the multigrid-preconditioned pressure PCG and the mixed-precision form (FP16
-    refinement vectors, BF16 departure velocity, fp32 state) by default, and in the plain form (fp32,
+    refinement vectors, fp32 state) by default, and in the plain form (fp32,
     the reference's Jacobi PCG, one request at a time, in arrival order).
     Latency = completion - arrival.  Each served result is compared with
     the plain result of the same request (relative L2 of T and u): both solve to the same relative
@@ -269,7 +269,7 @@ def bench_served(a) -> dict:
     return dict(metric="served p50/p99 latency and aggregate cell-substeps/s, continuous batching",
                 value=s_b["cell_substeps_per_s"], unit="cell-substeps/s", n_gpus=1, steps=None, warmup=0,
                 ms_per_step=None, higher_is_better=True, scaling="weak", vs_baseline=None,
-                dtype="f32" if prec == "fp32" else "f32 state, f16 PCG vectors, bf16 departure velocity",
+                dtype="f32" if prec == "fp32" else "f32 state, f16 PCG vectors",
                 data="synthetic",
                 config=dict(workload=f"{len(reqs)} Poisson requests at {a.rate}/s, n in {{48,64,96,128,192}} "
                                      f"(counts {[ns.count(m) for m in (48, 64, 96, 128, 192)]}), K uniform "
@@ -326,7 +326,7 @@ def bench_served_sweep(a) -> dict:
     return dict(metric="served p50/p99 latency and aggregate cell-substeps/s against arrival rate",
                 value=min(r["speedup"] for r in rows), unit="x plain aggregate cell-substeps/s (lowest over rates)",
                 n_gpus=1, steps=None, warmup=0, ms_per_step=None, higher_is_better=True, scaling="weak",
-                vs_baseline=None, dtype="f32" if prec == "fp32" else "f32 state, f16 PCG vectors, bf16 departure velocity",
+                vs_baseline=None, dtype="f32" if prec == "fp32" else "f32 state, f16 PCG vectors",
                 data="synthetic",
                 config=dict(workload=f"{a.requests} Poisson requests per rate, n in {{48,64,96,128,192}}, K uniform "
                                      f"{a.kmin}-{a.kmax}, PCG tol 1e-6 (pressure cap {a.cg_max_iter}), dt 2 ms",
@@ -338,7 +338,7 @@ def bench_served_sweep(a) -> dict:
 def bench_mixed(a) -> dict:
     """The coupled scene at n=192 (h = 2.5 mm) with the ellipsoid seeded to about 50k particles,
     K substeps from rest, once in fp32 and once in the mixed form (iterative refinement with FP16
-    PCG vectors, BF16 departure velocity) on the same GPU.  Reports both rates, the pressure-PCG
+    PCG vectors) on the same GPU.  Reports both rates, the pressure-PCG
     kernel times, and the relative L2 of the mixed T and u against fp32; with --fp64-n m also the
     error of both forms against the fp64 CPU oracle after K substeps of the same scene at n = m."""
     import dataclasses
@@ -402,7 +402,7 @@ def bench_mixed(a) -> dict:
     return dict(metric="cell-substeps/s, mixed precision (iterative refinement, FP16 PCG vectors)",
                 value=mixed["value"], unit="cell-substeps/s", n_gpus=1, steps=K, warmup=0,
                 ms_per_step=mixed["ms_per_step"], higher_is_better=True, scaling="weak", vs_baseline=None,
-                dtype="f32 state, f16 PCG vectors, bf16 departure velocity", data="synthetic",
+                dtype="f32 state, f16 PCG vectors", data="synthetic",
                 config=dict(workload=f"coupled scene n={n} (h={0.48 / n * 1e3:.2f} mm), static ellipsoid "
                                      f"P={mixed['particles']}, heater 400 W cap 120 C fan 2 m/s, PCG tol 1e-6 "
                                      f"(pressure cap {a.cg_max_iter} iterations), dt 2 ms, {K} substeps from rest",
@@ -616,7 +616,7 @@ def main() -> int:
 
     out = dict(metric=METRIC, value=value, unit="cell-substeps/s", n_gpus=1, steps=a.steps, warmup=a.warmup,
                ms_per_step=wall * 1e3 / a.steps, higher_is_better=True, scaling="weak", vs_baseline=None,
-               dtype="f32" if prec == "fp32" else "f32 state, f16 PCG vectors, bf16 departure velocity",
+               dtype="f32" if prec == "fp32" else "f32 state, f16 PCG vectors",
                data="synthetic",
                config=dict(workload=f"coupled scene n={n} (h=5 mm), static ellipsoid P={P}, heater 400 W cap 120 C fan "
                                     f"2 m/s, PCG tol {kw['grid'].cg_tol:g}, dt 2 ms", n=n, particles=P, dt=dt,

@@ -98,13 +98,6 @@ __host__ __device__ __forceinline__ void base_index1(float pos, float off, CellS
     split_floor(g, i, f);
 }
 
-// fp32 -> BF16 rounded to nearest even (finite values)
-__device__ __forceinline__ uint16_t to_bf16(float x) {
-    unsigned u = __float_as_uint(x);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-
 // (i, j, k) of flat index t of a lattice with ny x nz elements per i-slab, in 32-bit arithmetic
 // (every lattice holds at most (1024 + 1) * 1024^2 < 2^32 elements): two divisions instead of the
 // three 64-bit divisions of i = t / (ny nz), j = (t / nz) % ny, k = t % nz.
@@ -126,23 +119,13 @@ __host__ __device__ __forceinline__ void split_index(float q, float off, int& i,
 // The values at clamp(bk) and clamp(bk + 1) of a lattice row of nz >= 2 elements, from one load
 // of the adjacent pair at b = min(clamp(bk), nz - 2) (in the row for every bk): each index is b
 // or b + 1, so each value is the pair's low or high element.
-template <class E>
-__host__ __device__ __forceinline__ void row_pair(const E* __restrict__ row, int bk, int nz, float& v0, float& v1) {
+__host__ __device__ __forceinline__ void row_pair(const float* __restrict__ row, int bk, int nz, float& v0, float& v1) {
     const int i0 = clampi(bk, 0, nz - 1), i1 = clampi(bk + 1, 0, nz - 1);
     const int b = i0 < nz - 2 ? i0 : nz - 2;
-    float lo, hi;
-    if constexpr (sizeof(E) == 4) {   // fp32
-        unsigned p[2];
-        __builtin_memcpy(p, row + b, 8);
-        lo = __builtin_bit_cast(float, p[0]);
-        hi = __builtin_bit_cast(float, p[1]);
-    } else {                          // BF16 patterns (the upper half of an fp32 pattern), the lower
-                                      // address in the low half
-        unsigned p;
-        __builtin_memcpy(&p, row + b, 4);
-        lo = __builtin_bit_cast(float, p << 16);
-        hi = __builtin_bit_cast(float, p & 0xffff0000u);
-    }
+    unsigned p[2];
+    __builtin_memcpy(p, row + b, 8);
+    const float lo = __builtin_bit_cast(float, p[0]);
+    const float hi = __builtin_bit_cast(float, p[1]);
     v0 = i0 == b ? lo : hi;
     v1 = i1 == b ? lo : hi;
 }
@@ -151,8 +134,7 @@ __host__ __device__ __forceinline__ void row_pair(const E* __restrict__ row, int
 // fractions of a point: clamped indices, weights from the unclamped fractions, accumulation order
 // dx, dy, dz.  Index arithmetic is 32-bit (a lattice holds at most (1024 + 1) * 1024^2 < 2^32
 // elements); the two dz values of a row come from one load (row_pair).
-template <class E = float>
-__host__ __device__ __forceinline__ float trilinear(const E* __restrict__ a, int bi, float fx, int bj, float fy, int bk,
+__host__ __device__ __forceinline__ float trilinear(const float* __restrict__ a, int bi, float fx, int bj, float fy, int bk,
                                            float fz, int nx, int ny, int nz) {
     float s = 0.0f;
 #pragma unroll
@@ -179,15 +161,14 @@ __host__ __device__ __forceinline__ float trilinear(const E* __restrict__ a, int
 }
 
 // fields.py:25-36 — trilinear sample of a lattice with offsets (ox, oy, oz) at a point
-template <class E = float>
-__host__ __device__ __forceinline__ float sample(const E* __restrict__ a, float px, float py, float pz, float ox,
+__host__ __device__ __forceinline__ float sample(const float* __restrict__ a, float px, float py, float pz, float ox,
                                         float oy, float oz, CellSize h, int nx, int ny, int nz) {
     int bi, bj, bk;
     float fx, fy, fz;
     base_index1(px, ox, h, bi, fx);
     base_index1(py, oy, h, bj, fy);
     base_index1(pz, oz, h, bk, fz);
-    return trilinear<E>(a, bi, fx, bj, fy, bk, fz, nx, ny, nz);
+    return trilinear(a, bi, fx, bj, fy, bk, fz, nx, ny, nz);
 }
 
 // fields.py:51-65 — min and max of the eight lattice values around a point (the MacCormack
@@ -220,9 +201,8 @@ __device__ __forceinline__ void value_range(const float* __restrict__ a, float p
 }
 
 // fields.py:68-72
-template <class E = float>
-__host__ __device__ __forceinline__ void sample_velocity(const E* __restrict__ u, const E* __restrict__ v,
-                                                const E* __restrict__ w, float px, float py, float pz, CellSize h,
+__host__ __device__ __forceinline__ void sample_velocity(const float* __restrict__ u, const float* __restrict__ v,
+                                                const float* __restrict__ w, float px, float py, float pz, CellSize h,
                                                 int n, float& vx, float& vy, float& vz) {
     // the three face lattices take offsets 0 or 0.5 per axis: one quotient per axis, two splits
     const float qx = h.quotient(px), qy = h.quotient(py), qz = h.quotient(pz);
@@ -234,21 +214,20 @@ __host__ __device__ __forceinline__ void sample_velocity(const E* __restrict__ u
     split_index(qy, 0.5f, y5, fy5);
     split_index(qz, 0.0f, z0, fz0);
     split_index(qz, 0.5f, z5, fz5);
-    vx = trilinear<E>(u, x0, fx0, y5, fy5, z5, fz5, n + 1, n, n);
-    vy = trilinear<E>(v, x5, fx5, y0, fy0, z5, fz5, n, n + 1, n);
-    vz = trilinear<E>(w, x5, fx5, y5, fy5, z0, fz0, n, n, n + 1);
+    vx = trilinear(u, x0, fx0, y5, fy5, z5, fz5, n + 1, n, n);
+    vy = trilinear(v, x5, fx5, y0, fy0, z5, fz5, n, n + 1, n);
+    vz = trilinear(w, x5, fx5, y5, fy5, z0, fz0, n, n, n + 1);
 }
 
-// fields.py:111-116 — midpoint RK2 departure point (E = uint16_t: through BF16 copies of u, v, w)
-template <class E = float>
-__host__ __device__ __forceinline__ void backtrace(const E* __restrict__ u, const E* __restrict__ v,
-                                          const E* __restrict__ w, float& x, float& y, float& z, CellSize h, int n,
+// fields.py:111-116 — midpoint RK2 departure point
+__host__ __device__ __forceinline__ void backtrace(const float* __restrict__ u, const float* __restrict__ v,
+                                          const float* __restrict__ w, float& x, float& y, float& z, CellSize h, int n,
                                           float dt) {
     float vx, vy, vz;
-    sample_velocity<E>(u, v, w, x, y, z, h, n, vx, vy, vz);
+    sample_velocity(u, v, w, x, y, z, h, n, vx, vy, vz);
     float hd = 0.5f * dt;
     float x1 = x - hd * vx, y1 = y - hd * vy, z1 = z - hd * vz;
-    sample_velocity<E>(u, v, w, x1, y1, z1, h, n, vx, vy, vz);
+    sample_velocity(u, v, w, x1, y1, z1, h, n, vx, vy, vz);
     x = x - dt * vx;
     y = y - dt * vy;
     z = z - dt * vz;

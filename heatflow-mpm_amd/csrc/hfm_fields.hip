@@ -96,10 +96,7 @@ __device__ __forceinline__ void heat_tail(const A& a, long t, int i, int j, int 
 
 // heat.py:31-41 (T advection) fused with the conductivity and heat rhs/diagonal.  With
 // HFM_FLAG_MACCORMACK this pass only stores the semi-Lagrangian value; k_mc_T finishes the cell.
-// MIXED (the batch's precision mode): departure points through the BF16 copies of u, v, w.  One
-// instantiation per mode keeps the fp32 kernel at the registers of its own path (9 waves per
-// SIMD instead of 6).
-template <bool MIXED, class Src>
+template <class Src>
 __global__ __launch_bounds__(kBlock) void k_advect_T(Src src, float dt) {
     HFM_ENTRY(src);
     const CellSize hs(a.h, a.rh, a.fast_h);
@@ -115,10 +112,7 @@ __global__ __launch_bounds__(kBlock) void k_advect_T(Src src, float dt) {
         t2 = a.Tw[t];
     } else {
         float x = ((float)i + 0.5f) * h, y = ((float)j + 0.5f) * h, z = ((float)k + 0.5f) * h;
-        if constexpr (MIXED)
-            backtrace<uint16_t>(a.ub[0], a.ub[1], a.ub[2], x, y, z, hs, n, dt);
-        else
-            backtrace<float>(a.u[0], a.u[1], a.u[2], x, y, z, hs, n, dt);
+        backtrace(a.u[0], a.u[1], a.u[2], x, y, z, hs, n, dt);
         t2 = sample(a.Tw, x, y, z, 0.5f, 0.5f, 0.5f, hs, n, n, n);
     }
     if (a.maccormack) {
@@ -148,8 +142,8 @@ __global__ __launch_bounds__(kBlock) void k_mc_T(Src src, float dt) {
     if (!(ac > 0.5f)) {
         const float xc = ((float)i + 0.5f) * h, yc = ((float)j + 0.5f) * h, zc = ((float)k + 0.5f) * h;
         float dx = xc, dy = yc, dz = zc, fx = xc, fy = yc, fz = zc;
-        backtrace<float>(a.u[0], a.u[1], a.u[2], dx, dy, dz, hs, n, dt);
-        backtrace<float>(a.u[0], a.u[1], a.u[2], fx, fy, fz, hs, n, -dt);
+        backtrace(a.u[0], a.u[1], a.u[2], dx, dy, dz, hs, n, dt);
+        backtrace(a.u[0], a.u[1], a.u[2], fx, fy, fz, hs, n, -dt);
         const float back = sample(a.mc_T, fx, fy, fz, 0.5f, 0.5f, 0.5f, hs, n, n, n);
         float v = t2 + 0.5f * (a.Tw[t] - back);
         float lo, hi;
@@ -222,8 +216,8 @@ __device__ __forceinline__ float face_tail(const A& a, int axis, long f, int i, 
 
 // grid.py:25-33 (face advection of u, v, w from the old fields) fused with buoyancy and
 // relaxation.  With HFM_FLAG_MACCORMACK this pass only stores the semi-Lagrangian value;
-// k_mc_faces finishes the face.  MIXED as for k_advect_T.
-template <bool MIXED, class Src>
+// k_mc_faces finishes the face.
+template <class Src>
 __global__ __launch_bounds__(kBlock) void k_air_faces(Src src, float dt) {
     HFM_ENTRY(src);
     const CellSize hs(a.h, a.rh, a.fast_h);
@@ -243,10 +237,7 @@ __global__ __launch_bounds__(kBlock) void k_air_faces(Src src, float dt) {
     float x, y, z, ox, oy, oz;
     int dx, dy, dz;
     face_geometry(axis, i, j, k, h, n, x, y, z, ox, oy, oz, dx, dy, dz);
-    if constexpr (MIXED)
-        backtrace<uint16_t>(a.ub[0], a.ub[1], a.ub[2], x, y, z, hs, n, dt);
-    else
-        backtrace<float>(a.u[0], a.u[1], a.u[2], x, y, z, hs, n, dt);
+    backtrace(a.u[0], a.u[1], a.u[2], x, y, z, hs, n, dt);
     const float val = sample(a.u[axis], x, y, z, ox, oy, oz, hs, dx, dy, dz);
     out[f] = a.maccormack ? val : face_tail(a, axis, f, i, j, k, val, dt);
 }
@@ -275,8 +266,8 @@ __global__ __launch_bounds__(kBlock) void k_mc_faces(Src src, float dt) {
     int dx, dy, dz;
     face_geometry(axis, i, j, k, h, n, x, y, z, ox, oy, oz, dx, dy, dz);
     float px = x, py = y, pz = z, fx = x, fy = y, fz = z;
-    backtrace<float>(a.u[0], a.u[1], a.u[2], px, py, pz, hs, n, dt);
-    backtrace<float>(a.u[0], a.u[1], a.u[2], fx, fy, fz, hs, n, -dt);
+    backtrace(a.u[0], a.u[1], a.u[2], px, py, pz, hs, n, dt);
+    backtrace(a.u[0], a.u[1], a.u[2], fx, fy, fz, hs, n, -dt);
     const float fwd = a.mc_u[axis][f];
     const float back = sample(a.mc_u[axis], fx, fy, fz, ox, oy, oz, hs, dx, dy, dz);
     float v = fwd + 0.5f * (a.u[axis][f] - back);
@@ -470,7 +461,6 @@ __global__ __launch_bounds__(kBlock) void k_project(Src src, double dt) {
     const int idx = axis == 0 ? i : (axis == 1 ? j : k);
     if (idx == 0 || idx == n) {
         a.u[axis][f] = 0.0f;
-        if (a.mixed) a.ub[axis][f] = 0;
         return;
     }
     const float dor = (float)(dt / a.rho_d);  // dt/rho formed in double (grid.py:251)
@@ -479,14 +469,6 @@ __global__ __launch_bounds__(kBlock) void k_project(Src src, double dt) {
     const float gp = a.p[c] - a.p[cl];
     const float val = a.us[axis][f] - dor * gp / a.h;
     a.u[axis][f] = val;
-    if (a.mixed) a.ub[axis][f] = to_bf16(val);  // departure points of the next substep's advection
-}
-
-// BF16 copies of u, v, w (after the host wrote the velocity)
-__global__ __launch_bounds__(kBlock) void k_bf16_copy(const float* __restrict__ u, uint16_t* __restrict__ ub,
-                                                      long F) {
-    const long f = (long)blockIdx.x * kBlock + threadIdx.x;
-    if (f < F) ub[f] = to_bf16(u[f]);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -547,19 +529,9 @@ __global__ __launch_bounds__(kBlock) void k_apply_fixed(long N, const uint8_t* f
 }  // namespace
 
 void launch_heat_prep(const Launch& L, float dt) { HFM_LAUNCH(k_heat_prep, L, kBlock, nullptr, nullptr, dt); }
-void launch_advect_T(const Launch& L, float dt, bool mixed) {
-    if (mixed)
-        HFM_LAUNCH_T(k_advect_T, true, L, kBlock, nullptr, nullptr, dt);
-    else
-        HFM_LAUNCH_T(k_advect_T, false, L, kBlock, nullptr, nullptr, dt);
-}
+void launch_advect_T(const Launch& L, float dt) { HFM_LAUNCH(k_advect_T, L, kBlock, nullptr, nullptr, dt); }
 void launch_g2p(const Launch& L) { HFM_LAUNCH(k_g2p, L, kBlock, nullptr, nullptr); }
-void launch_air_faces(const Launch& L, float dt, bool mixed) {
-    if (mixed)
-        HFM_LAUNCH_T(k_air_faces, true, L, kBlock, nullptr, nullptr, dt);
-    else
-        HFM_LAUNCH_T(k_air_faces, false, L, kBlock, nullptr, nullptr, dt);
-}
+void launch_air_faces(const Launch& L, float dt) { HFM_LAUNCH(k_air_faces, L, kBlock, nullptr, nullptr, dt); }
 
 void launch_mc_T(const Launch& L, float dt) { HFM_LAUNCH(k_mc_T, L, kBlock, nullptr, nullptr, dt); }
 
@@ -567,12 +539,6 @@ void launch_mc_faces(const Launch& L, float dt) { HFM_LAUNCH(k_mc_faces, L, kBlo
 void launch_divergence(const Launch& L, double dt) { HFM_LAUNCH(k_divergence, L, kBlock, nullptr, nullptr, dt); }
 void launch_rhs_mean(const Launch& L) { HFM_LAUNCH(k_rhs_mean, L, kBlock, nullptr, nullptr); }
 void launch_project(const Launch& L, double dt) { HFM_LAUNCH(k_project, L, kBlock, nullptr, nullptr, dt); }
-
-void launch_bf16_copy(const Scene* s, hipStream_t st) {
-    if (!s->dev.mixed) return;
-    for (int a = 0; a < 3; ++a)
-        hipLaunchKernelGGL(k_bf16_copy, dim3(nblocks(s->F)), dim3(kBlock), 0, st, s->dev.u[a], s->dev.ub[a], s->F);
-}
 
 void launch_divergence_scale1(const Scene* s, hipStream_t st, float* out) {
     hipLaunchKernelGGL(k_divergence1, dim3(nblocks(s->N)), dim3(kBlock), 0, st, s->dev, out);

@@ -140,7 +140,6 @@ struct SceneView {
     ptr<float> mg_ax, mg_ay, mg_az, mg_w;   // coarse levels: -x, -y, -z face coefficients, omega / sum
     ptr<float> mg_s, mg_e;   // coarse levels: right-hand side and correction of the V-cycle
     ptr<_Float16> ir_r, ir_z, ir_d, ir_q;  // inner solve: r, z, p (two parity buffers), A p
-    ptr<uint16_t> ub[3];     // BF16 copies of u, v, w (departure points of the advection)
 };
 using SceneDev = SceneView<HostPtrs>;  // host copy, written by the host
 using SceneG = SceneView<DevPtrs>;     // the kernels' view of the same bytes
@@ -368,9 +367,9 @@ inline void launch_scene_kernel(KB kbatch, KS ksolo, const Launch& L, int thread
 
 // ---- batched kernel launchers (hfm_fields.hip) ----
 void launch_heat_prep(const Launch& L, float dt);
-void launch_advect_T(const Launch& L, float dt, bool mixed);   // mixed: the batch's precision mode
+void launch_advect_T(const Launch& L, float dt);
 void launch_g2p(const Launch& L);
-void launch_air_faces(const Launch& L, float dt, bool mixed);   // mixed: the batch's precision mode
+void launch_air_faces(const Launch& L, float dt);
 void launch_divergence(const Launch& L, double dt);
 void launch_rhs_mean(const Launch& L);
 void launch_project(const Launch& L, double dt);
@@ -412,7 +411,6 @@ void launch_pcg_upd(const Launch& L, int op, int k, hipEvent_t ev0 = nullptr, hi
 // then its scaling into the inner solve's FP16 vectors (or the end of the solve)
 void launch_ir_residual(const Launch& L, int op, float dt, bool first);
 void launch_ir_scale(const Launch& L, int op, int step);   // step: refinement step, 0 the first
-void launch_bf16_copy(const Scene* s, hipStream_t st);
 
 // ---- multigrid preconditioner of the pressure PCG (hfm_mg.hip) ----
 constexpr int kMgMaxLevels = 8;

@@ -967,14 +967,14 @@ int batch_substep(Batch* b, double dt) {
     if (listed) launch_p2g_listed(family(b, FAM_CELL));
     const bool mixed = !b->scenes.empty() && b->scenes[0]->dev.mixed != 0;  // one precision per batch
     launch_heat_prep(family(b, FAM_CELL), dt32);
-    launch_advect_T(family(b, FAM_CELL), dt32, mixed);
+    launch_advect_T(family(b, FAM_CELL), dt32);
     bool mc = false;
     for (Scene* s : b->scenes) mc |= s->dev.maccormack != 0;
     if (mc) launch_mc_T(family(b, FAM_CELL), dt32);
     CHECK_LAUNCH();
     RC(mixed ? run_ir(b, 0, dt32) : run_pcg(b, 0, dt32));
     launch_g2p(family(b, FAM_G2P));
-    launch_air_faces(family(b, FAM_FACE3), dt32, mixed);
+    launch_air_faces(family(b, FAM_FACE3), dt32);
     if (mc) launch_mc_faces(family(b, FAM_FACE3), dt32);
     launch_divergence(family(b, FAM_CELL), dt);
     launch_rhs_mean(family(b, FAM_MEAN));
@@ -1306,7 +1306,6 @@ int hfm_scene_create(const hfm_params* prm, int device, hfm_scene** out) {
         A(d.ir_z, N);
         A(d.ir_d, 2 * N);
         A(d.ir_q, N);
-        for (int a = 0; a < 3; ++a) A(d.ub[a], F);  // zero: the BF16 pattern of u = 0
     }
     A(d.ctl, 2);
     d.nchunks = (int)((N + 8191) / 8192);
@@ -1612,11 +1611,6 @@ int hfm_set_field(hfm_scene* h, int32_t field, const float* in) {
     HIP_OK(hipStreamSynchronize(s->stream));
     HIP_OK(hipMemcpy(p, in, size * sizeof(float), hipMemcpyHostToDevice));
     if (field == HFM_FIELD_ALPHA_S) s->mg_coef_dirty = 1;
-    if (field == HFM_FIELD_U || field == HFM_FIELD_V || field == HFM_FIELD_W) {
-        launch_bf16_copy(s, s->stream);   // mixed precision: the advection's copy of the velocity
-        CHECK_LAUNCH();
-        HIP_OK(hipStreamSynchronize(s->stream));
-    }
     return HFM_OK;
 }
 

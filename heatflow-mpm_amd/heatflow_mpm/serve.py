@@ -53,16 +53,16 @@ def footprint_estimate(n: int, particles: int, precision: str = "fp32", cg: str 
     """Device bytes of a scene before it is created: 21 face-sized and 29 cell-sized fp32
     arrays (fields, static parts, solver vectors, bins, contribution-list starts), 1 byte of
     flags per cell, an fp64 partial sum per 256 cells, 12 fp32 words and 4 ints per particle and
-    its 8 contribution-list terms (an int and a float each), and fixed tables (partial sums,
-    programs); the mixed form
-    adds 5 FP16 cell arrays (inner solve) and 3 BF16 face arrays (velocity copy), the multigrid
+    its 8 contribution-list terms (an int and a float each), and fixed tables (partial sums with
+    the persistent form's per-XCD copies, programs, about 1 MiB); the mixed form adds 5 FP16 cell
+    arrays (inner solve), the multigrid
     preconditioner 6 fp32 arrays over its coarse levels (fewer than N / 7 cells)."""
     N = n ** 3
     F = (n + 1) * n * n
-    extra = 2 * (5 * N + 3 * F) if precision == "mixed" else 0
+    extra = 2 * 5 * N if precision == "mixed" else 0
     if cg == "mg":
         extra += 24 * (N // 7 + 1)
-    return 4 * (21 * F + 29 * N) + N + 8 * (N // 256 + 1) + 128 * particles + 256 * 1024 + extra
+    return 4 * (21 * F + 29 * N) + N + 8 * (N // 256 + 1) + 128 * particles + 1024 * 1024 + extra
 
 
 class GpuBackend:

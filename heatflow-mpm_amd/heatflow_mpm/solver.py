@@ -213,8 +213,8 @@ class HeatflowMPM:
     that same iteration, with the same bits, as one cooperative launch per solve that keeps the
     vectors on chip (HFM_FLAG_PERSISTENT_CG: scenes stepped on their own whose 16^3 tiles the GPU
     holds resident at once, n <= 96 on MI355X; otherwise it is cg="fused");
-    `precision="mixed"` solves both systems by iterative refinement with FP16 inner vectors and
-    traces advection through a BF16 copy of the velocity (HFM_PREC_MIXED); the state stays fp32;
+    `precision="mixed"` solves both systems by iterative refinement with FP16 inner vectors
+    (HFM_PREC_MIXED); the state and the advection stay fp32;
     `cg="mg"` preconditions the pressure PCG with a geometric multigrid V-cycle instead of
     Jacobi (HFM_FLAG_MG_PCG: n = m 2^k with m <= 8 and n >= 16; with precision="mixed" it
     preconditions the FP16 inner solves): the same stopping rule in far fewer iterations, fields

@@ -74,8 +74,8 @@ typedef struct hfm_params {
  * arithmetic with its norm in fp64 and stops when ||r|| <= max(tol * ||b||, tol) (the
  * reference's threshold, now on the true residual); otherwise an inner Jacobi-PCG on r scaled to
  * unit RMS, with r, z, p and A p stored as FP16 (fp32 arithmetic, fp64 dot products), reduces it
- * by 1e-2 and adds its correction to x.  The departure points of semi-Lagrangian advection are
- * traced through a BF16 copy of u, v, w.  The iteration cap counts inner iterations; counts are
+ * by 1e-2 and adds its correction to x.  Advection traces its departure points through the fp32
+ * velocity.  The iteration cap counts inner iterations; counts are
  * reported exactly (no check_every quantisation).  Not combinable with HFM_FLAG_FUSED_CG. */
 #define HFM_PREC_MIXED 1
 

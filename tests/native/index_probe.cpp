@@ -38,10 +38,10 @@ int main() {
                 for (int c = 0; c < nb; c += 5) {
                     float x = big[a], y = big[b], z = big[c];
                     float vx, vy, vz;
-                    hfm::sample_velocity<float>(u, v, w, x, y, z, hs, n, vx, vy, vz);
+                    hfm::sample_velocity(u, v, w, x, y, z, hs, n, vx, vy, vz);
                     float bx = x, by = y, bz = z;
-                    hfm::backtrace<float>(u, v, w, bx, by, bz, hs, n, dt);
-                    const float t = hfm::sample<float>(T, bx, by, bz, 0.5f, 0.5f, 0.5f, hs, n, n, n);
+                    hfm::backtrace(u, v, w, bx, by, bz, hs, n, dt);
+                    const float t = hfm::sample(T, bx, by, bz, 0.5f, 0.5f, 0.5f, hs, n, n, n);
                     std::printf("%g %g %g %g\n", (double)vx, (double)vy, (double)vz, (double)t);
                 }
     // velocities that are themselves huge or non-finite (a diverged pressure projection)
@@ -54,8 +54,8 @@ int main() {
         }
         for (int a = 0; a < nb; a += 2) {
             float x = big[a], y = 0.5f * h, z = 0.5f * h;
-            hfm::backtrace<float>(u, v, w, x, y, z, h, n, dt);
-            std::printf("%g\n", (double)hfm::sample<float>(T, x, y, z, 0.5f, 0.5f, 0.5f, h, n, n, n));
+            hfm::backtrace(u, v, w, x, y, z, h, n, dt);
+            std::printf("%g\n", (double)hfm::sample(T, x, y, z, 0.5f, 0.5f, 0.5f, h, n, n, n));
         }
     }
     free(u);

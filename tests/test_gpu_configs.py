@@ -4,8 +4,7 @@
   particles) with the autotuned batch layouts, stepped in one batch for the configuration's 200
   substeps; one scene of every size is stepped alone with the same layout and must give the same bits.
 * C3, mixed precision: the coupled scene at n = 192 with about 50k particles, the configuration's 100
-  substeps in the mixed form (FP16 PCG vectors inside iterative refinement, BF16 departure velocity)
-  and in fp32: finite, projected (divergence_max small), mixed within 1e-3 of fp32.  The same scene at
+  substeps in the mixed form (FP16 PCG vectors inside iterative refinement) and in fp32: finite, projected (divergence_max small), mixed within 1e-3 of fp32.  The same scene at
   n = 48 against the fp64 oracle after 100 substeps: the error of T, u and div(u) of both forms, per
   result.
 * C5, served: seeded Poisson streams over n in {48, 64, 96, 128, 192} in the mixed form with

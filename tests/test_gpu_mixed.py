@@ -1,5 +1,5 @@
-"""Mixed precision (HFM_PREC_MIXED): iterative refinement with FP16 inner PCG vectors and BF16
-departure-point velocity, against the fp64 oracle (the truth the mixed form is held to) and
+"""Mixed precision (HFM_PREC_MIXED): iterative refinement with FP16 inner PCG vectors, against the
+fp64 oracle (the truth the mixed form is held to) and
 against the fp32 form on the GPU.
 
 Bars: T and u within relative L2 1e-3 of fp64 (the served form's tolerance); the pressure up to
@@ -117,8 +117,8 @@ def test_mixed_iteration_cap(gpu):
 
 
 def test_velocity_written_by_the_host_reaches_the_advection(gpu):
-    """Fields copied from one mixed scene into another (the BF16 velocity copy is refreshed by
-    set_field) make the next substep identical."""
+    """Fields copied from one mixed scene into another through set_field make the next substep
+    identical."""
     kw, dt, _ = plume_scene()
     a = HeatflowMPM(**kw, precision="mixed")
     for _ in range(5):
