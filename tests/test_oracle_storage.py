@@ -1,8 +1,10 @@
 """Which low-precision storage keeps the substep within the served tolerance (1e-3 relative L2
 against fp64)?  The oracle rounds the stored velocity or temperature to BF16 / FP16 after every
 write (or traces departure points through a BF16 velocity copy) and runs the heater plume
-against the fp64 oracle.  These are the measurements that fix the mixed form's design: BF16 or
-FP16 state storage fails the tolerance, the BF16 departure-point velocity does not."""
+against the fp64 oracle.  These are the measurements behind the mixed form's design: BF16 or
+FP16 state storage fails the tolerance.  The BF16 departure-point velocity keeps it over this short
+plume run, but a long one (689 substeps of a 192^3 plume) drifted 10 % from the fp32 result with it
+and 0.08 % without, so the mixed form advects with the fp32 velocity."""
 import numpy as np
 import pytest
 
@@ -44,6 +46,6 @@ def test_low_precision_state_fails_the_bar(runs, name):
     assert max(eT, eu) > BAR
 
 
-def test_bf16_departure_velocity_keeps_the_bar(runs):
+def test_bf16_departure_velocity_keeps_the_bar_over_a_short_run(runs):
     eT, eu = runs["backtrace_bf16"]
     assert eT < 2e-4 and eu < 2e-4
