@@ -1280,13 +1280,16 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init_fused(Src src) {
 // only what a neighbour needs: the r, s and w of the cells on the tile's faces, and the three partial
 // sums.  The launch boundary that separates two iterations of the one-launch form becomes the partial
 // sums themselves: iteration k stores its faces, waits for those stores to complete, then stores its
-// three sums in slot k % 3; iteration k + 1 starts when no sum of that slot reads as empty (a signalling
-// NaN pattern, which no arithmetic produces).  A workgroup runs at most one iteration ahead of any
+// three sums in slot k % 3 of every copy of the slots (kPersistReplicas, one per XCD, each on pages of
+// its own); iteration k + 1 starts when no sum of that slot in the copy of the workgroup's XCD reads as
+// empty (a signalling NaN pattern, which no arithmetic produces), so the polls of the whole device
+// spread over eight sets of pages.  A workgroup runs at most one iteration ahead of any
 // other (it cannot pass an iteration before every workgroup has posted its sums), so the faces keep the
-// one-launch form's two parity buffers, and a workgroup empties its entries of slot (k + 1) % 3 as
-// iteration k starts: every reader of what they held (the sums of iteration k - 2) has passed
+// one-launch form's two parity buffers, and a workgroup empties its entries of slot (k + 1) % 3 (in
+// every copy) as iteration k starts: every reader of what they held (the sums of iteration k - 2) has passed
 // iteration k - 1, and the next sums they take are those of iteration k + 1, posted after the emptying
-// (stores of one thread complete in order of its waits).  Every cross-workgroup access is a relaxed
+// (a copy's entries are emptied and posted by the same thread, whose stores complete in order of its
+// waits).  Every cross-workgroup access is a relaxed
 // agent-scope atomic, coherent across the XCDs without L2 write-back or invalidation.
 //
 // Each workgroup computes what its launch of k_pcg_fused computes, from the same values, in the same
