@@ -785,22 +785,30 @@ __global__ __launch_bounds__(kBlock) void k_ir_residual(Src src, float dt, int f
         s[1] += (double)bc * (double)bc;
     }
     store_partials<2>(s, a.partials + kIrPart, v.nb_upd, lb, lds);
-    if (first && lb == 0 && threadIdx.x == 0) {
-        c->done = 0;
-        c->brk = 0;
-        c->converged = 0;
-        c->it = 0;
-        c->ir_total = 0;
-        c->ir_outer = 0;
-        c->ir_end = 0;
+    if (lb == 0 && threadIdx.x == 0) {
+        if (first) {
+            c->done = 0;
+            c->brk = 0;
+            c->converged = 0;
+            c->it = 0;
+            c->ir_total = 0;
+            c->ir_outer = 0;
+            c->ir_end = 0;
+            c->ir_inner_conv = 0;
+        } else {
+            c->ir_inner_conv = c->converged == 1;   // the inner solve that just ended reached its tolerance
+        }
     }
 }
 
 // Decide the refinement step from the residual's partial sums: stop the solve when
 // ||r|| <= atol (atol = max(tol ||b||, tol), formed at the first step), when the last step cut
-// ||r|| by less than sqrt(2) (the true residual has reached the floor of its fp32 evaluation,
-// which can lie above atol: the fp32 solve, which tests its recursively updated residual, stops
-// there too), at the iteration cap, or on a non-finite residual; otherwise write r^ = r / sigma
+// ||r|| by less than sqrt(2), or, in a multigrid scene, by less than 4 after an inner solve that
+// reached its tolerance (its 1-3 multigrid iterations cut the true residual about 1/eta = 100 times
+// away from the floor; Jacobi inner solves run hundreds of FP16 iterations and can cut it less, so
+// they keep the first rule) -- either way the true residual has reached the floor of its fp32
+// evaluation, which can lie above atol: the fp32 solve, which tests its recursively updated residual,
+// stops there too -- at the iteration cap, or on a non-finite residual; otherwise write r^ = r / sigma
 // and z^ = (D / c) r^ in FP16 with their partial sums, which the inner solve's first pass A
 // reads, and set the inner solve's state.
 // step: the refinement step (0 the first), which the host counts; ||r||^2 of each step is kept in
@@ -831,7 +839,8 @@ __global__ __launch_bounds__(kBlock) void k_ir_scale(Src src, int step) {
     }
     const bool finite = rr <= 1.0e300;   // false for NaN and infinity
     const bool conv = finite && rr <= atol * atol;
-    const bool stalled = !first && finite && !conv && rr > 0.5 * rr_prev;
+    const bool stalled = !first && finite && !conv &&
+                         (rr > 0.5 * rr_prev || (a.cg_mg && c->ir_inner_conv && rr > 0.0625 * rr_prev));
     if (conv || stalled || !finite || total >= maxiter) {
         if (lb == 0 && threadIdx.x == 0) {
             c->ir_end = 1;

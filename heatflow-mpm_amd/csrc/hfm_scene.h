@@ -43,6 +43,7 @@ struct CgCtl {
     int ir_total;      // inner iterations of this solve so far (written when an inner solve stops)
     int ir_outer;      // refinement steps of this solve so far
     int ir_end;        // 1: the refinement has converged or reached the cap
+    int ir_inner_conv; // 1: the last inner solve reached its tolerance (written by the residual launch)
     int ir_maxiter;    // iteration cap (inner iterations over the whole solve)
     double ir_rr2[2];  // ||r||^2 (true residual) of refinement step s, in slot s & 1
     double src_dt;     // ctl[0]: sum of the source step's temperature rises of this substep (fp64)
