@@ -122,8 +122,9 @@ typedef struct hfm_stats {
     int32_t h_iters;          /* heat PCG count as warp returns it: HeatField.cg_iters */
     int32_t h_iters_exact;
     int32_t p_converged;      /* 1 when ||r||^2 <= atol^2 at exit, 0 at the iteration cap; 2: stopped at the
-                                 fp32 floor of the residual, above atol (HFM_PREC_MIXED: a refinement step no
-                                 longer cut the true residual; HFM_FLAG_MG_PCG in fp32: ||r||^2 set no new
+                                 fp32 floor of the residual, above atol (HFM_PREC_MIXED: a refinement step cut
+                                 the true residual by less than sqrt 2, or with HFM_FLAG_MG_PCG by less
+                                 than 4 after an inner solve that reached its tolerance; HFM_FLAG_MG_PCG in fp32: ||r||^2 set no new
                                  minimum for 8 iterations or grew 100 times above its minimum, where the
                                  multigrid-preconditioned iteration would diverge).  -1: stopped on a breakdown (no counterpart in warp, which
                                  runs on to its cap): p.Ap or r.z not positive, alpha or beta not finite,
