@@ -527,6 +527,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(Src src, float dt, int k,
     post_pace(pace, tag, k);
     HFM_ENTRY(src);
     if (batch_solve_done(src, a.ctl + OP)) return;
+    if (IR && a.ctl[OP].done) return;   // as k_pcg_dir_tile3: launches queued past an inner solve's stop
     __shared__ double lds[3][kWaves];
     const auto v = make_view<OP, IR>(a, lb, k, dt);
     PassAIn pre;
@@ -576,6 +577,9 @@ __global__ __launch_bounds__(kBlock3) void k_pcg_dir_tile3(Src src, int kit, uns
     post_pace(pace, tag, kit);
     HFM_ENTRY(src);
     if (batch_solve_done(src, a.ctl + 1)) return;
+    // an inner solve of the refinement takes a few iterations, so launches queued past its stop are
+    // common: they skip the frame's loads (a scene alone otherwise issues them with the prologue's)
+    if (IR && a.ctl[1].done) return;
     __shared__ float P[D + 2][kF3][kF3];
     __shared__ float S[D + 2][kF3][kF3];
     __shared__ double lds[1][kBlock3 / 64];
