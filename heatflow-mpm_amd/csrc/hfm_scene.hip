@@ -745,8 +745,10 @@ int pcg_iterate(Batch* b, int op, float dt, bool ir, int maxeff, int pred) {
         }
         if (inflight < 2 && launched < maxeff) {
             RC(refresh_active());
+            // past the predicted count, one iteration per group: the next group's first pass A
+            // finishes the last iteration, so a solve that ends as predicted leaves at most one queued
             int g = pred - launched;
-            g = g < 2 ? 2 : (g > 16 ? 16 : g);
+            g = launched >= pred ? 1 : (g < 2 ? 2 : (g > 16 ? 16 : g));
             if (g > maxeff - launched) g = maxeff - launched;
             for (int q = 0; q < g; ++q) {
                 const int k = launched + q;
