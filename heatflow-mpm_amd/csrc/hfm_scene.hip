@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <functional>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -597,7 +598,11 @@ int launch_persist_groups(Batch* b, const std::vector<char>& persist, bool timed
 // Run PCG iterations from iteration 0 for every scene of the batch whose mirror is not done
 // (the init launches are enqueued), and return when the device reports all of them done.
 // ir: the inner solves of a mixed-precision refinement step (FP16 kernels).
-int pcg_iterate(Batch* b, int op, float dt, bool ir, int maxeff, int pred) {
+// tail: launches that follow the solve in the stream; where the whole solve is already enqueued (the
+// persistent form) they are enqueued before the wait for it, so the GPU does not idle while the host
+// learns that the solve has ended; returns whether it did (else the caller enqueues them)
+int pcg_iterate(Batch* b, int op, float dt, bool ir, int maxeff, int pred, const std::function<void()>& tail,
+                bool* tail_done) {
     const unsigned tag = ++b->solve_tag;
     const int mg = op == 1 ? mg_depth(b, true) : 0;
     // pass A launches: the heat operator uses the flat kernel with the PCG geometry; the
@@ -679,6 +684,10 @@ int pcg_iterate(Batch* b, int op, float dt, bool ir, int maxeff, int pred) {
     // every solve.  Group size follows the iteration count of the previous solve so that few
     // launches are wasted after convergence.
     int launched = 0;
+    bool past_pred = false;
+    long cells = 0;
+    for (Scene* s : b->scenes) cells += (long)s->n * s->n * s->n;
+    const bool single_ok = ir || cells >= 96L * 96 * 96;
     int group_end[2] = {0, 0};
     int inflight = 0;
     bool closed = false;
@@ -702,6 +711,11 @@ int pcg_iterate(Batch* b, int op, float dt, bool ir, int maxeff, int pred) {
         }
         launched = maxeff;
         closed = true;
+        if (tail) {
+            tail();
+            CHECK_LAUNCH();
+            *tail_done = true;
+        }
     } else if (op == 1 && !ir && ns > 1) {
         // a batch: the scenes that can run the persistent form solve in cooperative groups first, the
         // others (if any) in the loop below, from which the persistent ones are left out
@@ -724,6 +738,11 @@ int pcg_iterate(Batch* b, int op, float dt, bool ir, int maxeff, int pred) {
             if (n_active == 0) {
                 launched = maxeff;
                 closed = true;
+                if (tail) {
+                    tail();
+                    CHECK_LAUNCH();
+                    *tail_done = true;
+                }
             } else {
                 RC(filter_families(b, iter_fams, niter, active, fl));
             }
@@ -745,10 +764,19 @@ int pcg_iterate(Batch* b, int op, float dt, bool ir, int maxeff, int pred) {
         }
         if (inflight < 2 && launched < maxeff) {
             RC(refresh_active());
-            // past the predicted count, one iteration per group: the next group's first pass A
-            // finishes the last iteration, so a solve that ends as predicted leaves at most one queued
+            // groups past the predicted count hold one iteration (the next group's first pass A finishes
+            // the last iteration, so a solve that ends as predicted leaves one queued, not two): the
+            // first of them always; every one for short solves (a prediction of at most 4: the heat
+            // solve, the refinement's inner solves) whose iterations outlast the host's turnaround (an
+            // inner solve's, with its V-cycle, or a batch of at least 96^3 cells); otherwise groups of
+            // two, so that the host does not pace short iterations one by one
             int g = pred - launched;
-            g = launched >= pred ? 1 : (g < 2 ? 2 : (g > 16 ? 16 : g));
+            if (launched >= pred) {
+                g = (pred <= 4 && single_ok) || !past_pred ? 1 : 2;
+                past_pred = true;
+            } else {
+                g = g < 2 ? 2 : (g > 16 ? 16 : g);
+            }
             if (g > maxeff - launched) g = maxeff - launched;
             for (int q = 0; q < g; ++q) {
                 const int k = launched + q;
@@ -796,7 +824,7 @@ int pcg_iterate(Batch* b, int op, float dt, bool ir, int maxeff, int pred) {
 
 // Enqueue one PCG solve (op 0 heat, 1 pressure) of every scene of the batch and return when
 // the device reports all of them done.
-int run_pcg(Batch* b, int op, float dt) {
+int run_pcg(Batch* b, int op, float dt, const std::function<void()>& tail, bool* tail_done) {
     int maxeff = 0, pred = 2;
     bool any_fused = false;
     for (Scene* s : b->scenes) {
@@ -815,7 +843,7 @@ int run_pcg(Batch* b, int op, float dt) {
         RC(launch_mg_vcycle(b, mg_depth(b, true)));
     }
     CHECK_LAUNCH();
-    RC(pcg_iterate(b, op, dt, false, maxeff, pred));
+    RC(pcg_iterate(b, op, dt, false, maxeff, pred, tail, tail_done));
     int bad = -1;
     for (size_t si = 0; si < b->scenes.size(); ++si) {
         Scene* s = b->scenes[si];
@@ -883,7 +911,7 @@ int run_ir(Batch* b, int op, float dt) {
     if (mg) RC(launch_mg_vcycle(b, mg, true));
     CHECK_LAUNCH();
     for (int step = 0;; ++step) {
-        RC(pcg_iterate(b, op, dt, true, maxeff, pred));
+        RC(pcg_iterate(b, op, dt, true, maxeff, pred, {}, nullptr));
         bool all_end = true;
         int longest = 2;
         for (Scene* s : b->scenes) {
@@ -974,15 +1002,17 @@ int batch_substep(Batch* b, double dt) {
     for (Scene* s : b->scenes) mc |= s->dev.maccormack != 0;
     if (mc) launch_mc_T(family(b, FAM_CELL), dt32);
     CHECK_LAUNCH();
-    RC(mixed ? run_ir(b, 0, dt32) : run_pcg(b, 0, dt32));
+    RC(mixed ? run_ir(b, 0, dt32) : run_pcg(b, 0, dt32, {}, nullptr));
     launch_g2p(family(b, FAM_G2P));
     launch_air_faces(family(b, FAM_FACE3), dt32);
     if (mc) launch_mc_faces(family(b, FAM_FACE3), dt32);
     launch_divergence(family(b, FAM_CELL), dt);
     launch_rhs_mean(family(b, FAM_MEAN));
     CHECK_LAUNCH();
-    RC(mixed ? run_ir(b, 1, dt32) : run_pcg(b, 1, dt32));
-    launch_project(family(b, FAM_FACE3), dt);
+    bool projected = false;   // the projection enqueued behind a persistent solve, before the wait for it
+    const auto project = [&] { launch_project(family(b, FAM_FACE3), dt); };
+    RC(mixed ? run_ir(b, 1, dt32) : run_pcg(b, 1, dt32, project, &projected));
+    if (!projected) launch_project(family(b, FAM_FACE3), dt);
     CHECK_LAUNCH();
     for (Scene* s : b->scenes) s->stats.substeps += 1;
     return HFM_OK;
